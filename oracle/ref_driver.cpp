@@ -1,0 +1,608 @@
+// Checker-side driver that runs Mjolnir's own force-and-step classes on an mjspec file.
+//
+// TEST INFRASTRUCTURE ONLY: this file is compiled against the reference sources where they
+// lie (see oracle/Makefile) into oracle/_ref/.  It is used to write golden vectors and to time
+// the reference CPU build as a baseline; nothing in the product links or calls it.
+//
+// It builds the objects the reference's TOML front end would build, without toml11 (absent
+// here): System (mjolnir/core/System.hpp:43), ForceField (mjolnir/core/ForceField.hpp:32),
+// GlobalPairInteraction with LorentzBerthelotRule / ExcludedVolumeParameterList /
+// DebyeHuckelParameterList and a CellList / VerletList / Naive partition
+// (mjolnir/input/read_global_interaction.hpp, read_spatial_partition.hpp:62-116), the local
+// BondLength / BondAngle / DihedralAngle / Contact interactions
+// (mjolnir/input/read_local_interaction.hpp), and BAOABLangevinIntegrator /
+// GJFNVTLangevinIntegrator (mjolnir/input/read_integrator.hpp).  Units are set exactly as
+// read_units does for "kcal/mol" and "angstrom" (mjolnir/input/read_units.hpp:32-92).
+//
+// Commands:
+//   ref_driver forces <spec> <out>         per-interaction energies, forces and pair lists
+//   ref_driver run    <spec> <nsteps> <out> [every]   integrate, dump x/v (every `every` steps)
+//   ref_driver bench  <spec> <nsteps>      time the step loop only, print one JSON line
+#include <mjolnir/core/SimulatorTraits.hpp>
+#include <mjolnir/core/BoundaryCondition.hpp>
+#include <mjolnir/core/System.hpp>
+#include <mjolnir/core/Unit.hpp>
+#include <mjolnir/core/ForceField.hpp>
+#include <mjolnir/core/RandomNumberGenerator.hpp>
+#include <mjolnir/core/BAOABLangevinIntegrator.hpp>
+#include <mjolnir/core/GJFNVTLangevinIntegrator.hpp>
+#include <mjolnir/core/SystemMotionRemover.hpp>
+#include <mjolnir/core/PeriodicGridCellList.hpp>
+#include <mjolnir/core/UnlimitedGridCellList.hpp>
+#include <mjolnir/core/VerletList.hpp>
+#include <mjolnir/core/NaivePairCalculation.hpp>
+#include <mjolnir/forcefield/global/GlobalPairInteraction.hpp>
+#include <mjolnir/forcefield/global/GlobalPairLennardJonesInteraction.hpp>
+#include <mjolnir/forcefield/global/GlobalPairExcludedVolumeInteraction.hpp>
+#include <mjolnir/forcefield/global/LennardJonesPotential.hpp>
+#include <mjolnir/forcefield/global/ExcludedVolumePotential.hpp>
+#include <mjolnir/forcefield/global/DebyeHuckelPotential.hpp>
+#include <mjolnir/forcefield/local/BondLengthInteraction.hpp>
+#include <mjolnir/forcefield/local/BondAngleInteraction.hpp>
+#include <mjolnir/forcefield/local/DihedralAngleInteraction.hpp>
+#include <mjolnir/forcefield/local/ContactInteraction.hpp>
+#include <mjolnir/forcefield/local/GoContactInteraction.hpp>
+#include <mjolnir/forcefield/local/HarmonicPotential.hpp>
+#include <mjolnir/forcefield/local/GaussianPotential.hpp>
+#include <mjolnir/forcefield/local/ClementiDihedralPotential.hpp>
+#include <mjolnir/forcefield/local/GoContactPotential.hpp>
+#ifdef MJOLNIR_WITH_OPENMP
+#include <mjolnir/omp/omp.hpp>
+#endif
+
+#include <chrono>
+#include <cstdio>
+#include <fstream>
+#include <functional>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace drv
+{
+using namespace mjolnir;
+
+struct LocalIn
+{
+    std::string interaction, potential, topology;
+    double margin;
+    std::vector<std::vector<std::size_t>> idx;
+    std::vector<std::vector<double>> par;
+};
+struct GlobalIn
+{
+    std::string potential, partition, ignore_mol;
+    double cutoff, epsilon, margin;
+    std::map<std::string, std::size_t> within;
+    std::vector<std::size_t> participants;
+    std::vector<std::vector<double>> par;
+};
+struct SpecIn
+{
+    bool periodic = false;
+    double lower[3] = {0, 0, 0}, upper[3] = {0, 0, 0};
+    std::map<std::string, double> attributes;
+    std::vector<double> mass, x, v;  // x, v: 3N
+    std::vector<LocalIn> local;
+    std::vector<GlobalIn> global;
+    bool has_integrator = false;
+    std::string integrator;
+    double dt = 0;
+    std::uint32_t seed = 0;
+    std::vector<double> coefs;
+};
+
+static SpecIn read_spec(const std::string& path)
+{
+    std::ifstream ifs(path);
+    if(!ifs) {throw std::runtime_error("cannot open " + path);}
+    SpecIn s;
+    std::string tok;
+    ifs >> tok; int ver; ifs >> ver;
+    if(tok != "mjspec") {throw std::runtime_error("not an mjspec file");}
+    while(ifs >> tok)
+    {
+        if(tok == "end") {break;}
+        if(tok == "boundary")
+        {
+            std::string kind; ifs >> kind;
+            s.periodic = (kind == "periodic");
+            if(s.periodic)
+            {
+                ifs >> s.lower[0] >> s.lower[1] >> s.lower[2] >> s.upper[0] >> s.upper[1] >> s.upper[2];
+            }
+        }
+        else if(tok == "attribute")
+        {
+            std::string k; double val; ifs >> k >> val; s.attributes[k] = val;
+        }
+        else if(tok == "particles")
+        {
+            std::size_t n; ifs >> n;
+            s.mass.resize(n); s.x.resize(3 * n); s.v.resize(3 * n);
+            for(std::size_t i = 0; i < n; ++i)
+            {
+                ifs >> s.mass[i] >> s.x[3*i] >> s.x[3*i+1] >> s.x[3*i+2]
+                    >> s.v[3*i] >> s.v[3*i+1] >> s.v[3*i+2];
+            }
+        }
+        else if(tok == "local")
+        {
+            LocalIn l; std::size_t n;
+            ifs >> l.interaction >> l.potential >> l.topology >> n >> l.margin;
+            std::size_t arity = 2, npar = 2;
+            if(l.interaction == "BondAngle") {arity = 3;}
+            if(l.interaction == "DihedralAngle") {arity = 4;}
+            if(l.potential == "Gaussian" || l.potential == "ClementiDihedral") {npar = 3;}
+            for(std::size_t t = 0; t < n; ++t)
+            {
+                std::vector<std::size_t> ix(arity); std::vector<double> p(npar);
+                for(auto& i : ix) {ifs >> i;}
+                for(auto& q : p)  {ifs >> q;}
+                l.idx.push_back(ix); l.par.push_back(p);
+            }
+            s.local.push_back(l);
+        }
+        else if(tok == "global")
+        {
+            GlobalIn g; std::size_t n; std::string within;
+            ifs >> g.potential >> n >> g.cutoff >> g.epsilon >> g.partition >> g.margin
+                >> g.ignore_mol >> within;
+            if(within != "-")
+            {
+                std::stringstream ss(within); std::string item;
+                while(std::getline(ss, item, ','))
+                {
+                    const auto c = item.find(':');
+                    g.within[item.substr(0, c)] = std::stoul(item.substr(c + 1));
+                }
+            }
+            const std::size_t npar = (g.potential == "LennardJones") ? 2 : 1;
+            for(std::size_t t = 0; t < n; ++t)
+            {
+                std::size_t i; std::vector<double> p(npar);
+                ifs >> i; for(auto& q : p) {ifs >> q;}
+                g.participants.push_back(i); g.par.push_back(p);
+            }
+            s.global.push_back(g);
+        }
+        else if(tok == "integrator")
+        {
+            s.has_integrator = true;
+            ifs >> s.integrator >> s.dt >> s.seed;
+            s.coefs.resize(s.mass.size());
+            for(auto& c : s.coefs) {ifs >> c;}
+        }
+        else {throw std::runtime_error("unknown token " + tok);}
+    }
+    return s;
+}
+
+// read_units for energy = "kcal/mol", length = "angstrom" (read_units.hpp:32-92).
+template<typename realT>
+static void set_units_kcal_angstrom()
+{
+    using phys_type = physics::constants<realT>;
+    using unit_type = unit::constants<realT>;
+    phys_type::set_kB(phys_type::kB() * (unit_type::J_to_cal() / 1000.0) * unit_type::avogadro_constant());
+    phys_type::set_eps0(phys_type::eps0() * (1000.0 / unit_type::J_to_cal()) / unit_type::avogadro_constant());
+    phys_type::set_energy_unit("kcal/mol");
+    phys_type::set_eps0(phys_type::eps0() / unit_type::m_to_angstrom());
+    phys_type::set_m_to_length(unit_type::m_to_angstrom());
+    phys_type::set_length_to_m(unit_type::angstrom_to_m());
+    phys_type::set_L_to_volume(1e-3 * std::pow(unit_type::m_to_angstrom(), 3));
+    phys_type::set_volume_to_L(1e+3 * std::pow(unit_type::angstrom_to_m(), 3));
+    phys_type::set_length_unit("angstrom");
+}
+
+template<typename traitsT>
+struct Built
+{
+    using real_type = typename traitsT::real_type;
+    using coord     = typename traitsT::coordinate_type;
+    using system_type = System<traitsT>;
+    using ff_type     = ForceField<traitsT>;
+
+    std::unique_ptr<system_type> sys;
+    std::unique_ptr<ForceFieldBase<traitsT>> ff;
+    // pair-list dumpers, one per global interaction, in input order
+    std::vector<std::function<void(std::ostream&)>> dump_pairs;
+};
+
+template<typename traitsT, typename potT, typename partitionBase>
+static std::unique_ptr<partitionBase> make_partition(const GlobalIn& g)
+{
+    using boundary_type = typename traitsT::boundary_type;
+    if(g.partition == "CellList")
+    {
+        if(is_cuboidal_periodic_boundary<boundary_type>::value)
+        {
+            return std::unique_ptr<partitionBase>(new PeriodicGridCellList<traitsT, potT>(g.margin));
+        }
+        return std::unique_ptr<partitionBase>(new UnlimitedGridCellList<traitsT, potT>(g.margin));
+    }
+    if(g.partition == "VerletList")
+    {
+        return std::unique_ptr<partitionBase>(new VerletList<traitsT, potT>(g.margin));
+    }
+    if(g.partition == "Naive")
+    {
+        return std::unique_ptr<partitionBase>(new NaivePairCalculation<traitsT, potT>());
+    }
+    throw std::runtime_error("unknown partition " + g.partition);
+}
+
+template<typename interactionT>
+static std::function<void(std::ostream&)> pair_dumper(const interactionT* inter, std::vector<std::size_t> participants)
+{
+    return [inter, participants](std::ostream& os) {
+        const auto& nl = inter->partition().neighbors();
+        std::vector<std::pair<std::size_t, std::size_t>> pairs;
+        for(std::size_t k = 0; k + 1 < participants.size(); ++k) // leading participants
+        {
+            const auto i = participants[k];
+            for(const auto& ptnr : inter->partition().partners(i))
+            {
+                pairs.emplace_back(i, ptnr.index);
+            }
+        }
+        (void)nl;
+        os << "pairs " << pairs.size() << "\n";
+        for(const auto& p : pairs) {os << p.first << ' ' << p.second << "\n";}
+    };
+}
+
+template<typename traitsT>
+static Built<traitsT> build(const SpecIn& s)
+{
+    using real_type   = typename traitsT::real_type;
+    using coord       = typename traitsT::coordinate_type;
+    using boundary_t  = typename traitsT::boundary_type;
+    Built<traitsT> b;
+
+    boundary_t bdry;
+    if(s.periodic)
+    {
+        bdry.set_boundary(coord(s.lower[0], s.lower[1], s.lower[2]), coord(s.upper[0], s.upper[1], s.upper[2]));
+    }
+    const std::size_t N = s.mass.size();
+    b.sys.reset(new System<traitsT>(N, bdry));
+    auto& sys = *b.sys;
+    for(const auto& kv : s.attributes) {sys.attribute(kv.first) = kv.second;}
+    for(std::size_t i = 0; i < N; ++i)
+    {
+        sys.mass(i)     = s.mass[i];
+        sys.rmass(i)    = 1.0 / s.mass[i];
+        sys.position(i) = coord(s.x[3*i], s.x[3*i+1], s.x[3*i+2]);
+        sys.velocity(i) = coord(s.v[3*i], s.v[3*i+1], s.v[3*i+2]);
+        sys.force(i)    = coord(0, 0, 0);
+        sys.name(i)     = "X";
+        sys.group(i)    = "NONE";
+    }
+    sys.velocity_initialized() = true;
+
+    LocalForceField<traitsT> local;
+    for(const auto& l : s.local)
+    {
+        if(l.interaction == "BondLength" && l.potential == "Harmonic")
+        {
+            using pot = HarmonicPotential<real_type>;
+            using I = BondLengthInteraction<traitsT, pot>;
+            typename I::container_type c;
+            for(std::size_t t = 0; t < l.idx.size(); ++t)
+            {
+                c.emplace_back(typename I::indices_type{{l.idx[t][0], l.idx[t][1]}}, pot(l.par[t][0], l.par[t][1]));
+            }
+            local.emplace(std::unique_ptr<LocalInteractionBase<traitsT>>(new I(l.topology, std::move(c))));
+        }
+        else if(l.interaction == "BondLength" && l.potential == "GoContact")
+        {
+            using pot = GoContactPotential<real_type>;
+            using I = BondLengthInteraction<traitsT, pot>;
+            typename I::container_type c;
+            for(std::size_t t = 0; t < l.idx.size(); ++t)
+            {
+                c.emplace_back(typename I::indices_type{{l.idx[t][0], l.idx[t][1]}}, pot(l.par[t][0], l.par[t][1]));
+            }
+            local.emplace(std::unique_ptr<LocalInteractionBase<traitsT>>(new I(l.topology, std::move(c))));
+        }
+        else if(l.interaction == "Contact" && l.potential == "GoContact")
+        {
+            using pot = GoContactPotential<real_type>;
+            using I = ContactInteraction<traitsT, pot>;
+            typename I::container_type c;
+            for(std::size_t t = 0; t < l.idx.size(); ++t)
+            {
+                c.emplace_back(typename I::indices_type{{l.idx[t][0], l.idx[t][1]}}, pot(l.par[t][0], l.par[t][1]));
+            }
+            local.emplace(std::unique_ptr<LocalInteractionBase<traitsT>>(new I(l.topology, std::move(c), l.margin)));
+        }
+        else if(l.interaction == "BondAngle" && l.potential == "Gaussian")
+        {
+            using pot = GaussianPotential<real_type>;
+            using I = BondAngleInteraction<traitsT, pot>;
+            typename I::container_type c;
+            for(std::size_t t = 0; t < l.idx.size(); ++t)
+            {
+                c.emplace_back(typename I::indices_type{{l.idx[t][0], l.idx[t][1], l.idx[t][2]}},
+                               pot(l.par[t][0], l.par[t][1], l.par[t][2]));
+            }
+            local.emplace(std::unique_ptr<LocalInteractionBase<traitsT>>(new I(l.topology, std::move(c))));
+        }
+        else if(l.interaction == "BondAngle" && l.potential == "Harmonic")
+        {
+            using pot = HarmonicPotential<real_type>;
+            using I = BondAngleInteraction<traitsT, pot>;
+            typename I::container_type c;
+            for(std::size_t t = 0; t < l.idx.size(); ++t)
+            {
+                c.emplace_back(typename I::indices_type{{l.idx[t][0], l.idx[t][1], l.idx[t][2]}},
+                               pot(l.par[t][0], l.par[t][1]));
+            }
+            local.emplace(std::unique_ptr<LocalInteractionBase<traitsT>>(new I(l.topology, std::move(c))));
+        }
+        else if(l.interaction == "DihedralAngle" && l.potential == "ClementiDihedral")
+        {
+            using pot = ClementiDihedralPotential<real_type>;
+            using I = DihedralAngleInteraction<traitsT, pot>;
+            typename I::container_type c;
+            for(std::size_t t = 0; t < l.idx.size(); ++t)
+            {
+                c.emplace_back(typename I::indices_type{{l.idx[t][0], l.idx[t][1], l.idx[t][2], l.idx[t][3]}},
+                               pot(l.par[t][0], l.par[t][1], l.par[t][2]));
+            }
+            local.emplace(std::unique_ptr<LocalInteractionBase<traitsT>>(new I(l.topology, std::move(c))));
+        }
+        else
+        {
+            throw std::runtime_error("unsupported local " + l.interaction + ":" + l.potential);
+        }
+    }
+
+    GlobalForceField<traitsT> global;
+    for(const auto& g : s.global)
+    {
+        IgnoreMolecule<typename Topology::molecule_id_type> ign_mol(g.ignore_mol);
+        IgnoreGroup<typename Topology::group_id_type> ign_grp({});
+        if(g.potential == "LennardJones")
+        {
+            using pot = LennardJonesPotential<real_type>;
+            using I = GlobalPairInteraction<traitsT, pot>;
+            std::vector<std::pair<std::size_t, typename pot::parameter_type>> ps;
+            for(std::size_t t = 0; t < g.participants.size(); ++t)
+            {
+                ps.emplace_back(g.participants[t], typename pot::parameter_type{g.par[t][0], g.par[t][1]});
+            }
+            ParameterList<traitsT, pot> pl(make_unique<LorentzBerthelotRule<traitsT, pot>>(ps, g.within, ign_mol, ign_grp));
+            SpatialPartition<traitsT, pot> part(make_partition<traitsT, pot, SpatialPartitionBase<traitsT, pot>>(g));
+            auto* inter = new I(pot(g.cutoff), std::move(pl), std::move(part));
+            b.dump_pairs.push_back(pair_dumper(inter, g.participants));
+            global.emplace(std::unique_ptr<GlobalInteractionBase<traitsT>>(inter));
+        }
+        else if(g.potential == "ExcludedVolume")
+        {
+            using pot = ExcludedVolumePotential<real_type>;
+            using I = GlobalPairInteraction<traitsT, pot>;
+            std::vector<std::pair<std::size_t, typename pot::parameter_type>> ps;
+            for(std::size_t t = 0; t < g.participants.size(); ++t)
+            {
+                ps.emplace_back(g.participants[t], typename pot::parameter_type{g.par[t][0]});
+            }
+            ParameterList<traitsT, pot> pl(make_unique<ExcludedVolumeParameterList<traitsT>>(ps, g.within, ign_mol, ign_grp));
+            SpatialPartition<traitsT, pot> part(make_partition<traitsT, pot, SpatialPartitionBase<traitsT, pot>>(g));
+            auto* inter = new I(pot(g.cutoff, g.epsilon), std::move(pl), std::move(part));
+            b.dump_pairs.push_back(pair_dumper(inter, g.participants));
+            global.emplace(std::unique_ptr<GlobalInteractionBase<traitsT>>(inter));
+        }
+        else if(g.potential == "DebyeHuckel")
+        {
+            using pot = DebyeHuckelPotential<real_type>;
+            using PL  = DebyeHuckelParameterList<traitsT>;
+            using I = GlobalPairInteraction<traitsT, pot>;
+            std::vector<std::pair<std::size_t, typename PL::parameter_type>> ps;
+            for(std::size_t t = 0; t < g.participants.size(); ++t)
+            {
+                ps.emplace_back(g.participants[t], typename PL::parameter_type{g.par[t][0]});
+            }
+            ParameterList<traitsT, pot> pl(make_unique<PL>(ps, g.within, ign_mol, ign_grp));
+            SpatialPartition<traitsT, pot> part(make_partition<traitsT, pot, SpatialPartitionBase<traitsT, pot>>(g));
+            auto* inter = new I(pot(g.cutoff), std::move(pl), std::move(part));
+            b.dump_pairs.push_back(pair_dumper(inter, g.participants));
+            global.emplace(std::unique_ptr<GlobalInteractionBase<traitsT>>(inter));
+        }
+        else
+        {
+            throw std::runtime_error("unsupported global " + g.potential);
+        }
+    }
+    b.ff.reset(new ForceField<traitsT>(std::move(local), std::move(global),
+                                        ExternalForceField<traitsT>{}, ConstraintForceField<traitsT>{}));
+    return b;
+}
+
+static void write_vec3(std::ostream& os, const char* tag, const std::vector<double>& v)
+{
+    const std::size_t n = v.size() / 3;
+    os << tag << ' ' << n << "\n";
+    char buf[128];
+    for(std::size_t i = 0; i < n; ++i)
+    {
+        std::snprintf(buf, sizeof(buf), "%.17g %.17g %.17g\n", v[3*i], v[3*i+1], v[3*i+2]);
+        os << buf;
+    }
+}
+
+template<typename traitsT>
+static std::vector<double> forces_of(const System<traitsT>& sys)
+{
+    std::vector<double> f(3 * sys.size());
+    for(std::size_t i = 0; i < sys.size(); ++i)
+    {
+        f[3*i] = math::X(sys.force(i)); f[3*i+1] = math::Y(sys.force(i)); f[3*i+2] = math::Z(sys.force(i));
+    }
+    return f;
+}
+
+template<typename traitsT>
+static void zero_forces(System<traitsT>& sys)
+{
+    using coord = typename traitsT::coordinate_type;
+    for(std::size_t i = 0; i < sys.size(); ++i) {sys.force(i) = coord(0, 0, 0);}
+}
+
+// forces: initialise the force field, then per interaction (local in order, then global in
+// order) the energy from calc_energy (what EnergyObserver prints, EnergyObserver.hpp:89-150)
+// and the force from calc_force with zeroed force arrays; then the total force from
+// ForceField::calc_force and the total from calc_force_and_energy.
+template<typename traitsT>
+static int cmd_forces(const SpecIn& s, const std::string& out)
+{
+    auto b = build<traitsT>(s);
+    auto& sys = *b.sys;
+    auto& ff  = *b.ff;
+    auto& ffc = dynamic_cast<ForceField<traitsT>&>(ff);
+    sys.initialize(*(new RandomNumberGenerator<traitsT>(1)));
+    ff.initialize(sys);
+
+    std::ofstream os(out);
+    char buf[128];
+    for(auto& inter : ffc.local())
+    {
+        zero_forces(sys);
+        const double e = inter->calc_energy(sys);
+        inter->calc_force(sys);
+        std::snprintf(buf, sizeof(buf), "%.17g", e);
+        os << "interaction " << inter->name() << " energy " << buf << "\n";
+        write_vec3(os, "force", forces_of(sys));
+    }
+    std::size_t gidx = 0;
+    for(auto& inter : ffc.global())
+    {
+        zero_forces(sys);
+        const double e = inter->calc_energy(sys);
+        inter->calc_force(sys);
+        std::snprintf(buf, sizeof(buf), "%.17g", e);
+        os << "interaction " << inter->name() << " energy " << buf << "\n";
+        write_vec3(os, "force", forces_of(sys));
+        b.dump_pairs.at(gidx++)(os);
+    }
+    zero_forces(sys);
+    const double etot = ff.calc_force_and_energy(sys);
+    std::snprintf(buf, sizeof(buf), "%.17g", etot);
+    os << "total energy " << buf << "\n";
+    write_vec3(os, "force", forces_of(sys));
+    os << "end\n";
+    return 0;
+}
+
+template<typename traitsT, typename integT>
+static int run_impl(const SpecIn& s, std::size_t nsteps, const std::string& out, std::size_t every, bool bench)
+{
+    using real_type = typename traitsT::real_type;
+    auto b = build<traitsT>(s);
+    auto& sys = *b.sys;
+    RandomNumberGenerator<traitsT> rng(s.seed);
+    sys.initialize(rng);
+    b.ff->initialize(sys);
+    std::vector<real_type> coefs(s.coefs.begin(), s.coefs.end());
+    integT integ(s.dt, std::move(coefs), SystemMotionRemover<traitsT>(false, false, false));
+    integ.initialize(sys, b.ff, rng);
+
+    std::ofstream os;
+    if(!bench) {os.open(out);}
+    auto dump = [&](std::size_t step) {
+        std::vector<double> x(3 * sys.size()), v(3 * sys.size());
+        for(std::size_t i = 0; i < sys.size(); ++i)
+        {
+            x[3*i] = math::X(sys.position(i)); x[3*i+1] = math::Y(sys.position(i)); x[3*i+2] = math::Z(sys.position(i));
+            v[3*i] = math::X(sys.velocity(i)); v[3*i+1] = math::Y(sys.velocity(i)); v[3*i+2] = math::Z(sys.velocity(i));
+        }
+        os << "step " << step << "\n";
+        write_vec3(os, "position", x);
+        write_vec3(os, "velocity", v);
+    };
+    real_type t = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    for(std::size_t step = 0; step < nsteps; ++step)
+    {
+        if(!bench && every != 0 && step % every == 0) {dump(step);}
+        t = integ.step(t, sys, b.ff, rng);
+    }
+    const auto t1 = std::chrono::steady_clock::now();
+    if(bench)
+    {
+        const double sec = std::chrono::duration<double>(t1 - t0).count();
+        int threads = 1;
+#ifdef MJOLNIR_WITH_OPENMP
+        threads = omp_get_max_threads();
+#endif
+        std::printf("{\"seconds\": %.9g, \"steps\": %zu, \"particles\": %zu, \"threads\": %d, "
+                    "\"particle_steps_per_s\": %.9g}\n", sec, nsteps, sys.size(), threads,
+                    double(sys.size()) * double(nsteps) / sec);
+        return 0;
+    }
+    dump(nsteps);
+    os << "end\n";
+    return 0;
+}
+
+template<typename traitsT>
+static int cmd_run(const SpecIn& s, std::size_t nsteps, const std::string& out, std::size_t every, bool bench)
+{
+    if(!s.has_integrator) {throw std::runtime_error("spec has no integrator");}
+    if(s.integrator == "BAOABLangevin")
+    {
+        return run_impl<traitsT, BAOABLangevinIntegrator<traitsT>>(s, nsteps, out, every, bench);
+    }
+    if(s.integrator == "GJFNVTLangevin")
+    {
+        return run_impl<traitsT, GJFNVTLangevinIntegrator<traitsT>>(s, nsteps, out, every, bench);
+    }
+    throw std::runtime_error("unsupported integrator " + s.integrator);
+}
+
+#ifdef MJOLNIR_WITH_OPENMP
+template<typename realT, template<typename, typename> class B>
+using Traits = OpenMPSimulatorTraits<realT, B>;
+#else
+template<typename realT, template<typename, typename> class B>
+using Traits = SimulatorTraits<realT, B>;
+#endif
+
+template<template<typename, typename> class B>
+static int dispatch(const std::string& cmd, const SpecIn& s, int argc, char** argv)
+{
+    using traits = Traits<double, B>;
+    if(cmd == "forces") {return cmd_forces<traits>(s, argv[3]);}
+    if(cmd == "run")
+    {
+        const std::size_t every = (argc > 5) ? std::stoul(argv[5]) : 0;
+        return cmd_run<traits>(s, std::stoul(argv[3]), argv[4], every, false);
+    }
+    if(cmd == "bench") {return cmd_run<traits>(s, std::stoul(argv[3]), "", 0, true);}
+    throw std::runtime_error("unknown command " + cmd);
+}
+} // drv
+
+int main(int argc, char** argv)
+{
+    if(argc < 4)
+    {
+        std::cerr << "usage: ref_driver forces|run|bench <spec> ...\n";
+        return 2;
+    }
+    try
+    {
+        mjolnir::LoggerManager::set_default_logger("/dev/null");
+        drv::set_units_kcal_angstrom<double>();
+        const auto spec = drv::read_spec(argv[2]);
+        if(spec.periodic) {return drv::dispatch<mjolnir::CuboidalPeriodicBoundary>(argv[1], spec, argc, argv);}
+        return drv::dispatch<mjolnir::UnlimitedBoundary>(argv[1], spec, argc, argv);
+    }
+    catch(const std::exception& e)
+    {
+        std::cerr << "ref_driver: " << e.what() << std::endl;
+        return 1;
+    }
+}
