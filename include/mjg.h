@@ -1,0 +1,162 @@
+/* mjg: MI355X-native force-and-integrate engine for Mjolnir-style coarse-grained MD.
+ *
+ * C ABI of libmjg.so.  Plain pointers and sizes only; every buffer passed in is owned by the
+ * caller and copied, every device buffer is owned by the engine.  Each entry point replaces a
+ * piece of the reference's class surface (paths relative to the Mjolnir source tree):
+ *
+ *   System<traits>                       mjolnir/core/System.hpp:18-207
+ *   ForceFieldBase<traits>               mjolnir/core/ForceFieldBase.hpp:13-52
+ *   LocalInteractionBase / Global...     mjolnir/core/LocalInteractionBase.hpp:10-41,
+ *                                        mjolnir/core/GlobalInteractionBase.hpp:10-38
+ *   SpatialPartition (neighbour lists)   mjolnir/core/SpatialPartitionBase.hpp:18-145
+ *   BAOABLangevinIntegrator              mjolnir/core/BAOABLangevinIntegrator.hpp:17-240
+ *   GJFNVTLangevinIntegrator             mjolnir/core/GJFNVTLangevinIntegrator.hpp:17-244
+ *
+ * Error convention: the reference throws std::runtime_error / std::out_of_range at set-up and
+ * keeps the hot path noexcept (ForceFieldBase.hpp:33-37).  Here every call returns an
+ * mjg_status; set-up errors carry a message available from mjg_last_error().  Device faults
+ * and non-finite state found during stepping are reported as MJG_DEVICE_ERROR /
+ * MJG_NAN_DETECTED by the call that observes them.
+ *
+ * Threading: one engine is driven by one host thread at a time (calls on the same engine are
+ * not re-entrant); engines are independent and own one HIP stream each.
+ */
+#ifndef MJG_H
+#define MJG_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mjg_engine mjg_engine;
+typedef int mjg_status;
+
+enum {
+    MJG_OK              = 0,
+    MJG_INVALID_ARG     = 1,
+    MJG_OOM             = 2,
+    MJG_DEVICE_ERROR    = 3,
+    MJG_NAN_DETECTED    = 4,
+    MJG_NOT_INITIALIZED = 5,
+    MJG_UNSUPPORTED     = 6
+};
+
+/* arithmetic of the force kernels */
+enum {
+    MJG_FP64       = 0, /* FP64 throughout */
+    MJG_FP32_FIXED = 1  /* FP32 pair/local arithmetic, int64 fixed-point force sums,
+                           FP64 integrator state */
+};
+/* boundary (read_system.hpp:100-299: boundary_type "Unlimited" / "PeriodicCuboid") */
+enum { MJG_UNLIMITED = 0, MJG_PERIODIC = 1 };
+
+/* local interactions: interaction:potential (read_local_interaction.hpp:31-715) */
+enum {
+    MJG_BOND_HARMONIC     = 1, /* BondLength:Harmonic  {k, v0}        BondLengthInteraction.hpp:126-148 */
+    MJG_BOND_GOCONTACT    = 2, /* BondLength:GoContact {k, v0}        BondLengthGoContactInteraction.hpp:126-167 */
+    MJG_CONTACT_GOCONTACT = 3, /* Contact:GoContact    {k, v0}        GoContactInteraction.hpp:159-228 */
+    MJG_ANGLE_HARMONIC    = 4, /* BondAngle:Harmonic   {k, v0}        BondAngleInteraction.hpp:129-189 */
+    MJG_ANGLE_GAUSSIAN    = 5, /* BondAngle:Gaussian   {k, sigma, v0} BondAngleInteraction.hpp:129-189 */
+    MJG_DIHEDRAL_CLEMENTI = 6  /* DihedralAngle:ClementiDihedral {k1, k3, v0} DihedralAngleInteraction.hpp:157-227 */
+};
+/* global pair potentials (read_global_potential.hpp:262-705) */
+enum {
+    MJG_PAIR_LJ = 1, /* LennardJones {sigma, epsilon} per participant */
+    MJG_PAIR_EV = 2, /* ExcludedVolume {radius} per participant, common epsilon */
+    MJG_PAIR_DH = 3  /* DebyeHuckel {charge} per participant */
+};
+/* spatial_partition.type (read_spatial_partition.hpp:62-116) */
+enum { MJG_PART_CELLLIST = 1, MJG_PART_VERLET = 2, MJG_PART_NAIVE = 3 };
+/* ignore.molecule (IgnoreMolecule.hpp:101-121) */
+enum { MJG_IGNORE_NOTHING = 0, MJG_IGNORE_SELF = 1, MJG_IGNORE_OTHERS = 2 };
+/* integrator.type (read_integrator.hpp) */
+enum { MJG_INTEG_BAOAB = 1, MJG_INTEG_GJF = 2 };
+
+/* Library version string. */
+const char* mjg_version(void);
+
+/* Create an engine for n_particles particles on HIP device `device`.
+ * Replaces System(num_particles, boundary) (System.hpp:43-51) and the choice of
+ * SimulatorTraits<precision, boundary> (read_input_file.hpp:22-145).
+ * lower/upper are read only for MJG_PERIODIC (CuboidalPeriodicBoundary, BoundaryCondition.hpp:75-78). */
+mjg_status mjg_engine_create(int precision, int boundary, const double* lower, const double* upper,
+                             size_t n_particles, int device, mjg_engine** out);
+void mjg_engine_destroy(mjg_engine* e);
+/* Text of the last error on this engine ("" if none). */
+const char* mjg_last_error(const mjg_engine* e);
+
+/* System state: masses (rmass = 1/mass), positions and velocities, each 3*N doubles for
+ * vectors (System.hpp:139-150).  Positions are wrapped into the box as System::initialize
+ * does (System.hpp:63-67). */
+mjg_status mjg_system_set_state(mjg_engine* e, const double* mass, const double* pos, const double* vel);
+/* Download positions, velocities and the current forces (any pointer may be NULL). */
+mjg_status mjg_system_get_state(mjg_engine* e, double* pos, double* vel, double* force);
+/* System attributes "temperature" [K] and "ionic_strength" [M] (System.hpp:164-167). */
+mjg_status mjg_system_set_attribute(mjg_engine* e, const char* key, double value);
+
+/* Add a local interaction ([[forcefields.local]]): n_terms terms of `arity` indices and
+ * the potential's parameters (see the enum).  `topology` is the connection kind the
+ * interaction writes into the topology ("bond", "contact", or "none"; write_topology,
+ * BondLengthInteraction.hpp:232-245).  `margin` is used by MJG_CONTACT_GOCONTACT only.
+ * The new interaction's id (its position in calc_energy / format_energy order) goes to *id. */
+mjg_status mjg_add_local(mjg_engine* e, int kind, const char* topology, size_t n_terms,
+                         const uint32_t* indices, const double* params, double margin, int* id);
+
+/* Add a global pair interaction ([[forcefields.global]] interaction = "Pair"):
+ * cutoff_ratio is the relative cutoff, epsilon the ExcludedVolume energy scale,
+ * ignore.molecule, ignore.particles_within as (kind, distance) pairs, and the participants
+ * with their per-particle parameters (parameters = [{index = ..., ...}]). */
+mjg_status mjg_add_global(mjg_engine* e, int potential, double cutoff_ratio, double epsilon,
+                          int partition, double margin, int ignore_molecule,
+                          size_t n_within, const char* const* within_kinds, const uint32_t* within_dist,
+                          size_t n_participants, const uint32_t* participants, const double* params,
+                          int* id);
+
+/* ForceField::initialize (ForceField.hpp:47-63): topology and molecules from the local
+ * interactions, exclusion lists, potential parameters, first neighbour lists. */
+mjg_status mjg_forcefield_initialize(mjg_engine* e);
+size_t mjg_num_interactions(const mjg_engine* e);
+
+/* ForceFieldBase::calc_force (ForceField.hpp:101-109) on the device state; forces are
+ * recomputed from zero. */
+mjg_status mjg_calc_force(mjg_engine* e);
+/* ForceFieldBase::calc_force_and_energy (ForceField.hpp:126-135); total energy to *energy. */
+mjg_status mjg_calc_force_and_energy(mjg_engine* e, double* energy);
+/* Per-interaction energies in interaction-id order (what format_energy prints,
+ * ForceField.hpp:154-161) and their sum; does not change the state. */
+mjg_status mjg_calc_energies(mjg_engine* e, double* per_interaction, double* total);
+/* Force of interaction `id` alone, downloaded into force_out (3*N doubles). */
+mjg_status mjg_calc_force_of(mjg_engine* e, int id, double* force_out);
+/* ForceFieldBase::reduce_margin (ForceField.hpp:75-92): rebuild lists whose skin is spent. */
+mjg_status mjg_reduce_margin(mjg_engine* e, double dmargin);
+/* Current neighbour pairs (i < j) of the global interaction with id `id`, as the
+ * reference's NeighborList holds them (NeighborList.hpp:151-266).  Writes min(n, capacity)
+ * pairs (2 uint32 each) and the total count to *n_pairs. */
+mjg_status mjg_neighbor_pairs(mjg_engine* e, int id, uint32_t* pairs, size_t capacity, size_t* n_pairs);
+
+/* Integrator: type, delta_t, per-particle gammas (BAOAB) or alphas (G-JF), and the seed of
+ * the counter-based noise stream (read_integrator.hpp:18-307). */
+mjg_status mjg_integrator_set(mjg_engine* e, int type, double dt, const double* coefs, uint64_t seed);
+/* Integrator::initialize: coefficients from the temperature, then the first calc_force
+ * (BAOABLangevinIntegrator.hpp:135-168, GJFNVTLangevinIntegrator.hpp:44-75). */
+mjg_status mjg_integrator_initialize(mjg_engine* e);
+/* n_steps calls of Integrator::step (BAOABLangevinIntegrator.hpp:170-240) on the device,
+ * including the skin check and list rebuilds; returns after the work is complete. */
+mjg_status mjg_step(mjg_engine* e, size_t n_steps);
+/* Steps taken and neighbour-list rebuilds performed so far. */
+mjg_status mjg_counters(mjg_engine* e, uint64_t* steps, uint64_t* rebuilds);
+/* Kinetic energy sum_i m_i v_i^2 / 2 (EnergyObserver.hpp:89-150 prints it beside the terms). */
+mjg_status mjg_kinetic_energy(mjg_engine* e, double* ke);
+
+/* Timing hook with no counterpart in the reference (it has no GPU kernels to time): launch
+ * the force kernel of interaction `id` `reps` times back to back on the engine's stream into a
+ * scratch force buffer, bracketed by HIP events, and return the average kernel time in ms.
+ * Also returns the number of half-list pairs the interaction currently holds (0 for locals). */
+mjg_status mjg_profile_interaction(mjg_engine* e, int id, int reps, double* avg_ms, uint64_t* half_pairs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

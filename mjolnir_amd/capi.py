@@ -1,0 +1,216 @@
+"""ctypes binding of libmjg.so (the C ABI in include/mjg.h) and a System/ForceField/Integrator
+facade over it.
+
+The facade mirrors the reference's calling sequence for the force-and-step path
+(MolecularDynamicsSimulator::initialize, mjolnir/core/MolecularDynamicsSimulator.hpp:72-81):
+System state -> ForceField::initialize -> Integrator::initialize -> Integrator::step.
+There is no fallback: if the HIP library is missing or no GPU is present, construction
+raises ``MjgError``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import List, Optional
+
+import numpy as np
+
+from .spec import SystemSpec
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmjg.so")
+
+OK, INVALID_ARG, OOM, DEVICE_ERROR, NAN_DETECTED, NOT_INITIALIZED, UNSUPPORTED = range(7)
+FP64, FP32_FIXED = 0, 1
+UNLIMITED, PERIODIC = 0, 1
+
+LOCAL_CODE = {
+    ("BondLength", "Harmonic"): 1,
+    ("BondLength", "GoContact"): 2,
+    ("Contact", "GoContact"): 3,
+    ("BondAngle", "Harmonic"): 4,
+    ("BondAngle", "Gaussian"): 5,
+    ("DihedralAngle", "ClementiDihedral"): 6,
+}
+GLOBAL_CODE = {"LennardJones": 1, "ExcludedVolume": 2, "DebyeHuckel": 3}
+PARTITION_CODE = {"CellList": 1, "VerletList": 2, "Naive": 3}
+IGNORE_CODE = {"Nothing": 0, "Self": 1, "Intra": 1, "Others": 2, "Inter": 2}
+INTEG_CODE = {"BAOABLangevin": 1, "GJFNVTLangevin": 2}
+
+
+class MjgError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"mjg status {status}: {msg}")
+        self.status = status
+
+
+_LIB = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+
+EXPORTED = [
+    "mjg_version", "mjg_engine_create", "mjg_engine_destroy", "mjg_last_error",
+    "mjg_system_set_state", "mjg_system_get_state", "mjg_system_set_attribute",
+    "mjg_add_local", "mjg_add_global", "mjg_forcefield_initialize", "mjg_num_interactions",
+    "mjg_calc_force", "mjg_calc_force_and_energy", "mjg_calc_energies", "mjg_calc_force_of",
+    "mjg_reduce_margin", "mjg_neighbor_pairs", "mjg_integrator_set", "mjg_integrator_initialize",
+    "mjg_step", "mjg_counters", "mjg_kinetic_energy", "mjg_profile_interaction",
+]
+
+
+def lib():
+    """Load libmjg.so (built in-tree by ``mjolnir_amd.build``); raises if it is missing."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise MjgError(NOT_INITIALIZED, f"{LIB_PATH} is missing: build it with `python -m mjolnir_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, sz, i32, u64, dbl = C.c_void_p, C.c_size_t, C.c_int, C.c_uint64, C.c_double
+    L.mjg_version.restype = C.c_char_p
+    L.mjg_engine_create.argtypes = [i32, i32, _dp, _dp, sz, i32, C.POINTER(vp)]
+    L.mjg_engine_destroy.argtypes = [vp]
+    L.mjg_last_error.restype = C.c_char_p
+    L.mjg_last_error.argtypes = [vp]
+    L.mjg_system_set_state.argtypes = [vp, _dp, _dp, _dp]
+    L.mjg_system_get_state.argtypes = [vp, vp, vp, vp]
+    L.mjg_system_set_attribute.argtypes = [vp, C.c_char_p, dbl]
+    L.mjg_add_local.argtypes = [vp, i32, C.c_char_p, sz, _up, _dp, dbl, C.POINTER(i32)]
+    L.mjg_add_global.argtypes = [vp, i32, dbl, dbl, i32, dbl, i32, sz, C.POINTER(C.c_char_p), _up, sz, _up, _dp,
+                                 C.POINTER(i32)]
+    L.mjg_forcefield_initialize.argtypes = [vp]
+    L.mjg_num_interactions.restype = sz
+    L.mjg_num_interactions.argtypes = [vp]
+    L.mjg_calc_force.argtypes = [vp]
+    L.mjg_calc_force_and_energy.argtypes = [vp, C.POINTER(dbl)]
+    L.mjg_calc_energies.argtypes = [vp, vp, C.POINTER(dbl)]
+    L.mjg_calc_force_of.argtypes = [vp, i32, _dp]
+    L.mjg_reduce_margin.argtypes = [vp, dbl]
+    L.mjg_neighbor_pairs.argtypes = [vp, i32, vp, sz, C.POINTER(sz)]
+    L.mjg_integrator_set.argtypes = [vp, i32, dbl, _dp, u64]
+    L.mjg_integrator_initialize.argtypes = [vp]
+    L.mjg_step.argtypes = [vp, sz]
+    L.mjg_counters.argtypes = [vp, C.POINTER(u64), C.POINTER(u64)]
+    L.mjg_kinetic_energy.argtypes = [vp, C.POINTER(dbl)]
+    L.mjg_profile_interaction.argtypes = [vp, i32, i32, C.POINTER(dbl), C.POINTER(u64)]
+    _LIB = L
+    return L
+
+
+class Engine:
+    """One system resident on the GPU, built from a ``SystemSpec``.
+
+    Interaction ids follow the reference's ``format_energy`` order: locals in insertion
+    order, then globals (mjolnir/core/ForceField.hpp:154-161).
+    """
+
+    def __init__(self, spec: SystemSpec, precision: int = FP64, device: int = 0, with_integrator: bool = True):
+        L = lib()
+        self.spec = spec
+        self.n = spec.n
+        h = C.c_void_p()
+        st = L.mjg_engine_create(precision, PERIODIC if spec.boundary == "periodic" else UNLIMITED,
+                                 np.ascontiguousarray(spec.lower), np.ascontiguousarray(spec.upper),
+                                 spec.n, device, C.byref(h))
+        if st != OK:
+            raise MjgError(st, "mjg_engine_create failed (is a GPU present?)")
+        self.h = h
+        self._check(L.mjg_system_set_state(self.h, spec.mass, spec.pos, spec.vel))
+        for k, v in spec.attributes.items():
+            self._check(L.mjg_system_set_attribute(self.h, k.encode(), float(v)))
+        iid = C.c_int()
+        for l in spec.local:
+            self._check(L.mjg_add_local(self.h, LOCAL_CODE[(l.interaction, l.potential)], l.topology.encode(),
+                                        len(l.indices), np.ascontiguousarray(l.indices.ravel()),
+                                        np.ascontiguousarray(l.params.ravel()), float(l.margin), C.byref(iid)))
+        for g in spec.global_:
+            kinds = list(g.ignore_within.keys())
+            arr = (C.c_char_p * max(1, len(kinds)))(*[k.encode() for k in kinds])
+            dists = np.array([g.ignore_within[k] for k in kinds] or [0], dtype=np.uint32)
+            self._check(L.mjg_add_global(self.h, GLOBAL_CODE[g.potential], float(g.cutoff), float(g.epsilon),
+                                         PARTITION_CODE[g.partition], float(g.margin),
+                                         IGNORE_CODE[g.ignore_molecule], len(kinds), arr, dists,
+                                         len(g.participants), np.ascontiguousarray(g.participants),
+                                         np.ascontiguousarray(g.params.ravel()), C.byref(iid)))
+        self._check(L.mjg_forcefield_initialize(self.h))
+        self.n_local = len(spec.local)
+        if with_integrator and spec.integrator is not None:
+            it = spec.integrator
+            self._check(L.mjg_integrator_set(self.h, INTEG_CODE[it.type], float(it.dt), it.coefs, int(it.seed)))
+            self._check(L.mjg_integrator_initialize(self.h))
+
+    def _check(self, st: int):
+        if st != OK:
+            msg = lib().mjg_last_error(self.h)
+            raise MjgError(st, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mjg_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- ForceFieldBase ------------------------------------------------------------------
+    @property
+    def num_interactions(self) -> int:
+        return lib().mjg_num_interactions(self.h)
+
+    def calc_force(self):
+        self._check(lib().mjg_calc_force(self.h))
+
+    def calc_force_and_energy(self) -> float:
+        e = C.c_double()
+        self._check(lib().mjg_calc_force_and_energy(self.h, C.byref(e)))
+        return e.value
+
+    def energies(self):
+        per = np.zeros(max(1, self.num_interactions))
+        tot = C.c_double()
+        self._check(lib().mjg_calc_energies(self.h, per.ctypes.data, C.byref(tot)))
+        return per[: self.num_interactions], tot.value
+
+    def force_of(self, k: int) -> np.ndarray:
+        out = np.zeros(3 * self.n)
+        self._check(lib().mjg_calc_force_of(self.h, k, out))
+        return out.reshape(-1, 3)
+
+    def reduce_margin(self, dmargin: float):
+        self._check(lib().mjg_reduce_margin(self.h, float(dmargin)))
+
+    def pairs(self, k: int) -> np.ndarray:
+        n = C.c_size_t()
+        self._check(lib().mjg_neighbor_pairs(self.h, k, None, 0, C.byref(n)))
+        out = np.zeros(2 * max(1, n.value), dtype=np.uint32)
+        self._check(lib().mjg_neighbor_pairs(self.h, k, out.ctypes.data, n.value, C.byref(n)))
+        return out[: 2 * n.value].reshape(-1, 2)
+
+    # -- Integrator ----------------------------------------------------------------------
+    def step(self, n: int):
+        self._check(lib().mjg_step(self.h, int(n)))
+
+    def counters(self):
+        s, r = C.c_uint64(), C.c_uint64()
+        self._check(lib().mjg_counters(self.h, C.byref(s), C.byref(r)))
+        return s.value, r.value
+
+    def kinetic_energy(self) -> float:
+        e = C.c_double()
+        self._check(lib().mjg_kinetic_energy(self.h, C.byref(e)))
+        return e.value
+
+    def profile_interaction(self, k: int, reps: int = 50):
+        """(average kernel ms, half-list pairs) of interaction k's force kernel."""
+        ms, hp = C.c_double(), C.c_uint64()
+        self._check(lib().mjg_profile_interaction(self.h, k, reps, C.byref(ms), C.byref(hp)))
+        return ms.value, hp.value
+
+    # -- System --------------------------------------------------------------------------
+    def state(self):
+        x = np.zeros(3 * self.n); v = np.zeros(3 * self.n); f = np.zeros(3 * self.n)
+        self._check(lib().mjg_system_get_state(self.h, x.ctypes.data, v.ctypes.data, f.ctypes.data))
+        return x.reshape(-1, 3), v.reshape(-1, 3), f.reshape(-1, 3)

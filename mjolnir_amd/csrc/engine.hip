@@ -1,0 +1,1150 @@
+// Host side of the engine (see engine.hpp) and the C ABI declared in include/mjg.h.
+#include "engine.hpp"
+#include "../../include/mjg.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+
+namespace mjg
+{
+
+// ------------------------------------------------------------------------------------
+// physical constants after read_units("kcal/mol", "angstrom") (read_units.hpp:32-92,
+// Unit.hpp:23-44,150-159), formed in the same operation order
+// ------------------------------------------------------------------------------------
+static double kB_kcal()
+{
+    const double J_to_cal = 1.0 / 4.184;
+    return 1.38064852e-23 * (J_to_cal / 1000.0) * 6.022140857e23;
+}
+static double eps0_kcal_angstrom()
+{
+    const double J_to_cal = 1.0 / 4.184;
+    double eps0 = (8.854187817e-12 / 1.6021766208e-19) / 1.6021766208e-19;
+    eps0 = eps0 * (1000.0 / J_to_cal) / 6.022140857e23;
+    return eps0 / 1e+10;
+}
+static double L_to_volume_angstrom() {return 1e-3 * std::pow(1e+10, 3);}
+
+static uint32_t blocks_for(size_t n, uint32_t bs) {return (uint32_t)std::max<size_t>(1, (n + bs - 1) / bs);}
+
+void Engine::check(hipError_t e, const char* what)
+{
+    if(e != hipSuccess)
+    {
+        throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+template<typename T> T* Engine::dalloc(size_t n)
+{
+    void* p = nullptr;
+    const size_t bytes = std::max<size_t>(1, n) * sizeof(T);
+    const hipError_t e = hipMalloc(&p, bytes);
+    if(e != hipSuccess)
+    {
+        (void)hipGetLastError();
+        throw std::bad_alloc();
+    }
+    allocations_.push_back(p);
+    return static_cast<T*>(p);
+}
+template<typename T> void Engine::upload(T* d, const T* h, size_t n)
+{
+    if(n == 0) {return;}
+    check(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, stream_), "upload");
+    check(hipStreamSynchronize(stream_), "upload sync");
+}
+template<typename T> void Engine::download(T* h, const T* d, size_t n)
+{
+    if(n == 0) {return;}
+    check(hipMemcpyAsync(h, d, n * sizeof(T), hipMemcpyDeviceToHost, stream_), "download");
+    check(hipStreamSynchronize(stream_), "download sync");
+}
+void Engine::sync() {check(hipStreamSynchronize(stream_), "synchronize");}
+
+Engine::Engine(int precision, int boundary, const double* lower, const double* upper, size_t n, int device)
+    : precision_(precision), device_(device), n_(n)
+{
+    if(precision != MJG_FP64 && precision != MJG_FP32_FIXED) {throw std::invalid_argument("unknown precision");}
+    if(boundary != MJG_UNLIMITED && boundary != MJG_PERIODIC) {throw std::invalid_argument("unknown boundary");}
+    if(n == 0 || n >= 0xffffffffull) {throw std::invalid_argument("particle count out of range");}
+    if(precision == MJG_FP32_FIXED) {throw std::invalid_argument("MJG_FP32_FIXED is served by the batched engine");}
+    check(hipSetDevice(device), "hipSetDevice");
+    check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    box_.periodic = (boundary == MJG_PERIODIC);
+    if(box_.periodic)
+    {
+        if(lower == nullptr || upper == nullptr) {throw std::invalid_argument("periodic boundary needs lower/upper");}
+        // CuboidalPeriodicBoundary::set_boundary (BoundaryCondition.hpp:115-123)
+        for(int d = 0; d < 3; ++d)
+        {
+            if(!(upper[d] > lower[d])) {throw std::invalid_argument("upper must exceed lower");}
+            box_.lower[d] = lower[d]; box_.upper[d] = upper[d];
+            box_.width[d] = upper[d] - lower[d];
+            box_.halfw[d] = box_.width[d] * 0.5;
+        }
+    }
+    d_pos_ = dalloc<double4>(n); d_vel_ = dalloc<double4>(n);
+    d_force_ = dalloc<double4>(n); d_force_tmp_ = dalloc<double4>(n);
+    d_bak_pos_ = dalloc<double4>(n); d_bak_vel_ = dalloc<double4>(n); d_bak_force_ = dalloc<double4>(n);
+    d_mass_ = dalloc<double>(n); d_rmass_ = dalloc<double>(n);
+    d_sctl_ = dalloc<StepCtl>(1); d_bak_sctl_ = dalloc<StepCtl>(1);
+    StepCtl z{};
+    upload(d_sctl_, &z, 1);
+    scratch_n_ = n;
+    d_scratch_ = dalloc<double>(n);
+    const char* ng = std::getenv("MJG_NO_GRAPH");
+    use_graph_ = !(ng && ng[0] == '1');
+}
+
+Engine::~Engine()
+{
+    if(graph_exec_) {(void)hipGraphExecDestroy(graph_exec_);}
+    if(stream_) {(void)hipStreamSynchronize(stream_);}
+    for(void* p : allocations_) {(void)hipFree(p);}
+    if(stream_) {(void)hipStreamDestroy(stream_);}
+}
+
+void Engine::set_state(const double* mass, const double* pos, const double* vel)
+{
+    if(!mass || !pos || !vel) {throw std::invalid_argument("null state pointer");}
+    mass_.assign(mass, mass + n_);
+    std::vector<double> rm(n_);
+    std::vector<double4> p(n_), v(n_);
+    for(size_t i = 0; i < n_; ++i)
+    {
+        if(!(mass[i] > 0.0)) {throw std::invalid_argument("masses must be positive");}
+        rm[i] = 1.0 / mass[i];
+        double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+        if(box_.periodic)
+        {
+            // System::initialize wraps positions (System.hpp:63-67, BoundaryCondition.hpp:93-102)
+            if(x < box_.lower[0]) {x += box_.width[0];} else if(x >= box_.upper[0]) {x -= box_.width[0];}
+            if(y < box_.lower[1]) {y += box_.width[1];} else if(y >= box_.upper[1]) {y -= box_.width[1];}
+            if(z < box_.lower[2]) {z += box_.width[2];} else if(z >= box_.upper[2]) {z -= box_.width[2];}
+        }
+        p[i] = make_double4(x, y, z, 0.0);
+        v[i] = make_double4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], 0.0);
+    }
+    upload(d_mass_, mass_.data(), n_);
+    upload(d_rmass_, rm.data(), n_);
+    upload(d_pos_, p.data(), n_);
+    upload(d_vel_, v.data(), n_);
+    std::vector<double4> zf(n_, make_double4(0, 0, 0, 0));
+    upload(d_force_, zf.data(), n_);
+    state_set_ = true;
+}
+
+void Engine::get_state(double* pos, double* vel, double* force)
+{
+    std::vector<double4> h(n_);
+    double4* src[3] = {d_pos_, d_vel_, d_force_};
+    double* dst[3] = {pos, vel, force};
+    for(int k = 0; k < 3; ++k)
+    {
+        if(!dst[k]) {continue;}
+        download(h.data(), src[k], n_);
+        for(size_t i = 0; i < n_; ++i) {dst[k][3 * i] = h[i].x; dst[k][3 * i + 1] = h[i].y; dst[k][3 * i + 2] = h[i].z;}
+    }
+}
+
+void Engine::set_attribute(const std::string& key, double value)
+{
+    if(key == "temperature") {T_ = value; has_T_ = true;}
+    else if(key == "ionic_strength") {I_ = value; has_I_ = true;}
+    else {throw std::invalid_argument("unknown attribute: " + key);}
+}
+
+int Engine::kind_id(const std::string& name)
+{
+    for(size_t i = 0; i < kinds_.size(); ++i) {if(kinds_[i] == name) {return (int)i;}}
+    kinds_.push_back(name);
+    return (int)kinds_.size() - 1;
+}
+
+int Engine::add_local(int kind, const char* topology, size_t n_terms, const uint32_t* idx, const double* par, double margin)
+{
+    if(initialized_) {throw std::invalid_argument("interactions must be added before initialize");}
+    if(!globals_.empty())
+    {
+        // interaction ids follow calc_energy order: every local, then every global
+        throw std::invalid_argument("add local interactions before global ones");
+    }
+    LocalIA L;
+    L.kind = kind;
+    switch(kind)
+    {
+        case kBondHarmonic: case kBondGo: case kContactGo: L.arity = 2; L.npar = 2; break;
+        case kAngleHarmonic: L.arity = 3; L.npar = 2; break;
+        case kAngleGaussian: L.arity = 3; L.npar = 3; break;
+        case kDihedralClementi: L.arity = 4; L.npar = 3; break;
+        default: throw std::invalid_argument("unknown local interaction kind");
+    }
+    if(n_terms > 0 && (!idx || !par)) {throw std::invalid_argument("null term arrays");}
+    for(size_t t = 0; t < n_terms * L.arity; ++t)
+    {
+        if(idx[t] >= n_) {throw std::out_of_range("local interaction index out of range");}
+    }
+    if(locals_.size() + 1 >= (size_t)kMaxLocalSegs) {throw std::invalid_argument("too many local interactions");}
+    const std::string topo = topology ? topology : "";
+    L.topo_kind = (topo.empty() || topo == "none") ? -1 : kind_id(topo);
+    L.idx.assign(idx, idx + n_terms * L.arity);
+    L.par.assign(par, par + n_terms * L.npar);
+    L.margin = margin;
+    locals_.push_back(std::move(L));
+    return (int)locals_.size() - 1;
+}
+
+int Engine::add_global(int pot, double cutoff_ratio, double eps, int partition, double margin, int ignore_mol,
+                       size_t n_within, const char* const* kinds, const uint32_t* dists,
+                       size_t np, const uint32_t* part, const double* par)
+{
+    if(initialized_) {throw std::invalid_argument("interactions must be added before initialize");}
+    if(globals_.size() >= (size_t)kMaxGlobals) {throw std::invalid_argument("too many global interactions");}
+    GlobalIA G;
+    G.pot = pot;
+    switch(pot)
+    {
+        case kPairLJ: G.npar = 2; break;
+        case kPairEV: case kPairDH: G.npar = 1; break;
+        default: throw std::invalid_argument("unknown pair potential");
+    }
+    if(partition != MJG_PART_CELLLIST && partition != MJG_PART_VERLET && partition != MJG_PART_NAIVE)
+    {
+        throw std::invalid_argument("unknown spatial partition");
+    }
+    if(ignore_mol < 0 || ignore_mol > 2) {throw std::invalid_argument("unknown ignore.molecule");}
+    if(np < 1 || !part || !par) {throw std::invalid_argument("a pair interaction needs participants");}
+    if(!(cutoff_ratio > 0.0) || !(margin >= 0.0)) {throw std::invalid_argument("cutoff and margin must be positive");}
+    for(size_t k = 0; k < np; ++k) {if(part[k] >= n_) {throw std::out_of_range("participant index out of range");}}
+    G.cutoff_ratio = cutoff_ratio; G.eps = eps; G.partition = partition; G.margin = margin; G.ignore_mol = ignore_mol;
+    for(size_t k = 0; k < n_within; ++k)
+    {
+        if(!kinds || !kinds[k] || !dists) {throw std::invalid_argument("null ignore.particles_within entry");}
+        G.within.emplace_back(kind_id(kinds[k]), dists[k]);
+    }
+    G.part.assign(part, part + np);
+    G.par.assign(par, par + np * G.npar);
+    globals_.push_back(std::move(G));
+    return (int)globals_.size() - 1;
+}
+
+// Topology::add_connection (Topology.hpp:161-191) for every local interaction with a
+// connection kind, in the order the force field writes them (ForceField.hpp:53-56), then
+// construct_molecules (Topology.hpp:280-317): only "bond" edges join molecules.
+void Engine::write_topology()
+{
+    adj_.assign(n_, {});
+    auto add = [&](uint32_t i, uint32_t j, int kind) {
+        for(int s = 0; s < 2; ++s)
+        {
+            const uint32_t a = s ? j : i, b = s ? i : j;
+            auto& v = adj_[a];
+            if(std::find(v.begin(), v.end(), std::make_pair(b, kind)) == v.end()) {v.emplace_back(b, kind);}
+        }
+    };
+    for(const auto& L : locals_)
+    {
+        if(L.topo_kind < 0) {continue;}
+        for(size_t t = 0; t < L.n_terms(); ++t)
+        {
+            const uint32_t* ix = L.idx.data() + t * L.arity;
+            for(int a = 0; a < L.arity; ++a)
+                for(int b = a + 1; b < L.arity; ++b) {add(ix[a], ix[b], L.topo_kind);}
+        }
+    }
+    const int bond = kind_id("bond");
+    const uint32_t UNINIT = 0xffffffffu;
+    mol_.assign(n_, UNINIT);
+    uint32_t next = 0;
+    for(size_t i = 0; i < n_; ++i)
+    {
+        for(const auto& e : adj_[i])
+        {
+            if(e.second != bond) {continue;}
+            if(mol_[e.first] != UNINIT) {mol_[i] = mol_[e.first]; break;}
+        }
+        if(mol_[i] == UNINIT) {mol_[i] = next++;}
+    }
+}
+
+// ExclusionList::make (ExclusionList.hpp:86-234): per particle, itself plus every particle
+// reachable within `dist` edges of each ignored connection kind
+// (Topology::list_adjacent_within, Topology.hpp:133-153,243-264), sorted and unique.
+void Engine::make_exclusions(GlobalIA& g)
+{
+    std::vector<uint32_t> start(n_ + 1), all;
+    std::vector<uint32_t> tmp;
+    std::vector<std::pair<uint32_t, uint32_t>> stack;
+    for(size_t i = 0; i < n_; ++i)
+    {
+        start[i] = (uint32_t)all.size();
+        tmp.clear();
+        tmp.push_back((uint32_t)i);
+        for(const auto& w : g.within)
+        {
+            // depth-first walk over `kind` edges up to `dist` steps
+            stack.clear();
+            stack.emplace_back((uint32_t)i, w.second);
+            while(!stack.empty())
+            {
+                const auto top = stack.back();
+                stack.pop_back();
+                if(top.second == 0) {continue;}
+                for(const auto& e : adj_[top.first])
+                {
+                    if(e.second != w.first) {continue;}
+                    tmp.push_back(e.first);
+                    stack.emplace_back(e.first, top.second - 1);
+                }
+            }
+        }
+        std::sort(tmp.begin(), tmp.end());
+        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+        all.insert(all.end(), tmp.begin(), tmp.end());
+    }
+    start[n_] = (uint32_t)all.size();
+    g.d_ex_start = dalloc<uint32_t>(n_ + 1);
+    g.d_ex = dalloc<uint32_t>(all.size());
+    upload(g.d_ex_start, start.data(), n_ + 1);
+    upload(g.d_ex, all.data(), all.size());
+    g.d_mol = dalloc<uint32_t>(n_);
+    upload(g.d_mol, mol_.data(), n_);
+}
+
+void Engine::alloc_list(GlobalIA& g, uint32_t cap)
+{
+    const uint32_t np = (uint32_t)g.part.size();
+    g.cap = cap;
+    g.stride = (np + 63u) & ~63u;
+    g.d_nl = dalloc<uint32_t>((size_t)g.stride * cap);
+}
+
+void Engine::setup_global(GlobalIA& g)
+{
+    const uint32_t np = (uint32_t)g.part.size();
+    // per-particle parameter table (zero for non-participants)
+    std::vector<double> ppar(n_ * g.npar, 0.0);
+    for(uint32_t k = 0; k < np; ++k)
+        for(int c = 0; c < g.npar; ++c) {ppar[(size_t)g.part[k] * g.npar + c] = g.par[(size_t)k * g.npar + c];}
+
+    PairParams& pp = g.pp;
+    pp = PairParams{};
+    pp.kind = g.pot;
+    pp.npar = g.npar;
+    if(g.pot == kPairLJ)
+    {
+        // LennardJonesPotential.hpp:34-38 and max_cutoff :82-98 over the parameter table
+        pp.cutoff_ratio2 = g.cutoff_ratio * g.cutoff_ratio;
+        pp.coef_at_cutoff = std::pow(1.0 / g.cutoff_ratio, 12) - std::pow(1.0 / g.cutoff_ratio, 6);
+        double ms = 0;
+        for(size_t i = 0; i < n_; ++i) {ms = std::max(ms, ppar[2 * i]);}
+        g.max_cutoff = ms * g.cutoff_ratio;
+    }
+    else if(g.pot == kPairEV)
+    {
+        // ExcludedVolumePotential.hpp:44-48, max_cutoff :90-106
+        pp.cutoff_ratio2 = g.cutoff_ratio * g.cutoff_ratio;
+        pp.coef_at_cutoff = std::pow(1.0 / g.cutoff_ratio, 12);
+        pp.eps = g.eps;
+        double mr = 0;
+        for(size_t i = 0; i < n_; ++i) {mr = std::max(mr, ppar[i]);}
+        g.max_cutoff = mr * 2 * g.cutoff_ratio;
+    }
+    else
+    {
+        // DebyeHuckelPotential.hpp:126-174
+        if(!has_T_ || !has_I_)
+        {
+            throw std::invalid_argument("DebyeHuckel requires `temperature` and `ionic_strength` attributes");
+        }
+        const double pi = 3.14159265358979323846264338;
+        const double kB = kB_kcal(), NA = 6.022140857e23, eps0 = eps0_kcal_angstrom();
+        const double T = T_, c = I_;
+        const double epsk = (249.4 - 0.788 * T + 7.2e-4 * T * T) *
+                            (1. - 2.551e-1 * c + 5.151e-2 * c * c - 6.889e-3 * c * c * c);
+        pp.inv4pieps = 1.0 / (4 * pi * eps0 * epsk);
+        const double I = c / L_to_volume_angstrom();
+        pp.debye = std::sqrt((eps0 * epsk * kB * T) / (2 * NA * I));
+        pp.inv_debye = 1. / pp.debye;
+        pp.abs_cutoff = g.cutoff_ratio * pp.debye;
+        pp.coef_at_cutoff = std::exp(-g.cutoff_ratio) / (pp.debye * g.cutoff_ratio);
+        g.max_cutoff = pp.debye * g.cutoff_ratio;
+        if(!std::isfinite(g.max_cutoff) || !(g.max_cutoff > 0)) {throw std::invalid_argument("invalid Debye length");}
+    }
+    g.r_list = g.max_cutoff * (1. + g.margin);
+
+    // participant slot per particle (kNoSlot for non-participants)
+    std::vector<uint32_t> pslot(n_, kNoSlot);
+    for(uint32_t k = 0; k < np; ++k) {pslot[g.part[k]] = k;}
+    g.rule.last_slot = np - 1;
+    g.rule.pairwise = (g.partition != MJG_PART_CELLLIST);
+    g.rule.use_distance = (g.partition != MJG_PART_NAIVE);
+    g.rule.ignore_mol = g.ignore_mol;
+    g.rule.r_list2 = g.r_list * g.r_list;
+
+    g.d_part = dalloc<uint32_t>(np);
+    upload(g.d_part, g.part.data(), np);
+    g.d_pslot = dalloc<uint32_t>(n_);
+    upload(g.d_pslot, pslot.data(), n_);
+    g.d_ppar = dalloc<double>(n_ * g.npar);
+    upload(g.d_ppar, ppar.data(), n_ * g.npar);
+    make_exclusions(g);
+
+    g.d_grid = dalloc<Grid>(1);
+    if(g.partition == MJG_PART_CELLLIST)
+    {
+        if(box_.periodic)
+        {
+            // PeriodicGridCellList::set_cutoff / initialize (:67-96, :152-160)
+            const double me = 1e-6;
+            const double r = 1 / (g.max_cutoff * (1 + g.margin) + me);
+            Grid gr{};
+            for(int d = 0; d < 3; ++d)
+            {
+                const double f = std::floor(box_.width[d] * r);
+                gr.dim[d] = (int)std::max<double>(3.0, f);
+                gr.inv_w[d] = 1.0 / (box_.width[d] / gr.dim[d]);
+                gr.lo[d] = box_.lower[d];
+            }
+            gr.periodic = 1;
+            const uint64_t nc = (uint64_t)gr.dim[0] * gr.dim[1] * gr.dim[2];
+            if(nc > kMaxCells) {throw std::invalid_argument("periodic box needs more than 2^18 cells");}
+            gr.ncell = (uint32_t)nc;
+            g.grid = gr;
+            upload(g.d_grid, &gr, 1);
+            g.key_bits = 1;
+            while((1u << g.key_bits) < gr.ncell) {++g.key_bits;}
+        }
+        else
+        {
+            g.key_bits = 18;
+        }
+        g.sb.n = np;
+        g.sb.tiles = blocks_for(np, kSortTile);
+        g.sb.keys = dalloc<uint32_t>(np); g.sb.vals = dalloc<uint32_t>(np);
+        g.sb.keys_alt = dalloc<uint32_t>(np); g.sb.vals_alt = dalloc<uint32_t>(np);
+        g.sb.hist = dalloc<uint32_t>(256 * (size_t)g.sb.tiles);
+        g.d_cell_start = dalloc<uint32_t>(kMaxCells);
+        g.d_cell_end = dalloc<uint32_t>(kMaxCells);
+    }
+    g.d_nn = dalloc<uint32_t>(np);
+    g.d_order = dalloc<uint32_t>(np);
+    g.d_epair = dalloc<double>(np);
+    alloc_list(g, 64);
+}
+
+// Launch the rebuild pipeline of global g; every kernel checks ListCtl::rebuild itself.
+void Engine::launch_rebuild(GlobalIA& g, int gi)
+{
+    const uint32_t np = (uint32_t)g.part.size();
+    ListCtl* ctl = d_lctl_ + gi;
+    if(g.partition == MJG_PART_CELLLIST)
+    {
+        if(!box_.periodic)
+        {
+            hipLaunchKernelGGL(k_bbox, dim3(std::min<uint32_t>(blocks_for(np, 256), 1024)), dim3(256), 0, stream_,
+                               d_pos_, g.d_part, np, ctl);
+            hipLaunchKernelGGL(k_grid_open, dim3(1), dim3(1), 0, stream_, ctl, g.d_grid, g.r_list);
+        }
+        hipLaunchKernelGGL(k_cellkey, dim3(blocks_for(np, 256)), dim3(256), 0, stream_, d_pos_, g.d_part, np,
+                           g.d_grid, g.sb.keys, g.sb.vals, ctl);
+        radix_sort(g.sb, g.key_bits, &ctl->rebuild, stream_);
+        const uint32_t ncell_launch = box_.periodic ? g.grid.ncell : kMaxCells;
+        hipLaunchKernelGGL(k_cell_clear, dim3(blocks_for(ncell_launch, 256)), dim3(256), 0, stream_,
+                           g.d_cell_start, g.d_cell_end, g.d_grid, ctl);
+        hipLaunchKernelGGL(k_cell_range, dim3(blocks_for(np, 256)), dim3(256), 0, stream_, g.sb.keys, np,
+                           g.d_cell_start, g.d_cell_end, ctl);
+        hipLaunchKernelGGL(k_verlet_cells, dim3(blocks_for(np, 256)), dim3(256), 0, stream_, d_pos_, g.d_part, np,
+                           g.sb.keys, g.sb.vals, g.d_cell_start, g.d_cell_end, g.d_grid, box_, g.rule, g.d_pslot,
+                           g.d_mol, g.d_ex_start, g.d_ex, g.d_nl, g.d_nn, g.d_order, g.stride, g.cap, ctl);
+    }
+    else
+    {
+        hipLaunchKernelGGL(k_verlet_allpairs, dim3(blocks_for(np, 256)), dim3(256), 0, stream_, d_pos_, g.d_part, np,
+                           box_, g.rule, g.d_pslot, g.d_mol, g.d_ex_start, g.d_ex, g.d_nl, g.d_nn, g.d_order,
+                           g.stride, g.cap, ctl);
+    }
+}
+
+// Build the lists of every global interaction now (initialize / explicit reduce_margin),
+// growing the list capacity until every partner list fits.
+void Engine::build_lists_now(bool all)
+{
+    std::vector<ListCtl> ctl(globals_.size());
+    download(ctl.data(), d_lctl_, globals_.size());
+    for(int pass = 0; pass < 8; ++pass)
+    {
+        for(size_t g = 0; g < globals_.size(); ++g)
+        {
+            if(!all && !ctl[g].rebuild) {continue;}
+            ctl[g].rebuild = 1;
+            ctl[g].maxcount = 0;
+            for(int d = 0; d < 3; ++d) {ctl[g].bbox_min[d] = ~0ull; ctl[g].bbox_max[d] = 0ull;}
+            ctl[g].margin = globals_[g].partition == MJG_PART_NAIVE ? INFINITY : globals_[g].max_cutoff * globals_[g].margin;
+        }
+        upload(d_lctl_, ctl.data(), globals_.size());
+        for(size_t g = 0; g < globals_.size(); ++g) {launch_rebuild(globals_[g], (int)g);}
+        check(hipGetLastError(), "rebuild launch");
+        download(ctl.data(), d_lctl_, globals_.size());
+        bool again = false;
+        for(size_t g = 0; g < globals_.size(); ++g)
+        {
+            if(ctl[g].rebuild && ctl[g].maxcount > globals_[g].cap)
+            {
+                alloc_list(globals_[g], (uint32_t)(ctl[g].maxcount + ctl[g].maxcount / 4 + 16));
+                again = true;
+                if(graph_exec_) {(void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr;}
+            }
+            else
+            {
+                ctl[g].rebuild = 0;
+            }
+        }
+        if(!again) {break;}
+        all = false;
+    }
+    for(auto& c : ctl) {c.rebuild = 0;}
+    upload(d_lctl_, ctl.data(), globals_.size());
+}
+
+void Engine::initialize()
+{
+    if(!state_set_) {throw std::invalid_argument("system state must be set before initialize");}
+    if(initialized_) {throw std::invalid_argument("force field already initialized");}
+    write_topology();
+
+    // local tables: one segment per interaction, body slots in interaction/term/body order
+    std::vector<uint32_t> lidx;
+    std::vector<double> lpar;
+    std::vector<std::vector<uint32_t>> per_particle(n_);
+    segtab_.n = 0;
+    uint32_t block = 0, slot = 0, term = 0;
+    for(size_t l = 0; l < locals_.size(); ++l)
+    {
+        LocalIA& L = locals_[l];
+        LocalSeg s{};
+        s.kind = L.kind; s.arity = L.arity; s.npar = L.npar;
+        s.n_terms = (uint32_t)L.n_terms();
+        s.block_begin = block;
+        s.idx_off = (uint32_t)lidx.size();
+        s.par_off = (uint32_t)lpar.size();
+        s.slot_off = slot;
+        s.term_off = term;
+        L.slot_off = slot; L.term_off = term;
+        lidx.insert(lidx.end(), L.idx.begin(), L.idx.end());
+        lpar.insert(lpar.end(), L.par.begin(), L.par.end());
+        for(uint32_t t = 0; t < s.n_terms; ++t)
+            for(int b = 0; b < L.arity; ++b) {per_particle[L.idx[(size_t)t * L.arity + b]].push_back(slot + t * L.arity + b);}
+        segtab_.seg[segtab_.n++] = s;
+        block += blocks_for(s.n_terms, 256) * (s.n_terms > 0 ? 1 : 0);
+        slot += s.n_terms * L.arity;
+        term += s.n_terms;
+    }
+    local_blocks_ = block;
+    n_slots_ = slot;
+    n_terms_ = term;
+    if(!locals_.empty())
+    {
+        std::vector<uint32_t> gstart(n_ + 1), gslot;
+        for(size_t i = 0; i < n_; ++i)
+        {
+            gstart[i] = (uint32_t)gslot.size();
+            gslot.insert(gslot.end(), per_particle[i].begin(), per_particle[i].end());
+        }
+        gstart[n_] = (uint32_t)gslot.size();
+        d_lidx_ = dalloc<uint32_t>(lidx.size()); upload(d_lidx_, lidx.data(), lidx.size());
+        d_lpar_ = dalloc<double>(lpar.size()); upload(d_lpar_, lpar.data(), lpar.size());
+        d_tf_ = dalloc<double>(3 * (size_t)n_slots_);
+        d_te_ = dalloc<double>(n_terms_);
+        d_gstart_ = dalloc<uint32_t>(n_ + 1); upload(d_gstart_, gstart.data(), n_ + 1);
+        d_gslot_ = dalloc<uint32_t>(gslot.size()); upload(d_gslot_, gslot.data(), gslot.size());
+    }
+
+    d_lctl_ = dalloc<ListCtl>(std::max<size_t>(1, globals_.size()));
+    d_bak_lctl_ = dalloc<ListCtl>(std::max<size_t>(1, globals_.size()));
+    d_abs_margin_ = dalloc<double>(std::max<size_t>(1, globals_.size()));
+    d_never_ = dalloc<int>(std::max<size_t>(1, globals_.size()));
+    d_energy_ = dalloc<double>(num_interactions() + 1);
+    std::vector<double> absm;
+    std::vector<int> never;
+    for(auto& g : globals_)
+    {
+        setup_global(g);
+        absm.push_back(g.max_cutoff * g.margin);
+        never.push_back(g.partition == MJG_PART_NAIVE);
+    }
+    if(!globals_.empty())
+    {
+        upload(d_abs_margin_, absm.data(), absm.size());
+        upload(d_never_, never.data(), never.size());
+        std::vector<ListCtl> ctl(globals_.size());
+        upload(d_lctl_, ctl.data(), ctl.size());
+        build_lists_now(true);
+    }
+    initialized_ = true;
+}
+
+// ForceField::calc_force into f: locals (term kernel + per-particle gather), then every pair
+// interaction adds its sum; with `energy`, per-interaction energies land in d_energy_.
+// only >= 0 restricts the call to one interaction id.
+void Engine::forces(double4* f, bool energy, int only)
+{
+    const int nl = (int)locals_.size();
+    if(local_blocks_ > 0 && (only < 0 || only < nl))
+    {
+        if(only >= 0)
+        {
+            // other segments' slots must read as zero
+            hipLaunchKernelGGL(k_zero_doubles, dim3(blocks_for(3 * (size_t)n_slots_, 256)), dim3(256), 0, stream_,
+                               d_tf_, 3 * n_slots_);
+        }
+        if(energy)
+        {
+            hipLaunchKernelGGL(k_local_terms<true>, dim3(local_blocks_), dim3(256), 0, stream_, d_pos_, box_, segtab_,
+                               d_lidx_, d_lpar_, d_tf_, d_te_, only);
+        }
+        else
+        {
+            hipLaunchKernelGGL(k_local_terms<false>, dim3(local_blocks_), dim3(256), 0, stream_, d_pos_, box_, segtab_,
+                               d_lidx_, d_lpar_, d_tf_, d_te_, only);
+        }
+        hipLaunchKernelGGL(k_gather_local, dim3(blocks_for(n_, 256)), dim3(256), 0, stream_, (uint32_t)n_, d_gstart_,
+                           d_gslot_, d_tf_, f);
+    }
+    else
+    {
+        hipLaunchKernelGGL(k_zero_double4, dim3(blocks_for(n_, 256)), dim3(256), 0, stream_, f, (uint32_t)n_);
+    }
+    if(energy)
+    {
+        for(int l = 0; l < nl; ++l)
+        {
+            if(only >= 0 && only != l) {continue;}
+            const auto& L = locals_[l];
+            hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, stream_, d_te_ + L.term_off, (uint32_t)L.n_terms(),
+                               d_energy_ + l);
+        }
+    }
+    for(size_t gi = 0; gi < globals_.size(); ++gi)
+    {
+        const int id = nl + (int)gi;
+        if(only >= 0 && only != id) {continue;}
+        GlobalIA& g = globals_[gi];
+        const uint32_t np = (uint32_t)g.part.size();
+        const dim3 grid(blocks_for(np, 256)), blk(256);
+#define MJG_PAIR_LAUNCH(KIND, E)                                                                          \
+        hipLaunchKernelGGL((k_pair_full<KIND, E>), grid, blk, 0, stream_, d_pos_, f, g.d_part, g.d_order, np, \
+                           g.d_nl, g.d_nn, g.stride, g.d_ppar, g.pp, box_, g.d_epair)
+        if(g.pot == kPairLJ) {if(energy) {MJG_PAIR_LAUNCH(kPairLJ, true);} else {MJG_PAIR_LAUNCH(kPairLJ, false);}}
+        else if(g.pot == kPairEV) {if(energy) {MJG_PAIR_LAUNCH(kPairEV, true);} else {MJG_PAIR_LAUNCH(kPairEV, false);}}
+        else {if(energy) {MJG_PAIR_LAUNCH(kPairDH, true);} else {MJG_PAIR_LAUNCH(kPairDH, false);}}
+#undef MJG_PAIR_LAUNCH
+        if(energy)
+        {
+            hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, stream_, g.d_epair, np, d_energy_ + id);
+        }
+    }
+}
+
+void Engine::calc_force()
+{
+    if(!initialized_) {throw std::logic_error("not initialized");}
+    forces(d_force_, false, -1);
+    check(hipGetLastError(), "calc_force launch");
+    sync();
+}
+
+double Engine::calc_force_and_energy()
+{
+    if(!initialized_) {throw std::logic_error("not initialized");}
+    forces(d_force_, true, -1);
+    check(hipGetLastError(), "calc_force_and_energy launch");
+    std::vector<double> e(num_interactions());
+    download(e.data(), d_energy_, e.size());
+    double tot = 0;
+    for(double x : e) {tot += x;}
+    return tot;
+}
+
+void Engine::calc_energies(double* per, double* total)
+{
+    if(!initialized_) {throw std::logic_error("not initialized");}
+    forces(d_force_tmp_, true, -1);
+    check(hipGetLastError(), "calc_energies launch");
+    std::vector<double> e(num_interactions());
+    download(e.data(), d_energy_, e.size());
+    double tot = 0;
+    for(size_t k = 0; k < e.size(); ++k) {if(per) {per[k] = e[k];} tot += e[k];}
+    if(total) {*total = tot;}
+}
+
+void Engine::calc_force_of(int id, double* out)
+{
+    if(!initialized_) {throw std::logic_error("not initialized");}
+    if(id < 0 || (size_t)id >= num_interactions()) {throw std::out_of_range("interaction id out of range");}
+    forces(d_force_tmp_, false, id);
+    check(hipGetLastError(), "calc_force_of launch");
+    std::vector<double4> h(n_);
+    download(h.data(), d_force_tmp_, n_);
+    for(size_t i = 0; i < n_; ++i) {out[3 * i] = h[i].x; out[3 * i + 1] = h[i].y; out[3 * i + 2] = h[i].z;}
+}
+
+void Engine::reduce_margin(double dmargin)
+{
+    if(!initialized_) {throw std::logic_error("not initialized");}
+    if(globals_.empty()) {return;}
+    std::vector<ListCtl> ctl(globals_.size());
+    download(ctl.data(), d_lctl_, globals_.size());
+    bool any = false;
+    for(size_t g = 0; g < globals_.size(); ++g)
+    {
+        if(globals_[g].partition == MJG_PART_NAIVE) {ctl[g].rebuild = 0; continue;}
+        ctl[g].margin -= dmargin;
+        ctl[g].rebuild = ctl[g].margin < 0;
+        any = any || ctl[g].rebuild;
+    }
+    upload(d_lctl_, ctl.data(), globals_.size());
+    if(any) {build_lists_now(false);}
+}
+
+size_t Engine::neighbor_pairs(int id, uint32_t* pairs, size_t cap)
+{
+    if(!initialized_) {throw std::logic_error("not initialized");}
+    const int gi = id - (int)locals_.size();
+    if(gi < 0 || (size_t)gi >= globals_.size()) {throw std::out_of_range("not a global interaction id");}
+    GlobalIA& g = globals_[gi];
+    const uint32_t np = (uint32_t)g.part.size();
+    std::vector<uint32_t> nn(np), order(np), nl((size_t)g.stride * g.cap);
+    download(nn.data(), g.d_nn, np);
+    download(order.data(), g.d_order, np);
+    download(nl.data(), g.d_nl, nl.size());
+    std::vector<std::pair<uint32_t, uint32_t>> out;
+    for(uint32_t t = 0; t < np; ++t)
+    {
+        const uint32_t i = g.part[order[t]];
+        for(uint32_t k = 0; k < nn[t]; ++k)
+        {
+            const uint32_t j = nl[(size_t)k * g.stride + t];
+            if(i < j) {out.emplace_back(i, j);}
+        }
+    }
+    std::sort(out.begin(), out.end());
+    if(pairs)
+    {
+        for(size_t k = 0; k < out.size() && k < cap; ++k) {pairs[2 * k] = out[k].first; pairs[2 * k + 1] = out[k].second;}
+    }
+    return out.size();
+}
+
+void Engine::set_integrator(int type, double dt, const double* coefs, uint64_t seed)
+{
+    if(type != MJG_INTEG_BAOAB && type != MJG_INTEG_GJF) {throw std::invalid_argument("unknown integrator");}
+    if(!(dt > 0) || !coefs) {throw std::invalid_argument("invalid integrator parameters");}
+    integ_ = type; dt_ = dt; seed_ = seed;
+    coefs_.assign(coefs, coefs + n_);
+    integ_ready_ = false;
+    if(graph_exec_) {(void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr;}
+}
+
+// reset_parameters (BAOABLangevinIntegrator.hpp:73-99; GJFNVTLangevinIntegrator.hpp:106-134)
+// and the first force call (:151-166 / :58-73).
+void Engine::integrator_initialize()
+{
+    if(!initialized_) {throw std::logic_error("force field not initialized");}
+    if(integ_ == 0) {throw std::logic_error("integrator not set");}
+    if(!has_T_) {throw std::out_of_range("Langevin integrator requires the `temperature` attribute");}
+    const double kBT = kB_kcal() * T_;
+    std::vector<double> c1(n_), c2(n_), c3(n_);
+    for(size_t i = 0; i < n_; ++i)
+    {
+        if(integ_ == MJG_INTEG_BAOAB)
+        {
+            const double gamma_dt = -1 * coefs_[i] * dt_;
+            c1[i] = std::exp(gamma_dt);
+            c2[i] = std::sqrt(kBT * (1 - std::exp(2 * gamma_dt)) * (1.0 / mass_[i]));
+        }
+        else
+        {
+            const double alpha = coefs_[i], m = mass_[i];
+            c1[i] = 1.0 / (1.0 + alpha * dt_ * 0.5 / m);
+            c2[i] = std::sqrt(2 * alpha * kBT * dt_);
+            c3[i] = 1.0 - alpha * c1[i] * dt_ / m;
+        }
+    }
+    if(!d_c1_) {d_c1_ = dalloc<double>(n_); d_c2_ = dalloc<double>(n_); d_c3_ = dalloc<double>(n_);}
+    upload(d_c1_, c1.data(), n_);
+    upload(d_c2_, c2.data(), n_);
+    upload(d_c3_, c3.data(), n_);
+    forces(d_force_, false, -1);
+    check(hipGetLastError(), "integrator_initialize launch");
+    sync();
+    integ_ready_ = true;
+}
+
+void Engine::launch_step()
+{
+    const uint32_t nb = blocks_for(n_, 256);
+    if(integ_ == MJG_INTEG_BAOAB)
+    {
+        hipLaunchKernelGGL(k_baoab_a, dim3(nb), dim3(256), 0, stream_, (uint32_t)n_, d_pos_, d_vel_, d_force_,
+                           d_rmass_, d_c1_, d_c2_, dt_ / 2, box_, seed_, d_sctl_);
+    }
+    else
+    {
+        hipLaunchKernelGGL(k_gjf_a, dim3(nb), dim3(256), 0, stream_, (uint32_t)n_, d_pos_, d_vel_, d_force_,
+                           d_rmass_, d_c1_, d_c2_, d_c3_, dt_, box_, seed_, d_sctl_);
+    }
+    if(!globals_.empty())
+    {
+        hipLaunchKernelGGL(k_margin, dim3(1), dim3(64), 0, stream_, d_sctl_, d_lctl_, (int)globals_.size(),
+                           d_abs_margin_, d_never_);
+        for(size_t g = 0; g < globals_.size(); ++g)
+        {
+            if(globals_[g].partition != MJG_PART_NAIVE) {launch_rebuild(globals_[g], (int)g);}
+        }
+    }
+    else
+    {
+        hipLaunchKernelGGL(k_reset_maxdisp, dim3(1), dim3(1), 0, stream_, d_sctl_);
+    }
+    forces(d_force_, false, -1);
+    if(integ_ == MJG_INTEG_BAOAB)
+    {
+        hipLaunchKernelGGL(k_kick_b, dim3(nb), dim3(256), 0, stream_, (uint32_t)n_, d_vel_, d_force_, d_rmass_,
+                           dt_ / 2, d_sctl_);
+    }
+    else
+    {
+        hipLaunchKernelGGL(k_gjf_b, dim3(nb), dim3(256), 0, stream_, (uint32_t)n_, d_vel_, d_force_, d_rmass_,
+                           dt_, d_sctl_);
+    }
+}
+
+void Engine::run_segment(size_t n)
+{
+    const size_t ng = std::max<size_t>(1, globals_.size());
+    for(int attempt = 0; attempt < 6; ++attempt)
+    {
+        check(hipMemcpyAsync(d_bak_pos_, d_pos_, n_ * sizeof(double4), hipMemcpyDeviceToDevice, stream_), "backup");
+        check(hipMemcpyAsync(d_bak_vel_, d_vel_, n_ * sizeof(double4), hipMemcpyDeviceToDevice, stream_), "backup");
+        check(hipMemcpyAsync(d_bak_force_, d_force_, n_ * sizeof(double4), hipMemcpyDeviceToDevice, stream_), "backup");
+        check(hipMemcpyAsync(d_bak_sctl_, d_sctl_, sizeof(StepCtl), hipMemcpyDeviceToDevice, stream_), "backup");
+        if(!globals_.empty())
+        {
+            check(hipMemcpyAsync(d_bak_lctl_, d_lctl_, ng * sizeof(ListCtl), hipMemcpyDeviceToDevice, stream_), "backup");
+        }
+        if(use_graph_)
+        {
+            if(graph_exec_ == nullptr || graph_steps_ != n)
+            {
+                if(graph_exec_) {(void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr;}
+                hipGraph_t graph = nullptr;
+                check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
+                for(size_t s = 0; s < n; ++s) {launch_step();}
+                check(hipStreamEndCapture(stream_, &graph), "end capture");
+                check(hipGraphInstantiate(&graph_exec_, graph, nullptr, nullptr, 0), "graph instantiate");
+                (void)hipGraphDestroy(graph);
+                graph_steps_ = n;
+            }
+            check(hipGraphLaunch(graph_exec_, stream_), "graph launch");
+        }
+        else
+        {
+            for(size_t s = 0; s < n; ++s) {launch_step();}
+            check(hipGetLastError(), "step launch");
+        }
+        StepCtl sc{};
+        download(&sc, d_sctl_, 1);
+        bool overflow = false;
+        if(!globals_.empty())
+        {
+            std::vector<ListCtl> ctl(globals_.size());
+            download(ctl.data(), d_lctl_, globals_.size());
+            for(size_t g = 0; g < globals_.size(); ++g)
+            {
+                if(ctl[g].maxcount > globals_[g].cap)
+                {
+                    alloc_list(globals_[g], (uint32_t)(ctl[g].maxcount + ctl[g].maxcount / 4 + 16));
+                    overflow = true;
+                }
+            }
+        }
+        if(!overflow)
+        {
+            if(sc.nonfinite) {throw NonFinite("non-finite particle position");}
+            return;
+        }
+        // a partner list did not fit: restore the segment's start, force a rebuild with the
+        // larger capacity and run the segment again
+        if(graph_exec_) {(void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr;}
+        check(hipMemcpyAsync(d_pos_, d_bak_pos_, n_ * sizeof(double4), hipMemcpyDeviceToDevice, stream_), "restore");
+        check(hipMemcpyAsync(d_vel_, d_bak_vel_, n_ * sizeof(double4), hipMemcpyDeviceToDevice, stream_), "restore");
+        check(hipMemcpyAsync(d_force_, d_bak_force_, n_ * sizeof(double4), hipMemcpyDeviceToDevice, stream_), "restore");
+        check(hipMemcpyAsync(d_sctl_, d_bak_sctl_, sizeof(StepCtl), hipMemcpyDeviceToDevice, stream_), "restore");
+        check(hipMemcpyAsync(d_lctl_, d_bak_lctl_, ng * sizeof(ListCtl), hipMemcpyDeviceToDevice, stream_), "restore");
+        sync();
+        build_lists_now(true);
+        forces(d_force_, false, -1);
+        sync();
+    }
+    throw DeviceError("neighbour list capacity did not converge");
+}
+
+void Engine::step(size_t n)
+{
+    if(!integ_ready_) {throw std::logic_error("integrator not initialized");}
+    const size_t seg = 100;
+    while(n > 0)
+    {
+        const size_t k = std::min(n, seg);
+        run_segment(k);
+        n -= k;
+    }
+}
+
+void Engine::counters(uint64_t* steps, uint64_t* rebuilds)
+{
+    StepCtl sc{};
+    download(&sc, d_sctl_, 1);
+    if(steps) {*steps = sc.step;}
+    if(rebuilds) {*rebuilds = sc.rebuilds;}
+}
+
+double Engine::kinetic_energy()
+{
+    hipLaunchKernelGGL(k_kinetic, dim3(blocks_for(n_, 256)), dim3(256), 0, stream_, (uint32_t)n_, d_vel_, d_mass_,
+                       d_scratch_);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, stream_, d_scratch_, (uint32_t)n_, d_energy_ + num_interactions());
+    check(hipGetLastError(), "kinetic launch");
+    double ke = 0;
+    download(&ke, d_energy_ + num_interactions(), 1);
+    return ke;
+}
+
+double Engine::profile_interaction(int id, int reps, uint64_t* half_pairs)
+{
+    if(!initialized_) {throw std::logic_error("not initialized");}
+    if(id < 0 || (size_t)id >= num_interactions() || reps < 1) {throw std::out_of_range("bad profile request");}
+    hipEvent_t a, b;
+    check(hipEventCreate(&a), "event");
+    check(hipEventCreate(&b), "event");
+    const int nl = (int)locals_.size();
+    uint64_t hp = 0;
+    check(hipEventRecord(a, stream_), "event record");
+    for(int r = 0; r < reps; ++r)
+    {
+        if(id < nl)
+        {
+            hipLaunchKernelGGL(k_local_terms<false>, dim3(local_blocks_), dim3(256), 0, stream_, d_pos_, box_, segtab_,
+                               d_lidx_, d_lpar_, d_tf_, d_te_, id);
+        }
+        else
+        {
+            GlobalIA& g = globals_[id - nl];
+            const uint32_t np = (uint32_t)g.part.size();
+            const dim3 grid(blocks_for(np, 256)), blk(256);
+            if(g.pot == kPairLJ)
+            {
+                hipLaunchKernelGGL((k_pair_full<kPairLJ, false>), grid, blk, 0, stream_, d_pos_, d_force_tmp_, g.d_part,
+                                   g.d_order, np, g.d_nl, g.d_nn, g.stride, g.d_ppar, g.pp, box_, g.d_epair);
+            }
+            else if(g.pot == kPairEV)
+            {
+                hipLaunchKernelGGL((k_pair_full<kPairEV, false>), grid, blk, 0, stream_, d_pos_, d_force_tmp_, g.d_part,
+                                   g.d_order, np, g.d_nl, g.d_nn, g.stride, g.d_ppar, g.pp, box_, g.d_epair);
+            }
+            else
+            {
+                hipLaunchKernelGGL((k_pair_full<kPairDH, false>), grid, blk, 0, stream_, d_pos_, d_force_tmp_, g.d_part,
+                                   g.d_order, np, g.d_nl, g.d_nn, g.stride, g.d_ppar, g.pp, box_, g.d_epair);
+            }
+        }
+    }
+    check(hipEventRecord(b, stream_), "event record");
+    check(hipEventSynchronize(b), "event sync");
+    check(hipGetLastError(), "profile launch");
+    float ms = 0;
+    check(hipEventElapsedTime(&ms, a, b), "event elapsed");
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if(id >= nl)
+    {
+        GlobalIA& g = globals_[id - nl];
+        const uint32_t np = (uint32_t)g.part.size();
+        std::vector<uint32_t> nn(np);
+        download(nn.data(), g.d_nn, np);
+        for(uint32_t c : nn) {hp += c;}
+        hp /= 2;
+    }
+    if(half_pairs) {*half_pairs = hp;}
+    return (double)ms / reps;
+}
+
+} // mjg
+
+// ======================================================================================
+// C ABI
+// ======================================================================================
+struct mjg_engine
+{
+    mjg::Engine* impl = nullptr;
+    std::string err;
+};
+
+namespace
+{
+template<typename F>
+mjg_status guarded(mjg_engine* e, F&& f)
+{
+    if(e == nullptr || e->impl == nullptr) {return MJG_INVALID_ARG;}
+    try
+    {
+        f(*e->impl);
+        e->err.clear();
+        return MJG_OK;
+    }
+    catch(const mjg::NonFinite& x) {e->err = x.what(); return MJG_NAN_DETECTED;}
+    catch(const mjg::DeviceError& x) {e->err = x.what(); return MJG_DEVICE_ERROR;}
+    catch(const std::bad_alloc&) {e->err = "device or host allocation failed"; return MJG_OOM;}
+    catch(const std::logic_error& x)
+    {
+        e->err = x.what();
+        if(dynamic_cast<const std::invalid_argument*>(&x) || dynamic_cast<const std::out_of_range*>(&x))
+        {
+            return MJG_INVALID_ARG;
+        }
+        return MJG_NOT_INITIALIZED;
+    }
+    catch(const std::exception& x) {e->err = x.what(); return MJG_INVALID_ARG;}
+}
+} // namespace
+
+extern "C" {
+
+const char* mjg_version(void) {return "mjg 0.1 (gfx950)";}
+
+mjg_status mjg_engine_create(int precision, int boundary, const double* lower, const double* upper,
+                             size_t n_particles, int device, mjg_engine** out)
+{
+    if(out == nullptr) {return MJG_INVALID_ARG;}
+    *out = nullptr;
+    mjg_engine* e = new(std::nothrow) mjg_engine;
+    if(e == nullptr) {return MJG_OOM;}
+    try
+    {
+        e->impl = new mjg::Engine(precision, boundary, lower, upper, n_particles, device);
+    }
+    catch(const std::bad_alloc&) {delete e; return MJG_OOM;}
+    catch(const mjg::DeviceError&) {delete e; return MJG_DEVICE_ERROR;}
+    catch(const std::exception&) {delete e; return MJG_INVALID_ARG;}
+    *out = e;
+    return MJG_OK;
+}
+
+void mjg_engine_destroy(mjg_engine* e)
+{
+    if(e == nullptr) {return;}
+    delete e->impl;
+    delete e;
+}
+
+const char* mjg_last_error(const mjg_engine* e) {return e ? e->err.c_str() : "null engine";}
+
+mjg_status mjg_system_set_state(mjg_engine* e, const double* mass, const double* pos, const double* vel)
+{
+    return guarded(e, [&](mjg::Engine& x) {x.set_state(mass, pos, vel);});
+}
+mjg_status mjg_system_get_state(mjg_engine* e, double* pos, double* vel, double* force)
+{
+    return guarded(e, [&](mjg::Engine& x) {x.get_state(pos, vel, force);});
+}
+mjg_status mjg_system_set_attribute(mjg_engine* e, const char* key, double value)
+{
+    if(key == nullptr) {return MJG_INVALID_ARG;}
+    return guarded(e, [&](mjg::Engine& x) {x.set_attribute(key, value);});
+}
+mjg_status mjg_add_local(mjg_engine* e, int kind, const char* topology, size_t n_terms, const uint32_t* indices,
+                         const double* params, double margin, int* id)
+{
+    return guarded(e, [&](mjg::Engine& x) {
+        const int k = x.add_local(kind, topology, n_terms, indices, params, margin);
+        if(id) {*id = k;}
+    });
+}
+mjg_status mjg_add_global(mjg_engine* e, int potential, double cutoff_ratio, double epsilon, int partition,
+                          double margin, int ignore_molecule, size_t n_within, const char* const* within_kinds,
+                          const uint32_t* within_dist, size_t n_participants, const uint32_t* participants,
+                          const double* params, int* id)
+{
+    return guarded(e, [&](mjg::Engine& x) {
+        const int k = x.add_global(potential, cutoff_ratio, epsilon, partition, margin, ignore_molecule, n_within,
+                                   within_kinds, within_dist, n_participants, participants, params);
+        // interaction ids follow calc_energy order: locals first, then globals
+        if(id) {*id = (int)x.num_locals() + k;}
+    });
+}
+mjg_status mjg_forcefield_initialize(mjg_engine* e)
+{
+    return guarded(e, [&](mjg::Engine& x) {x.initialize();});
+}
+size_t mjg_num_interactions(const mjg_engine* e) {return (e && e->impl) ? e->impl->num_interactions() : 0;}
+mjg_status mjg_calc_force(mjg_engine* e) {return guarded(e, [&](mjg::Engine& x) {x.calc_force();});}
+mjg_status mjg_calc_force_and_energy(mjg_engine* e, double* energy)
+{
+    return guarded(e, [&](mjg::Engine& x) {const double v = x.calc_force_and_energy(); if(energy) {*energy = v;}});
+}
+mjg_status mjg_calc_energies(mjg_engine* e, double* per_interaction, double* total)
+{
+    return guarded(e, [&](mjg::Engine& x) {x.calc_energies(per_interaction, total);});
+}
+mjg_status mjg_calc_force_of(mjg_engine* e, int id, double* force_out)
+{
+    if(force_out == nullptr) {return MJG_INVALID_ARG;}
+    return guarded(e, [&](mjg::Engine& x) {x.calc_force_of(id, force_out);});
+}
+mjg_status mjg_reduce_margin(mjg_engine* e, double dmargin)
+{
+    return guarded(e, [&](mjg::Engine& x) {x.reduce_margin(dmargin);});
+}
+mjg_status mjg_neighbor_pairs(mjg_engine* e, int id, uint32_t* pairs, size_t capacity, size_t* n_pairs)
+{
+    return guarded(e, [&](mjg::Engine& x) {
+        const size_t n = x.neighbor_pairs(id, pairs, capacity);
+        if(n_pairs) {*n_pairs = n;}
+    });
+}
+mjg_status mjg_integrator_set(mjg_engine* e, int type, double dt, const double* coefs, uint64_t seed)
+{
+    return guarded(e, [&](mjg::Engine& x) {x.set_integrator(type, dt, coefs, seed);});
+}
+mjg_status mjg_integrator_initialize(mjg_engine* e)
+{
+    return guarded(e, [&](mjg::Engine& x) {x.integrator_initialize();});
+}
+mjg_status mjg_step(mjg_engine* e, size_t n_steps)
+{
+    return guarded(e, [&](mjg::Engine& x) {x.step(n_steps);});
+}
+mjg_status mjg_counters(mjg_engine* e, uint64_t* steps, uint64_t* rebuilds)
+{
+    return guarded(e, [&](mjg::Engine& x) {x.counters(steps, rebuilds);});
+}
+mjg_status mjg_kinetic_energy(mjg_engine* e, double* ke)
+{
+    return guarded(e, [&](mjg::Engine& x) {const double v = x.kinetic_energy(); if(ke) {*ke = v;}});
+}
+
+mjg_status mjg_profile_interaction(mjg_engine* e, int id, int reps, double* avg_ms, uint64_t* half_pairs)
+{
+    return guarded(e, [&](mjg::Engine& x) {
+        const double v = x.profile_interaction(id, reps, half_pairs);
+        if(avg_ms) {*avg_ms = v;}
+    });
+}
+
+} // extern "C"

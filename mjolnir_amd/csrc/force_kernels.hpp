@@ -1,0 +1,342 @@
+// Force kernels of the FP64 path.
+//
+// Local terms: one thread per term writes each body's force into its own slot of a term-force
+// buffer (no atomics); k_gather_local then sums, per particle, the slots that touch it in
+// interaction order and term order, i.e. in the order the reference's LocalForceField loop
+// accumulates them (LocalForceField.hpp:111-118).
+//
+// Pair terms: one thread per participant walks its full (two-sided) ELL list and accumulates
+// its own force in registers, so no atomics are needed and the sum order is fixed by the list.
+// The per-pair arithmetic follows the reference specialisations (cited below); the pair
+// parameter is recomputed from the per-particle table with the lower index first, which is
+// what prepare_params cached for the pair (i < j) in the reference.
+#pragma once
+#include "device_common.hpp"
+
+namespace mjg
+{
+
+enum LocalKind : int
+{
+    kBondHarmonic = 1, kBondGo = 2, kContactGo = 3, kAngleHarmonic = 4, kAngleGaussian = 5, kDihedralClementi = 6
+};
+enum PairKind : int {kPairLJ = 1, kPairEV = 2, kPairDH = 3};
+
+struct LocalSeg
+{
+    int kind;
+    int arity;
+    int npar;
+    uint32_t n_terms;
+    uint32_t block_begin;   // first block of this segment in the launch
+    uint32_t idx_off;       // into the flat index array
+    uint32_t par_off;       // into the flat parameter array
+    uint32_t slot_off;      // first body slot (3 doubles per slot)
+    uint32_t term_off;      // into the per-term energy array
+};
+constexpr int kMaxLocalSegs = 32;
+struct LocalSegTable
+{
+    LocalSeg seg[kMaxLocalSegs];
+    int n;
+};
+
+__device__ __forceinline__ void store3(double* f, uint32_t slot, double x, double y, double z)
+{
+    f[3 * (size_t)slot + 0] = x; f[3 * (size_t)slot + 1] = y; f[3 * (size_t)slot + 2] = z;
+}
+
+template<bool ENERGY>
+__device__ void local_term(const LocalSeg& s, uint32_t t, const double4* __restrict__ pos, const Box& box,
+                           const uint32_t* __restrict__ idx, const double* __restrict__ par,
+                           double* __restrict__ tf, double* __restrict__ te)
+{
+    const uint32_t* ix = idx + s.idx_off + (size_t)t * s.arity;
+    const double* p = par + s.par_off + (size_t)t * s.npar;
+    const uint32_t slot = s.slot_off + t * s.arity;
+    double energy = 0.0;
+    if(s.kind == kBondHarmonic)
+    {
+        // BondLengthInteraction.hpp:126-148 (force), :112-124 (energy); HarmonicPotential.hpp:23-32
+        const double4 a = pos[ix[0]], b = pos[ix[1]];
+        double dx, dy, dz;
+        adjust_direction(box, a.x, a.y, a.z, b.x, b.y, b.z, dx, dy, dz);
+        const double len2 = length_sq(dx, dy, dz);
+        const double rlen = 1.0 / sqrt(len2);
+        const double len = len2 * rlen;
+        const double k = p[0], v0 = p[1];
+        const double force = -1 * (2 * k * (len - v0));
+        const double c = force * rlen;
+        store3(tf, slot + 0, -(dx * c), -(dy * c), -(dz * c));
+        store3(tf, slot + 1, dx * c, dy * c, dz * c);
+        if(ENERGY) {const double dv = sqrt(len2) - v0; energy = k * dv * dv;}
+    }
+    else if(s.kind == kBondGo || s.kind == kContactGo)
+    {
+        // BondLengthGoContactInteraction.hpp:126-167, GoContactInteraction.hpp:185-228; the
+        // contact interaction's active list only drops terms beyond the cutoff, which the
+        // cutoff test below drops as well.
+        const double4 a = pos[ix[0]], b = pos[ix[1]];
+        double dx, dy, dz;
+        adjust_direction(box, a.x, a.y, a.z, b.x, b.y, b.z, dx, dy, dz);
+        const double len2 = length_sq(dx, dy, dz);
+        const double k = p[0], v0 = p[1];
+        const double cut = v0 * 2.5;
+        double fx = 0, fy = 0, fz = 0;
+        if(!(cut * cut <= len2))
+        {
+            const double r2 = 1.0 / len2;
+            const double v0r_2 = v0 * v0 * r2;
+            const double v0r_6 = v0r_2 * v0r_2 * v0r_2;
+            const double v0r_10 = v0r_6 * v0r_2 * v0r_2;
+            const double v0r_12 = v0r_10 * v0r_2;
+            const double coef = -60 * k * r2 * (v0r_10 - v0r_12);
+            fx = coef * dx; fy = coef * dy; fz = coef * dz;
+            if(ENERGY)
+            {
+                // GoContactPotential.hpp:26-35 at r = sqrt(len2)
+                const double r = sqrt(len2);
+                const double rd = v0 / r;
+                const double rd3 = rd * rd * rd;
+                const double rd9 = rd3 * rd3 * rd3;
+                energy = (v0 * 2.5 <= r) ? 0.0 : k * (5 * rd9 * rd3 - 6 * rd9 * rd);
+            }
+        }
+        store3(tf, slot + 0, -fx, -fy, -fz);
+        store3(tf, slot + 1, fx, fy, fz);
+    }
+    else if(s.kind == kAngleHarmonic || s.kind == kAngleGaussian)
+    {
+        // BondAngleInteraction.hpp:129-189
+        const double4 p0 = pos[ix[0]], p1 = pos[ix[1]], p2 = pos[ix[2]];
+        double ijx, ijy, ijz, kjx, kjy, kjz;
+        adjust_direction(box, p1.x, p1.y, p1.z, p0.x, p0.y, p0.z, ijx, ijy, ijz);
+        const double inv_ij = 1.0 / sqrt(length_sq(ijx, ijy, ijz));
+        const double eix = ijx * inv_ij, eiy = ijy * inv_ij, eiz = ijz * inv_ij;
+        adjust_direction(box, p1.x, p1.y, p1.z, p2.x, p2.y, p2.z, kjx, kjy, kjz);
+        const double inv_kj = 1.0 / sqrt(length_sq(kjx, kjy, kjz));
+        const double ekx = kjx * inv_kj, eky = kjy * inv_kj, ekz = kjz * inv_kj;
+        const double cos_theta = fmin(fmax(eix * ekx + eiy * eky + eiz * ekz, -1.0), 1.0);
+        const double theta = acos(cos_theta);
+        double coef, pot = 0.0;
+        if(s.kind == kAngleHarmonic)
+        {
+            const double dv = theta - p[1];
+            coef = -(2 * p[0] * dv);
+            if(ENERGY) {pot = p[0] * dv * dv;}
+        }
+        else
+        {
+            // GaussianPotential.hpp:20-38
+            const double k = p[0], sigma = p[1], v0 = p[2];
+            const double inv_sigma2 = -1. / (2 * sigma * sigma);
+            const double dval = theta - v0;
+            const double ex = exp(inv_sigma2 * dval * dval);
+            coef = -(2 * inv_sigma2 * dval * k * ex);
+            if(ENERGY) {pot = k * ex;}
+        }
+        const double sin_theta = sin(theta);
+        const double cis = coef / fmax(sin_theta, 1e-8);
+        const double ci = cis * inv_ij, ck = cis * inv_kj;
+        const double Fix = ci * (cos_theta * eix - ekx), Fiy = ci * (cos_theta * eiy - eky), Fiz = ci * (cos_theta * eiz - ekz);
+        const double Fkx = ck * (cos_theta * ekx - eix), Fky = ck * (cos_theta * eky - eiy), Fkz = ck * (cos_theta * ekz - eiz);
+        store3(tf, slot + 0, Fix, Fiy, Fiz);
+        store3(tf, slot + 1, -Fix - Fkx, -Fiy - Fky, -Fiz - Fkz);
+        store3(tf, slot + 2, Fkx, Fky, Fkz);
+        energy = pot;
+    }
+    else if(s.kind == kDihedralClementi)
+    {
+        // DihedralAngleInteraction.hpp:157-227; ClementiDihedralPotential.hpp:27-42
+        const double4 ri = pos[ix[0]], rj = pos[ix[1]], rk = pos[ix[2]], rl = pos[ix[3]];
+        double ijx, ijy, ijz, kjx, kjy, kjz, lkx, lky, lkz;
+        adjust_direction(box, rj.x, rj.y, rj.z, ri.x, ri.y, ri.z, ijx, ijy, ijz);
+        adjust_direction(box, rj.x, rj.y, rj.z, rk.x, rk.y, rk.z, kjx, kjy, kjz);
+        adjust_direction(box, rk.x, rk.y, rk.z, rl.x, rl.y, rl.z, lkx, lky, lkz);
+        const double klx = -1.0 * lkx, kly = -1.0 * lky, klz = -1.0 * lkz;
+        const double r_kj_lensq = length_sq(kjx, kjy, kjz);
+        const double r_kj_rlen = 1.0 / sqrt(r_kj_lensq);
+        const double r_kj_rlensq = r_kj_rlen * r_kj_rlen;
+        const double r_kj_len = r_kj_rlen * r_kj_lensq;
+        const double mx = ijy * kjz - ijz * kjy, my = ijz * kjx - ijx * kjz, mz = ijx * kjy - ijy * kjx;
+        const double nx = kjy * klz - kjz * kly, ny = kjz * klx - kjx * klz, nz = kjx * kly - kjy * klx;
+        const double m_lensq = length_sq(mx, my, mz), n_lensq = length_sq(nx, ny, nz);
+        const double dot_mn = (mx * nx + my * ny + mz * nz) * (1.0 / sqrt(m_lensq * n_lensq));
+        const double cos_phi = fmin(fmax(dot_mn, -1.0), 1.0);
+        const double phi = copysign(acos(cos_phi), ijx * nx + ijy * ny + ijz * nz);
+        const double k1 = p[0], k3 = p[1], v0 = p[2];
+        const double dphi = phi - v0;
+        const double sin1 = sin(dphi);
+        const double sin3 = sin1 * (3.0 - 4.0 * sin1 * sin1);
+        const double coef = -(k1 * sin1 + 3.0 * k3 * sin3);
+        if(ENERGY)
+        {
+            const double cos1 = cos(dphi);
+            const double cos3 = cos1 * (4.0 * cos1 * cos1 - 3.0);
+            energy = k1 * (1.0 - cos1) + k3 * (1.0 - cos3);
+        }
+        const double ci = coef * r_kj_len / m_lensq;
+        const double cl = -coef * r_kj_len / n_lensq;
+        const double Fix = ci * mx, Fiy = ci * my, Fiz = ci * mz;
+        const double Flx = cl * nx, Fly = cl * ny, Flz = cl * nz;
+        const double coef_ijk = (ijx * kjx + ijy * kjy + ijz * kjz) * r_kj_rlensq;
+        const double coef_jkl = (klx * kjx + kly * kjy + klz * kjz) * r_kj_rlensq;
+        store3(tf, slot + 0, Fix, Fiy, Fiz);
+        store3(tf, slot + 1, (coef_ijk - 1.0) * Fix - coef_jkl * Flx, (coef_ijk - 1.0) * Fiy - coef_jkl * Fly,
+               (coef_ijk - 1.0) * Fiz - coef_jkl * Flz);
+        store3(tf, slot + 2, (coef_jkl - 1.0) * Flx - coef_ijk * Fix, (coef_jkl - 1.0) * Fly - coef_ijk * Fiy,
+               (coef_jkl - 1.0) * Flz - coef_ijk * Fiz);
+        store3(tf, slot + 3, Flx, Fly, Flz);
+    }
+    if(ENERGY) {te[s.term_off + t] = energy;}
+}
+
+// One launch covers every local interaction: each block belongs to one segment, found from
+// the segment table, so the branch on the kind is uniform per block.
+template<bool ENERGY>
+__global__ void __launch_bounds__(256)
+k_local_terms(const double4* __restrict__ pos, Box box, LocalSegTable tab,
+              const uint32_t* __restrict__ idx, const double* __restrict__ par,
+              double* __restrict__ tf, double* __restrict__ te, int only_seg)
+{
+    int si = 0;
+    while(si + 1 < tab.n && blockIdx.x >= tab.seg[si + 1].block_begin) {++si;}
+    if(only_seg >= 0 && si != only_seg) {return;}
+    const LocalSeg s = tab.seg[si];
+    const uint32_t t = (blockIdx.x - s.block_begin) * blockDim.x + threadIdx.x;
+    if(t >= s.n_terms) {return;}
+    local_term<ENERGY>(s, t, pos, box, idx, par, tf, te);
+}
+
+// force[i] = sum of the local-term slots of particle i (gstart/gslot: CSR per particle).
+__global__ void k_gather_local(uint32_t n, const uint32_t* __restrict__ gstart, const uint32_t* __restrict__ gslot,
+                               const double* __restrict__ tf, double4* __restrict__ force)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) {return;}
+    double x = 0, y = 0, z = 0;
+    const uint32_t b = gstart[i], e = gstart[i + 1];
+    for(uint32_t k = b; k < e; ++k)
+    {
+        const uint32_t s = gslot[k];
+        x += tf[3 * (size_t)s + 0]; y += tf[3 * (size_t)s + 1]; z += tf[3 * (size_t)s + 2];
+    }
+    force[i] = make_double4(x, y, z, 0.0);
+}
+
+struct PairParams
+{
+    int kind;
+    double cutoff_ratio2;    // LJ / EV: cutoff_ratio^2
+    double coef_at_cutoff;
+    double eps;              // EV epsilon
+    double debye, inv_debye, abs_cutoff, inv4pieps; // DH
+    int npar;                // per-particle parameter columns
+};
+
+// Pair force on the owner i from j (full list).  d = adjust_direction(r_i, r_j).
+template<int KIND, bool ENERGY>
+__device__ __forceinline__ void pair_term(const PairParams& pp, uint32_t i, uint32_t j, const double* __restrict__ ppar,
+                                          double dx, double dy, double dz,
+                                          double& fx, double& fy, double& fz, double& e)
+{
+    const double l_sq = length_sq(dx, dy, dz);
+    if(KIND == kPairLJ)
+    {
+        // GlobalPairLennardJonesInteraction.hpp:157-188 with LorentzBerthelotRule
+        // (ParameterList.hpp:240-252)
+        const double s1 = ppar[2 * i], e1 = ppar[2 * i + 1];
+        const double s2 = ppar[2 * j], e2 = ppar[2 * j + 1];
+        const double sigma = (s1 + s2) / 2;
+        const double eps = (e1 == e2) ? e1 : sqrt(e1 * e2);
+        const double sigma_sq = sigma * sigma;
+        if(sigma_sq * pp.cutoff_ratio2 < l_sq) {return;}
+        const double rcp = 1 / l_sq;
+        const double s2l2 = sigma_sq * rcp;
+        const double s6l6 = s2l2 * s2l2 * s2l2;
+        const double c = 24 * eps * (s6l6 - 2 * s6l6 * s6l6) * rcp;
+        fx += dx * c; fy += dy * c; fz += dz * c;
+        if(ENERGY && i < j) {e += 4 * eps * (s6l6 * s6l6 - s6l6 - pp.coef_at_cutoff);}
+    }
+    else if(KIND == kPairEV)
+    {
+        // GlobalPairExcludedVolumeInteraction.hpp:159-193; radius sum
+        // (ExcludedVolumePotential.hpp:192-195)
+        const double radius = ppar[i] + ppar[j];
+        const double sigma_sq = radius * radius;
+        if(sigma_sq * pp.cutoff_ratio2 < l_sq) {return;}
+        const double rcp = 1.0 / l_sq;
+        const double s2l2 = sigma_sq * rcp;
+        const double s6l6 = s2l2 * s2l2 * s2l2;
+        const double c = -(pp.eps * 12) * s6l6 * s6l6 * rcp;
+        fx += dx * c; fy += dy * c; fz += dz * c;
+        if(ENERGY && i < j) {e += pp.eps * (s6l6 * s6l6 - pp.coef_at_cutoff);}
+    }
+    else
+    {
+        // GlobalPairInteraction.hpp:164-185 with DebyeHuckelPotential.hpp:47-65; charge
+        // product cached as (1/(4 pi eps0 epsk) q_lo) q_hi (DebyeHuckelPotential.hpp:249-254)
+        const uint32_t lo = (i < j) ? i : j, hi = (i < j) ? j : i;
+        const double qq = pp.inv4pieps * ppar[lo] * ppar[hi];
+        const double rl = 1.0 / sqrt(l_sq);
+        const double l = l_sq * rl;
+        if(pp.abs_cutoff <= l) {return;}
+        const double ex = exp(-l * pp.inv_debye);
+        const double f_mag = -qq * (pp.debye + l) * pp.inv_debye / (l * l) * ex;
+        const double c = f_mag * rl;
+        fx += dx * c; fy += dy * c; fz += dz * c;
+        if(ENERGY && i < j) {e += qq * (ex / l - pp.coef_at_cutoff);}
+    }
+}
+
+template<int KIND, bool ENERGY>
+__global__ void __launch_bounds__(256)
+k_pair_full(const double4* __restrict__ pos, double4* __restrict__ force, const uint32_t* __restrict__ part,
+            const uint32_t* __restrict__ order, uint32_t np, const uint32_t* __restrict__ nl,
+            const uint32_t* __restrict__ nn, uint32_t stride, const double* __restrict__ ppar,
+            PairParams pp, Box box, double* __restrict__ e_out)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if(t >= np) {return;}
+    const uint32_t i = part[order[t]];
+    const double4 ri = pos[i];
+    double fx = 0, fy = 0, fz = 0, e = 0;
+    const uint32_t cnt = nn[t];
+    for(uint32_t k = 0; k < cnt; ++k)
+    {
+        const uint32_t j = nl[(size_t)k * stride + t];
+        const double4 rj = pos[j];
+        double dx, dy, dz;
+        adjust_direction(box, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, dx, dy, dz);
+        pair_term<KIND, ENERGY>(pp, i, j, ppar, dx, dy, dz, fx, fy, fz, e);
+    }
+    double4 f = force[i];
+    f.x += fx; f.y += fy; f.z += fz;
+    force[i] = f;
+    if(ENERGY) {e_out[t] = e;}
+}
+
+__global__ void k_zero_doubles(double* __restrict__ p, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i < n) {p[i] = 0.0;}
+}
+__global__ void k_zero_double4(double4* __restrict__ p, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i < n) {p[i] = make_double4(0.0, 0.0, 0.0, 0.0);}
+}
+
+// Deterministic sum of n doubles (one block of 1024 threads): thread k adds elements
+// k, k+1024, ... in order, then a fixed-shape tree.
+__global__ void __launch_bounds__(1024)
+k_sum(const double* __restrict__ in, uint32_t n, double* __restrict__ out)
+{
+    __shared__ double lds[16];
+    double s = 0.0;
+    for(uint32_t k = threadIdx.x; k < n; k += blockDim.x) {s += in[k];}
+    s = block_sum(s, lds);
+    if(threadIdx.x == 0) {*out = s;}
+}
+
+} // mjg

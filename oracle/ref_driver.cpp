@@ -233,6 +233,14 @@ static std::unique_ptr<partitionBase> make_partition(const GlobalIn& g)
     throw std::runtime_error("unknown partition " + g.partition);
 }
 
+#ifdef MJOLNIR_WITH_OPENMP
+// the OpenMP specialisations keep their partition private; the timing build never dumps lists
+template<typename interactionT>
+static std::function<void(std::ostream&)> pair_dumper(const interactionT*, std::vector<std::size_t>)
+{
+    return [](std::ostream& os) {os << "pairs 0\n";};
+}
+#else
 template<typename interactionT>
 static std::function<void(std::ostream&)> pair_dumper(const interactionT* inter, std::vector<std::size_t> participants)
 {
@@ -252,6 +260,7 @@ static std::function<void(std::ostream&)> pair_dumper(const interactionT* inter,
         for(const auto& p : pairs) {os << p.first << ' ' << p.second << "\n";}
     };
 }
+#endif
 
 template<typename traitsT>
 static Built<traitsT> build(const SpecIn& s)

@@ -1,0 +1,151 @@
+"""HIP engine (through the C ABI) against the reference's golden vectors and the oracle.
+
+Bars (FP64 path):
+* neighbour lists: bit-exact as sets against the reference's lists;
+* per-particle forces: |F_gpu - F_ref| <= 1e-10 (|F_ref,i| + F_rms) per particle;
+* energies: |E_gpu - E_ref| <= 1e-10 max(|E_ref|, 1 kcal/mol);
+* zero-temperature (gamma = 0) trajectories: max |x_gpu - x_ref| <= 1e-8 A after 1000 steps;
+* T > 0: the engine and the oracle draw the same Philox noise, so short trajectories agree
+  to 1e-7 A; thermostat statistics within their standard error.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import force_tolerance, golden_case
+import make_golden
+
+pytestmark = pytest.mark.gpu
+
+CASES = list(make_golden.cases().keys())
+
+
+def _engine(spec, **kw):
+    from mjolnir_amd import capi
+    print(f"[gpu] engine {spec.name} N={spec.n}", flush=True)
+    return capi.Engine(spec, **kw)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_forces_energies_pairs_vs_reference(name):
+    spec, g = golden_case(name)
+    e = _engine(spec, with_integrator=False)
+    names = [str(x) for x in g["names"]]
+    per, tot = e.energies()
+    for k in range(len(names)):
+        ref_f = g[f"force_{k}"]
+        f = e.force_of(k)
+        err = np.linalg.norm(f - ref_f, axis=1)
+        assert np.all(err <= force_tolerance(ref_f, 1e-10)), (names[k], err.max())
+        ref_e = float(g[f"energy_{k}"])
+        assert abs(per[k] - ref_e) <= 1e-10 * max(abs(ref_e), 1.0), (names[k], per[k], ref_e)
+        if f"npairs_{k}" in g:
+            p = e.pairs(k)
+            assert len(p) == int(g[f"npairs_{k}"]), names[k]
+            assert make_golden.pair_key_hash(p, spec.n) == str(g[f"pairhash_{k}"]), names[k]
+    e.calc_force()
+    _, _, ftot = e.state()
+    err = np.linalg.norm(ftot - g["total_force"], axis=1)
+    assert np.all(err <= force_tolerance(g["total_force"], 1e-10)), err.max()
+    assert abs(tot - float(g["total_energy"])) <= 1e-10 * max(abs(float(g["total_energy"])), 1.0)
+
+
+@pytest.mark.parametrize("name", [n for n in CASES if make_golden.cases()[n][2] > 0])
+def test_zero_temperature_trajectory_vs_reference(name):
+    spec, g = golden_case(name)
+    e = _engine(spec)
+    e.step(int(g["zt_steps"]))
+    x, v, _ = e.state()
+    d = x - g["zt_pos"]
+    if spec.boundary == "periodic":
+        w = spec.upper - spec.lower
+        d = d - w * np.round(d / w)
+    assert np.abs(d).max() <= 1e-8, np.abs(d).max()
+    assert np.abs(v - g["zt_vel"]).max() <= 1e-8
+
+
+def test_langevin_trajectory_vs_oracle_same_noise():
+    import oracle
+    from mjolnir_amd import workloads
+    spec = make_golden.perturb(workloads.lj_fluid(n_side=8, seed=5, gamma=0.5), 0.1, 1)
+    e = _engine(spec)
+    o = oracle.OracleEngine(spec)
+    for _ in range(4):
+        e.step(50)
+        o.step(50)
+    x, v, _ = e.state()
+    xo, vo, _ = o.state()
+    d = x - xo
+    w = spec.upper - spec.lower
+    d = d - w * np.round(d / w)
+    assert np.abs(d).max() < 1e-7
+    assert np.abs(v - vo).max() < 1e-7
+    steps, rebuilds = e.counters()
+    assert steps == 200
+
+
+def test_gjf_trajectory_vs_oracle_same_noise():
+    import copy
+    import oracle
+    from mjolnir_amd import workloads
+    spec = make_golden.perturb(workloads.cg_chain(n=120, seed=7), 0.2, 2)
+    spec = copy.deepcopy(spec)
+    spec.integrator.type = "GJFNVTLangevin"
+    spec.integrator.coefs = np.full(spec.n, 0.1)
+    spec.integrator.dt = 0.02
+    e = _engine(spec)
+    o = oracle.OracleEngine(spec)
+    e.step(150)
+    o.step(150)
+    x, v, _ = e.state()
+    xo, vo, _ = o.state()
+    assert np.abs(x - xo).max() < 1e-7
+    assert np.abs(v - vo).max() < 1e-7
+
+
+def test_kinetic_temperature_lj():
+    """<KE> = 3/2 N kB T for the thermostatted LJ fluid (equipartition)."""
+    from mjolnir_amd import workloads
+    spec = workloads.lj_fluid(n_side=8, seed=9, gamma=1.0)
+    e = _engine(spec)
+    e.step(2000)
+    kes = []
+    for _ in range(60):
+        e.step(50)
+        kes.append(e.kinetic_energy())
+    kes = np.array(kes)
+    target = 1.5 * spec.n * workloads.KB_KCAL * 300.0
+    # each sample has std sqrt(3N/2) kB T; samples 50 steps apart are nearly independent
+    sem = np.sqrt(1.5 * spec.n) * workloads.KB_KCAL * 300.0 / np.sqrt(len(kes))
+    assert abs(kes.mean() - target) < 4 * sem + 0.01 * target
+
+
+def test_neighbor_sets_after_rebuilds_vs_oracle():
+    """After many steps (several list rebuilds): a list built from the current positions equals
+    the oracle's, and the lists the stepped engine holds still contain every pair inside the
+    cutoff (the skin guarantee of reduce_margin, PeriodicGridCellList.hpp:100-110)."""
+    import oracle
+    from mjolnir_amd import workloads
+    spec = workloads.zero_temperature(make_golden.perturb(workloads.cg_chain(n=300, seed=3), 0.2, 4))
+    e = _engine(spec)
+    e.step(600)
+    steps, rebuilds = e.counters()
+    assert steps == 600 and rebuilds > 0
+    x, v, _ = e.state()
+    s2 = make_golden.perturb(spec, 0.0, 0)
+    s2.pos = x
+    s2.vel = v
+    o = oracle.OracleEngine(s2, with_integrator=False)
+    e2 = _engine(s2, with_integrator=False)
+    for gid, k in enumerate(range(len(spec.local), len(spec.interaction_names()))):
+        want = o.pairs(gid)
+        assert set(map(tuple, e2.pairs(k).tolist())) == set(map(tuple, want.tolist()))
+        g = spec.global_[gid]
+        rc = (2 * g.params[:, 0].max() * g.cutoff) if g.potential == "ExcludedVolume" else None
+        if rc is None:
+            continue
+        d = x[want[:, 1]] - x[want[:, 0]]
+        inside = want[np.einsum("ij,ij->i", d, d) < rc * rc]
+        held = set(map(tuple, e.pairs(k).tolist()))
+        assert set(map(tuple, inside.tolist())) <= held
