@@ -130,8 +130,26 @@ def cases():
         "two_chains_self": (lambda: two_chains(ignore="Self"), True, 500),
         "two_chains_others": (lambda: two_chains(ignore="Others"), True, 0),
         "lj_pair_scan": (lj_pair_scan, True, 0),
-        "lj_boundary": (lj_boundary_cases, True, 300),
+        "lj_boundary": (lj_boundary_cases, True, 0),
     }
+
+
+def sensitivity(spec: SystemSpec, steps: int) -> np.ndarray:
+    import oracle
+    a = oracle.OracleEngine(spec)
+    s2 = copy.deepcopy(spec)
+    s2.pos = s2.pos * (1 + 1e-15)
+    b = oracle.OracleEngine(s2)
+    out = []
+    for _ in range(steps // 100):
+        a.step(100)
+        b.step(100)
+        d = a.state()[0] - b.state()[0]
+        if spec.boundary == "periodic":
+            w = spec.upper - spec.lower
+            d = d - w * np.round(d / w)
+        out.append(np.abs(d).max())
+    return np.array(out)
 
 
 def make(name: str, factory, full_pairs: bool, zt_steps: int):
@@ -153,11 +171,19 @@ def make(name: str, factory, full_pairs: bool, zt_steps: int):
                 out[f"pairs_{k}"] = p.astype(np.uint32)
     out["names"] = np.array(names)
     if zt_steps:
-        frames = refrun.run(spec, zt_steps, every=0)
+        # frames every 100 steps for the small cases; only the last for the large ones
+        every = 100 if spec.n <= 600 else 0
+        frames = refrun.run(spec, zt_steps, every=every)
         step, x, v = frames[-1]
         out["zt_steps"] = np.array(step)
         out["zt_pos"] = x
         out["zt_vel"] = v
+        if every:
+            out["zt_frame_steps"] = np.array([f[0] for f in frames])
+            out["zt_frames"] = np.stack([f[1] for f in frames])
+        # the reference's own sensitivity: the oracle (bit-identical to the reference) started
+        # from positions scaled by (1 + 1e-15); max |dx| every 100 steps
+        out["zt_sens"] = sensitivity(spec, zt_steps)
     path = os.path.join(HERE, f"{name}.npz")
     np.savez_compressed(path, **out)
     print(f"{name}: {len(names)} interactions, {os.path.getsize(path) / 1024:.1f} KiB")

@@ -51,18 +51,39 @@ def test_forces_energies_pairs_vs_reference(name):
     assert abs(tot - float(g["total_energy"])) <= 1e-10 * max(abs(float(g["total_energy"])), 1.0)
 
 
-@pytest.mark.parametrize("name", [n for n in CASES if make_golden.cases()[n][2] > 0])
-def test_zero_temperature_trajectory_vs_reference(name):
-    spec, g = golden_case(name)
-    e = _engine(spec)
-    e.step(int(g["zt_steps"]))
-    x, v, _ = e.state()
-    d = x - g["zt_pos"]
+def _wrapped_diff(spec, a, b):
+    d = a - b
     if spec.boundary == "periodic":
         w = spec.upper - spec.lower
         d = d - w * np.round(d / w)
-    assert np.abs(d).max() <= 1e-8, np.abs(d).max()
-    assert np.abs(v - g["zt_vel"]).max() <= 1e-8
+    return np.abs(d).max()
+
+
+@pytest.mark.parametrize("name", [n for n in CASES if make_golden.cases()[n][2] > 0])
+def test_zero_temperature_trajectory_vs_reference(name):
+    """gamma = 0 trajectories against the reference's, every 100 steps.  Bar: 1e-8 A, or ten
+    times the reference's own sensitivity when that is larger -- zt_sens is how far the
+    reference itself moves when its start is scaled by (1 + 1e-15); no implementation whose
+    libm differs from glibc's in the last ulp can stay closer than that."""
+    spec, g = golden_case(name)
+    e = _engine(spec)
+    sens = g["zt_sens"]
+    if "zt_frames" in g:
+        done = 0
+        for k, step in enumerate(g["zt_frame_steps"]):
+            e.step(int(step) - done)
+            done = int(step)
+            x, _, _ = e.state()
+            bar = 1e-8 if step == 0 else max(1e-8, 10 * sens[int(step) // 100 - 1])
+            d = _wrapped_diff(spec, x, g["zt_frames"][k])
+            print(f"[gpu] {name} step {step}: max|dx| = {d:.2e} (bar {bar:.1e})", flush=True)
+            assert d <= bar, (step, d, bar)
+    else:
+        e.step(int(g["zt_steps"]))
+    x, v, _ = e.state()
+    bar = max(1e-8, 10 * sens[-1])
+    assert _wrapped_diff(spec, x, g["zt_pos"]) <= bar
+    assert np.abs(v - g["zt_vel"]).max() <= max(1e-8, 100 * sens[-1])
 
 
 def test_langevin_trajectory_vs_oracle_same_noise():
