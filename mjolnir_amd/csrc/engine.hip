@@ -320,8 +320,15 @@ void Engine::alloc_list(GlobalIA& g, uint32_t cap)
 {
     const uint32_t np = (uint32_t)g.part.size();
     g.cap = cap;
-    g.stride = (np + 63u) & ~63u;
-    g.d_nl = dalloc<uint32_t>((size_t)g.stride * cap);
+    g.d_nl = dalloc<uint32_t>((size_t)np * cap);
+}
+
+// lanes per participant: about six partners per lane, a power of two in [1, 64]
+static uint32_t lanes_for(uint32_t maxcount)
+{
+    uint32_t t = 1;
+    while(t < 64 && t * 6 < maxcount) {t <<= 1;}
+    return t;
 }
 
 void Engine::setup_global(GlobalIA& g)
@@ -432,82 +439,174 @@ void Engine::setup_global(GlobalIA& g)
         g.d_cell_start = dalloc<uint32_t>(kMaxCells);
         g.d_cell_end = dalloc<uint32_t>(kMaxCells);
     }
+    // one-workgroup rebuild: the participants and the cells must fit its LDS tables
+    g.small = (np <= kSmallMaxPart) && (!box_.periodic || g.partition != MJG_PART_CELLLIST ||
+                                        g.grid.ncell <= kSmallMaxCells);
+    if(g.partition != MJG_PART_CELLLIST && g.sb.keys == nullptr)
+    {
+        g.sb.keys = dalloc<uint32_t>(np); g.sb.vals = dalloc<uint32_t>(np);
+    }
     g.d_nn = dalloc<uint32_t>(np);
     g.d_order = dalloc<uint32_t>(np);
     g.d_epair = dalloc<double>(np);
+    g.d_fg = dalloc<double4>(n_);
+    check(hipMemsetAsync(g.d_fg, 0, n_ * sizeof(double4), stream_), "memset");
+    sync();
+    g.tpp = 8;
     alloc_list(g, 64);
 }
 
-// Launch the rebuild pipeline of global g; every kernel checks ListCtl::rebuild itself.
-void Engine::launch_rebuild(GlobalIA& g, int gi)
+ListSegTable Engine::list_table()
 {
-    const uint32_t np = (uint32_t)g.part.size();
-    ListCtl* ctl = d_lctl_ + gi;
-    if(g.partition == MJG_PART_CELLLIST)
+    ListSegTable t{};
+    uint32_t block = 0;
+    for(size_t gi = 0; gi < globals_.size(); ++gi)
     {
-        if(!box_.periodic)
+        GlobalIA& g = globals_[gi];
+        ListSeg& s = t.seg[t.n++];
+        s.np = (uint32_t)g.part.size();
+        s.tpp = g.tpp;
+        s.block_begin = block;
+        s.cap = g.cap;
+        s.allpairs = (g.partition != MJG_PART_CELLLIST);
+        s.part = g.d_part;
+        s.skeys = g.sb.keys;
+        s.svals = g.sb.vals;
+        s.cell_start = g.d_cell_start;
+        s.cell_end = g.d_cell_end;
+        s.grid = g.d_grid;
+        s.pslot = g.d_pslot;
+        s.mol = g.d_mol;
+        s.ex_start = g.d_ex_start;
+        s.ex = g.d_ex;
+        s.nl = g.d_nl;
+        s.nn = g.d_nn;
+        s.order = g.d_order;
+        s.ctl = d_lctl_ + gi;
+        s.rule = g.rule;
+        block += blocks_for((size_t)s.np * s.tpp, 256);
+    }
+    return t;
+}
+
+PairSegTable Engine::pair_table(int only)
+{
+    PairSegTable t{};
+    uint32_t block = 0;
+    const int nl = (int)locals_.size();
+    for(size_t gi = 0; gi < globals_.size(); ++gi)
+    {
+        if(only >= 0 && only != nl + (int)gi) {continue;}
+        GlobalIA& g = globals_[gi];
+        PairSeg& s = t.seg[t.n++];
+        s.kind = g.pot;
+        s.tpp = g.tpp;
+        s.np = (uint32_t)g.part.size();
+        s.block_begin = block;
+        s.cap = g.cap;
+        s.part = g.d_part;
+        s.order = g.d_order;
+        s.nl = g.d_nl;
+        s.nn = g.d_nn;
+        s.ppar = g.d_ppar;
+        s.fout = g.d_fg;
+        s.eout = g.d_epair;
+        s.pp = g.pp;
+        block += blocks_for((size_t)s.np * s.tpp, 256);
+    }
+    return t;
+}
+
+// Launch the skin check and the rebuild pipeline of every global interaction; the kernels
+// check ListCtl::rebuild themselves, so a captured step always launches the same sequence.
+// force != 0 rebuilds every list now (initialize, explicit reduce_margin, capacity growth).
+void Engine::launch_rebuilds(int force)
+{
+    PrepSegTable small{}, large{};
+    std::vector<int> large_ids;
+    for(size_t gi = 0; gi < globals_.size(); ++gi)
+    {
+        GlobalIA& g = globals_[gi];
+        PrepSeg ps{};
+        ps.np = (uint32_t)g.part.size();
+        ps.periodic_grid = box_.periodic;
+        ps.never_rebuild = (g.partition == MJG_PART_NAIVE);
+        ps.allpairs = (g.partition != MJG_PART_CELLLIST);
+        ps.abs_margin = g.max_cutoff * g.margin;
+        ps.r_list = g.r_list;
+        ps.part = g.d_part;
+        ps.skeys = g.sb.keys;
+        ps.svals = g.sb.vals;
+        ps.cell_start = g.d_cell_start;
+        ps.cell_end = g.d_cell_end;
+        ps.grid = g.d_grid;
+        ps.ctl = d_lctl_ + gi;
+        if(g.small || g.partition != MJG_PART_CELLLIST) {small.seg[small.n++] = ps;}
+        else {large.seg[large.n++] = ps; large_ids.push_back((int)gi);}
+    }
+    if(small.n > 0)
+    {
+        hipLaunchKernelGGL(k_prep_small, dim3(small.n), dim3(1024), 0, stream_, d_pos_, small, d_sctl_, force);
+    }
+    if(large.n > 0)
+    {
+        hipLaunchKernelGGL(k_margin_large, dim3(1), dim3(64), 0, stream_, large, d_sctl_, force);
+        for(int gi : large_ids)
         {
-            hipLaunchKernelGGL(k_bbox, dim3(std::min<uint32_t>(blocks_for(np, 256), 1024)), dim3(256), 0, stream_,
-                               d_pos_, g.d_part, np, ctl);
-            hipLaunchKernelGGL(k_grid_open, dim3(1), dim3(1), 0, stream_, ctl, g.d_grid, g.r_list);
+            GlobalIA& g = globals_[gi];
+            const uint32_t np = (uint32_t)g.part.size();
+            ListCtl* ctl = d_lctl_ + gi;
+            if(!box_.periodic)
+            {
+                hipLaunchKernelGGL(k_bbox, dim3(std::min<uint32_t>(blocks_for(np, 256), 1024)), dim3(256), 0, stream_,
+                                   d_pos_, g.d_part, np, ctl);
+                hipLaunchKernelGGL(k_grid_open, dim3(1), dim3(1), 0, stream_, ctl, g.d_grid, g.r_list);
+            }
+            hipLaunchKernelGGL(k_cellkey, dim3(blocks_for(np, 256)), dim3(256), 0, stream_, d_pos_, g.d_part, np,
+                               g.d_grid, g.sb.keys, g.sb.vals, ctl);
+            radix_sort(g.sb, g.key_bits, &ctl->rebuild, stream_);
+            const uint32_t ncell_launch = box_.periodic ? g.grid.ncell : kMaxCells;
+            hipLaunchKernelGGL(k_cell_clear, dim3(blocks_for(ncell_launch, 256)), dim3(256), 0, stream_,
+                               g.d_cell_start, g.d_cell_end, g.d_grid, ctl);
+            hipLaunchKernelGGL(k_cell_range, dim3(blocks_for(np, 256)), dim3(256), 0, stream_, g.sb.keys, np,
+                               g.d_cell_start, g.d_cell_end, ctl);
         }
-        hipLaunchKernelGGL(k_cellkey, dim3(blocks_for(np, 256)), dim3(256), 0, stream_, d_pos_, g.d_part, np,
-                           g.d_grid, g.sb.keys, g.sb.vals, ctl);
-        radix_sort(g.sb, g.key_bits, &ctl->rebuild, stream_);
-        const uint32_t ncell_launch = box_.periodic ? g.grid.ncell : kMaxCells;
-        hipLaunchKernelGGL(k_cell_clear, dim3(blocks_for(ncell_launch, 256)), dim3(256), 0, stream_,
-                           g.d_cell_start, g.d_cell_end, g.d_grid, ctl);
-        hipLaunchKernelGGL(k_cell_range, dim3(blocks_for(np, 256)), dim3(256), 0, stream_, g.sb.keys, np,
-                           g.d_cell_start, g.d_cell_end, ctl);
-        hipLaunchKernelGGL(k_verlet_cells, dim3(blocks_for(np, 256)), dim3(256), 0, stream_, d_pos_, g.d_part, np,
-                           g.sb.keys, g.sb.vals, g.d_cell_start, g.d_cell_end, g.d_grid, box_, g.rule, g.d_pslot,
-                           g.d_mol, g.d_ex_start, g.d_ex, g.d_nl, g.d_nn, g.d_order, g.stride, g.cap, ctl);
     }
-    else
-    {
-        hipLaunchKernelGGL(k_verlet_allpairs, dim3(blocks_for(np, 256)), dim3(256), 0, stream_, d_pos_, g.d_part, np,
-                           box_, g.rule, g.d_pslot, g.d_mol, g.d_ex_start, g.d_ex, g.d_nl, g.d_nn, g.d_order,
-                           g.stride, g.cap, ctl);
-    }
+    const ListSegTable lt = list_table();
+    uint32_t blocks = 0;
+    for(int k = 0; k < lt.n; ++k) {blocks += blocks_for((size_t)lt.seg[k].np * lt.seg[k].tpp, 256);}
+    hipLaunchKernelGGL(k_verlet, dim3(blocks), dim3(256), 0, stream_, d_pos_, box_, lt);
 }
 
 // Build the lists of every global interaction now (initialize / explicit reduce_margin),
-// growing the list capacity until every partner list fits.
-void Engine::build_lists_now(bool all)
+// growing the list capacity until every partner list fits, then size the lanes per
+// participant from the longest list.
+void Engine::build_lists_now(bool)
 {
     std::vector<ListCtl> ctl(globals_.size());
-    download(ctl.data(), d_lctl_, globals_.size());
     for(int pass = 0; pass < 8; ++pass)
     {
-        for(size_t g = 0; g < globals_.size(); ++g)
-        {
-            if(!all && !ctl[g].rebuild) {continue;}
-            ctl[g].rebuild = 1;
-            ctl[g].maxcount = 0;
-            for(int d = 0; d < 3; ++d) {ctl[g].bbox_min[d] = ~0ull; ctl[g].bbox_max[d] = 0ull;}
-            ctl[g].margin = globals_[g].partition == MJG_PART_NAIVE ? INFINITY : globals_[g].max_cutoff * globals_[g].margin;
-        }
-        upload(d_lctl_, ctl.data(), globals_.size());
-        for(size_t g = 0; g < globals_.size(); ++g) {launch_rebuild(globals_[g], (int)g);}
+        launch_rebuilds(1);
         check(hipGetLastError(), "rebuild launch");
         download(ctl.data(), d_lctl_, globals_.size());
         bool again = false;
         for(size_t g = 0; g < globals_.size(); ++g)
         {
-            if(ctl[g].rebuild && ctl[g].maxcount > globals_[g].cap)
+            GlobalIA& G = globals_[g];
+            if(ctl[g].maxcount > G.cap)
             {
-                alloc_list(globals_[g], (uint32_t)(ctl[g].maxcount + ctl[g].maxcount / 4 + 16));
+                alloc_list(G, (uint32_t)(ctl[g].maxcount + ctl[g].maxcount / 4 + 16));
                 again = true;
-                if(graph_exec_) {(void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr;}
             }
-            else
-            {
-                ctl[g].rebuild = 0;
-            }
+            const uint32_t t = lanes_for(ctl[g].maxcount);
+            if(t != G.tpp) {G.tpp = t; again = again || false;}
         }
+        if(graph_exec_) {(void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr;}
         if(!again) {break;}
-        all = false;
     }
+    // the decision flags were consumed by this build
+    for(auto& c : ctl) {c.rebuild = 0;}
+    download(ctl.data(), d_lctl_, globals_.size());
     for(auto& c : ctl) {c.rebuild = 0;}
     upload(d_lctl_, ctl.data(), globals_.size());
 }
@@ -589,13 +688,15 @@ void Engine::initialize()
     initialized_ = true;
 }
 
-// ForceField::calc_force into f: locals (term kernel + per-particle gather), then every pair
-// interaction adds its sum; with `energy`, per-interaction energies land in d_energy_.
-// only >= 0 restricts the call to one interaction id.
-void Engine::forces(double4* f, bool energy, int only)
+// ForceField::calc_force into f: locals (term kernel), every pair interaction (fused kernel),
+// then k_finish sums them per particle (and with `kick` applies the closing half-kick of the
+// integrator step).  With `energy`, per-interaction energies land in d_energy_.  only >= 0
+// restricts the call to one interaction id.
+void Engine::forces(double4* f, bool energy, int only, bool kick)
 {
     const int nl = (int)locals_.size();
-    if(local_blocks_ > 0 && (only < 0 || only < nl))
+    const bool run_locals = local_blocks_ > 0 && (only < 0 || only < nl);
+    if(run_locals)
     {
         if(only >= 0)
         {
@@ -613,12 +714,14 @@ void Engine::forces(double4* f, bool energy, int only)
             hipLaunchKernelGGL(k_local_terms<false>, dim3(local_blocks_), dim3(256), 0, stream_, d_pos_, box_, segtab_,
                                d_lidx_, d_lpar_, d_tf_, d_te_, only);
         }
-        hipLaunchKernelGGL(k_gather_local, dim3(blocks_for(n_, 256)), dim3(256), 0, stream_, (uint32_t)n_, d_gstart_,
-                           d_gslot_, d_tf_, f);
     }
-    else
+    const PairSegTable pt = pair_table(only);
+    if(pt.n > 0)
     {
-        hipLaunchKernelGGL(k_zero_double4, dim3(blocks_for(n_, 256)), dim3(256), 0, stream_, f, (uint32_t)n_);
+        uint32_t blocks = 0;
+        for(int k = 0; k < pt.n; ++k) {blocks += blocks_for((size_t)pt.seg[k].np * pt.seg[k].tpp, 256);}
+        if(energy) {hipLaunchKernelGGL(k_pairs<true>, dim3(blocks), dim3(256), 0, stream_, d_pos_, box_, pt);}
+        else {hipLaunchKernelGGL(k_pairs<false>, dim3(blocks), dim3(256), 0, stream_, d_pos_, box_, pt);}
     }
     if(energy)
     {
@@ -629,32 +732,30 @@ void Engine::forces(double4* f, bool energy, int only)
             hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, stream_, d_te_ + L.term_off, (uint32_t)L.n_terms(),
                                d_energy_ + l);
         }
-    }
-    for(size_t gi = 0; gi < globals_.size(); ++gi)
-    {
-        const int id = nl + (int)gi;
-        if(only >= 0 && only != id) {continue;}
-        GlobalIA& g = globals_[gi];
-        const uint32_t np = (uint32_t)g.part.size();
-        const dim3 grid(blocks_for(np, 256)), blk(256);
-#define MJG_PAIR_LAUNCH(KIND, E)                                                                          \
-        hipLaunchKernelGGL((k_pair_full<KIND, E>), grid, blk, 0, stream_, d_pos_, f, g.d_part, g.d_order, np, \
-                           g.d_nl, g.d_nn, g.stride, g.d_ppar, g.pp, box_, g.d_epair)
-        if(g.pot == kPairLJ) {if(energy) {MJG_PAIR_LAUNCH(kPairLJ, true);} else {MJG_PAIR_LAUNCH(kPairLJ, false);}}
-        else if(g.pot == kPairEV) {if(energy) {MJG_PAIR_LAUNCH(kPairEV, true);} else {MJG_PAIR_LAUNCH(kPairEV, false);}}
-        else {if(energy) {MJG_PAIR_LAUNCH(kPairDH, true);} else {MJG_PAIR_LAUNCH(kPairDH, false);}}
-#undef MJG_PAIR_LAUNCH
-        if(energy)
+        for(size_t gi = 0; gi < globals_.size(); ++gi)
         {
-            hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, stream_, g.d_epair, np, d_energy_ + id);
+            const int id = nl + (int)gi;
+            if(only >= 0 && only != id) {continue;}
+            hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, stream_, globals_[gi].d_epair,
+                               (uint32_t)globals_[gi].part.size(), d_energy_ + id);
         }
     }
+    FinishArgs fa{};
+    fa.gstart = run_locals ? d_gstart_ : nullptr;
+    fa.gslot = d_gslot_;
+    fa.tf = d_tf_;
+    fa.ng = pt.n;
+    for(int k = 0; k < pt.n; ++k) {fa.fg[k] = pt.seg[k].fout;}
+    fa.halfdt = dt_ / 2;
+    fa.kick = kick ? 1 : 0;
+    hipLaunchKernelGGL(k_finish, dim3(blocks_for(n_, 256)), dim3(256), 0, stream_, (uint32_t)n_, fa, f, d_vel_,
+                       d_rmass_, &d_sctl_->step);
 }
 
 void Engine::calc_force()
 {
     if(!initialized_) {throw std::logic_error("not initialized");}
-    forces(d_force_, false, -1);
+    forces(d_force_, false, -1, false);
     check(hipGetLastError(), "calc_force launch");
     sync();
 }
@@ -662,7 +763,7 @@ void Engine::calc_force()
 double Engine::calc_force_and_energy()
 {
     if(!initialized_) {throw std::logic_error("not initialized");}
-    forces(d_force_, true, -1);
+    forces(d_force_, true, -1, false);
     check(hipGetLastError(), "calc_force_and_energy launch");
     std::vector<double> e(num_interactions());
     download(e.data(), d_energy_, e.size());
@@ -674,7 +775,7 @@ double Engine::calc_force_and_energy()
 void Engine::calc_energies(double* per, double* total)
 {
     if(!initialized_) {throw std::logic_error("not initialized");}
-    forces(d_force_tmp_, true, -1);
+    forces(d_force_tmp_, true, -1, false);
     check(hipGetLastError(), "calc_energies launch");
     std::vector<double> e(num_interactions());
     download(e.data(), d_energy_, e.size());
@@ -687,7 +788,7 @@ void Engine::calc_force_of(int id, double* out)
 {
     if(!initialized_) {throw std::logic_error("not initialized");}
     if(id < 0 || (size_t)id >= num_interactions()) {throw std::out_of_range("interaction id out of range");}
-    forces(d_force_tmp_, false, id);
+    forces(d_force_tmp_, false, id, false);
     check(hipGetLastError(), "calc_force_of launch");
     std::vector<double4> h(n_);
     download(h.data(), d_force_tmp_, n_);
@@ -703,13 +804,13 @@ void Engine::reduce_margin(double dmargin)
     bool any = false;
     for(size_t g = 0; g < globals_.size(); ++g)
     {
-        if(globals_[g].partition == MJG_PART_NAIVE) {ctl[g].rebuild = 0; continue;}
+        if(globals_[g].partition == MJG_PART_NAIVE) {continue;}
         ctl[g].margin -= dmargin;
-        ctl[g].rebuild = ctl[g].margin < 0;
-        any = any || ctl[g].rebuild;
+        any = any || ctl[g].margin < 0;
     }
     upload(d_lctl_, ctl.data(), globals_.size());
-    if(any) {build_lists_now(false);}
+    // a rebuild resets the margin of every list (rebuilding a list early is always valid)
+    if(any) {build_lists_now(true);}
 }
 
 size_t Engine::neighbor_pairs(int id, uint32_t* pairs, size_t cap)
@@ -719,7 +820,7 @@ size_t Engine::neighbor_pairs(int id, uint32_t* pairs, size_t cap)
     if(gi < 0 || (size_t)gi >= globals_.size()) {throw std::out_of_range("not a global interaction id");}
     GlobalIA& g = globals_[gi];
     const uint32_t np = (uint32_t)g.part.size();
-    std::vector<uint32_t> nn(np), order(np), nl((size_t)g.stride * g.cap);
+    std::vector<uint32_t> nn(np), order(np), nl((size_t)np * g.cap);
     download(nn.data(), g.d_nn, np);
     download(order.data(), g.d_order, np);
     download(nl.data(), g.d_nl, nl.size());
@@ -729,7 +830,7 @@ size_t Engine::neighbor_pairs(int id, uint32_t* pairs, size_t cap)
         const uint32_t i = g.part[order[t]];
         for(uint32_t k = 0; k < nn[t]; ++k)
         {
-            const uint32_t j = nl[(size_t)k * g.stride + t];
+            const uint32_t j = nl[(size_t)t * g.cap + k];
             if(i < j) {out.emplace_back(i, j);}
         }
     }
@@ -780,7 +881,7 @@ void Engine::integrator_initialize()
     upload(d_c1_, c1.data(), n_);
     upload(d_c2_, c2.data(), n_);
     upload(d_c3_, c3.data(), n_);
-    forces(d_force_, false, -1);
+    forces(d_force_, false, -1, false);
     check(hipGetLastError(), "integrator_initialize launch");
     sync();
     integ_ready_ = true;
@@ -799,30 +900,8 @@ void Engine::launch_step()
         hipLaunchKernelGGL(k_gjf_a, dim3(nb), dim3(256), 0, stream_, (uint32_t)n_, d_pos_, d_vel_, d_force_,
                            d_rmass_, d_c1_, d_c2_, d_c3_, dt_, box_, seed_, d_sctl_);
     }
-    if(!globals_.empty())
-    {
-        hipLaunchKernelGGL(k_margin, dim3(1), dim3(64), 0, stream_, d_sctl_, d_lctl_, (int)globals_.size(),
-                           d_abs_margin_, d_never_);
-        for(size_t g = 0; g < globals_.size(); ++g)
-        {
-            if(globals_[g].partition != MJG_PART_NAIVE) {launch_rebuild(globals_[g], (int)g);}
-        }
-    }
-    else
-    {
-        hipLaunchKernelGGL(k_reset_maxdisp, dim3(1), dim3(1), 0, stream_, d_sctl_);
-    }
-    forces(d_force_, false, -1);
-    if(integ_ == MJG_INTEG_BAOAB)
-    {
-        hipLaunchKernelGGL(k_kick_b, dim3(nb), dim3(256), 0, stream_, (uint32_t)n_, d_vel_, d_force_, d_rmass_,
-                           dt_ / 2, d_sctl_);
-    }
-    else
-    {
-        hipLaunchKernelGGL(k_gjf_b, dim3(nb), dim3(256), 0, stream_, (uint32_t)n_, d_vel_, d_force_, d_rmass_,
-                           dt_, d_sctl_);
-    }
+    if(!globals_.empty()) {launch_rebuilds(0);}
+    forces(d_force_, false, -1, true);
 }
 
 void Engine::run_segment(size_t n)
@@ -889,7 +968,7 @@ void Engine::run_segment(size_t n)
         check(hipMemcpyAsync(d_lctl_, d_bak_lctl_, ng * sizeof(ListCtl), hipMemcpyDeviceToDevice, stream_), "restore");
         sync();
         build_lists_now(true);
-        forces(d_force_, false, -1);
+        forces(d_force_, false, -1, false);
         sync();
     }
     throw DeviceError("neighbour list capacity did not converge");
@@ -936,6 +1015,9 @@ double Engine::profile_interaction(int id, int reps, uint64_t* half_pairs)
     const int nl = (int)locals_.size();
     uint64_t hp = 0;
     check(hipEventRecord(a, stream_), "event record");
+    const PairSegTable pt = pair_table(id);
+    uint32_t pblocks = 0;
+    for(int k = 0; k < pt.n; ++k) {pblocks += blocks_for((size_t)pt.seg[k].np * pt.seg[k].tpp, 256);}
     for(int r = 0; r < reps; ++r)
     {
         if(id < nl)
@@ -945,24 +1027,7 @@ double Engine::profile_interaction(int id, int reps, uint64_t* half_pairs)
         }
         else
         {
-            GlobalIA& g = globals_[id - nl];
-            const uint32_t np = (uint32_t)g.part.size();
-            const dim3 grid(blocks_for(np, 256)), blk(256);
-            if(g.pot == kPairLJ)
-            {
-                hipLaunchKernelGGL((k_pair_full<kPairLJ, false>), grid, blk, 0, stream_, d_pos_, d_force_tmp_, g.d_part,
-                                   g.d_order, np, g.d_nl, g.d_nn, g.stride, g.d_ppar, g.pp, box_, g.d_epair);
-            }
-            else if(g.pot == kPairEV)
-            {
-                hipLaunchKernelGGL((k_pair_full<kPairEV, false>), grid, blk, 0, stream_, d_pos_, d_force_tmp_, g.d_part,
-                                   g.d_order, np, g.d_nl, g.d_nn, g.stride, g.d_ppar, g.pp, box_, g.d_epair);
-            }
-            else
-            {
-                hipLaunchKernelGGL((k_pair_full<kPairDH, false>), grid, blk, 0, stream_, d_pos_, d_force_tmp_, g.d_part,
-                                   g.d_order, np, g.d_nl, g.d_nn, g.stride, g.d_ppar, g.pp, box_, g.d_epair);
-            }
+            hipLaunchKernelGGL(k_pairs<false>, dim3(pblocks), dim3(256), 0, stream_, d_pos_, box_, pt);
         }
     }
     check(hipEventRecord(b, stream_), "event record");

@@ -59,8 +59,11 @@ struct GlobalIA
     SortBuffers sb;
     uint32_t *d_cell_start = nullptr, *d_cell_end = nullptr;
     uint32_t *d_nl = nullptr, *d_nn = nullptr, *d_order = nullptr;
-    uint32_t stride = 0, cap = 0;
+    uint32_t cap = 0;             // row length of the list: nl[t * cap + k]
+    uint32_t tpp = 8;             // lanes per participant in k_verlet / k_pairs
+    bool small = false;           // one-workgroup rebuild (np <= kSmallMaxPart)
     double* d_epair = nullptr;
+    double4* d_fg = nullptr;      // this interaction's force on every particle
 };
 
 class Engine
@@ -102,9 +105,11 @@ class Engine
     void make_exclusions(GlobalIA& g);
     void setup_global(GlobalIA& g);
     void alloc_list(GlobalIA& g, uint32_t cap);
-    void launch_rebuild(GlobalIA& g, int gi);
+    void launch_rebuilds(int force);
     void build_lists_now(bool all);
-    void forces(double4* f, bool energy, int only);
+    ListSegTable list_table();
+    PairSegTable pair_table(int only);
+    void forces(double4* f, bool energy, int only, bool kick);
     void launch_step();
     void run_segment(size_t n);
     void sync();

@@ -1,12 +1,12 @@
 // Force kernels of the FP64 path.
 //
 // Local terms: one thread per term writes each body's force into its own slot of a term-force
-// buffer (no atomics); k_gather_local then sums, per particle, the slots that touch it in
+// buffer (no atomics); k_finish then sums, per particle, the slots that touch it in
 // interaction order and term order, i.e. in the order the reference's LocalForceField loop
 // accumulates them (LocalForceField.hpp:111-118).
 //
-// Pair terms: one thread per participant walks its full (two-sided) ELL list and accumulates
-// its own force in registers, so no atomics are needed and the sum order is fixed by the list.
+// Pair terms: a group of lanes per participant walks its full (two-sided) list and reduces in
+// registers, so no atomics are needed and the sum order is fixed by the list.
 // The per-pair arithmetic follows the reference specialisations (cited below); the pair
 // parameter is recomputed from the per-particle table with the lower index first, which is
 // what prepare_params cached for the pair (i < j) in the reference.
@@ -208,22 +208,6 @@ k_local_terms(const double4* __restrict__ pos, Box box, LocalSegTable tab,
     local_term<ENERGY>(s, t, pos, box, idx, par, tf, te);
 }
 
-// force[i] = sum of the local-term slots of particle i (gstart/gslot: CSR per particle).
-__global__ void k_gather_local(uint32_t n, const uint32_t* __restrict__ gstart, const uint32_t* __restrict__ gslot,
-                               const double* __restrict__ tf, double4* __restrict__ force)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if(i >= n) {return;}
-    double x = 0, y = 0, z = 0;
-    const uint32_t b = gstart[i], e = gstart[i + 1];
-    for(uint32_t k = b; k < e; ++k)
-    {
-        const uint32_t s = gslot[k];
-        x += tf[3 * (size_t)s + 0]; y += tf[3 * (size_t)s + 1]; z += tf[3 * (size_t)s + 2];
-    }
-    force[i] = make_double4(x, y, z, 0.0);
-}
-
 struct PairParams
 {
     int kind;
@@ -289,31 +273,127 @@ __device__ __forceinline__ void pair_term(const PairParams& pp, uint32_t i, uint
     }
 }
 
-template<int KIND, bool ENERGY>
-__global__ void __launch_bounds__(256)
-k_pair_full(const double4* __restrict__ pos, double4* __restrict__ force, const uint32_t* __restrict__ part,
-            const uint32_t* __restrict__ order, uint32_t np, const uint32_t* __restrict__ nl,
-            const uint32_t* __restrict__ nn, uint32_t stride, const double* __restrict__ ppar,
-            PairParams pp, Box box, double* __restrict__ e_out)
+// ------------------------------------------------------------------------------------
+// Pair forces, fused over every global interaction: blocks are split into one segment per
+// interaction; inside a segment `tpp` lanes share one participant (sorted slot t), split its
+// row nl[t * cap + k] with stride tpp, and combine their partial forces with a fixed
+// butterfly, so the sum order is fixed.  The interaction's force on i lands in its own array
+// fout[i] (zero for non-participants), which k_finish adds up.
+// ------------------------------------------------------------------------------------
+struct PairSeg
 {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if(t >= np) {return;}
-    const uint32_t i = part[order[t]];
+    int kind;
+    uint32_t tpp, np, block_begin, cap;
+    const uint32_t* part;
+    const uint32_t* order;
+    const uint32_t* nl;
+    const uint32_t* nn;
+    const double* ppar;
+    double4* fout;
+    double* eout;
+    PairParams pp;
+};
+constexpr int kMaxPairSegs = 8;
+struct PairSegTable
+{
+    PairSeg seg[kMaxPairSegs];
+    int n;
+};
+
+template<int KIND, bool ENERGY>
+__device__ __forceinline__ void pair_row(const PairSeg& s, uint32_t t, uint32_t l, const double4* __restrict__ pos,
+                                         const Box& box, double& fx, double& fy, double& fz, double& e, uint32_t& i)
+{
+    i = s.part[s.order[t]];
     const double4 ri = pos[i];
-    double fx = 0, fy = 0, fz = 0, e = 0;
-    const uint32_t cnt = nn[t];
-    for(uint32_t k = 0; k < cnt; ++k)
+    const uint32_t cnt = s.nn[t];
+    const uint32_t* row = s.nl + (size_t)t * s.cap;
+    for(uint32_t k = l; k < cnt; k += s.tpp)
     {
-        const uint32_t j = nl[(size_t)k * stride + t];
+        const uint32_t j = row[k];
         const double4 rj = pos[j];
         double dx, dy, dz;
         adjust_direction(box, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, dx, dy, dz);
-        pair_term<KIND, ENERGY>(pp, i, j, ppar, dx, dy, dz, fx, fy, fz, e);
+        pair_term<KIND, ENERGY>(s.pp, i, j, s.ppar, dx, dy, dz, fx, fy, fz, e);
     }
-    double4 f = force[i];
-    f.x += fx; f.y += fy; f.z += fz;
-    force[i] = f;
-    if(ENERGY) {e_out[t] = e;}
+}
+
+template<bool ENERGY>
+__global__ void __launch_bounds__(256)
+k_pairs(const double4* __restrict__ pos, Box box, PairSegTable tab)
+{
+    int si = 0;
+    while(si + 1 < tab.n && blockIdx.x >= tab.seg[si + 1].block_begin) {++si;}
+    const PairSeg& s = tab.seg[si];
+    const uint32_t gt = (blockIdx.x - s.block_begin) * blockDim.x + threadIdx.x;
+    const uint32_t t = gt / s.tpp, l = gt % s.tpp;
+    if(t >= s.np) {return;}   // whole groups leave together
+    double fx = 0, fy = 0, fz = 0, e = 0;
+    uint32_t i = 0;
+    if(s.kind == kPairLJ) {pair_row<kPairLJ, ENERGY>(s, t, l, pos, box, fx, fy, fz, e, i);}
+    else if(s.kind == kPairEV) {pair_row<kPairEV, ENERGY>(s, t, l, pos, box, fx, fy, fz, e, i);}
+    else {pair_row<kPairDH, ENERGY>(s, t, l, pos, box, fx, fy, fz, e, i);}
+    for(uint32_t o = s.tpp / 2; o > 0; o >>= 1)
+    {
+        fx += __shfl_xor(fx, o, kWave);
+        fy += __shfl_xor(fy, o, kWave);
+        fz += __shfl_xor(fz, o, kWave);
+        if(ENERGY) {e += __shfl_xor(e, o, kWave);}
+    }
+    if(l == 0)
+    {
+        s.fout[i] = make_double4(fx, fy, fz, 0.0);
+        if(ENERGY) {s.eout[t] = e;}
+    }
+}
+
+// Per particle: the local-term slots (in interaction, term order) plus every pair
+// interaction's force; stores the force and, with kick != 0, applies the velocity half-kick
+// v += (dt/2) (1/m) f of BAOAB (BAOABLangevinIntegrator.hpp:222-229) and G-JF
+// (GJFNVTLangevinIntegrator.hpp:222-229, (dt 0.5) rm == (dt/2) rm) and advances the step.
+struct FinishArgs
+{
+    const uint32_t* gstart;
+    const uint32_t* gslot;
+    const double* tf;
+    const double4* fg[kMaxPairSegs];
+    int ng;
+    double halfdt;
+    int kick;
+};
+
+__global__ void __launch_bounds__(256)
+k_finish(uint32_t n, FinishArgs a, double4* __restrict__ force, double4* __restrict__ vel,
+         const double* __restrict__ rmass, unsigned long long* __restrict__ step)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i < n)
+    {
+        double x = 0, y = 0, z = 0;
+        if(a.gstart != nullptr)
+        {
+            const uint32_t b = a.gstart[i], e = a.gstart[i + 1];
+            for(uint32_t k = b; k < e; ++k)
+            {
+                const uint32_t sl = a.gslot[k];
+                x += a.tf[3 * (size_t)sl + 0]; y += a.tf[3 * (size_t)sl + 1]; z += a.tf[3 * (size_t)sl + 2];
+            }
+        }
+        for(int g = 0; g < a.ng; ++g)
+        {
+            const double4 f = a.fg[g][i];
+            x += f.x; y += f.y; z += f.z;
+        }
+        force[i] = make_double4(x, y, z, 0.0);
+        if(a.kick)
+        {
+            const double hrm = a.halfdt * rmass[i];
+            double4 v = vel[i];
+            v.x += hrm * x; v.y += hrm * y; v.z += hrm * z;
+            vel[i] = v;
+        }
+    }
+    if(a.kick && i == 0) {*step += 1;}
 }
 
 __global__ void k_zero_doubles(double* __restrict__ p, uint32_t n)

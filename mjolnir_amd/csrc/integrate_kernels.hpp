@@ -9,7 +9,6 @@
 // the engine's Philox stream indexed by (seed, step, particle).
 #pragma once
 #include "device_common.hpp"
-#include "nlist_kernels.hpp"
 
 namespace mjg
 {
@@ -18,15 +17,18 @@ constexpr int kMaxGlobals = 16;
 struct StepCtl
 {
     unsigned long long step;          // integrator step counter
-    unsigned long long maxdisp2;      // bits of max |dp|^2 this step (non-negative double)
+    unsigned long long maxdisp2[2];   // bits of max |dp|^2 (non-negative double), slot step & 1
     unsigned long long rebuilds;      // total list rebuilds
     int nonfinite;                    // set when a position becomes non-finite
 };
 
-__device__ __forceinline__ void block_max_to(double v, unsigned long long* dst)
+// max |dp|^2 of this step into slot step & 1; the other slot (read by the previous step's
+// skin check) is cleared for the next step
+__device__ __forceinline__ void block_max_to(double v, StepCtl* ctl, uint64_t step)
 {
     v = wave_max(v);
-    if(threadIdx.x % kWave == 0) {atomicMax(dst, (unsigned long long)__double_as_longlong(v));}
+    if(threadIdx.x % kWave == 0) {atomicMax(&ctl->maxdisp2[step & 1], (unsigned long long)__double_as_longlong(v));}
+    if(blockIdx.x == 0 && threadIdx.x == 0) {ctl->maxdisp2[(step + 1) & 1] = 0ull;}
 }
 
 __global__ void __launch_bounds__(256)
@@ -36,9 +38,9 @@ k_baoab_a(uint32_t n, double4* __restrict__ pos, double4* __restrict__ vel, cons
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     double d2 = 0.0;
+    const uint64_t step = ctl->step;
     if(i < n)
     {
-        const uint64_t step = ctl->step;
         double Rx, Ry, Rz;
         noise3(seed, step, i, Rx, Ry, Rz);
         const double rm = rmass[i];
@@ -58,7 +60,7 @@ k_baoab_a(uint32_t n, double4* __restrict__ pos, double4* __restrict__ vel, cons
         d2 = length_sq(dpx, dpy, dpz);
         if(!isfinite(x) || !isfinite(y) || !isfinite(z)) {ctl->nonfinite = 1;}
     }
-    block_max_to(d2, &ctl->maxdisp2);
+    block_max_to(d2, ctl, step);
 }
 
 __global__ void __launch_bounds__(256)
@@ -68,9 +70,9 @@ k_gjf_a(uint32_t n, double4* __restrict__ pos, double4* __restrict__ vel, const 
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     double d2 = 0.0;
+    const uint64_t step = ctl->step;
     if(i < n)
     {
-        const uint64_t step = ctl->step;
         double Rx, Ry, Rz;
         noise3(seed, step, i, Rx, Ry, Rz);
         const double rm = rmass[i], r2m = 0.5 * rm;
@@ -93,75 +95,11 @@ k_gjf_a(uint32_t n, double4* __restrict__ pos, double4* __restrict__ vel, const 
         d2 = length_sq(dpx, dpy, dpz);
         if(!isfinite(x) || !isfinite(y) || !isfinite(z)) {ctl->nonfinite = 1;}
     }
-    block_max_to(d2, &ctl->maxdisp2);
+    block_max_to(d2, ctl, step);
 }
 
-// reduce_margin(2 sqrt(max |dp|^2)) on every partition; one thread per global interaction.
-__global__ void k_margin(StepCtl* __restrict__ sctl, ListCtl* __restrict__ lctl, int n_global,
-                         const double* __restrict__ abs_margin, const int* __restrict__ never_rebuild)
-{
-    const int g = threadIdx.x;
-    const double dmargin = 2 * sqrt(__longlong_as_double((long long)sctl->maxdisp2));
-    __syncthreads();
-    if(g < n_global)
-    {
-        ListCtl& c = lctl[g];
-        if(never_rebuild[g]) {c.rebuild = 0;}
-        else
-        {
-            const double m = c.margin - dmargin;
-            if(m < 0)
-            {
-                c.rebuild = 1;
-                c.margin = abs_margin[g];
-                c.maxcount = 0;
-                for(int d = 0; d < 3; ++d) {c.bbox_min[d] = ~0ull; c.bbox_max[d] = 0ull;}
-                atomicAdd(&sctl->rebuilds, 1ull);
-            }
-            else
-            {
-                c.rebuild = 0;
-                c.margin = m;
-            }
-        }
-    }
-    __syncthreads();
-    if(g == 0) {sctl->maxdisp2 = 0ull;}
-}
-
-__global__ void k_reset_maxdisp(StepCtl* __restrict__ ctl) {ctl->maxdisp2 = 0ull;}
-
-__global__ void __launch_bounds__(256)
-k_kick_b(uint32_t n, double4* __restrict__ vel, const double4* __restrict__ force,
-         const double* __restrict__ rmass, double halfdt, StepCtl* __restrict__ ctl)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if(i < n)
-    {
-        const double hrm = halfdt * rmass[i];
-        double4 v = vel[i];
-        const double4 f = force[i];
-        v.x += hrm * f.x; v.y += hrm * f.y; v.z += hrm * f.z;
-        vel[i] = v;
-    }
-    if(i == 0) {ctl->step += 1;}
-}
-
-__global__ void __launch_bounds__(256)
-k_gjf_b(uint32_t n, double4* __restrict__ vel, const double4* __restrict__ force,
-        const double* __restrict__ rmass, double dt, StepCtl* __restrict__ ctl)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if(i < n)
-    {
-        const double c = dt * 0.5 * rmass[i];
-        double4 v = vel[i];
-        const double4 f = force[i];
-        v.x += c * f.x; v.y += c * f.y; v.z += c * f.z;
-        vel[i] = v;
-    }
-    if(i == 0) {ctl->step += 1;}
-}
+// The velocity half-kick after the force call is part of k_finish (force_kernels.hpp), and
+// the skin check is part of k_prep_small / k_margin_large (nlist_kernels.hpp).
 
 __global__ void __launch_bounds__(256)
 k_kinetic(uint32_t n, const double4* __restrict__ vel, const double* __restrict__ mass, double* __restrict__ out)

@@ -11,11 +11,12 @@
 // (cells of width r_c(1+m)(1+1e-6), widened when there would be more than kMaxCells) instead
 // of the reference's 8x8x8 hash, which gives the same set with fewer candidates.
 //
-// Storage is a full list (both directions) in ELL layout, column-major by sorted slot:
-// nl[k * stride + t] is the k-th partner of the participant in sorted slot t, so lane t of a
-// wave reads consecutive words at every k.
+// Storage is a full list (both directions), one fixed-capacity row per sorted slot:
+// nl[t * cap + k] is the k-th partner of the participant in sorted slot t; the lanes that
+// share a participant read consecutive words of its row.
 #pragma once
 #include "device_common.hpp"
+#include "integrate_kernels.hpp"
 
 namespace mjg
 {
@@ -193,102 +194,338 @@ __device__ __forceinline__ bool pair_allowed(uint32_t lo, uint32_t hi, const Lis
     return true;
 }
 
-// One thread per sorted slot t: scan the 27 surrounding cells (z outer, x inner, as the
-// reference orders them) and append every allowed partner within r_list.
-__global__ void __launch_bounds__(256)
-k_verlet_cells(const double4* __restrict__ pos, const uint32_t* __restrict__ part, uint32_t np,
-               const uint32_t* __restrict__ skeys, const uint32_t* __restrict__ svals,
-               const uint32_t* __restrict__ cell_start, const uint32_t* __restrict__ cell_end,
-               const Grid* __restrict__ grid, Box box, ListRule rule,
-               const uint32_t* __restrict__ pslot, const uint32_t* __restrict__ mol,
-               const uint32_t* __restrict__ ex_start, const uint32_t* __restrict__ ex,
-               uint32_t* __restrict__ nl, uint32_t* __restrict__ nn, uint32_t* __restrict__ order,
-               uint32_t stride, uint32_t cap, ListCtl* __restrict__ ctl)
+// ------------------------------------------------------------------------------------
+// Verlet build, fused over every global interaction that rebuilds this step.
+//
+// Each segment (one global interaction) gives `tpp` lanes to every sorted slot t; the lanes
+// walk the candidates of the 27 surrounding cells (z outer, x inner, as the reference orders
+// them) with stride tpp and append accepted partners with a ballot: a candidate's position
+// is the number of accepted candidates before it, so the list order is the serial order and
+// does not depend on timing.  Output is row-major: nl[t * cap + k].
+// ------------------------------------------------------------------------------------
+struct ListSeg
 {
-    if(ctl->rebuild == 0) {return;}
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if(t >= np) {return;}
-    const Grid g = *grid;
-    const uint32_t p = svals[t];
-    const uint32_t i = part[p];
-    order[t] = p;
-    const uint32_t c = skeys[t];
-    const int cx = (int)(c % (uint32_t)g.dim[0]);
-    const int cy = (int)((c / (uint32_t)g.dim[0]) % (uint32_t)g.dim[1]);
-    const int cz = (int)(c / ((uint32_t)g.dim[0] * (uint32_t)g.dim[1]));
+    uint32_t np, tpp, block_begin, cap;
+    int allpairs;                      // Verlet / Naive partitions: scan every participant
+    const uint32_t* part;
+    const uint32_t* skeys;
+    const uint32_t* svals;
+    const uint32_t* cell_start;
+    const uint32_t* cell_end;
+    const Grid* grid;
+    const uint32_t* pslot;
+    const uint32_t* mol;
+    const uint32_t* ex_start;
+    const uint32_t* ex;
+    uint32_t* nl;
+    uint32_t* nn;
+    uint32_t* order;
+    ListCtl* ctl;
+    ListRule rule;
+};
+constexpr int kMaxListSegs = 8;
+struct ListSegTable
+{
+    ListSeg seg[kMaxListSegs];
+    int n;
+};
+
+__device__ __forceinline__ uint64_t group_mask_of(uint32_t tpp, uint32_t lane)
+{
+    const uint64_t m = (tpp >= 64) ? ~0ull : ((1ull << tpp) - 1ull);
+    return m << (lane & ~(tpp - 1u));
+}
+
+__global__ void __launch_bounds__(256)
+k_verlet(const double4* __restrict__ pos, Box box, ListSegTable tab)
+{
+    int si = 0;
+    while(si + 1 < tab.n && blockIdx.x >= tab.seg[si + 1].block_begin) {++si;}
+    const ListSeg& s = tab.seg[si];
+    if(s.ctl->rebuild == 0) {return;}
+    const uint32_t gt = (blockIdx.x - s.block_begin) * blockDim.x + threadIdx.x;
+    const uint32_t t = gt / s.tpp;
+    const uint32_t lane = threadIdx.x % kWave;
+    const uint32_t l = gt % s.tpp;
+    const bool active = t < s.np;
+    // all lanes of a group share t, so whole groups are active or inactive together
+    if(!active) {return;}
+    const uint64_t gmask = group_mask_of(s.tpp, lane);
+    const uint64_t below = gmask & ((1ull << lane) - 1ull);
+    uint32_t p, i;
+    if(s.allpairs) {p = t;} else {p = s.svals[t];}
+    i = s.part[p];
+    if(l == 0) {s.order[t] = p;}
     const double4 ri = pos[i];
     uint32_t count = 0;
-    for(int dz = -1; dz <= 1; ++dz)
-    for(int dy = -1; dy <= 1; ++dy)
-    for(int dx = -1; dx <= 1; ++dx)
-    {
-        int x = cx + dx, y = cy + dy, z = cz + dz;
-        if(g.periodic)
+    uint32_t* row = s.nl + (size_t)t * s.cap;
+
+    auto consider = [&](uint32_t j, bool valid) {
+        bool accept = false;
+        if(valid && j != i)
         {
-            x = (x < 0) ? g.dim[0] - 1 : (x >= g.dim[0] ? 0 : x);
-            y = (y < 0) ? g.dim[1] - 1 : (y >= g.dim[1] ? 0 : y);
-            z = (z < 0) ? g.dim[2] - 1 : (z >= g.dim[2] ? 0 : z);
-        }
-        else if(x < 0 || y < 0 || z < 0 || x >= g.dim[0] || y >= g.dim[1] || z >= g.dim[2])
-        {
-            continue;
-        }
-        const uint32_t c2 = (uint32_t)x + (uint32_t)g.dim[0] * ((uint32_t)y + (uint32_t)g.dim[1] * (uint32_t)z);
-        const uint32_t sb = cell_start[c2], se = cell_end[c2];
-        for(uint32_t s = sb; s < se; ++s)
-        {
-            const uint32_t j = part[svals[s]];
-            if(j == i) {continue;}
             const uint32_t lo = (i < j) ? i : j, hi = (i < j) ? j : i;
-            if(!pair_allowed(lo, hi, rule, pslot, mol, ex_start, ex)) {continue;}
-            const double4 a = pos[lo], b = pos[hi];
-            double ddx, ddy, ddz;
-            adjust_direction(box, a.x, a.y, a.z, b.x, b.y, b.z, ddx, ddy, ddz);
-            if(length_sq(ddx, ddy, ddz) < rule.r_list2)
+            if(pair_allowed(lo, hi, s.rule, s.pslot, s.mol, s.ex_start, s.ex))
             {
-                if(count < cap) {nl[(size_t)count * stride + t] = j;}
-                ++count;
+                if(s.rule.use_distance)
+                {
+                    const double4 a = pos[lo], b = pos[hi];
+                    double ddx, ddy, ddz;
+                    adjust_direction(box, a.x, a.y, a.z, b.x, b.y, b.z, ddx, ddy, ddz);
+                    accept = length_sq(ddx, ddy, ddz) < s.rule.r_list2;
+                }
+                else
+                {
+                    accept = true;
+                }
+            }
+        }
+        const uint64_t b = __ballot(accept) & gmask;
+        if(accept)
+        {
+            const uint32_t k = count + (uint32_t)__popcll(b & below);
+            if(k < s.cap) {row[k] = j;}
+        }
+        count += (uint32_t)__popcll(b);
+    };
+
+    if(s.allpairs)
+    {
+        for(uint32_t base = 0; base < s.np; base += s.tpp)
+        {
+            const uint32_t q = base + l;
+            consider((q < s.np) ? s.part[q] : 0u, q < s.np);
+        }
+    }
+    else
+    {
+        const Grid g = *s.grid;
+        const uint32_t c = s.skeys[t];
+        const int cx = (int)(c % (uint32_t)g.dim[0]);
+        const int cy = (int)((c / (uint32_t)g.dim[0]) % (uint32_t)g.dim[1]);
+        const int cz = (int)(c / ((uint32_t)g.dim[0] * (uint32_t)g.dim[1]));
+        for(int dz = -1; dz <= 1; ++dz)
+        for(int dy = -1; dy <= 1; ++dy)
+        for(int dx = -1; dx <= 1; ++dx)
+        {
+            int x = cx + dx, y = cy + dy, z = cz + dz;
+            if(g.periodic)
+            {
+                x = (x < 0) ? g.dim[0] - 1 : (x >= g.dim[0] ? 0 : x);
+                y = (y < 0) ? g.dim[1] - 1 : (y >= g.dim[1] ? 0 : y);
+                z = (z < 0) ? g.dim[2] - 1 : (z >= g.dim[2] ? 0 : z);
+            }
+            else if(x < 0 || y < 0 || z < 0 || x >= g.dim[0] || y >= g.dim[1] || z >= g.dim[2])
+            {
+                continue;
+            }
+            const uint32_t c2 = (uint32_t)x + (uint32_t)g.dim[0] * ((uint32_t)y + (uint32_t)g.dim[1] * (uint32_t)z);
+            const uint32_t sb = s.cell_start[c2], se = s.cell_end[c2];
+            for(uint32_t base = sb; base < se; base += s.tpp)
+            {
+                const uint32_t q = base + l;
+                consider((q < se) ? s.part[s.svals[q]] : 0u, q < se);
             }
         }
     }
     (void)ri;
-    nn[t] = (count < cap) ? count : cap;
-    atomicMax(&ctl->maxcount, count);
+    if(l == 0)
+    {
+        s.nn[t] = (count < s.cap) ? count : s.cap;
+        atomicMax(&s.ctl->maxcount, count);
+    }
 }
 
-// All-pairs build (small participant sets, Naive partition): thread per participant slot t
-// scans every other participant in input order.
-__global__ void __launch_bounds__(256)
-k_verlet_allpairs(const double4* __restrict__ pos, const uint32_t* __restrict__ part, uint32_t np,
-                  Box box, ListRule rule, const uint32_t* __restrict__ pslot, const uint32_t* __restrict__ mol,
-                  const uint32_t* __restrict__ ex_start, const uint32_t* __restrict__ ex,
-                  uint32_t* __restrict__ nl, uint32_t* __restrict__ nn, uint32_t* __restrict__ order,
-                  uint32_t stride, uint32_t cap, ListCtl* __restrict__ ctl)
+// ------------------------------------------------------------------------------------
+// One-workgroup rebuild preparation for interactions with at most kSmallMaxPart
+// participants: the skin check of this interaction (reduce_margin), then the grid, the cell
+// keys, a counting sort in LDS (slots taken with LDS atomics, then each cell's run sorted by
+// participant slot, which gives the stable order) and the cell ranges, written to global
+// memory for k_verlet.  One block per interaction.
+// ------------------------------------------------------------------------------------
+constexpr uint32_t kSmallMaxPart = 4096;
+constexpr uint32_t kSmallMaxCells = 2048;
+
+struct PrepSeg
 {
-    if(ctl->rebuild == 0) {return;}
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if(t >= np) {return;}
-    const uint32_t i = part[t];
-    order[t] = t;
-    uint32_t count = 0;
-    for(uint32_t q = 0; q < np; ++q)
+    uint32_t np;
+    int periodic_grid;       // 1: fixed host grid (periodic box); 0: bounding-box grid
+    int never_rebuild;       // Naive partition
+    int allpairs;            // Verlet / Naive: no cells to prepare
+    double abs_margin;       // cutoff * margin
+    double r_list;
+    const uint32_t* part;
+    uint32_t* skeys;
+    uint32_t* svals;
+    uint32_t* cell_start;
+    uint32_t* cell_end;
+    Grid* grid;
+    ListCtl* ctl;
+};
+struct PrepSegTable
+{
+    PrepSeg seg[kMaxListSegs];
+    int n;
+};
+
+// margin update of one partition for this step; returns the rebuild decision
+__device__ __forceinline__ int margin_decision(ListCtl& c, double dmargin, double abs_margin, int never, int force)
+{
+    if(never && !force) {c.rebuild = 0; return 0;}
+    if(force)
     {
-        const uint32_t j = part[q];
-        if(j == i) {continue;}
-        const uint32_t lo = (i < j) ? i : j, hi = (i < j) ? j : i;
-        if(!pair_allowed(lo, hi, rule, pslot, mol, ex_start, ex)) {continue;}
-        if(rule.use_distance)
-        {
-            const double4 a = pos[lo], b = pos[hi];
-            double ddx, ddy, ddz;
-            adjust_direction(box, a.x, a.y, a.z, b.x, b.y, b.z, ddx, ddy, ddz);
-            if(!(length_sq(ddx, ddy, ddz) < rule.r_list2)) {continue;}
-        }
-        if(count < cap) {nl[(size_t)count * stride + t] = j;}
-        ++count;
+        c.rebuild = 1; c.margin = never ? INFINITY : abs_margin; c.maxcount = 0;
+        return 1;
     }
-    nn[t] = (count < cap) ? count : cap;
-    atomicMax(&ctl->maxcount, count);
+    const double m = c.margin - dmargin;
+    if(m < 0)
+    {
+        c.rebuild = 1; c.margin = abs_margin; c.maxcount = 0;
+        for(int d = 0; d < 3; ++d) {c.bbox_min[d] = ~0ull; c.bbox_max[d] = 0ull;}
+        return 1;
+    }
+    c.rebuild = 0; c.margin = m;
+    return 0;
+}
+
+__global__ void __launch_bounds__(1024)
+k_prep_small(const double4* __restrict__ pos, PrepSegTable tab, StepCtl* __restrict__ sctl, int force)
+{
+    __shared__ uint32_t key[kSmallMaxPart];
+    __shared__ uint32_t sval[kSmallMaxPart];
+    __shared__ uint32_t start[kSmallMaxCells + 1];
+    __shared__ uint32_t cursor[kSmallMaxCells];
+    __shared__ double red[2][3][16];
+    __shared__ int go;
+    __shared__ Grid gl;
+    const PrepSeg& s = tab.seg[blockIdx.x];
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    if(tid == 0)
+    {
+        const double dmargin = 2 * sqrt(__longlong_as_double((long long)sctl->maxdisp2[sctl->step & 1]));
+        go = margin_decision(*s.ctl, dmargin, s.abs_margin, s.never_rebuild, force);
+        if(go) {atomicAdd(&sctl->rebuilds, 1ull);}
+    }
+    __syncthreads();
+    if(!go || s.allpairs) {return;}
+
+    if(s.periodic_grid)
+    {
+        if(tid == 0) {gl = *s.grid;}
+    }
+    else
+    {
+        double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for(uint32_t p = tid; p < s.np; p += nt)
+        {
+            const double4 r = pos[s.part[p]];
+            mn[0] = fmin(mn[0], r.x); mn[1] = fmin(mn[1], r.y); mn[2] = fmin(mn[2], r.z);
+            mx[0] = fmax(mx[0], r.x); mx[1] = fmax(mx[1], r.y); mx[2] = fmax(mx[2], r.z);
+        }
+        for(int d = 0; d < 3; ++d)
+        {
+            double a = mn[d], b = mx[d];
+            for(int o = kWave / 2; o > 0; o >>= 1)
+            {
+                a = fmin(a, __shfl_xor(a, o, kWave));
+                b = fmax(b, __shfl_xor(b, o, kWave));
+            }
+            if(tid % kWave == 0) {red[0][d][tid / kWave] = a; red[1][d][tid / kWave] = b;}
+        }
+        __syncthreads();
+        if(tid == 0)
+        {
+            double lo[3], ext[3];
+            for(int d = 0; d < 3; ++d)
+            {
+                double a = INFINITY, b = -INFINITY;
+                for(uint32_t w = 0; w < nt / kWave; ++w) {a = fmin(a, red[0][d][w]); b = fmax(b, red[1][d][w]);}
+                lo[d] = isfinite(a) ? a : 0.0;
+                ext[d] = (isfinite(b - a) && b >= a) ? (b - a) : 0.0;
+            }
+            double w = s.r_list * (1.0 + 1e-6);
+            if(!(w > 0.0)) {w = 1.0;}
+            int dim[3];
+            for(int it = 0; it < 200; ++it)
+            {
+                unsigned long long prod = 1;
+                for(int d = 0; d < 3; ++d)
+                {
+                    const double cc = floor(ext[d] / w) + 1.0;
+                    dim[d] = (cc > 4096.0) ? 4096 : (int)cc;
+                    prod *= (unsigned long long)dim[d];
+                }
+                if(prod <= kSmallMaxCells) {break;}
+                w *= 1.1;
+            }
+            for(int d = 0; d < 3; ++d) {gl.lo[d] = lo[d]; gl.inv_w[d] = 1.0 / w; gl.dim[d] = dim[d];}
+            gl.periodic = 0;
+            gl.ncell = (uint32_t)dim[0] * (uint32_t)dim[1] * (uint32_t)dim[2];
+            *s.grid = gl;
+        }
+    }
+    __syncthreads();
+    const uint32_t ncell = gl.ncell;
+    for(uint32_t c = tid; c <= ncell; c += nt) {start[c] = 0;}
+    __syncthreads();
+    for(uint32_t p = tid; p < s.np; p += nt)
+    {
+        const double4 r = pos[s.part[p]];
+        const int cx = cell_coord(r.x, gl.lo[0], gl.inv_w[0], gl.dim[0]);
+        const int cy = cell_coord(r.y, gl.lo[1], gl.inv_w[1], gl.dim[1]);
+        const int cz = cell_coord(r.z, gl.lo[2], gl.inv_w[2], gl.dim[2]);
+        const uint32_t k = (uint32_t)cx + (uint32_t)gl.dim[0] * ((uint32_t)cy + (uint32_t)gl.dim[1] * (uint32_t)cz);
+        key[p] = k;
+        atomicAdd(&start[k + 1], 1u);
+    }
+    __syncthreads();
+    if(tid == 0)
+    {
+        // counts -> exclusive offsets (ncell <= 2048)
+        uint32_t run = 0;
+        for(uint32_t c = 0; c <= ncell; ++c) {run += start[c]; start[c] = run;}
+    }
+    __syncthreads();
+    for(uint32_t c = tid; c < ncell; c += nt) {cursor[c] = start[c];}
+    __syncthreads();
+    for(uint32_t p = tid; p < s.np; p += nt)
+    {
+        const uint32_t slot = atomicAdd(&cursor[key[p]], 1u);
+        sval[slot] = p;
+    }
+    __syncthreads();
+    for(uint32_t c = tid; c < ncell; c += nt)
+    {
+        const uint32_t b = start[c], e = start[c + 1];
+        for(uint32_t a = b + 1; a < e; ++a)
+        {
+            const uint32_t v = sval[a];
+            uint32_t q = a;
+            while(q > b && sval[q - 1] > v) {sval[q] = sval[q - 1]; --q;}
+            sval[q] = v;
+        }
+        s.cell_start[c] = b;
+        s.cell_end[c] = e;
+    }
+    __syncthreads();
+    for(uint32_t t = tid; t < s.np; t += nt)
+    {
+        const uint32_t v = sval[t];
+        s.svals[t] = v;
+        s.skeys[t] = key[v];
+    }
+}
+
+// Skin check for interactions on the multi-kernel (radix-sort) path.
+__global__ void k_margin_large(PrepSegTable tab, StepCtl* __restrict__ sctl, int force)
+{
+    const int g = threadIdx.x;
+    if(g < tab.n)
+    {
+        const PrepSeg& s = tab.seg[g];
+        const double dmargin = 2 * sqrt(__longlong_as_double((long long)sctl->maxdisp2[sctl->step & 1]));
+        if(margin_decision(*s.ctl, dmargin, s.abs_margin, s.never_rebuild, force)) {atomicAdd(&sctl->rebuilds, 1ull);}
+    }
 }
 
 } // mjg
