@@ -156,6 +156,61 @@ mjg_status mjg_kinetic_energy(mjg_engine* e, double* ke);
  * Also returns the number of half-list pairs the interaction currently holds (0 for locals). */
 mjg_status mjg_profile_interaction(mjg_engine* e, int id, int reps, double* avg_ms, uint64_t* half_pairs);
 
+/* ------------------------------------------------------------------------------------
+ * Batched replicas.  The reference runs one System per process
+ * (MolecularDynamicsSimulator holds a single system_, MolecularDynamicsSimulator.hpp:153-165);
+ * a batch holds many independent ones -- each with the same description as an engine above --
+ * and advances all resident replicas in one persistent launch per segment of steps, one
+ * workgroup per replica (at most 2048 particles each).  Replicas can be added and retired
+ * between calls; their state stays in device memory.  MJG_FP32_FIXED runs pair and local
+ * arithmetic in FP32 with int64 fixed-point force sums and the FP64 integrator; MJG_FP64 runs
+ * the same kernel in FP64.
+ * ------------------------------------------------------------------------------------ */
+typedef struct mjg_batch mjg_batch;
+
+mjg_status mjg_batch_create(int precision, int device, mjg_batch** out);
+void mjg_batch_destroy(mjg_batch* b);
+const char* mjg_batch_last_error(const mjg_batch* b);
+/* A new replica: System(num_particles, boundary) with masses, positions, velocities
+ * (System.hpp:43-51,139-150); its id goes to *replica. */
+mjg_status mjg_batch_add_replica(mjg_batch* b, int boundary, const double* lower, const double* upper, size_t n,
+                                 const double* mass, const double* pos, const double* vel, int* replica);
+/* System attributes of one replica (System.hpp:164-167). */
+mjg_status mjg_batch_set_attribute(mjg_batch* b, int replica, const char* key, double value);
+/* Local / global interactions of one replica; same meaning as mjg_add_local / mjg_add_global. */
+mjg_status mjg_batch_add_local(mjg_batch* b, int replica, int kind, const char* topology, size_t n_terms,
+                               const uint32_t* indices, const double* params);
+mjg_status mjg_batch_add_global(mjg_batch* b, int replica, int potential, double cutoff_ratio, double epsilon,
+                                int partition, double margin, int ignore_molecule, size_t n_within,
+                                const char* const* within_kinds, const uint32_t* within_dist, size_t n_participants,
+                                const uint32_t* participants, const double* params);
+/* Integrator of one replica (as mjg_integrator_set). */
+mjg_status mjg_batch_set_integrator(mjg_batch* b, int replica, int type, double dt, const double* coefs, uint64_t seed);
+/* ForceField::initialize + Integrator::initialize for every replica added since the last call
+ * (ForceField.hpp:47-63, BAOABLangevinIntegrator.hpp:135-168); they become resident. */
+mjg_status mjg_batch_initialize(mjg_batch* b);
+/* Every resident replica takes n_steps Integrator::step calls (BAOABLangevinIntegrator.hpp:170-240). */
+mjg_status mjg_batch_step(mjg_batch* b, size_t n_steps);
+/* Steps one replica still has to take under mjg_batch_advance. */
+mjg_status mjg_batch_set_steps(mjg_batch* b, int replica, uint64_t steps);
+/* One launch of up to `segment` steps for every resident replica with steps remaining. */
+mjg_status mjg_batch_advance(mjg_batch* b, int segment);
+/* Free a replica's device memory. */
+mjg_status mjg_batch_retire(mjg_batch* b, int replica);
+mjg_status mjg_batch_get_state(mjg_batch* b, int replica, double* pos, double* vel, double* force);
+/* Steps taken, steps remaining, list rebuilds and device bytes of one replica. */
+mjg_status mjg_batch_replica_info(mjg_batch* b, int replica, uint64_t* step, uint64_t* remaining, uint64_t* rebuilds,
+                                  size_t* footprint_bytes);
+/* Forces and per-interaction energies of one replica at its current positions
+ * (calc_force_and_energy / format_energy, ForceField.hpp:126-161). */
+mjg_status mjg_batch_evaluate(mjg_batch* b, int replica, double* per_interaction, double* total, double* force);
+/* Sum over replicas of particles x steps taken, and list rebuilds. */
+mjg_status mjg_batch_counters(mjg_batch* b, uint64_t* particle_steps, uint64_t* rebuilds);
+/* Timing hook with no counterpart in the reference: `reps` launches of a `segment_steps` segment
+ * over every resident replica, bracketed by HIP events; average launch time in ms and the
+ * particle-steps of one launch. */
+mjg_status mjg_batch_profile(mjg_batch* b, int segment_steps, int reps, double* avg_ms, uint64_t* particle_steps);
+
 #ifdef __cplusplus
 }
 #endif

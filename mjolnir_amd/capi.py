@@ -55,6 +55,11 @@ EXPORTED = [
     "mjg_calc_force", "mjg_calc_force_and_energy", "mjg_calc_energies", "mjg_calc_force_of",
     "mjg_reduce_margin", "mjg_neighbor_pairs", "mjg_integrator_set", "mjg_integrator_initialize",
     "mjg_step", "mjg_counters", "mjg_kinetic_energy", "mjg_profile_interaction",
+    "mjg_batch_create", "mjg_batch_destroy", "mjg_batch_last_error", "mjg_batch_add_replica",
+    "mjg_batch_set_attribute", "mjg_batch_add_local", "mjg_batch_add_global", "mjg_batch_set_integrator",
+    "mjg_batch_initialize", "mjg_batch_step", "mjg_batch_set_steps", "mjg_batch_advance", "mjg_batch_retire",
+    "mjg_batch_get_state", "mjg_batch_replica_info", "mjg_batch_evaluate", "mjg_batch_counters",
+    "mjg_batch_profile",
 ]
 
 
@@ -93,6 +98,26 @@ def lib():
     L.mjg_counters.argtypes = [vp, C.POINTER(u64), C.POINTER(u64)]
     L.mjg_kinetic_energy.argtypes = [vp, C.POINTER(dbl)]
     L.mjg_profile_interaction.argtypes = [vp, i32, i32, C.POINTER(dbl), C.POINTER(u64)]
+    L.mjg_batch_create.argtypes = [i32, i32, C.POINTER(vp)]
+    L.mjg_batch_destroy.argtypes = [vp]
+    L.mjg_batch_last_error.restype = C.c_char_p
+    L.mjg_batch_last_error.argtypes = [vp]
+    L.mjg_batch_add_replica.argtypes = [vp, i32, _dp, _dp, sz, _dp, _dp, _dp, C.POINTER(i32)]
+    L.mjg_batch_set_attribute.argtypes = [vp, i32, C.c_char_p, dbl]
+    L.mjg_batch_add_local.argtypes = [vp, i32, i32, C.c_char_p, sz, _up, _dp]
+    L.mjg_batch_add_global.argtypes = [vp, i32, i32, dbl, dbl, i32, dbl, i32, sz, C.POINTER(C.c_char_p), _up, sz,
+                                       _up, _dp]
+    L.mjg_batch_set_integrator.argtypes = [vp, i32, i32, dbl, _dp, u64]
+    L.mjg_batch_initialize.argtypes = [vp]
+    L.mjg_batch_step.argtypes = [vp, sz]
+    L.mjg_batch_set_steps.argtypes = [vp, i32, u64]
+    L.mjg_batch_advance.argtypes = [vp, i32]
+    L.mjg_batch_retire.argtypes = [vp, i32]
+    L.mjg_batch_get_state.argtypes = [vp, i32, vp, vp, vp]
+    L.mjg_batch_replica_info.argtypes = [vp, i32, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), C.POINTER(sz)]
+    L.mjg_batch_evaluate.argtypes = [vp, i32, vp, C.POINTER(dbl), vp]
+    L.mjg_batch_counters.argtypes = [vp, C.POINTER(u64), C.POINTER(u64)]
+    L.mjg_batch_profile.argtypes = [vp, i32, i32, C.POINTER(dbl), C.POINTER(u64)]
     _LIB = L
     return L
 
@@ -214,3 +239,109 @@ class Engine:
         x = np.zeros(3 * self.n); v = np.zeros(3 * self.n); f = np.zeros(3 * self.n)
         self._check(lib().mjg_system_get_state(self.h, x.ctypes.data, v.ctypes.data, f.ctypes.data))
         return x.reshape(-1, 3), v.reshape(-1, 3), f.reshape(-1, 3)
+
+
+class Batch:
+    """Independent replicas advanced together on one GPU (one persistent workgroup each).
+
+    ``add(spec)`` uploads one ``SystemSpec`` as a replica and returns its id; ``initialize()``
+    makes every new replica resident (lists, first forces); ``step(n)`` advances all of them.
+    """
+
+    def __init__(self, precision: int = FP32_FIXED, device: int = 0):
+        L = lib()
+        h = C.c_void_p()
+        st = L.mjg_batch_create(precision, device, C.byref(h))
+        if st != OK:
+            raise MjgError(st, "mjg_batch_create failed (is a GPU present?)")
+        self.h = h
+        self.specs = {}
+
+    def _check(self, st: int):
+        if st != OK:
+            msg = lib().mjg_batch_last_error(self.h)
+            raise MjgError(st, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mjg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, spec: SystemSpec) -> int:
+        L = lib()
+        rid = C.c_int()
+        self._check(L.mjg_batch_add_replica(self.h, PERIODIC if spec.boundary == "periodic" else UNLIMITED,
+                                            np.ascontiguousarray(spec.lower), np.ascontiguousarray(spec.upper),
+                                            spec.n, spec.mass, spec.pos, spec.vel, C.byref(rid)))
+        r = rid.value
+        for k, v in spec.attributes.items():
+            self._check(L.mjg_batch_set_attribute(self.h, r, k.encode(), float(v)))
+        for l in spec.local:
+            self._check(L.mjg_batch_add_local(self.h, r, LOCAL_CODE[(l.interaction, l.potential)], l.topology.encode(),
+                                              len(l.indices), np.ascontiguousarray(l.indices.ravel()),
+                                              np.ascontiguousarray(l.params.ravel())))
+        for g in spec.global_:
+            kinds = list(g.ignore_within.keys())
+            arr = (C.c_char_p * max(1, len(kinds)))(*[k.encode() for k in kinds])
+            dists = np.array([g.ignore_within[k] for k in kinds] or [0], dtype=np.uint32)
+            self._check(L.mjg_batch_add_global(self.h, r, GLOBAL_CODE[g.potential], float(g.cutoff), float(g.epsilon),
+                                               PARTITION_CODE[g.partition], float(g.margin),
+                                               IGNORE_CODE[g.ignore_molecule], len(kinds), arr, dists,
+                                               len(g.participants), np.ascontiguousarray(g.participants),
+                                               np.ascontiguousarray(g.params.ravel())))
+        it = spec.integrator
+        self._check(L.mjg_batch_set_integrator(self.h, r, INTEG_CODE[it.type], float(it.dt), it.coefs, int(it.seed)))
+        self.specs[r] = spec
+        return r
+
+    def initialize(self):
+        self._check(lib().mjg_batch_initialize(self.h))
+
+    def step(self, n: int):
+        self._check(lib().mjg_batch_step(self.h, int(n)))
+
+    def set_steps(self, r: int, n: int):
+        self._check(lib().mjg_batch_set_steps(self.h, r, int(n)))
+
+    def advance(self, segment: int):
+        self._check(lib().mjg_batch_advance(self.h, int(segment)))
+
+    def retire(self, r: int):
+        self._check(lib().mjg_batch_retire(self.h, r))
+        self.specs.pop(r, None)
+
+    def state(self, r: int):
+        n = self.specs[r].n
+        x = np.zeros(3 * n); v = np.zeros(3 * n); f = np.zeros(3 * n)
+        self._check(lib().mjg_batch_get_state(self.h, r, x.ctypes.data, v.ctypes.data, f.ctypes.data))
+        return x.reshape(-1, 3), v.reshape(-1, 3), f.reshape(-1, 3)
+
+    def info(self, r: int):
+        s, rem, rb, fp = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_size_t()
+        self._check(lib().mjg_batch_replica_info(self.h, r, C.byref(s), C.byref(rem), C.byref(rb), C.byref(fp)))
+        return {"step": s.value, "remaining": rem.value, "rebuilds": rb.value, "footprint": fp.value}
+
+    def evaluate(self, r: int):
+        spec = self.specs[r]
+        ne = len(spec.local) + len(spec.global_)
+        per = np.zeros(max(1, ne))
+        tot = C.c_double()
+        f = np.zeros(3 * spec.n)
+        self._check(lib().mjg_batch_evaluate(self.h, r, per.ctypes.data, C.byref(tot), f.ctypes.data))
+        return per[:ne], tot.value, f.reshape(-1, 3)
+
+    def counters(self):
+        ps, rb = C.c_uint64(), C.c_uint64()
+        self._check(lib().mjg_batch_counters(self.h, C.byref(ps), C.byref(rb)))
+        return ps.value, rb.value
+
+    def profile(self, segment_steps: int, reps: int = 3):
+        ms, ps = C.c_double(), C.c_uint64()
+        self._check(lib().mjg_batch_profile(self.h, segment_steps, reps, C.byref(ms), C.byref(ps)))
+        return ms.value, ps.value

@@ -1,0 +1,916 @@
+// Host side of the batched replica engine (kernels in batch_kernels.hpp) and its C ABI.
+//
+// A batch holds independent replicas (each one a System + ForceField + Integrator of its own,
+// as the reference would hold in separate processes).  Replicas can be added while others are
+// resident: each one owns a device arena sized by its footprint, and every launch runs the
+// replicas that are active, largest first, one workgroup each.
+#include "batch_kernels.hpp"
+#include "topology.hpp"
+#include "../../include/mjg.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace mjg
+{
+
+struct BatchDeviceError : std::runtime_error {using std::runtime_error::runtime_error;};
+struct BatchNonFinite : std::runtime_error {using std::runtime_error::runtime_error;};
+
+struct RepLocal
+{
+    int kind = 0, arity = 0, npar = 0, topo = -1;
+    std::vector<uint32_t> idx;
+    std::vector<double> par;
+};
+struct RepGlobalIn
+{
+    int pot = 0, partition = 0, ignore_mol = 0, npar = 0;
+    double cutoff_ratio = 0, eps = 0, margin = 0;
+    std::vector<std::pair<int, uint32_t>> within;
+    std::vector<uint32_t> part;
+    std::vector<double> par;
+    uint32_t cap = 0;
+};
+
+struct Replica
+{
+    // host description
+    Box box{};
+    size_t n = 0;
+    std::vector<double> mass, pos, vel;
+    double T = 0, I = 0;
+    bool has_T = false, has_I = false;
+    Topo topo;
+    std::vector<RepLocal> locals;
+    std::vector<RepGlobalIn> globals;
+    int integ = 0;
+    double dt = 0;
+    uint64_t seed = 0;
+    std::vector<double> coefs;
+    // device
+    void* arena = nullptr;
+    size_t footprint = 0;
+    RepDesc desc{};
+    int slot = -1;                 // index into the batch's state array
+    bool ready = false, active = false;
+};
+
+static void hip_check(hipError_t e, const char* what)
+{
+    if(e != hipSuccess) {throw BatchDeviceError(std::string(what) + ": " + hipGetErrorString(e));}
+}
+static size_t align256(size_t x) {return (x + 255) & ~(size_t)255;}
+
+class Batch
+{
+  public:
+    Batch(int precision, int device) : precision_(precision)
+    {
+        if(precision != MJG_FP32_FIXED && precision != MJG_FP64) {throw std::invalid_argument("unknown precision");}
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+        const char* seg = std::getenv("MJG_SEGMENT");
+        if(seg) {segment_ = std::max(1, std::atoi(seg));}
+        // the segment kernel keeps 48 B per particle in dynamic LDS (up to 96 KiB)
+        const int lds = (int)(kRepMaxN * 48);
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_replica_segment<float, true>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds), "lds attribute");
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_replica_segment<float, false>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds), "lds attribute");
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_replica_segment<double, true>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds), "lds attribute");
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_replica_segment<double, false>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds), "lds attribute");
+    }
+    ~Batch()
+    {
+        if(stream_) {(void)hipStreamSynchronize(stream_);}
+        for(auto& r : reps_) {if(r && r->arena) {(void)hipFree(r->arena);}}
+        if(d_descs_) {(void)hipFree(d_descs_);}
+        if(d_states_) {(void)hipFree(d_states_);}
+        if(stream_) {(void)hipStreamDestroy(stream_);}
+    }
+
+    int begin_replica(int boundary, const double* lower, const double* upper, size_t n, const double* mass,
+                      const double* pos, const double* vel)
+    {
+        if(n == 0 || n > kRepMaxN) {throw std::invalid_argument("replica size must be 1.." + std::to_string(kRepMaxN));}
+        if(!mass || !pos || !vel) {throw std::invalid_argument("null state pointer");}
+        auto r = std::make_unique<Replica>();
+        r->n = n;
+        r->box.periodic = (boundary == MJG_PERIODIC);
+        if(boundary != MJG_PERIODIC && boundary != MJG_UNLIMITED) {throw std::invalid_argument("unknown boundary");}
+        if(r->box.periodic)
+        {
+            if(!lower || !upper) {throw std::invalid_argument("periodic boundary needs lower/upper");}
+            for(int d = 0; d < 3; ++d)
+            {
+                if(!(upper[d] > lower[d])) {throw std::invalid_argument("upper must exceed lower");}
+                r->box.lower[d] = lower[d]; r->box.upper[d] = upper[d];
+                r->box.width[d] = upper[d] - lower[d]; r->box.halfw[d] = r->box.width[d] * 0.5;
+            }
+        }
+        r->mass.assign(mass, mass + n);
+        r->pos.assign(pos, pos + 3 * n);
+        r->vel.assign(vel, vel + 3 * n);
+        for(size_t i = 0; i < n; ++i) {if(!(mass[i] > 0)) {throw std::invalid_argument("masses must be positive");}}
+        r->topo.reset(n);
+        reps_.push_back(std::move(r));
+        return (int)reps_.size() - 1;
+    }
+
+    Replica& rep(int id)
+    {
+        if(id < 0 || (size_t)id >= reps_.size() || !reps_[id]) {throw std::out_of_range("replica id out of range");}
+        return *reps_[id];
+    }
+
+    void set_attribute(int id, const std::string& key, double v)
+    {
+        Replica& r = rep(id);
+        if(key == "temperature") {r.T = v; r.has_T = true;}
+        else if(key == "ionic_strength") {r.I = v; r.has_I = true;}
+        else {throw std::invalid_argument("unknown attribute: " + key);}
+    }
+
+    void add_local(int id, int kind, const char* topology, size_t nt, const uint32_t* idx, const double* par)
+    {
+        Replica& r = rep(id);
+        if(r.ready) {throw std::invalid_argument("replica already initialized");}
+        if(!r.globals.empty()) {throw std::invalid_argument("add local interactions before global ones");}
+        if(r.locals.size() >= (size_t)kRepMaxLocals) {throw std::invalid_argument("too many local interactions");}
+        RepLocal L;
+        L.kind = kind;
+        switch(kind)
+        {
+            case kBondHarmonic: case kBondGo: case kContactGo: L.arity = 2; L.npar = 2; break;
+            case kAngleHarmonic: L.arity = 3; L.npar = 2; break;
+            case kAngleGaussian: L.arity = 3; L.npar = 3; break;
+            case kDihedralClementi: L.arity = 4; L.npar = 3; break;
+            default: throw std::invalid_argument("unknown local interaction kind");
+        }
+        if(nt > 0 && (!idx || !par)) {throw std::invalid_argument("null term arrays");}
+        for(size_t t = 0; t < nt * L.arity; ++t) {if(idx[t] >= r.n) {throw std::out_of_range("term index out of range");}}
+        const std::string topo = topology ? topology : "";
+        L.topo = (topo.empty() || topo == "none") ? -1 : r.topo.kind_id(topo);
+        L.idx.assign(idx, idx + nt * L.arity);
+        L.par.assign(par, par + nt * L.npar);
+        r.locals.push_back(std::move(L));
+    }
+
+    void add_global(int id, int pot, double cutoff_ratio, double eps, int partition, double margin, int ignore_mol,
+                    size_t n_within, const char* const* kinds, const uint32_t* dists, size_t np, const uint32_t* part,
+                    const double* par)
+    {
+        Replica& r = rep(id);
+        if(r.ready) {throw std::invalid_argument("replica already initialized");}
+        if(r.globals.size() >= (size_t)kRepMaxGlobals) {throw std::invalid_argument("too many global interactions");}
+        RepGlobalIn G;
+        G.pot = pot;
+        switch(pot)
+        {
+            case kPairLJ: G.npar = 2; break;
+            case kPairEV: case kPairDH: G.npar = 1; break;
+            default: throw std::invalid_argument("unknown pair potential");
+        }
+        if(partition != MJG_PART_CELLLIST && partition != MJG_PART_VERLET && partition != MJG_PART_NAIVE)
+        {
+            throw std::invalid_argument("unknown spatial partition");
+        }
+        if(np < 1 || !part || !par) {throw std::invalid_argument("a pair interaction needs participants");}
+        if(ignore_mol < 0 || ignore_mol > 2) {throw std::invalid_argument("unknown ignore.molecule");}
+        for(size_t k = 0; k < np; ++k) {if(part[k] >= r.n) {throw std::out_of_range("participant index out of range");}}
+        G.cutoff_ratio = cutoff_ratio; G.eps = eps; G.partition = partition; G.margin = margin; G.ignore_mol = ignore_mol;
+        for(size_t k = 0; k < n_within; ++k)
+        {
+            if(!kinds || !kinds[k] || !dists) {throw std::invalid_argument("null ignore.particles_within entry");}
+            G.within.emplace_back(r.topo.kind_id(kinds[k]), dists[k]);
+        }
+        G.part.assign(part, part + np);
+        G.par.assign(par, par + np * G.npar);
+        r.globals.push_back(std::move(G));
+    }
+
+    void set_integrator(int id, int type, double dt, const double* coefs, uint64_t seed)
+    {
+        Replica& r = rep(id);
+        if(type != MJG_INTEG_BAOAB && type != MJG_INTEG_GJF) {throw std::invalid_argument("unknown integrator");}
+        if(!(dt > 0) || !coefs) {throw std::invalid_argument("invalid integrator parameters");}
+        r.integ = type; r.dt = dt; r.seed = seed;
+        r.coefs.assign(coefs, coefs + r.n);
+    }
+
+    // Prepare every replica that is not resident yet, make it active, build its lists and
+    // its first forces.
+    void initialize()
+    {
+        std::vector<int> fresh;
+        for(size_t id = 0; id < reps_.size(); ++id)
+        {
+            if(reps_[id] && !reps_[id]->ready) {prepare((int)id); fresh.push_back((int)id);}
+        }
+        if(fresh.empty()) {return;}
+        std::vector<int> todo = fresh;
+        for(int pass = 0; pass < 8 && !todo.empty(); ++pass)
+        {
+            launch(todo, 0, 1);
+            download_states();
+            std::vector<int> again;
+            for(int id : todo)
+            {
+                Replica& r = *reps_[id];
+                const RepState& st = h_states_[r.slot];
+                bool over = false;
+                for(size_t g = 0; g < r.globals.size(); ++g)
+                {
+                    if(st.maxpairs[g] > r.globals[g].cap) {r.globals[g].cap = st.maxpairs[g] + st.maxpairs[g] / 4 + 256; over = true;}
+                }
+                if(over) {r.topo.reset(r.n); r.ready = false; prepare(id); again.push_back(id);}
+            }
+            todo.swap(again);
+        }
+        if(!todo.empty()) {throw BatchDeviceError("replica list capacity did not converge");}
+        for(int id : fresh) {reps_[id]->active = true;}
+        descs_dirty_ = true;
+    }
+
+    // Advance every active replica by up to n steps (fewer if it has fewer remaining).
+    void step(size_t n)
+    {
+        std::vector<int> act = active_ids();
+        if(act.empty()) {return;}
+        download_states();
+        for(int id : act) {h_states_[reps_[id]->slot].remaining = n;}
+        upload_states();
+        size_t done = 0;
+        while(done < n)
+        {
+            const int seg = (int)std::min<size_t>(segment_, n - done);
+            run_segment(act, seg);
+            done += seg;
+        }
+    }
+
+    void set_steps(int id, uint64_t steps) {set_remaining(rep(id), steps);}
+
+    // run `seg` steps of the replicas whose remaining count is positive (serving)
+    void advance(int seg)
+    {
+        std::vector<int> act = active_ids();
+        if(!act.empty()) {run_segment(act, seg);}
+    }
+
+    void retire(int id)
+    {
+        Replica& r = rep(id);
+        hip_check(hipStreamSynchronize(stream_), "sync");
+        if(r.arena) {hip_check(hipFree(r.arena), "hipFree");}
+        free_slots_.push_back(r.slot);
+        reps_[id].reset();
+        descs_dirty_ = true;
+    }
+
+    RepState state(int id)
+    {
+        Replica& r = rep(id);
+        if(!r.ready) {throw std::logic_error("replica not initialized");}
+        return state_of(r);
+    }
+
+    void get_state(int id, double* pos, double* vel, double* force)
+    {
+        Replica& r = rep(id);
+        if(!r.ready) {throw std::logic_error("replica not initialized");}
+        std::vector<double4> h(r.n);
+        double4* src[3] = {r.desc.pos, r.desc.vel, r.desc.force};
+        double* dst[3] = {pos, vel, force};
+        for(int k = 0; k < 3; ++k)
+        {
+            if(!dst[k]) {continue;}
+            hip_check(hipMemcpyAsync(h.data(), src[k], r.n * sizeof(double4), hipMemcpyDeviceToHost, stream_), "download");
+            hip_check(hipStreamSynchronize(stream_), "sync");
+            for(size_t i = 0; i < r.n; ++i) {dst[k][3 * i] = h[i].x; dst[k][3 * i + 1] = h[i].y; dst[k][3 * i + 2] = h[i].z;}
+        }
+    }
+
+    // forces and per-interaction energies at the current positions (no step)
+    void evaluate(int id, double* energies, double* total, double* force)
+    {
+        Replica& r = rep(id);
+        if(!r.ready) {throw std::logic_error("replica not initialized");}
+        launch({id}, 0, 2);
+        RepState s = state(id);
+        double tot = 0;
+        const int ne = (int)r.locals.size() + (int)r.globals.size();
+        for(int k = 0; k < ne; ++k) {if(energies) {energies[k] = s.energy[k];} tot += s.energy[k];}
+        if(total) {*total = tot;}
+        if(force) {get_state(id, nullptr, nullptr, force);}
+    }
+
+    size_t footprint(int id) {return rep(id).footprint;}
+    size_t num_interactions(int id) {Replica& r = rep(id); return r.locals.size() + r.globals.size();}
+
+    void counters(uint64_t* steps, uint64_t* rebuilds)
+    {
+        uint64_t s = 0, b = 0;
+        download_states();
+        for(size_t id = 0; id < reps_.size(); ++id)
+        {
+            if(!reps_[id] || !reps_[id]->ready) {continue;}
+            const RepState& st = h_states_[reps_[id]->slot];
+            s += st.step * reps_[id]->n;
+            b += st.rebuilds;
+        }
+        if(steps) {*steps = s;}
+        if(rebuilds) {*rebuilds = b;}
+    }
+
+    // time `reps` launches of an nsteps segment over every active replica (HIP events)
+    double profile(int nsteps, int reps, uint64_t* particle_steps)
+    {
+        std::vector<int> act = active_ids();
+        if(act.empty() || reps < 1) {throw std::logic_error("nothing to profile");}
+        uint64_t ps = 0;
+        download_states();
+        for(int id : act) {h_states_[reps_[id]->slot].remaining = (uint64_t)nsteps * reps; ps += reps_[id]->n;}
+        upload_states();
+        upload_descs(act);
+        hipEvent_t a, b;
+        hip_check(hipEventCreate(&a), "event");
+        hip_check(hipEventCreate(&b), "event");
+        hip_check(hipEventRecord(a, stream_), "record");
+        for(int k = 0; k < reps; ++k) {launch_descs(act, nsteps, 0);}
+        hip_check(hipEventRecord(b, stream_), "record");
+        hip_check(hipEventSynchronize(b), "event sync");
+        hip_check(hipGetLastError(), "profile launch");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, a, b), "elapsed");
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        download_states();
+        for(int id : act) {if(h_states_[reps_[id]->slot].nonfinite) {throw BatchNonFinite("non-finite position in a replica");}}
+        if(particle_steps) {*particle_steps = ps * (uint64_t)nsteps;}
+        return (double)ms / reps;
+    }
+
+  private:
+
+    std::vector<int> active_ids()
+    {
+        std::vector<int> v;
+        for(size_t id = 0; id < reps_.size(); ++id) {if(reps_[id] && reps_[id]->active) {v.push_back((int)id);}}
+        return v;
+    }
+
+    void set_remaining(Replica& r, uint64_t n)
+    {
+        RepState st = state_of(r);
+        st.remaining = n;
+        hip_check(hipMemcpyAsync(r.desc.st, &st, sizeof(RepState), hipMemcpyHostToDevice, stream_), "upload");
+        hip_check(hipStreamSynchronize(stream_), "sync");
+    }
+
+    // state array shared by every replica (one slot each), mirrored on the host
+    int take_slot()
+    {
+        if(!free_slots_.empty()) {const int k = free_slots_.back(); free_slots_.pop_back(); return k;}
+        if(n_slots_ == states_cap_)
+        {
+            const size_t cap = std::max<size_t>(64, 2 * states_cap_);
+            RepState* d = nullptr;
+            hip_check(hipMalloc(&d, cap * sizeof(RepState)), "hipMalloc");
+            if(d_states_)
+            {
+                hip_check(hipMemcpyAsync(d, d_states_, n_slots_ * sizeof(RepState), hipMemcpyDeviceToDevice, stream_), "copy");
+                hip_check(hipStreamSynchronize(stream_), "sync");
+                hip_check(hipFree(d_states_), "hipFree");
+            }
+            d_states_ = d;
+            states_cap_ = cap;
+            // every replica's state pointer moves with the array
+            for(auto& r : reps_) {if(r && r->slot >= 0) {r->desc.st = d_states_ + r->slot;}}
+            descs_dirty_ = true;
+        }
+        h_states_.resize(n_slots_ + 1);
+        return (int)n_slots_++;
+    }
+    void download_states()
+    {
+        if(n_slots_ == 0) {return;}
+        h_states_.resize(n_slots_);
+        hip_check(hipMemcpyAsync(h_states_.data(), d_states_, n_slots_ * sizeof(RepState), hipMemcpyDeviceToHost, stream_),
+                  "download");
+        hip_check(hipStreamSynchronize(stream_), "sync");
+    }
+    void upload_states()
+    {
+        if(n_slots_ == 0) {return;}
+        hip_check(hipMemcpyAsync(d_states_, h_states_.data(), n_slots_ * sizeof(RepState), hipMemcpyHostToDevice, stream_),
+                  "upload");
+        hip_check(hipStreamSynchronize(stream_), "sync");
+    }
+
+    // Lay out the replica's device arena and upload everything (set-up only).
+    void prepare(int id)
+    {
+        Replica& r = *reps_[id];
+        if(r.integ == 0) {throw std::invalid_argument("replica " + std::to_string(id) + " has no integrator");}
+        if(!r.has_T) {throw std::invalid_argument("Langevin integrator requires the `temperature` attribute");}
+        // topology, molecules
+        for(auto& L : r.locals) {r.topo.add_terms(L.idx, L.arity, L.topo);}
+        r.topo.construct_molecules();
+        const size_t n = r.n;
+
+        // local terms, one record per term, grouped by interaction
+        std::vector<RepTerm> terms;
+        for(size_t l = 0; l < r.locals.size(); ++l)
+        {
+            const RepLocal& L = r.locals[l];
+            const size_t nt = L.idx.size() / L.arity;
+            for(size_t t = 0; t < nt; ++t)
+            {
+                RepTerm T{};
+                T.kind = (uint16_t)L.kind;
+                T.inter = (uint16_t)l;
+                for(int a = 0; a < L.arity; ++a) {T.idx[a] = (uint16_t)L.idx[t * L.arity + a];}
+                for(int c = 0; c < L.npar; ++c) {T.p[c] = L.par[t * L.npar + c];}
+                terms.push_back(T);
+            }
+        }
+
+        // per global: tables
+        struct GHost
+        {
+            std::vector<uint16_t> part;
+            std::vector<uint32_t> pslot, ex_start, ex;
+            std::vector<double> ppar;
+        };
+        std::vector<GHost> gh(r.globals.size());
+        for(size_t g = 0; g < r.globals.size(); ++g)
+        {
+            RepGlobalIn& G = r.globals[g];
+            GHost& h = gh[g];
+            h.part.assign(G.part.begin(), G.part.end());
+            h.pslot.assign(n, 0xffffffffu);
+            for(size_t k = 0; k < G.part.size(); ++k) {h.pslot[G.part[k]] = (uint32_t)k;}
+            h.ppar.assign(n * G.npar, 0.0);
+            for(size_t k = 0; k < G.part.size(); ++k)
+                for(int c = 0; c < G.npar; ++c) {h.ppar[(size_t)G.part[k] * G.npar + c] = G.par[k * G.npar + c];}
+            r.topo.exclusions(G.within, h.ex_start, h.ex);
+            if(G.cap == 0)
+            {
+                // first guess: a tenth of all participant pairs, at least 1024; grown on overflow.
+                // MJG_BATCH_INITIAL_CAP overrides the guess (tests use it to exercise the growth path)
+                const uint64_t np = G.part.size();
+                G.cap = (uint32_t)std::min<uint64_t>(np * (np - 1) / 2 + 1, std::max<uint64_t>(1024, np * (np - 1) / 10));
+                if(const char* c = std::getenv("MJG_BATCH_INITIAL_CAP")) {G.cap = (uint32_t)std::max(1, std::atoi(c));}
+            }
+        }
+
+        // arena layout
+        size_t off = 0;
+        auto take = [&](size_t bytes) {const size_t o = off; off += align256(bytes); return o;};
+        const size_t o_pos = take(n * sizeof(double4)), o_vel = take(n * sizeof(double4)), o_force = take(n * sizeof(double4));
+        const size_t o_bpos = take(n * sizeof(double4)), o_bvel = take(n * sizeof(double4)), o_bforce = take(n * sizeof(double4));
+        const size_t o_rm = take(n * 8), o_c1 = take(n * 8), o_c2 = take(n * 8), o_c3 = take(n * 8);
+        const size_t o_mol = take(n * 4), o_terms = take(terms.size() * sizeof(RepTerm));
+        const size_t o_bst = take(sizeof(RepState));
+        if(r.slot < 0) {r.slot = take_slot();}
+        std::vector<size_t> o_part(gh.size()), o_pslot(gh.size()), o_ppar(gh.size()), o_exs(gh.size()), o_ex(gh.size()),
+            o_pairs(gh.size());
+        for(size_t g = 0; g < gh.size(); ++g)
+        {
+            o_part[g] = take(gh[g].part.size() * 2);
+            o_pslot[g] = take(n * 4);
+            o_ppar[g] = take(gh[g].ppar.size() * 8);
+            o_exs[g] = take((n + 1) * 4);
+            o_ex[g] = take(std::max<size_t>(1, gh[g].ex.size()) * 4);
+            o_pairs[g] = take((size_t)r.globals[g].cap * 4);
+        }
+        if(r.arena) {hip_check(hipFree(r.arena), "hipFree");}
+        r.footprint = off;
+        if(hipMalloc(&r.arena, off) != hipSuccess) {(void)hipGetLastError(); r.arena = nullptr; throw std::bad_alloc();}
+        char* A = static_cast<char*>(r.arena);
+
+        // state and coefficients
+        const double kBT = kB_kcal() * r.T;
+        std::vector<double4> p(n), v(n), z(n, make_double4(0, 0, 0, 0));
+        std::vector<double> rm(n), c1(n), c2(n), c3(n);
+        for(size_t i = 0; i < n; ++i)
+        {
+            double x = r.pos[3 * i], y = r.pos[3 * i + 1], zz = r.pos[3 * i + 2];
+            adjust_position_host(r.box, x, y, zz);
+            p[i] = make_double4(x, y, zz, 0);
+            v[i] = make_double4(r.vel[3 * i], r.vel[3 * i + 1], r.vel[3 * i + 2], 0);
+            rm[i] = 1.0 / r.mass[i];
+            if(r.integ == MJG_INTEG_BAOAB)
+            {
+                const double gamma_dt = -1 * r.coefs[i] * r.dt;
+                c1[i] = std::exp(gamma_dt);
+                c2[i] = std::sqrt(kBT * (1 - std::exp(2 * gamma_dt)) * rm[i]);
+                c3[i] = 0;
+            }
+            else
+            {
+                const double alpha = r.coefs[i], m = r.mass[i];
+                c1[i] = 1.0 / (1.0 + alpha * r.dt * 0.5 / m);
+                c2[i] = std::sqrt(2 * alpha * kBT * r.dt);
+                c3[i] = 1.0 - alpha * c1[i] * r.dt / m;
+            }
+        }
+        RepState st{};
+        for(size_t g = 0; g < r.globals.size(); ++g) {st.margin[g] = -1.0;}
+        st.force_rebuild = 1;
+        auto up = [&](size_t o, const void* src, size_t bytes) {
+            if(bytes) {hip_check(hipMemcpyAsync(A + o, src, bytes, hipMemcpyHostToDevice, stream_), "upload");}
+        };
+        up(o_pos, p.data(), n * sizeof(double4));
+        up(o_vel, v.data(), n * sizeof(double4));
+        up(o_force, z.data(), n * sizeof(double4));
+        up(o_rm, rm.data(), n * 8); up(o_c1, c1.data(), n * 8); up(o_c2, c2.data(), n * 8); up(o_c3, c3.data(), n * 8);
+        up(o_mol, r.topo.mol.data(), n * 4);
+        up(o_terms, terms.data(), terms.size() * sizeof(RepTerm));
+        hip_check(hipMemcpyAsync(d_states_ + r.slot, &st, sizeof(RepState), hipMemcpyHostToDevice, stream_), "upload");
+
+        RepDesc& D = r.desc;
+        D = RepDesc{};
+        D.n = (uint32_t)n;
+        D.n_terms = (uint32_t)terms.size();
+        D.n_local = (int)r.locals.size();
+        D.ng = (int)r.globals.size();
+        D.integ = r.integ;
+        D.dt = r.dt;
+        D.halfdt = r.dt / 2;
+        D.seed = r.seed;
+        D.box = r.box;
+        D.pos = reinterpret_cast<double4*>(A + o_pos);
+        D.vel = reinterpret_cast<double4*>(A + o_vel);
+        D.force = reinterpret_cast<double4*>(A + o_force);
+        D.bak_pos = reinterpret_cast<double4*>(A + o_bpos);
+        D.bak_vel = reinterpret_cast<double4*>(A + o_bvel);
+        D.bak_force = reinterpret_cast<double4*>(A + o_bforce);
+        D.rmass = reinterpret_cast<const double*>(A + o_rm);
+        D.c1 = reinterpret_cast<const double*>(A + o_c1);
+        D.c2 = reinterpret_cast<const double*>(A + o_c2);
+        D.c3 = reinterpret_cast<const double*>(A + o_c3);
+        D.mol = reinterpret_cast<const uint32_t*>(A + o_mol);
+        D.terms = reinterpret_cast<const RepTerm*>(A + o_terms);
+        D.st = d_states_ + r.slot;
+        D.bak_st = reinterpret_cast<RepState*>(A + o_bst);
+        for(size_t g = 0; g < r.globals.size(); ++g)
+        {
+            RepGlobalIn& G = r.globals[g];
+            GHost& h = gh[g];
+            up(o_part[g], h.part.data(), h.part.size() * 2);
+            up(o_pslot[g], h.pslot.data(), n * 4);
+            up(o_ppar[g], h.ppar.data(), h.ppar.size() * 8);
+            up(o_exs[g], h.ex_start.data(), (n + 1) * 4);
+            up(o_ex[g], h.ex.data(), h.ex.size() * 4);
+            RepGlobal& RG = D.g[g];
+            RG = RepGlobal{};
+            RG.kind = G.pot;
+            RG.pairwise = (G.partition != MJG_PART_CELLLIST);
+            RG.use_distance = (G.partition != MJG_PART_NAIVE);
+            RG.never_rebuild = (G.partition == MJG_PART_NAIVE);
+            RG.ignore_mol = G.ignore_mol;
+            RG.np = (uint32_t)G.part.size();
+            RG.last_slot = RG.np - 1;
+            RG.cap = G.cap;
+            RG.part = reinterpret_cast<const uint16_t*>(A + o_part[g]);
+            RG.pslot = reinterpret_cast<const uint32_t*>(A + o_pslot[g]);
+            RG.ppar = reinterpret_cast<const double*>(A + o_ppar[g]);
+            RG.ex_start = reinterpret_cast<const uint32_t*>(A + o_exs[g]);
+            RG.ex = reinterpret_cast<const uint32_t*>(A + o_ex[g]);
+            RG.pairs = reinterpret_cast<uint32_t*>(A + o_pairs[g]);
+            double max_cutoff = 0;
+            if(G.pot == kPairLJ)
+            {
+                RG.cutoff_ratio2 = G.cutoff_ratio * G.cutoff_ratio;
+                RG.coef_at_cutoff = std::pow(1.0 / G.cutoff_ratio, 12) - std::pow(1.0 / G.cutoff_ratio, 6);
+                double ms = 0;
+                for(size_t i = 0; i < n; ++i) {ms = std::max(ms, h.ppar[2 * i]);}
+                max_cutoff = ms * G.cutoff_ratio;
+            }
+            else if(G.pot == kPairEV)
+            {
+                RG.cutoff_ratio2 = G.cutoff_ratio * G.cutoff_ratio;
+                RG.coef_at_cutoff = std::pow(1.0 / G.cutoff_ratio, 12);
+                RG.eps = G.eps;
+                double mr = 0;
+                for(size_t i = 0; i < n; ++i) {mr = std::max(mr, h.ppar[i]);}
+                max_cutoff = mr * 2 * G.cutoff_ratio;
+            }
+            else
+            {
+                if(!r.has_I) {throw std::invalid_argument("DebyeHuckel requires `temperature` and `ionic_strength`");}
+                const DHParams dh = debye_huckel_params(r.T, r.I, G.cutoff_ratio);
+                RG.debye = dh.debye; RG.inv_debye = dh.inv_debye; RG.abs_cutoff = dh.abs_cutoff;
+                RG.inv4pieps = dh.inv4pieps; RG.coef_at_cutoff = dh.coef_at_cutoff;
+                max_cutoff = dh.debye * G.cutoff_ratio;
+            }
+            const double r_list = max_cutoff * (1. + G.margin);
+            RG.r_list2 = r_list * r_list;
+            RG.abs_margin = max_cutoff * G.margin;
+        }
+        hip_check(hipStreamSynchronize(stream_), "upload sync");
+        r.ready = true;
+        descs_dirty_ = true;
+    }
+
+    static void adjust_position_host(const Box& b, double& x, double& y, double& z)
+    {
+        if(!b.periodic) {return;}
+        if(x < b.lower[0]) {x += b.width[0];} else if(x >= b.upper[0]) {x -= b.width[0];}
+        if(y < b.lower[1]) {y += b.width[1];} else if(y >= b.upper[1]) {y -= b.width[1];}
+        if(z < b.lower[2]) {z += b.width[2];} else if(z >= b.upper[2]) {z -= b.width[2];}
+    }
+
+    void upload_descs(const std::vector<int>& ids)
+    {
+        if(!descs_dirty_ && ids == last_ids_) {return;}
+        last_ids_ = ids;
+        descs_dirty_ = false;
+        // largest replicas first: they have the longest critical path
+        std::vector<int> order(ids);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {return reps_[a]->n > reps_[b]->n;});
+        std::vector<RepDesc> h;
+        max_n_ = 0;
+        for(int id : order) {h.push_back(reps_[id]->desc); max_n_ = std::max<uint32_t>(max_n_, reps_[id]->desc.n);}
+        if(h.size() > desc_cap_)
+        {
+            if(d_descs_) {hip_check(hipFree(d_descs_), "hipFree");}
+            desc_cap_ = std::max<size_t>(h.size(), 2 * desc_cap_);
+            hip_check(hipMalloc(&d_descs_, desc_cap_ * sizeof(RepDesc)), "hipMalloc");
+        }
+        hip_check(hipMemcpyAsync(d_descs_, h.data(), h.size() * sizeof(RepDesc), hipMemcpyHostToDevice, stream_), "upload");
+        n_descs_ = h.size();
+    }
+
+    // mode 0: step segment, 1: evaluate after a forced rebuild (set-up), 2: evaluate with energies
+    void launch_descs(const std::vector<int>&, int nsteps, int mode)
+    {
+        const size_t lds = (size_t)max_n_ * 48;
+        const dim3 grid((uint32_t)n_descs_), blk(kRepThreads);
+        const bool fp32 = (precision_ == MJG_FP32_FIXED);
+        if(mode == 2)
+        {
+            if(fp32) {hipLaunchKernelGGL((k_replica_segment<float, true>), grid, blk, lds, stream_, d_descs_, 0, 1);}
+            else {hipLaunchKernelGGL((k_replica_segment<double, true>), grid, blk, lds, stream_, d_descs_, 0, 1);}
+        }
+        else if(mode == 1)
+        {
+            if(fp32) {hipLaunchKernelGGL((k_replica_segment<float, false>), grid, blk, lds, stream_, d_descs_, 0, 1);}
+            else {hipLaunchKernelGGL((k_replica_segment<double, false>), grid, blk, lds, stream_, d_descs_, 0, 1);}
+        }
+        else
+        {
+            if(fp32) {hipLaunchKernelGGL((k_replica_segment<float, false>), grid, blk, lds, stream_, d_descs_, nsteps, 0);}
+            else {hipLaunchKernelGGL((k_replica_segment<double, false>), grid, blk, lds, stream_, d_descs_, nsteps, 0);}
+        }
+    }
+
+    void launch(const std::vector<int>& ids, int nsteps, int mode)
+    {
+        upload_descs(ids);
+        launch_descs(ids, nsteps, mode);
+        hip_check(hipGetLastError(), "replica launch");
+        hip_check(hipStreamSynchronize(stream_), "replica sync");
+    }
+
+    // after a step launch: grow the list capacities that overflowed and restart the replica from
+    // its segment-start copy (true if the replica must rerun the segment)
+    bool grow_if_overflow(Replica& r)
+    {
+        const RepState s = state_of(r);
+        bool over = false;
+        for(size_t g = 0; g < r.globals.size(); ++g)
+        {
+            if(s.maxpairs[g] > r.globals[g].cap)
+            {
+                r.globals[g].cap = s.maxpairs[g] + s.maxpairs[g] / 4 + 256;
+                over = true;
+            }
+        }
+        if(!over) {return false;}
+        std::vector<double> pos(3 * r.n), vel(3 * r.n), force(3 * r.n);
+        download_xyz(r, r.desc.bak_pos, pos);
+        download_xyz(r, r.desc.bak_vel, vel);
+        download_xyz(r, r.desc.bak_force, force);
+        RepState bak{};
+        hip_check(hipMemcpyAsync(&bak, r.desc.bak_st, sizeof(RepState), hipMemcpyDeviceToHost, stream_), "download");
+        hip_check(hipStreamSynchronize(stream_), "sync");
+        r.pos = pos;
+        r.vel = vel;
+        r.topo.reset(r.n);
+        r.ready = false;
+        int id = -1;
+        for(size_t k = 0; k < reps_.size(); ++k) {if(reps_[k].get() == &r) {id = (int)k;}}
+        prepare(id);
+        // positions are already wrapped; forces and counters are the segment start's
+        std::vector<double4> f(r.n);
+        for(size_t i = 0; i < r.n; ++i) {f[i] = make_double4(force[3 * i], force[3 * i + 1], force[3 * i + 2], 0.0);}
+        hip_check(hipMemcpyAsync(r.desc.force, f.data(), r.n * sizeof(double4), hipMemcpyHostToDevice, stream_), "upload");
+        bak.force_rebuild = 1;
+        bak.overflow = 0;
+        for(size_t g = 0; g < r.globals.size(); ++g) {bak.maxpairs[g] = 0;}
+        hip_check(hipMemcpyAsync(r.desc.st, &bak, sizeof(RepState), hipMemcpyHostToDevice, stream_), "upload");
+        hip_check(hipStreamSynchronize(stream_), "sync");
+        return true;
+    }
+
+    void download_xyz(Replica& r, const double4* d, std::vector<double>& out)
+    {
+        std::vector<double4> h(r.n);
+        hip_check(hipMemcpyAsync(h.data(), d, r.n * sizeof(double4), hipMemcpyDeviceToHost, stream_), "download");
+        hip_check(hipStreamSynchronize(stream_), "sync");
+        for(size_t i = 0; i < r.n; ++i) {out[3 * i] = h[i].x; out[3 * i + 1] = h[i].y; out[3 * i + 2] = h[i].z;}
+    }
+
+    RepState state_of(Replica& r)
+    {
+        RepState s{};
+        hip_check(hipMemcpyAsync(&s, r.desc.st, sizeof(RepState), hipMemcpyDeviceToHost, stream_), "download");
+        hip_check(hipStreamSynchronize(stream_), "sync");
+        return s;
+    }
+
+    void run_segment(const std::vector<int>& act, int seg)
+    {
+        launch(act, seg, 0);
+        download_states();
+        // a replica whose list overflowed reruns the segment from its segment-start copy
+        std::vector<int> again;
+        for(int id : act)
+        {
+            const RepState& s = h_states_[reps_[id]->slot];
+            if(s.nonfinite) {throw BatchNonFinite("non-finite position in a replica");}
+            if(s.overflow) {again.push_back(id);}
+        }
+        for(int pass = 0; pass < 6 && !again.empty(); ++pass)
+        {
+            for(int id : again) {grow_if_overflow(*reps_[id]);}
+            launch(again, seg, 0);
+            download_states();
+            std::vector<int> still;
+            for(int id : again) {if(h_states_[reps_[id]->slot].overflow) {still.push_back(id);}}
+            again.swap(still);
+        }
+        if(!again.empty()) {throw BatchDeviceError("replica list capacity did not converge");}
+    }
+
+    int precision_;
+    hipStream_t stream_ = nullptr;
+    std::vector<std::unique_ptr<Replica>> reps_;
+    RepDesc* d_descs_ = nullptr;
+    size_t desc_cap_ = 0, n_descs_ = 0;
+    uint32_t max_n_ = 0;
+    int segment_ = 100;
+    bool descs_dirty_ = true;
+    std::vector<int> last_ids_;
+    RepState* d_states_ = nullptr;
+    size_t states_cap_ = 0, n_slots_ = 0;
+    std::vector<RepState> h_states_;
+    std::vector<int> free_slots_;
+};
+
+} // mjg
+
+// ======================================================================================
+// C ABI
+// ======================================================================================
+struct mjg_batch
+{
+    mjg::Batch* impl = nullptr;
+    std::string err;
+};
+
+namespace
+{
+template<typename F>
+mjg_status bguard(mjg_batch* b, F&& f)
+{
+    if(b == nullptr || b->impl == nullptr) {return MJG_INVALID_ARG;}
+    try
+    {
+        f(*b->impl);
+        b->err.clear();
+        return MJG_OK;
+    }
+    catch(const mjg::BatchNonFinite& x) {b->err = x.what(); return MJG_NAN_DETECTED;}
+    catch(const mjg::BatchDeviceError& x) {b->err = x.what(); return MJG_DEVICE_ERROR;}
+    catch(const std::bad_alloc&) {b->err = "device or host allocation failed"; return MJG_OOM;}
+    catch(const std::logic_error& x)
+    {
+        b->err = x.what();
+        if(dynamic_cast<const std::invalid_argument*>(&x) || dynamic_cast<const std::out_of_range*>(&x))
+        {
+            return MJG_INVALID_ARG;
+        }
+        return MJG_NOT_INITIALIZED;
+    }
+    catch(const std::exception& x) {b->err = x.what(); return MJG_INVALID_ARG;}
+}
+} // namespace
+
+extern "C" {
+
+mjg_status mjg_batch_create(int precision, int device, mjg_batch** out)
+{
+    if(out == nullptr) {return MJG_INVALID_ARG;}
+    *out = nullptr;
+    mjg_batch* b = new(std::nothrow) mjg_batch;
+    if(b == nullptr) {return MJG_OOM;}
+    try {b->impl = new mjg::Batch(precision, device);}
+    catch(const std::bad_alloc&) {delete b; return MJG_OOM;}
+    catch(const mjg::BatchDeviceError&) {delete b; return MJG_DEVICE_ERROR;}
+    catch(const std::exception&) {delete b; return MJG_INVALID_ARG;}
+    *out = b;
+    return MJG_OK;
+}
+void mjg_batch_destroy(mjg_batch* b)
+{
+    if(!b) {return;}
+    delete b->impl;
+    delete b;
+}
+const char* mjg_batch_last_error(const mjg_batch* b) {return b ? b->err.c_str() : "null batch";}
+
+mjg_status mjg_batch_add_replica(mjg_batch* b, int boundary, const double* lower, const double* upper, size_t n,
+                                 const double* mass, const double* pos, const double* vel, int* replica)
+{
+    return bguard(b, [&](mjg::Batch& x) {
+        const int id = x.begin_replica(boundary, lower, upper, n, mass, pos, vel);
+        if(replica) {*replica = id;}
+    });
+}
+mjg_status mjg_batch_set_attribute(mjg_batch* b, int replica, const char* key, double value)
+{
+    if(!key) {return MJG_INVALID_ARG;}
+    return bguard(b, [&](mjg::Batch& x) {x.set_attribute(replica, key, value);});
+}
+mjg_status mjg_batch_add_local(mjg_batch* b, int replica, int kind, const char* topology, size_t n_terms,
+                               const uint32_t* indices, const double* params)
+{
+    return bguard(b, [&](mjg::Batch& x) {x.add_local(replica, kind, topology, n_terms, indices, params);});
+}
+mjg_status mjg_batch_add_global(mjg_batch* b, int replica, int potential, double cutoff_ratio, double epsilon,
+                                int partition, double margin, int ignore_molecule, size_t n_within,
+                                const char* const* within_kinds, const uint32_t* within_dist, size_t n_participants,
+                                const uint32_t* participants, const double* params)
+{
+    return bguard(b, [&](mjg::Batch& x) {
+        x.add_global(replica, potential, cutoff_ratio, epsilon, partition, margin, ignore_molecule, n_within,
+                     within_kinds, within_dist, n_participants, participants, params);
+    });
+}
+mjg_status mjg_batch_set_integrator(mjg_batch* b, int replica, int type, double dt, const double* coefs, uint64_t seed)
+{
+    return bguard(b, [&](mjg::Batch& x) {x.set_integrator(replica, type, dt, coefs, seed);});
+}
+mjg_status mjg_batch_initialize(mjg_batch* b) {return bguard(b, [&](mjg::Batch& x) {x.initialize();});}
+mjg_status mjg_batch_step(mjg_batch* b, size_t n_steps) {return bguard(b, [&](mjg::Batch& x) {x.step(n_steps);});}
+mjg_status mjg_batch_set_steps(mjg_batch* b, int replica, uint64_t steps)
+{
+    return bguard(b, [&](mjg::Batch& x) {x.set_steps(replica, steps);});
+}
+mjg_status mjg_batch_advance(mjg_batch* b, int segment) {return bguard(b, [&](mjg::Batch& x) {x.advance(segment);});}
+mjg_status mjg_batch_retire(mjg_batch* b, int replica) {return bguard(b, [&](mjg::Batch& x) {x.retire(replica);});}
+mjg_status mjg_batch_get_state(mjg_batch* b, int replica, double* pos, double* vel, double* force)
+{
+    return bguard(b, [&](mjg::Batch& x) {x.get_state(replica, pos, vel, force);});
+}
+mjg_status mjg_batch_replica_info(mjg_batch* b, int replica, uint64_t* step, uint64_t* remaining, uint64_t* rebuilds,
+                                  size_t* footprint_bytes)
+{
+    return bguard(b, [&](mjg::Batch& x) {
+        const mjg::RepState s = x.state(replica);
+        if(step) {*step = s.step;}
+        if(remaining) {*remaining = s.remaining;}
+        if(rebuilds) {*rebuilds = s.rebuilds;}
+        if(footprint_bytes) {*footprint_bytes = x.footprint(replica);}
+    });
+}
+mjg_status mjg_batch_evaluate(mjg_batch* b, int replica, double* per_interaction, double* total, double* force)
+{
+    return bguard(b, [&](mjg::Batch& x) {x.evaluate(replica, per_interaction, total, force);});
+}
+mjg_status mjg_batch_counters(mjg_batch* b, uint64_t* particle_steps, uint64_t* rebuilds)
+{
+    return bguard(b, [&](mjg::Batch& x) {x.counters(particle_steps, rebuilds);});
+}
+mjg_status mjg_batch_profile(mjg_batch* b, int segment_steps, int reps, double* avg_ms, uint64_t* particle_steps)
+{
+    return bguard(b, [&](mjg::Batch& x) {
+        const double v = x.profile(segment_steps, reps, particle_steps);
+        if(avg_ms) {*avg_ms = v;}
+    });
+}
+
+} // extern "C"

@@ -14,6 +14,13 @@ namespace mjg
 
 constexpr int kWave = 64;
 
+// interaction kinds (the values of the C ABI's MJG_* enums)
+enum LocalKind : int
+{
+    kBondHarmonic = 1, kBondGo = 2, kContactGo = 3, kAngleHarmonic = 4, kAngleGaussian = 5, kDihedralClementi = 6
+};
+enum PairKind : int {kPairLJ = 1, kPairEV = 2, kPairDH = 3};
+
 struct Box
 {
     double lower[3], upper[3], width[3], halfw[3];

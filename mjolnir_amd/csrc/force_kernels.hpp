@@ -16,11 +16,6 @@
 namespace mjg
 {
 
-enum LocalKind : int
-{
-    kBondHarmonic = 1, kBondGo = 2, kContactGo = 3, kAngleHarmonic = 4, kAngleGaussian = 5, kDihedralClementi = 6
-};
-enum PairKind : int {kPairLJ = 1, kPairEV = 2, kPairDH = 3};
 
 struct LocalSeg
 {
