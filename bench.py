@@ -1,14 +1,18 @@
 """Benchmark of the force-and-integrate step on one MI355X.
 
-``python bench.py [--steps K] [--warmup W] [--workload NAME]`` prints one JSON line:
+``python bench.py [--steps K] [--warmup W] [--workload NAME]`` prints one JSON line: whole-job
 particle-steps/s of the timed K steps (inputs resident in HBM), the roofline position of the
-dominant force kernel measured live with HIP events, and the reference CPU build timed on this
-host's cores on a bounded sample of the same workload.
+dominant kernel measured live with HIP events, and the reference CPU build timed on this host's
+cores on a bounded sample of the same workload.
 
 Workloads (seeded synthetic inputs, ``mjolnir_amd.workloads``):
-  chain1000  protein-like CG chain, N = 1000, FP64 (bonds, Gaussian angles, Clementi dihedrals,
-             Go contacts, excluded volume, Debye-Hueckel; BAOAB gamma = 0.1, dt = 0.01)
-  lj4096     LJ fluid, N = 4096, rho sigma^3 = 0.6, FP64 (BAOAB gamma = 0.1, dt = 0.01)
+  replicas512  (default) 512 independent CG chains, lengths log-uniform in [50, 2000] beads,
+               temperatures uniform in [280, 400] K (bonds, Gaussian angles, Clementi dihedrals,
+               Go contacts, excluded volume, Debye-Hueckel; BAOAB gamma = 0.1, dt = 0.01), run
+               as one batch: FP32 pair/local arithmetic, int64 fixed-point force sums, FP64
+               integrator, one persistent launch per 100-step segment
+  chain1000    one CG chain, N = 1000, FP64 engine
+  lj4096       LJ fluid, N = 4096, rho sigma^3 = 0.6, FP64 engine
 """
 from __future__ import annotations
 
@@ -24,21 +28,25 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def make_workload(name: str):
-    from mjolnir_amd import workloads
-    if name == "chain1000":
-        return workloads.cg_chain(n=1000, seed=2)
-    if name == "lj4096":
-        return workloads.lj_fluid(n_side=16, seed=1)
-    raise SystemExit(f"unknown workload {name}")
+def host_cores() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
 
 
-def cpu_baseline(spec, target_s: float = 12.0):
+def _stratified(specs, k):
+    order = sorted(range(len(specs)), key=lambda i: specs[i].n)
+    step = max(1, len(order) // k)
+    return [specs[order[i]] for i in range(step // 2, len(order), step)][:k]
+
+
+# ------------------------------------------------------------------------------------------
+# single system (FP64 engine)
+# ------------------------------------------------------------------------------------------
+def cpu_baseline_single(spec, target_s: float = 12.0):
     """The reference's OpenMP build (oracle/_ref/ref_driver_omp) on this host: a short probe,
     then a run sized to about target_s seconds of CPU work."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import refrun
-    threads = os.cpu_count() or 1
+    threads = host_cores()
     if not refrun.available(omp=True):
         return None
     probe = refrun.bench(spec, 50, threads=threads)
@@ -50,22 +58,8 @@ def cpu_baseline(spec, target_s: float = 12.0):
                       f"(OMP_NUM_THREADS={r['threads']}), step loop only, {r['seconds']:.1f} s"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
-    ap.add_argument("--warmup", type=int, default=500)
-    ap.add_argument("--workload", default="chain1000")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (replicas and jobs "
-              "shard across GPUs with no exchange; see the design notes)", file=sys.stderr)
-        sys.exit(2)
-
+def bench_single(args, spec):
     from mjolnir_amd import capi
-    spec = make_workload(args.workload)
-    print(f"[bench] workload {spec.name} N={spec.n}", file=sys.stderr, flush=True)
     eng = capi.Engine(spec)
     eng.step(args.warmup)
     s0, r0 = eng.counters()
@@ -94,9 +88,8 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
                 "kernel": f"k_pair_full<{g.potential}>", "avg_us": round(ms * 1e3, 3),
                 "half_pairs": hp, "participants": np_}
-
-    cpu = None if args.no_cpu_baseline else cpu_baseline(spec)
-    out = {
+    cpu = None if args.no_cpu_baseline else cpu_baseline_single(spec)
+    return {
         "metric": "particle-steps/s", "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec * 1e3 / args.steps, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
@@ -106,7 +99,133 @@ def main():
                    "rebuilds_in_timed_region": int(r1 - r0)},
         "roofline": roofline, "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+
+
+# ------------------------------------------------------------------------------------------
+# replica batch (headline)
+# ------------------------------------------------------------------------------------------
+SEGMENT = 100
+
+
+def batch_alg_bytes(b, ids, specs, steps):
+    """Algorithmic bytes of one segment launch of `steps` steps over the batch.
+
+    Per step and replica: 36 B per local term (8 B packed indices, 4 B kind, 24 B parameters),
+    4 B per neighbour-list entry, 24 B per particle (inverse mass and two integrator
+    coefficients).  Per launch: 288 B per particle (FP64 position, velocity, force read as
+    double4, written back, and copied once as the segment-start backup)."""
+    total = 0
+    for r, s in zip(ids, specs):
+        n_terms = sum(len(l.indices) for l in s.local)
+        entries = int(sum(b.list_sizes(r)))
+        total += steps * (36 * n_terms + 4 * entries + 24 * s.n) + 288 * s.n
+    return total
+
+
+def cpu_baseline_batch(specs, target_s: float = 15.0):
+    """The reference's OpenMP build, one single-threaded process per replica and one process per
+    core, on a stratified sample of the ensemble; steps per replica sized to about target_s s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refrun
+    if not refrun.available(omp=True):
+        return None
+    procs = host_cores()
+    # largest first, so the longest runs do not start last
+    sample = sorted(_stratified(specs, 2 * procs), key=lambda s: -s.n)
+    # a short pass over the whole sample sizes the timed one
+    tot_n = sum(s.n for s in sample)
+    _, wall0, res0 = refrun.bench_many(sample, [200] * len(sample), procs)
+    loop0 = max(r["seconds"] for r in res0)
+    steps = max(200, int(200 * target_s / max(wall0, loop0, 1e-3)))
+    ps, wall, _ = refrun.bench_many(sample, [steps] * len(sample), procs)
+    return {"value": round(ps / wall, 1), "unit": "particle-steps/s", "cores": procs, "kind": "reference",
+            "sample": f"{len(sample)} replicas of the ensemble (stratified by length, {tot_n} particles), "
+                      f"{steps} BAOAB steps each on the reference OpenMP build with OMP_NUM_THREADS=1, "
+                      f"{procs} processes at a time, step loops only, {wall:.1f} s wall"}
+
+
+def fp64_engine_rate(specs, steps: int = 300):
+    """The plain FP64 engine run replica by replica on a stratified sample (particle-steps/s)."""
+    from mjolnir_amd import capi
+    tps, tt = 0, 0.0
+    for s in _stratified(specs, 12):
+        e = capi.Engine(s, capi.FP64)
+        e.step(50)
+        t = time.perf_counter()
+        e.step(steps)
+        tt += time.perf_counter() - t
+        tps += s.n * steps
+        e.close()
+    return tps / tt
+
+
+def bench_batch(args):
+    from mjolnir_amd import capi, workloads
+    t = time.perf_counter()
+    specs = workloads.replica_ensemble(512)
+    print(f"[bench] replicas512: {len(specs)} replicas, {sum(s.n for s in specs)} particles "
+          f"(generated in {time.perf_counter() - t:.1f} s)", file=sys.stderr, flush=True)
+    b = capi.Batch(capi.FP32_FIXED)
+    ids = [b.add(s) for s in specs]
+    b.initialize()
+    b.step(args.warmup)
+    ps0, rb0 = b.counters()
+    t0 = time.perf_counter()
+    b.step(args.steps)             # returns after the device work is complete
+    t1 = time.perf_counter()
+    ps1, rb1 = b.counters()
+    sec = t1 - t0
+    value = (ps1 - ps0) / sec
+    print(f"[bench] timed {args.steps} steps in {sec:.3f} s", file=sys.stderr, flush=True)
+
+    # dominant kernel: the segment kernel (all size classes of one launch), HIP events
+    ms, ps_launch = b.profile(SEGMENT, reps=5)
+    alg = batch_alg_bytes(b, ids, specs, SEGMENT)
+    achieved = alg / (ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                "kernel": "k_replica_segment<float> (3 size classes, one launch each per segment)",
+                "avg_ms": round(ms, 4), "segment_steps": SEGMENT, "alg_bytes_per_launch": int(alg),
+                "kernel_particle_steps_per_s": round(ps_launch / (ms * 1e-3), 1)}
+    fp64 = fp64_engine_rate(specs)
+    cpu = None if args.no_cpu_baseline else cpu_baseline_batch(specs)
+    return {
+        "metric": "particle-steps/s", "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec * 1e3 / args.steps, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+i64fixed+f64",
+        "data": "synthetic",
+        "config": {"workload": "replicas512", "replicas": len(specs), "particles": int(sum(s.n for s in specs)),
+                   "lengths": "log-uniform [50, 2000]", "temperatures": "uniform [280, 400] K",
+                   "precision": "fp32 pair/local, int64 fixed-point sums, fp64 integrator",
+                   "integrator": "BAOABLangevin", "dt": 0.01, "segment_steps": SEGMENT,
+                   "rebuilds_in_timed_region": int(rb1 - rb0)},
+        "fp64_engine": {"value": round(fp64, 1), "unit": "particle-steps/s",
+                        "sample": "12 replicas (stratified by length), FP64 engine one at a time",
+                        "speedup_of_batch": round(value / fp64, 2)},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--workload", default="replicas512", choices=["replicas512", "chain1000", "lj4096"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (replicas and jobs "
+              "shard across GPUs with no exchange)", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "replicas512":
+        out = bench_batch(args)
+    else:
+        from mjolnir_amd import workloads
+        spec = workloads.cg_chain(n=1000, seed=2) if args.workload == "chain1000" else workloads.lj_fluid(n_side=16, seed=1)
+        print(f"[bench] workload {spec.name} N={spec.n}", file=sys.stderr, flush=True)
+        out = bench_single(args, spec)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

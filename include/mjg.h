@@ -201,6 +201,10 @@ mjg_status mjg_batch_get_state(mjg_batch* b, int replica, double* pos, double* v
 /* Steps taken, steps remaining, list rebuilds and device bytes of one replica. */
 mjg_status mjg_batch_replica_info(mjg_batch* b, int replica, uint64_t* step, uint64_t* remaining, uint64_t* rebuilds,
                                   size_t* footprint_bytes);
+/* Current neighbour-list entries of each global interaction of one replica, in the order they
+ * were added (the size of SpatialPartitionBase::neighbors(), SpatialPartitionBase.hpp:138-139);
+ * writes min(max_globals, number of globals) counts. */
+mjg_status mjg_batch_list_sizes(mjg_batch* b, int replica, uint64_t* entries, int max_globals);
 /* Forces and per-interaction energies of one replica at its current positions
  * (calc_force_and_energy / format_energy, ForceField.hpp:126-161). */
 mjg_status mjg_batch_evaluate(mjg_batch* b, int replica, double* per_interaction, double* total, double* force);

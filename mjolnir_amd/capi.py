@@ -58,7 +58,7 @@ EXPORTED = [
     "mjg_batch_create", "mjg_batch_destroy", "mjg_batch_last_error", "mjg_batch_add_replica",
     "mjg_batch_set_attribute", "mjg_batch_add_local", "mjg_batch_add_global", "mjg_batch_set_integrator",
     "mjg_batch_initialize", "mjg_batch_step", "mjg_batch_set_steps", "mjg_batch_advance", "mjg_batch_retire",
-    "mjg_batch_get_state", "mjg_batch_replica_info", "mjg_batch_evaluate", "mjg_batch_counters",
+    "mjg_batch_get_state", "mjg_batch_replica_info", "mjg_batch_list_sizes", "mjg_batch_evaluate", "mjg_batch_counters",
     "mjg_batch_profile",
 ]
 
@@ -117,6 +117,7 @@ def lib():
     L.mjg_batch_replica_info.argtypes = [vp, i32, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), C.POINTER(sz)]
     L.mjg_batch_evaluate.argtypes = [vp, i32, vp, C.POINTER(dbl), vp]
     L.mjg_batch_counters.argtypes = [vp, C.POINTER(u64), C.POINTER(u64)]
+    L.mjg_batch_list_sizes.argtypes = [vp, i32, vp, i32]
     L.mjg_batch_profile.argtypes = [vp, i32, i32, C.POINTER(dbl), C.POINTER(u64)]
     _LIB = L
     return L
@@ -326,6 +327,12 @@ class Batch:
         s, rem, rb, fp = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_size_t()
         self._check(lib().mjg_batch_replica_info(self.h, r, C.byref(s), C.byref(rem), C.byref(rb), C.byref(fp)))
         return {"step": s.value, "remaining": rem.value, "rebuilds": rb.value, "footprint": fp.value}
+
+    def list_sizes(self, r: int):
+        ng = len(self.specs[r].global_)
+        out = np.zeros(max(1, ng), dtype=np.uint64)
+        self._check(lib().mjg_batch_list_sizes(self.h, r, out.ctypes.data, ng))
+        return out[:ng]
 
     def evaluate(self, r: int):
         spec = self.specs[r]

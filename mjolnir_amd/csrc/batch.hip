@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -59,6 +60,8 @@ struct Replica
     size_t footprint = 0;
     RepDesc desc{};
     int slot = -1;                 // index into the batch's state array
+    size_t lds = 0;                // dynamic LDS of its workgroup
+    int cls = 0;                   // size class (block shape)
     bool ready = false, active = false;
 };
 
@@ -75,27 +78,28 @@ class Batch
     {
         if(precision != MJG_FP32_FIXED && precision != MJG_FP64) {throw std::invalid_argument("unknown precision");}
         hip_check(hipSetDevice(device), "hipSetDevice");
-        hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+        for(int c = 0; c < kRepClasses; ++c)
+        {
+            hip_check(hipStreamCreateWithFlags(&cls_stream_[c], hipStreamNonBlocking), "hipStreamCreate");
+        }
+        stream_ = cls_stream_[0];
+        hip_check(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming), "hipEventCreate");
+        for(int c = 0; c < kRepClasses; ++c) {hip_check(hipEventCreateWithFlags(&ev_join_[c], hipEventDisableTiming), "hipEventCreate");}
         const char* seg = std::getenv("MJG_SEGMENT");
         if(seg) {segment_ = std::max(1, std::atoi(seg));}
-        // the segment kernel keeps 48 B per particle in dynamic LDS (up to 96 KiB)
-        const int lds = (int)(kRepMaxN * 48);
-        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_replica_segment<float, true>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds), "lds attribute");
-        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_replica_segment<float, false>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds), "lds attribute");
-        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_replica_segment<double, true>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds), "lds attribute");
-        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_replica_segment<double, false>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds), "lds attribute");
+        trace_ = std::getenv("MJG_BATCH_TRACE") != nullptr;
+        set_lds_limits<float>();
+        set_lds_limits<double>();
     }
     ~Batch()
     {
-        if(stream_) {(void)hipStreamSynchronize(stream_);}
+        for(int c = 0; c < kRepClasses; ++c) {if(cls_stream_[c]) {(void)hipStreamSynchronize(cls_stream_[c]);}}
         for(auto& r : reps_) {if(r && r->arena) {(void)hipFree(r->arena);}}
         if(d_descs_) {(void)hipFree(d_descs_);}
         if(d_states_) {(void)hipFree(d_states_);}
-        if(stream_) {(void)hipStreamDestroy(stream_);}
+        if(ev_fork_) {(void)hipEventDestroy(ev_fork_);}
+        for(int c = 0; c < kRepClasses; ++c) {if(ev_join_[c]) {(void)hipEventDestroy(ev_join_[c]);}}
+        for(int c = 0; c < kRepClasses; ++c) {if(cls_stream_[c]) {(void)hipStreamDestroy(cls_stream_[c]);}}
     }
 
     int begin_replica(int boundary, const double* lower, const double* upper, size_t n, const double* mass,
@@ -230,7 +234,7 @@ class Batch
                 bool over = false;
                 for(size_t g = 0; g < r.globals.size(); ++g)
                 {
-                    if(st.maxpairs[g] > r.globals[g].cap) {r.globals[g].cap = st.maxpairs[g] + st.maxpairs[g] / 4 + 256; over = true;}
+                    if(st.maxpairs[g] > r.globals[g].cap) {r.globals[g].cap = grown_cap(st.maxpairs[g]); over = true;}
                 }
                 if(over) {r.topo.reset(r.n); r.ready = false; prepare(id); again.push_back(id);}
             }
@@ -316,6 +320,7 @@ class Batch
 
     size_t footprint(int id) {return rep(id).footprint;}
     size_t num_interactions(int id) {Replica& r = rep(id); return r.locals.size() + r.globals.size();}
+    int num_globals(int id) {return (int)rep(id).globals.size();}
 
     void counters(uint64_t* steps, uint64_t* rebuilds)
     {
@@ -346,7 +351,7 @@ class Batch
         hip_check(hipEventCreate(&a), "event");
         hip_check(hipEventCreate(&b), "event");
         hip_check(hipEventRecord(a, stream_), "record");
-        for(int k = 0; k < reps; ++k) {launch_descs(act, nsteps, 0);}
+        for(int k = 0; k < reps; ++k) {launch_descs(nsteps, 0);}
         hip_check(hipEventRecord(b, stream_), "record");
         hip_check(hipEventSynchronize(b), "event sync");
         hip_check(hipGetLastError(), "profile launch");
@@ -428,20 +433,26 @@ class Batch
         r.topo.construct_molecules();
         const size_t n = r.n;
 
-        // local terms, one record per term, grouped by interaction
-        std::vector<RepTerm> terms;
-        for(size_t l = 0; l < r.locals.size(); ++l)
+        // local terms, grouped by interaction, as structure of arrays
+        size_t n_terms = 0;
+        for(const RepLocal& L : r.locals) {n_terms += L.idx.size() / L.arity;}
+        std::vector<unsigned long long> t_idx(n_terms, 0ull);
+        std::vector<uint32_t> t_kind(n_terms, 0u);
+        std::vector<double> t_par(3 * n_terms, 0.0);
         {
-            const RepLocal& L = r.locals[l];
-            const size_t nt = L.idx.size() / L.arity;
-            for(size_t t = 0; t < nt; ++t)
+            size_t t = 0;
+            for(size_t l = 0; l < r.locals.size(); ++l)
             {
-                RepTerm T{};
-                T.kind = (uint16_t)L.kind;
-                T.inter = (uint16_t)l;
-                for(int a = 0; a < L.arity; ++a) {T.idx[a] = (uint16_t)L.idx[t * L.arity + a];}
-                for(int c = 0; c < L.npar; ++c) {T.p[c] = L.par[t * L.npar + c];}
-                terms.push_back(T);
+                const RepLocal& L = r.locals[l];
+                const size_t nt = L.idx.size() / L.arity;
+                for(size_t q = 0; q < nt; ++q, ++t)
+                {
+                    unsigned long long ix = 0;
+                    for(int a = 0; a < L.arity; ++a) {ix |= (unsigned long long)L.idx[q * L.arity + a] << (16 * a);}
+                    t_idx[t] = ix;
+                    t_kind[t] = (uint32_t)L.kind | ((uint32_t)l << 8);
+                    for(int c = 0; c < L.npar; ++c) {t_par[c * n_terms + t] = L.par[q * L.npar + c];}
+                }
             }
         }
 
@@ -480,7 +491,8 @@ class Batch
         const size_t o_pos = take(n * sizeof(double4)), o_vel = take(n * sizeof(double4)), o_force = take(n * sizeof(double4));
         const size_t o_bpos = take(n * sizeof(double4)), o_bvel = take(n * sizeof(double4)), o_bforce = take(n * sizeof(double4));
         const size_t o_rm = take(n * 8), o_c1 = take(n * 8), o_c2 = take(n * 8), o_c3 = take(n * 8);
-        const size_t o_mol = take(n * 4), o_terms = take(terms.size() * sizeof(RepTerm));
+        const size_t o_mol = take(n * 4), o_tidx = take(n_terms * 8), o_tkind = take(n_terms * 4);
+        const size_t o_tpar = take(n_terms * 24);
         const size_t o_bst = take(sizeof(RepState));
         if(r.slot < 0) {r.slot = take_slot();}
         std::vector<size_t> o_part(gh.size()), o_pslot(gh.size()), o_ppar(gh.size()), o_exs(gh.size()), o_ex(gh.size()),
@@ -536,13 +548,15 @@ class Batch
         up(o_force, z.data(), n * sizeof(double4));
         up(o_rm, rm.data(), n * 8); up(o_c1, c1.data(), n * 8); up(o_c2, c2.data(), n * 8); up(o_c3, c3.data(), n * 8);
         up(o_mol, r.topo.mol.data(), n * 4);
-        up(o_terms, terms.data(), terms.size() * sizeof(RepTerm));
+        up(o_tidx, t_idx.data(), n_terms * 8);
+        up(o_tkind, t_kind.data(), n_terms * 4);
+        up(o_tpar, t_par.data(), n_terms * 24);
         hip_check(hipMemcpyAsync(d_states_ + r.slot, &st, sizeof(RepState), hipMemcpyHostToDevice, stream_), "upload");
 
         RepDesc& D = r.desc;
         D = RepDesc{};
         D.n = (uint32_t)n;
-        D.n_terms = (uint32_t)terms.size();
+        D.n_terms = (uint32_t)n_terms;
         D.n_local = (int)r.locals.size();
         D.ng = (int)r.globals.size();
         D.integ = r.integ;
@@ -561,7 +575,9 @@ class Batch
         D.c2 = reinterpret_cast<const double*>(A + o_c2);
         D.c3 = reinterpret_cast<const double*>(A + o_c3);
         D.mol = reinterpret_cast<const uint32_t*>(A + o_mol);
-        D.terms = reinterpret_cast<const RepTerm*>(A + o_terms);
+        D.t_idx = reinterpret_cast<const unsigned long long*>(A + o_tidx);
+        D.t_kind = reinterpret_cast<const uint32_t*>(A + o_tkind);
+        D.t_par = reinterpret_cast<const double*>(A + o_tpar);
         D.st = d_states_ + r.slot;
         D.bak_st = reinterpret_cast<RepState*>(A + o_bst);
         for(size_t g = 0; g < r.globals.size(); ++g)
@@ -582,6 +598,12 @@ class Batch
             RG.ignore_mol = G.ignore_mol;
             RG.np = (uint32_t)G.part.size();
             RG.last_slot = RG.np - 1;
+            RG.sorted = std::is_sorted(G.part.begin(), G.part.end()) ? 1 : 0;
+            RG.npar = G.npar;
+            RG.par_col = D.par_cols;
+            RG.part_off = D.part_total;
+            D.par_cols += (uint32_t)G.npar;
+            D.part_total += (RG.np + 7u) & ~7u;
             RG.cap = G.cap;
             RG.part = reinterpret_cast<const uint16_t*>(A + o_part[g]);
             RG.pslot = reinterpret_cast<const uint32_t*>(A + o_pslot[g]);
@@ -619,6 +641,14 @@ class Batch
             RG.r_list2 = r_list * r_list;
             RG.abs_margin = max_cutoff * G.margin;
         }
+        r.lds = rep_lds_bytes(D.n, D.par_cols, D.part_total, real_size());
+        if(r.lds > kRepLdsLimit)
+        {
+            throw std::invalid_argument("replica needs " + std::to_string(r.lds) + " B of LDS (limit " +
+                                        std::to_string(kRepLdsLimit) + ")");
+        }
+        r.cls = 0;
+        while(r.n > kRepClassMaxN[r.cls]) {++r.cls;}
         hip_check(hipStreamSynchronize(stream_), "upload sync");
         r.ready = true;
         descs_dirty_ = true;
@@ -632,17 +662,32 @@ class Batch
         if(z < b.lower[2]) {z += b.width[2];} else if(z >= b.upper[2]) {z -= b.width[2];}
     }
 
+    // Upload the descriptors of `ids` grouped by size class, largest replicas first within a
+    // class (they have the longest critical path).
     void upload_descs(const std::vector<int>& ids)
     {
         if(!descs_dirty_ && ids == last_ids_) {return;}
         last_ids_ = ids;
         descs_dirty_ = false;
-        // largest replicas first: they have the longest critical path
         std::vector<int> order(ids);
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {return reps_[a]->n > reps_[b]->n;});
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            const Replica& x = *reps_[a];
+            const Replica& y = *reps_[b];
+            if(x.cls != y.cls) {return x.cls > y.cls;}
+            return x.n > y.n;
+        });
         std::vector<RepDesc> h;
-        max_n_ = 0;
-        for(int id : order) {h.push_back(reps_[id]->desc); max_n_ = std::max<uint32_t>(max_n_, reps_[id]->desc.n);}
+        for(int c = 0; c < kRepClasses; ++c) {grp_count_[c] = 0; grp_lds_[c] = 0;}
+        for(int id : order)
+        {
+            const Replica& r = *reps_[id];
+            h.push_back(r.desc);
+            grp_count_[r.cls] += 1;
+            grp_lds_[r.cls] = std::max(grp_lds_[r.cls], r.lds);
+        }
+        // groups are stored class 2, 1, 0
+        size_t off = 0;
+        for(int c = kRepClasses - 1; c >= 0; --c) {grp_first_[c] = off; off += grp_count_[c];}
         if(h.size() > desc_cap_)
         {
             if(d_descs_) {hip_check(hipFree(d_descs_), "hipFree");}
@@ -650,36 +695,83 @@ class Batch
             hip_check(hipMalloc(&d_descs_, desc_cap_ * sizeof(RepDesc)), "hipMalloc");
         }
         hip_check(hipMemcpyAsync(d_descs_, h.data(), h.size() * sizeof(RepDesc), hipMemcpyHostToDevice, stream_), "upload");
-        n_descs_ = h.size();
+        hip_check(hipStreamSynchronize(stream_), "upload sync");
     }
 
-    // mode 0: step segment, 1: evaluate after a forced rebuild (set-up), 2: evaluate with energies
-    void launch_descs(const std::vector<int>&, int nsteps, int mode)
+    size_t real_size() const {return precision_ == MJG_FP32_FIXED ? sizeof(float) : sizeof(double);}
+
+    // list capacity after an overflow at `seen` entries
+    static uint32_t grown_cap(uint32_t seen) {return seen + seen / 2 + 1024;}
+
+    template<typename Real, bool ENERGY, int BS, int PPT>
+    void set_lds_limit()
     {
-        const size_t lds = (size_t)max_n_ * 48;
-        const dim3 grid((uint32_t)n_descs_), blk(kRepThreads);
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_replica_segment<Real, ENERGY, BS, PPT>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRepLdsLimit),
+                  "LDS attribute");
+    }
+    // block shape of each size class: threads and particles per thread (the FP64 kernel needs
+    // more registers per thread, so its largest class runs 512 threads with 4 particles each)
+    template<typename Real, int C> struct Shape
+    {
+        static constexpr int bs = C == 0 ? 256 : (C == 1 || sizeof(Real) == 8) ? 512 : 1024;
+        static constexpr int ppt = (int)(kRepClassMaxN[C] / bs);
+    };
+    template<typename Real, int C>
+    void set_lds_limits_class()
+    {
+        set_lds_limit<Real, false, Shape<Real, C>::bs, Shape<Real, C>::ppt>();
+        set_lds_limit<Real, true, Shape<Real, C>::bs, Shape<Real, C>::ppt>();
+    }
+    template<typename Real>
+    void set_lds_limits()
+    {
+        set_lds_limits_class<Real, 0>();
+        set_lds_limits_class<Real, 1>();
+        set_lds_limits_class<Real, 2>();
+    }
+
+    template<typename Real, bool ENERGY, int C>
+    void launch_shape(int nsteps, int eval_only)
+    {
+        constexpr int BS = Shape<Real, C>::bs, PPT = Shape<Real, C>::ppt;
+        static_assert(BS * PPT == (int)kRepClassMaxN[C], "class shape must cover its size");
+        hipLaunchKernelGGL((k_replica_segment<Real, ENERGY, BS, PPT>), dim3((uint32_t)grp_count_[C]), dim3(BS),
+                           grp_lds_[C], cls_stream_[C], d_descs_ + grp_first_[C], nsteps, eval_only);
+    }
+    template<typename Real, bool ENERGY>
+    void launch_class(int c, int nsteps, int eval_only)
+    {
+        if(c == 0) {launch_shape<Real, ENERGY, 0>(nsteps, eval_only);}
+        else if(c == 1) {launch_shape<Real, ENERGY, 1>(nsteps, eval_only);}
+        else {launch_shape<Real, ENERGY, 2>(nsteps, eval_only);}
+    }
+
+    // mode 0: step segment, 1: evaluate after a forced rebuild (set-up), 2: evaluate with energies.
+    // Each size class runs on its own stream; the classes fork from and join into stream_.
+    void launch_descs(int nsteps, int mode)
+    {
         const bool fp32 = (precision_ == MJG_FP32_FIXED);
-        if(mode == 2)
+        hip_check(hipEventRecord(ev_fork_, stream_), "record");
+        for(int c = kRepClasses - 1; c >= 0; --c)
         {
-            if(fp32) {hipLaunchKernelGGL((k_replica_segment<float, true>), grid, blk, lds, stream_, d_descs_, 0, 1);}
-            else {hipLaunchKernelGGL((k_replica_segment<double, true>), grid, blk, lds, stream_, d_descs_, 0, 1);}
-        }
-        else if(mode == 1)
-        {
-            if(fp32) {hipLaunchKernelGGL((k_replica_segment<float, false>), grid, blk, lds, stream_, d_descs_, 0, 1);}
-            else {hipLaunchKernelGGL((k_replica_segment<double, false>), grid, blk, lds, stream_, d_descs_, 0, 1);}
-        }
-        else
-        {
-            if(fp32) {hipLaunchKernelGGL((k_replica_segment<float, false>), grid, blk, lds, stream_, d_descs_, nsteps, 0);}
-            else {hipLaunchKernelGGL((k_replica_segment<double, false>), grid, blk, lds, stream_, d_descs_, nsteps, 0);}
+            if(grp_count_[c] == 0) {continue;}
+            if(c != 0) {hip_check(hipStreamWaitEvent(cls_stream_[c], ev_fork_, 0), "wait");}
+            const int ns = (mode == 0) ? nsteps : 0, ev = (mode == 0) ? 0 : 1;
+            if(mode == 2) {if(fp32) {launch_class<float, true>(c, ns, ev);} else {launch_class<double, true>(c, ns, ev);}}
+            else {if(fp32) {launch_class<float, false>(c, ns, ev);} else {launch_class<double, false>(c, ns, ev);}}
+            if(c != 0)
+            {
+                hip_check(hipEventRecord(ev_join_[c], cls_stream_[c]), "record");
+                hip_check(hipStreamWaitEvent(stream_, ev_join_[c], 0), "wait");
+            }
         }
     }
 
     void launch(const std::vector<int>& ids, int nsteps, int mode)
     {
         upload_descs(ids);
-        launch_descs(ids, nsteps, mode);
+        launch_descs(nsteps, mode);
         hip_check(hipGetLastError(), "replica launch");
         hip_check(hipStreamSynchronize(stream_), "replica sync");
     }
@@ -694,7 +786,7 @@ class Batch
         {
             if(s.maxpairs[g] > r.globals[g].cap)
             {
-                r.globals[g].cap = s.maxpairs[g] + s.maxpairs[g] / 4 + 256;
+                r.globals[g].cap = grown_cap(s.maxpairs[g]);
                 over = true;
             }
         }
@@ -755,6 +847,7 @@ class Batch
         }
         for(int pass = 0; pass < 6 && !again.empty(); ++pass)
         {
+            if(trace_) {std::fprintf(stderr, "[mjg batch] %zu replica(s) overflowed a list; rerunning the segment\n", again.size());}
             for(int id : again) {grow_if_overflow(*reps_[id]);}
             launch(again, seg, 0);
             download_states();
@@ -766,12 +859,15 @@ class Batch
     }
 
     int precision_;
-    hipStream_t stream_ = nullptr;
+    hipStream_t stream_ = nullptr;                 // class 0's stream; set-up copies and joins
+    hipStream_t cls_stream_[kRepClasses] = {};
+    hipEvent_t ev_fork_ = nullptr, ev_join_[kRepClasses] = {};
+    size_t grp_count_[kRepClasses] = {}, grp_first_[kRepClasses] = {}, grp_lds_[kRepClasses] = {};
     std::vector<std::unique_ptr<Replica>> reps_;
     RepDesc* d_descs_ = nullptr;
-    size_t desc_cap_ = 0, n_descs_ = 0;
-    uint32_t max_n_ = 0;
+    size_t desc_cap_ = 0;
     int segment_ = 100;
+    bool trace_ = false;           // MJG_BATCH_TRACE: report segment reruns on stderr
     bool descs_dirty_ = true;
     std::vector<int> last_ids_;
     RepState* d_states_ = nullptr;
@@ -895,6 +991,15 @@ mjg_status mjg_batch_replica_info(mjg_batch* b, int replica, uint64_t* step, uin
         if(remaining) {*remaining = s.remaining;}
         if(rebuilds) {*rebuilds = s.rebuilds;}
         if(footprint_bytes) {*footprint_bytes = x.footprint(replica);}
+    });
+}
+mjg_status mjg_batch_list_sizes(mjg_batch* b, int replica, uint64_t* entries, int max_globals)
+{
+    if(!entries && max_globals > 0) {return MJG_INVALID_ARG;}
+    return bguard(b, [&](mjg::Batch& x) {
+        const mjg::RepState s = x.state(replica);
+        const int ng = std::min(max_globals, x.num_globals(replica));
+        for(int g = 0; g < ng; ++g) {entries[g] = s.npairs[g];}
     });
 }
 mjg_status mjg_batch_evaluate(mjg_batch* b, int replica, double* per_interaction, double* total, double* force)

@@ -2,44 +2,49 @@
 // steps without leaving the kernel.
 //
 // Per replica (replica-local indices, at most kRepMaxN particles):
-// * FP64 state (position, velocity, force) lives in the registers of the thread that owns the
-//   particle for the whole segment; the integrator update is the reference's FP64 BAOAB / G-JF
-//   loop (BAOABLangevinIntegrator.hpp:176-229, GJFNVTLangevinIntegrator.hpp:172-229);
-// * LDS holds the FP64 positions (for displacements) and int64 fixed-point force accumulators
-//   (2^-32 kcal/mol/A per unit), so force sums are exact integer sums: the result does not
-//   depend on the order in which terms land, and a replica's trajectory does not depend on
-//   which other replicas share the launch;
+// * velocities and the per-particle integrator coefficients live in the registers of the thread
+//   that owns the particle for the whole segment; the update is the reference's FP64 BAOAB /
+//   G-JF loop (BAOABLangevinIntegrator.hpp:176-229, GJFNVTLangevinIntegrator.hpp:172-229);
+// * LDS holds the FP64 positions, int64 fixed-point force accumulators (2^-32 kcal/mol/A per
+//   unit), the per-particle pair parameters and the participant lists.  Force sums are exact
+//   integer sums, so the result does not depend on the order in which terms land, and a
+//   replica's trajectory does not depend on which other replicas share the launch;
 // * displacements are formed in FP64 (minimum image, BoundaryCondition.hpp:81-91) and the pair
 //   and angle/dihedral/contact arithmetic runs in Real (float in the mixed mode); harmonic
 //   bonds stay FP64 because r - r0 cancels badly in FP32 for stiff springs;
-// * the skin check (reduce_margin, PeriodicGridCellList.hpp:100-110) and the list rebuild run
+// * the skin check (reduce_margin, UnlimitedGridCellList.hpp:71-81) and the list rebuild run
 //   inside the workgroup; lists are flat arrays of packed (i << 16 | j) pairs with i < j, built
-//   all-pairs from the FP64 positions with the reference's rules, so the pair set is exactly the
-//   reference's.
+//   from the FP64 positions with the reference's rules, so the pair set is the reference's;
+// * global memory is touched in batches (several terms / list entries per thread per load
+//   round), since one workgroup per CU cannot hide a dependent load per item.
 #pragma once
 #include "device_common.hpp"
 
 namespace mjg
 {
-constexpr int kRepThreads = 512;
-constexpr int kRepPPT = 4;                         // particles per thread
-constexpr uint32_t kRepMaxN = kRepThreads * kRepPPT;
+constexpr uint32_t kRepMaxN = 2048;
 constexpr int kRepMaxGlobals = 3;
 constexpr int kRepMaxLocals = 8;
 constexpr double kFixScale = 4294967296.0;         // 2^32
 constexpr double kFixInv = 2.3283064365386963e-10; // 2^-32
+// dynamic LDS available to one workgroup (160 KiB minus the kernel's static arrays)
+constexpr size_t kRepLdsLimit = 160 * 1024 - 1024;
+
+// size classes: a replica runs in the smallest class that holds it
+constexpr int kRepClasses = 3;
+constexpr uint32_t kRepClassMaxN[kRepClasses] = {256, 1024, 2048};
+
+// terms and list entries per thread per load round (fewer at 1024 threads, where a thread
+// has at most 128 VGPRs)
+template<int BS> struct RepBatching
+{
+    static constexpr int terms = BS >= 1024 ? 2 : 4;
+    static constexpr int pairs = BS >= 1024 ? 4 : 8;
+};
 
 enum RepTermKind : int
 {
     kRBondHarmonic = 1, kRBondGo = 2, kRContactGo = 3, kRAngleHarmonic = 4, kRAngleGaussian = 5, kRDihedralClementi = 6
-};
-
-struct RepTerm
-{
-    uint16_t kind;
-    uint16_t inter;      // local interaction index within the replica (energy slot)
-    uint16_t idx[4];
-    double p[3];
 };
 
 struct RepGlobal
@@ -49,6 +54,10 @@ struct RepGlobal
     int use_distance;
     int never_rebuild;
     int ignore_mol;
+    int sorted;          // participants in increasing order (partner b > a  <=>  index b > index a)
+    int npar;            // parameters per particle
+    uint32_t par_col;    // first LDS parameter column of this interaction
+    uint32_t part_off;   // offset of the participant list in the LDS participant area (uint16 units)
     uint32_t np;
     uint32_t last_slot;
     uint32_t cap;
@@ -84,6 +93,8 @@ struct RepDesc
     int n_local;               // local interactions (energy slots 0..n_local-1)
     int ng;
     int integ;                 // 1 BAOAB, 2 G-JF
+    uint32_t par_cols;         // total LDS parameter columns
+    uint32_t part_total;       // LDS participant entries (uint16, padded per interaction)
     double dt, halfdt;
     uint64_t seed;
     Box box;
@@ -95,7 +106,11 @@ struct RepDesc
     const double* c2;
     const double* c3;
     const uint32_t* mol;
-    const RepTerm* terms;
+    // local terms, structure of arrays: 4 packed uint16 indices, kind | interaction << 8,
+    // three parameter columns of n_terms each
+    const unsigned long long* t_idx;
+    const uint32_t* t_kind;
+    const double* t_par;
     RepState* st;
     // segment-start copies for restarting a segment after a list overflow
     double4* bak_pos;
@@ -104,6 +119,12 @@ struct RepDesc
     RepState* bak_st;
     RepGlobal g[kRepMaxGlobals];
 };
+
+// dynamic LDS bytes of one replica (layout in k_replica_segment)
+__host__ __device__ inline size_t rep_lds_bytes(uint32_t n, uint32_t par_cols, uint32_t part_total, size_t real_size)
+{
+    return (size_t)n * 48 + (size_t)n * par_cols * real_size + (size_t)part_total * 2;
+}
 
 // ------------------------------------------------------------------------------------
 // fixed-point accumulation
@@ -117,6 +138,10 @@ __device__ __forceinline__ void acc3(unsigned long long* F, uint32_t i, double x
     atomicAdd(&F[3 * i + 0], (unsigned long long)to_fix(x));
     atomicAdd(&F[3 * i + 1], (unsigned long long)to_fix(y));
     atomicAdd(&F[3 * i + 2], (unsigned long long)to_fix(z));
+}
+__device__ __forceinline__ double from_fix(unsigned long long v)
+{
+    return (double)(long long)v * kFixInv;
 }
 
 struct PosL
@@ -135,34 +160,33 @@ __device__ __forceinline__ void disp(const Box& b, const PosL& P, uint32_t from,
 // ------------------------------------------------------------------------------------
 // local terms (Real arithmetic except the harmonic bond)
 // ------------------------------------------------------------------------------------
-template<typename Real>
-__device__ __forceinline__ Real r_sqrt(Real x) {return sqrt(x);}
-
 template<typename Real, bool ENERGY>
-__device__ void rep_local(const RepTerm& T, const Box& box, const PosL& P, unsigned long long* F, double& e)
+__device__ __forceinline__ void rep_local(int kind, unsigned long long ix, double p0, double p1, double p2,
+                                          const Box& box, const PosL& P, unsigned long long* F, double& e)
 {
-    if(T.kind == kRBondHarmonic)
+    const uint32_t i0 = (uint32_t)(ix & 0xffff), i1 = (uint32_t)((ix >> 16) & 0xffff);
+    const uint32_t i2 = (uint32_t)((ix >> 32) & 0xffff), i3 = (uint32_t)(ix >> 48);
+    if(kind == kRBondHarmonic)
     {
         // BondLengthInteraction.hpp:126-148, HarmonicPotential.hpp:23-32 (FP64)
         double dx, dy, dz;
-        disp(box, P, T.idx[0], T.idx[1], dx, dy, dz);
+        disp(box, P, i0, i1, dx, dy, dz);
         const double len2 = length_sq(dx, dy, dz);
         const double rlen = 1.0 / sqrt(len2);
         const double len = len2 * rlen;
-        const double k = T.p[0], v0 = T.p[1];
-        const double c = (-1 * (2 * k * (len - v0))) * rlen;
-        acc3(F, T.idx[0], -(dx * c), -(dy * c), -(dz * c));
-        acc3(F, T.idx[1], dx * c, dy * c, dz * c);
-        if(ENERGY) {const double dv = len - v0; e += k * dv * dv;}
+        const double c = (-1 * (2 * p0 * (len - p1))) * rlen;
+        acc3(F, i0, -(dx * c), -(dy * c), -(dz * c));
+        acc3(F, i1, dx * c, dy * c, dz * c);
+        if(ENERGY) {const double dv = len - p1; e = p0 * dv * dv;}
     }
-    else if(T.kind == kRBondGo || T.kind == kRContactGo)
+    else if(kind == kRBondGo || kind == kRContactGo)
     {
         // BondLengthGoContactInteraction.hpp:126-167
         double ddx, ddy, ddz;
-        disp(box, P, T.idx[0], T.idx[1], ddx, ddy, ddz);
+        disp(box, P, i0, i1, ddx, ddy, ddz);
         const Real dx = (Real)ddx, dy = (Real)ddy, dz = (Real)ddz;
         const Real len2 = dx * dx + dy * dy + dz * dz;
-        const Real k = (Real)T.p[0], v0 = (Real)T.p[1];
+        const Real k = (Real)p0, v0 = (Real)p1;
         const Real cut = v0 * (Real)2.5;
         if(cut * cut <= len2) {return;}
         const Real r2 = (Real)1 / len2;
@@ -171,35 +195,35 @@ __device__ void rep_local(const RepTerm& T, const Box& box, const PosL& P, unsig
         const Real v0r_10 = v0r_6 * v0r_2 * v0r_2;
         const Real v0r_12 = v0r_10 * v0r_2;
         const Real coef = (Real)(-60) * k * r2 * (v0r_10 - v0r_12);
-        acc3(F, T.idx[0], -(double)(coef * dx), -(double)(coef * dy), -(double)(coef * dz));
-        acc3(F, T.idx[1], (double)(coef * dx), (double)(coef * dy), (double)(coef * dz));
-        if(ENERGY) {e += (double)(k * ((Real)5 * v0r_12 - (Real)6 * v0r_10));}
+        acc3(F, i0, -(double)(coef * dx), -(double)(coef * dy), -(double)(coef * dz));
+        acc3(F, i1, (double)(coef * dx), (double)(coef * dy), (double)(coef * dz));
+        if(ENERGY) {e = (double)(k * ((Real)5 * v0r_12 - (Real)6 * v0r_10));}
     }
-    else if(T.kind == kRAngleHarmonic || T.kind == kRAngleGaussian)
+    else if(kind == kRAngleHarmonic || kind == kRAngleGaussian)
     {
         // BondAngleInteraction.hpp:129-189; GaussianPotential.hpp:20-38
         double a, b, c;
-        disp(box, P, T.idx[1], T.idx[0], a, b, c);
+        disp(box, P, i1, i0, a, b, c);
         const Real ijx = (Real)a, ijy = (Real)b, ijz = (Real)c;
-        disp(box, P, T.idx[1], T.idx[2], a, b, c);
+        disp(box, P, i1, i2, a, b, c);
         const Real kjx = (Real)a, kjy = (Real)b, kjz = (Real)c;
-        const Real inv_ij = (Real)1 / r_sqrt<Real>(ijx * ijx + ijy * ijy + ijz * ijz);
-        const Real inv_kj = (Real)1 / r_sqrt<Real>(kjx * kjx + kjy * kjy + kjz * kjz);
+        const Real inv_ij = (Real)1 / sqrt(ijx * ijx + ijy * ijy + ijz * ijz);
+        const Real inv_kj = (Real)1 / sqrt(kjx * kjx + kjy * kjy + kjz * kjz);
         const Real eix = ijx * inv_ij, eiy = ijy * inv_ij, eiz = ijz * inv_ij;
         const Real ekx = kjx * inv_kj, eky = kjy * inv_kj, ekz = kjz * inv_kj;
         Real cos_t = eix * ekx + eiy * eky + eiz * ekz;
         cos_t = fmin(fmax(cos_t, (Real)-1), (Real)1);
         const Real theta = acos(cos_t);
         Real coef, pot = 0;
-        if(T.kind == kRAngleHarmonic)
+        if(kind == kRAngleHarmonic)
         {
-            const Real dv = theta - (Real)T.p[1];
-            coef = -((Real)2 * (Real)T.p[0] * dv);
-            if(ENERGY) {pot = (Real)T.p[0] * dv * dv;}
+            const Real dv = theta - (Real)p1;
+            coef = -((Real)2 * (Real)p0 * dv);
+            if(ENERGY) {pot = (Real)p0 * dv * dv;}
         }
         else
         {
-            const Real k = (Real)T.p[0], sg = (Real)T.p[1], v0 = (Real)T.p[2];
+            const Real k = (Real)p0, sg = (Real)p1, v0 = (Real)p2;
             const Real inv_s2 = (Real)-1 / ((Real)2 * sg * sg);
             const Real dv = theta - v0;
             const Real ex = exp(inv_s2 * dv * dv);
@@ -211,33 +235,33 @@ __device__ void rep_local(const RepTerm& T, const Box& box, const PosL& P, unsig
         const Real ci = cis * inv_ij, ck = cis * inv_kj;
         const Real Fix = ci * (cos_t * eix - ekx), Fiy = ci * (cos_t * eiy - eky), Fiz = ci * (cos_t * eiz - ekz);
         const Real Fkx = ck * (cos_t * ekx - eix), Fky = ck * (cos_t * eky - eiy), Fkz = ck * (cos_t * ekz - eiz);
-        acc3(F, T.idx[0], Fix, Fiy, Fiz);
-        acc3(F, T.idx[1], -(double)Fix - (double)Fkx, -(double)Fiy - (double)Fky, -(double)Fiz - (double)Fkz);
-        acc3(F, T.idx[2], Fkx, Fky, Fkz);
-        if(ENERGY) {e += (double)pot;}
+        acc3(F, i0, Fix, Fiy, Fiz);
+        acc3(F, i1, -(double)Fix - (double)Fkx, -(double)Fiy - (double)Fky, -(double)Fiz - (double)Fkz);
+        acc3(F, i2, Fkx, Fky, Fkz);
+        if(ENERGY) {e = (double)pot;}
     }
-    else if(T.kind == kRDihedralClementi)
+    else if(kind == kRDihedralClementi)
     {
         // DihedralAngleInteraction.hpp:157-227; ClementiDihedralPotential.hpp:27-42
         double a, b, c;
-        disp(box, P, T.idx[1], T.idx[0], a, b, c);
+        disp(box, P, i1, i0, a, b, c);
         const Real ijx = (Real)a, ijy = (Real)b, ijz = (Real)c;
-        disp(box, P, T.idx[1], T.idx[2], a, b, c);
+        disp(box, P, i1, i2, a, b, c);
         const Real kjx = (Real)a, kjy = (Real)b, kjz = (Real)c;
-        disp(box, P, T.idx[2], T.idx[3], a, b, c);
+        disp(box, P, i2, i3, a, b, c);
         const Real klx = -(Real)a, kly = -(Real)b, klz = -(Real)c;
         const Real r_kj_lensq = kjx * kjx + kjy * kjy + kjz * kjz;
-        const Real r_kj_rlen = (Real)1 / r_sqrt<Real>(r_kj_lensq);
+        const Real r_kj_rlen = (Real)1 / sqrt(r_kj_lensq);
         const Real r_kj_rlensq = r_kj_rlen * r_kj_rlen;
         const Real r_kj_len = r_kj_rlen * r_kj_lensq;
         const Real mx = ijy * kjz - ijz * kjy, my = ijz * kjx - ijx * kjz, mz = ijx * kjy - ijy * kjx;
         const Real nx = kjy * klz - kjz * kly, ny = kjz * klx - kjx * klz, nz = kjx * kly - kjy * klx;
         const Real m_lensq = mx * mx + my * my + mz * mz, n_lensq = nx * nx + ny * ny + nz * nz;
-        Real cos_p = (mx * nx + my * ny + mz * nz) / r_sqrt<Real>(m_lensq * n_lensq);
+        Real cos_p = (mx * nx + my * ny + mz * nz) / sqrt(m_lensq * n_lensq);
         cos_p = fmin(fmax(cos_p, (Real)-1), (Real)1);
         const Real phi = copysign(acos(cos_p), ijx * nx + ijy * ny + ijz * nz);
-        const Real k1 = (Real)T.p[0], k3 = (Real)T.p[1];
-        const Real dphi = phi - (Real)T.p[2];
+        const Real k1 = (Real)p0, k3 = (Real)p1;
+        const Real dphi = phi - (Real)p2;
         const Real sin1 = sin(dphi);
         const Real sin3 = sin1 * ((Real)3 - (Real)4 * sin1 * sin1);
         const Real coef = -(k1 * sin1 + (Real)3 * k3 * sin3);
@@ -245,7 +269,7 @@ __device__ void rep_local(const RepTerm& T, const Box& box, const PosL& P, unsig
         {
             const Real cos1 = cos(dphi);
             const Real cos3 = cos1 * ((Real)4 * cos1 * cos1 - (Real)3);
-            e += (double)(k1 * ((Real)1 - cos1) + k3 * ((Real)1 - cos3));
+            e = (double)(k1 * ((Real)1 - cos1) + k3 * ((Real)1 - cos3));
         }
         const Real ci = coef * r_kj_len / m_lensq;
         const Real cl = -coef * r_kj_len / n_lensq;
@@ -253,21 +277,22 @@ __device__ void rep_local(const RepTerm& T, const Box& box, const PosL& P, unsig
         const Real Flx = cl * nx, Fly = cl * ny, Flz = cl * nz;
         const Real cijk = (ijx * kjx + ijy * kjy + ijz * kjz) * r_kj_rlensq;
         const Real cjkl = (klx * kjx + kly * kjy + klz * kjz) * r_kj_rlensq;
-        acc3(F, T.idx[0], Fix, Fiy, Fiz);
-        acc3(F, T.idx[1], (cijk - (Real)1) * Fix - cjkl * Flx, (cijk - (Real)1) * Fiy - cjkl * Fly,
+        acc3(F, i0, Fix, Fiy, Fiz);
+        acc3(F, i1, (cijk - (Real)1) * Fix - cjkl * Flx, (cijk - (Real)1) * Fiy - cjkl * Fly,
              (cijk - (Real)1) * Fiz - cjkl * Flz);
-        acc3(F, T.idx[2], (cjkl - (Real)1) * Flx - cijk * Fix, (cjkl - (Real)1) * Fly - cijk * Fiy,
+        acc3(F, i2, (cjkl - (Real)1) * Flx - cijk * Fix, (cjkl - (Real)1) * Fly - cijk * Fiy,
              (cjkl - (Real)1) * Flz - cijk * Fiz);
-        acc3(F, T.idx[3], Flx, Fly, Flz);
+        acc3(F, i3, Flx, Fly, Flz);
     }
 }
 
 // ------------------------------------------------------------------------------------
-// pair term (i < j): force on i is +f, on j is -f (GlobalPair*Interaction loops)
+// pair term (i < j): force on i is +f, on j is -f (GlobalPair*Interaction loops); the
+// per-particle parameters come from LDS (`par`, npar columns)
 // ------------------------------------------------------------------------------------
 template<typename Real, bool ENERGY>
-__device__ __forceinline__ void rep_pair(const RepGlobal& G, uint32_t i, uint32_t j, const Box& box, const PosL& P,
-                                         unsigned long long* F, double& e)
+__device__ __forceinline__ void rep_pair(const RepGlobal& G, const Real* par, uint32_t i, uint32_t j, const Box& box,
+                                         const PosL& P, unsigned long long* F, double& e)
 {
     double ddx, ddy, ddz;
     disp(box, P, i, j, ddx, ddy, ddz);
@@ -276,9 +301,10 @@ __device__ __forceinline__ void rep_pair(const RepGlobal& G, uint32_t i, uint32_
     Real c;
     if(G.kind == kPairLJ)
     {
-        const double s1 = G.ppar[2 * i], e1 = G.ppar[2 * i + 1], s2 = G.ppar[2 * j], e2 = G.ppar[2 * j + 1];
-        const Real sigma = (Real)((s1 + s2) / 2);
-        const Real eps = (Real)((e1 == e2) ? e1 : sqrt(e1 * e2));
+        // LorentzBerthelotRule (ParameterList.hpp:240-252), LennardJonesPotential.hpp:34-63
+        const Real s1 = par[2 * i], e1 = par[2 * i + 1], s2 = par[2 * j], e2 = par[2 * j + 1];
+        const Real sigma = (s1 + s2) / (Real)2;
+        const Real eps = (e1 == e2) ? e1 : sqrt(e1 * e2);
         const Real sigma_sq = sigma * sigma;
         if(sigma_sq * (Real)G.cutoff_ratio2 < l_sq) {return;}
         const Real rcp = (Real)1 / l_sq;
@@ -289,7 +315,8 @@ __device__ __forceinline__ void rep_pair(const RepGlobal& G, uint32_t i, uint32_
     }
     else if(G.kind == kPairEV)
     {
-        const Real radius = (Real)(G.ppar[i] + G.ppar[j]);
+        // ExcludedVolumePotential.hpp:44-133
+        const Real radius = par[i] + par[j];
         const Real sigma_sq = radius * radius;
         if(sigma_sq * (Real)G.cutoff_ratio2 < l_sq) {return;}
         const Real rcp = (Real)1 / l_sq;
@@ -300,8 +327,9 @@ __device__ __forceinline__ void rep_pair(const RepGlobal& G, uint32_t i, uint32_
     }
     else
     {
-        const Real qq = (Real)(G.inv4pieps * G.ppar[i] * G.ppar[j]);
-        const Real rl = (Real)1 / r_sqrt<Real>(l_sq);
+        // DebyeHuckelPotential.hpp:47-65
+        const Real qq = (Real)G.inv4pieps * par[i] * par[j];
+        const Real rl = (Real)1 / sqrt(l_sq);
         const Real l = l_sq * rl;
         if((Real)G.abs_cutoff <= l) {return;}
         const Real ex = exp(-l * (Real)G.inv_debye);
@@ -331,77 +359,83 @@ __device__ __forceinline__ bool rep_allowed(const RepGlobal& G, const uint32_t* 
     return true;
 }
 
-// all-pairs list build of global g inside the workgroup (count in LDS)
-__device__ void rep_build(const RepGlobal& G, const RepDesc& R, const PosL& P, uint32_t* s_count, RepState& st, int gi)
+// List build of one interaction inside the workgroup: every participant pair (each once, owned
+// by the lower particle index) within the list radius that the rules allow.  Rows are dealt in
+// snake order so the triangular loop is balanced over threads; `part` is the LDS copy.
+template<int BS>
+__device__ void rep_build_rows(const RepGlobal& G, const RepDesc& R, const PosL& P, const uint16_t* part,
+                               uint32_t* count)
 {
-    if(threadIdx.x == 0) {*s_count = 0;}
-    __syncthreads();
     const uint32_t np = G.np;
-    const uint64_t total = (uint64_t)np * (np - 1) / 2;
-    // thread t walks rows a = t, t + bs, ...; partners b > a in input order
-    for(uint32_t a = threadIdx.x; a < np; a += blockDim.x)
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nblk = (np + BS - 1) / BS;
+    for(uint32_t m = 0; m < nblk; ++m)
     {
-        const uint32_t pa = G.part[a];
-        for(uint32_t b = 0; b < np; ++b)
+        const uint32_t a = (m & 1) ? m * BS + (BS - 1 - tid) : m * BS + tid;
+        if(a >= np) {continue;}
+        const uint32_t pa = part[a];
+        const double xa = P.x[pa], ya = P.y[pa], za = P.z[pa];
+        for(uint32_t b = G.sorted ? a + 1 : 0; b < np; ++b)
         {
-            if(b == a) {continue;}
-            const uint32_t pb = G.part[b];
-            if(pb <= pa) {continue;}        // each pair once, owned by its lower index
+            const uint32_t pb = part[b];
+            if(pb <= pa) {continue;}
             if(G.use_distance)
             {
                 double dx, dy, dz;
-                disp(R.box, P, pa, pb, dx, dy, dz);
+                adjust_direction(R.box, xa, ya, za, P.x[pb], P.y[pb], P.z[pb], dx, dy, dz);
                 if(!(length_sq(dx, dy, dz) < G.r_list2)) {continue;}
             }
             if(!rep_allowed(G, R.mol, pa, pb)) {continue;}
-            const uint32_t k = atomicAdd(s_count, 1u);
+            const uint32_t k = atomicAdd(count, 1u);
             if(k < G.cap) {G.pairs[k] = (pa << 16) | pb;}
         }
     }
-    (void)total;
-    __syncthreads();
-    if(threadIdx.x == 0)
-    {
-        const uint32_t c = *s_count;
-        st.npairs[gi] = (c < G.cap) ? c : G.cap;
-        if(c > st.maxpairs[gi]) {st.maxpairs[gi] = c;}
-        if(c > G.cap) {st.overflow = 1;}
-    }
-    __syncthreads();
 }
 
 // ------------------------------------------------------------------------------------
 // one segment: `nsteps` steps (fewer if the replica has fewer remaining), or a pure force /
-// energy evaluation when nsteps == 0
+// energy evaluation when eval_only (the lists are rebuilt first if the state asks for it)
+//
+// LDS: x[n], y[n], z[n] (double); F[3n] (int64 fixed point); parameters[par_cols][n] as
+// npar-column rows per interaction (Real); participant lists (uint16)
 // ------------------------------------------------------------------------------------
-template<typename Real, bool ENERGY>
-__global__ void __launch_bounds__(kRepThreads)
+template<typename Real, bool ENERGY, int BS, int PPT>
+__global__ void __launch_bounds__(BS)
 k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
 {
-    extern __shared__ unsigned char smem[];
+    extern __shared__ __align__(16) unsigned char smem[];
     const RepDesc& R = reps[blockIdx.x];
     RepState& st = *R.st;
     const uint32_t n = R.n;
+    const int ng = R.ng;
     PosL P;
     P.x = reinterpret_cast<double*>(smem);
     P.y = P.x + n;
     P.z = P.y + n;
     unsigned long long* F = reinterpret_cast<unsigned long long*>(P.z + n);
-    __shared__ double s_red[kRepThreads / kWave];
-    __shared__ int s_rebuild[kRepMaxGlobals];
-    __shared__ uint32_t s_count;
-    __shared__ uint32_t s_npairs[kRepMaxGlobals];
-    __shared__ double s_energy[kRepMaxLocals + kRepMaxGlobals];
-    const uint32_t tid = threadIdx.x, bs = blockDim.x;
+    Real* par_lds = reinterpret_cast<Real*>(F + 3 * n);
+    uint16_t* part_lds = reinterpret_cast<uint16_t*>(par_lds + (size_t)n * R.par_cols);
 
-    // own particles: velocities in registers; positions in LDS (FP64); forces in the LDS
-    // fixed-point accumulators, which carry the force from one step to the next (the FP64 force
-    // of a step is the accumulator value times 2^-32, so the round trip through memory is exact)
-    double vx[kRepPPT], vy[kRepPPT], vz[kRepPPT];
+    __shared__ double s_red[BS / kWave];
+    __shared__ double s_margin[kRepMaxGlobals];
+    __shared__ uint32_t s_npairs[kRepMaxGlobals], s_maxpairs[kRepMaxGlobals], s_count[kRepMaxGlobals];
+    __shared__ int s_rebuild[kRepMaxGlobals];
+    __shared__ int s_any_rebuild, s_flags, s_force_rebuild;
+    __shared__ unsigned long long s_rebuilds;
+    __shared__ double s_energy[kRepMaxLocals + kRepMaxGlobals];
+    const uint32_t tid = threadIdx.x;
+    constexpr int kTermBatch = RepBatching<BS>::terms;
+    constexpr int kPairBatch = RepBatching<BS>::pairs;
+
+    // own particles: velocity in registers; position and force in LDS (the force round trip
+    // through the fixed-point accumulator is exact); the integrator coefficients are re-read
+    // each step (cache hits, issued ahead of the noise arithmetic)
+    double vx[PPT], vy[PPT], vz[PPT];
 #pragma unroll
-    for(int k = 0; k < kRepPPT; ++k)
+    for(int k = 0; k < PPT; ++k)
     {
-        const uint32_t p = tid + k * bs;
+        const uint32_t p = tid + k * BS;
+        vx[k] = vy[k] = vz[k] = 0.0;
         if(p < n)
         {
             const double4 a = R.pos[p], v = R.vel[p], f = R.force[p];
@@ -413,12 +447,20 @@ k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
             if(!eval_only) {R.bak_pos[p] = a; R.bak_vel[p] = v; R.bak_force[p] = f;}
         }
     }
-    __shared__ int s_force_rebuild;
+    for(int g = 0; g < ng; ++g)
+    {
+        const RepGlobal& G = R.g[g];
+        Real* par = par_lds + (size_t)n * G.par_col;
+        for(uint32_t q = tid; q < n * (uint32_t)G.npar; q += BS) {par[q] = (Real)G.ppar[q];}
+        for(uint32_t q = tid; q < G.np; q += BS) {part_lds[G.part_off + q] = G.part[q];}
+    }
     if(tid == 0)
     {
         if(!eval_only) {*R.bak_st = st;}
         s_force_rebuild = st.force_rebuild;
-        st.force_rebuild = 0;
+        s_rebuilds = st.rebuilds;
+        s_flags = 0;
+        for(int g = 0; g < ng; ++g) {s_margin[g] = st.margin[g]; s_npairs[g] = st.npairs[g]; s_maxpairs[g] = st.maxpairs[g];}
     }
     __syncthreads();
     unsigned long long steps_to_do = eval_only ? 1ull : (unsigned long long)nsteps;
@@ -431,21 +473,20 @@ k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
         if(!eval_only)
         {
             // first half of the step (BAOA / G-JF position update)
+            bool bad = false;
 #pragma unroll
-            for(int k = 0; k < kRepPPT; ++k)
+            for(int k = 0; k < PPT; ++k)
             {
-                const uint32_t p = tid + k * bs;
+                const uint32_t p = tid + k * BS;
                 if(p >= n) {continue;}
-                const double fxk = (double)(long long)F[3 * p + 0] * kFixInv;
-                const double fyk = (double)(long long)F[3 * p + 1] * kFixInv;
-                const double fzk = (double)(long long)F[3 * p + 2] * kFixInv;
+                const double rmk = R.rmass[p], c1k = R.c1[p], c2k = R.c2[p];
+                const double fxk = from_fix(F[3 * p + 0]), fyk = from_fix(F[3 * p + 1]), fzk = from_fix(F[3 * p + 2]);
                 double Rx, Ry, Rz;
                 noise3(R.seed, step, p, Rx, Ry, Rz);
-                const double rm = R.rmass[p];
                 double dpx, dpy, dpz;
                 if(R.integ == 1)
                 {
-                    const double hrm = R.halfdt * rm, eg = R.c1[p], nc = R.c2[p];
+                    const double hrm = R.halfdt * rmk, eg = c1k, nc = c2k;
                     vx[k] += hrm * fxk; vy[k] += hrm * fyk; vz[k] += hrm * fzk;
                     dpx = R.halfdt * vx[k]; dpy = R.halfdt * vy[k]; dpz = R.halfdt * vz[k];
                     vx[k] *= eg; vy[k] *= eg; vz[k] *= eg;
@@ -454,12 +495,12 @@ k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
                 }
                 else
                 {
-                    const double r2m = 0.5 * rm, b = R.c1[p], bt = R.c2[p], a = R.c3[p];
+                    const double r2m = 0.5 * rmk, b = c1k, bt = c2k, a = R.c3[p];
                     const double bx = Rx * bt, by = Ry * bt, bz = Rz * bt, bdt = b * R.dt;
                     dpx = bdt * (vx[k] + r2m * (R.dt * fxk + bx));
                     dpy = bdt * (vy[k] + r2m * (R.dt * fyk + by));
                     dpz = bdt * (vz[k] + r2m * (R.dt * fzk + bz));
-                    const double dtr2m = R.dt * r2m, brm = b * rm;
+                    const double dtr2m = R.dt * r2m, brm = b * rmk;
                     vx[k] = a * (vx[k] + dtr2m * fxk) + brm * bx;
                     vy[k] = a * (vy[k] + dtr2m * fyk) + brm * by;
                     vz[k] = a * (vz[k] + dtr2m * fzk) + brm * bz;
@@ -468,13 +509,13 @@ k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
                 adjust_position(R.box, x, y, z);
                 P.x[p] = x; P.y[p] = y; P.z[p] = z;
                 d2 = fmax(d2, length_sq(dpx, dpy, dpz));
-                if(!isfinite(x) || !isfinite(y) || !isfinite(z)) {st.nonfinite = 1;}
+                if(!isfinite(x) || !isfinite(y) || !isfinite(z)) {bad = true;}
             }
+            if(bad) {atomicOr(&s_flags, 2);}
         }
         __syncthreads();
-        // zero accumulators
-        for(uint32_t q = tid; q < 3 * n; q += bs) {F[q] = 0ull;}
-        // skin check
+        for(uint32_t q = tid; q < 3 * n; q += BS) {F[q] = 0ull;}
+        // skin check (ForceField::reduce_margin): 2 sqrt(max |dp|^2) off every margin
         d2 = wave_max(d2);
         if(tid % kWave == 0) {s_red[tid / kWave] = d2;}
         if(ENERGY && tid < (uint32_t)(kRepMaxLocals + kRepMaxGlobals)) {s_energy[tid] = 0.0;}
@@ -482,53 +523,98 @@ k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
         if(tid == 0)
         {
             double m = 0.0;
-            for(uint32_t w = 0; w < bs / kWave; ++w) {m = fmax(m, s_red[w]);}
+            for(int w = 0; w < BS / kWave; ++w) {m = fmax(m, s_red[w]);}
             const double dmargin = 2 * sqrt(m);
-            for(int g = 0; g < R.ng; ++g)
+            int any = 0;
+            for(int g = 0; g < ng; ++g)
             {
                 int rb = 0;
                 if(s_force_rebuild && s == 0) {rb = 1;}
                 else if(!eval_only && !R.g[g].never_rebuild)
                 {
-                    st.margin[g] -= dmargin;
-                    rb = st.margin[g] < 0;
+                    s_margin[g] -= dmargin;
+                    rb = s_margin[g] < 0;
                 }
-                if(rb) {st.margin[g] = R.g[g].never_rebuild ? INFINITY : R.g[g].abs_margin; st.rebuilds += 1;}
+                if(rb) {s_margin[g] = R.g[g].never_rebuild ? INFINITY : R.g[g].abs_margin; s_rebuilds += 1;}
                 s_rebuild[g] = rb;
+                s_count[g] = 0;
+                any |= rb;
             }
+            s_any_rebuild = any;
         }
         __syncthreads();
-        for(int g = 0; g < R.ng; ++g)
+        if(s_any_rebuild)
         {
-            if(s_rebuild[g]) {rep_build(R.g[g], R, P, &s_count, st, g);}
-        }
-        if(tid < (uint32_t)R.ng) {s_npairs[tid] = st.npairs[tid];}
-        __syncthreads();
-        // local terms
-        double e_loc = 0.0;
-        int cur_inter = -1;
-        for(uint32_t t = tid; t < R.n_terms; t += bs)
-        {
-            const RepTerm T = R.terms[t];
-            if(ENERGY && cur_inter != T.inter)
+            for(int g = 0; g < ng; ++g)
             {
-                if(cur_inter >= 0) {atomicAdd(&s_energy[cur_inter], e_loc);}
-                e_loc = 0.0;
-                cur_inter = T.inter;
+                if(s_rebuild[g]) {rep_build_rows<BS>(R.g[g], R, P, part_lds + R.g[g].part_off, &s_count[g]);}
             }
-            rep_local<Real, ENERGY>(T, R.box, P, F, e_loc);
+            __syncthreads();
+            if(tid == 0)
+            {
+                for(int g = 0; g < ng; ++g)
+                {
+                    if(!s_rebuild[g]) {continue;}
+                    const uint32_t c = s_count[g], cap = R.g[g].cap;
+                    s_npairs[g] = (c < cap) ? c : cap;
+                    if(c > s_maxpairs[g]) {s_maxpairs[g] = c;}
+                    if(c > cap) {s_flags |= 1;}
+                }
+            }
+            __syncthreads();
         }
-        if(ENERGY && cur_inter >= 0) {atomicAdd(&s_energy[cur_inter], e_loc);}
-        // pair terms
-        for(int g = 0; g < R.ng; ++g)
+        // local terms, kTermBatch per thread per load round
+        const uint32_t nt = R.n_terms;
+        for(uint32_t base = 0; base < nt; base += BS * kTermBatch)
+        {
+            unsigned long long ix[kTermBatch];
+            uint32_t kd[kTermBatch];
+            double p0[kTermBatch], p1[kTermBatch], p2[kTermBatch];
+#pragma unroll
+            for(int m = 0; m < kTermBatch; ++m)
+            {
+                const uint32_t t = base + m * BS + tid;
+                kd[m] = 0;
+                if(t < nt)
+                {
+                    ix[m] = R.t_idx[t];
+                    kd[m] = R.t_kind[t];
+                    p0[m] = R.t_par[t];
+                    p1[m] = R.t_par[nt + t];
+                    p2[m] = R.t_par[2 * nt + t];
+                }
+            }
+#pragma unroll
+            for(int m = 0; m < kTermBatch; ++m)
+            {
+                if(kd[m] == 0) {continue;}
+                double e = 0.0;
+                rep_local<Real, ENERGY>((int)(kd[m] & 0xff), ix[m], p0[m], p1[m], p2[m], R.box, P, F, e);
+                if(ENERGY) {atomicAdd(&s_energy[kd[m] >> 8], e);}
+            }
+        }
+        // pair terms, kPairBatch list entries per thread per load round
+        for(int g = 0; g < ng; ++g)
         {
             const RepGlobal& G = R.g[g];
+            const Real* par = par_lds + (size_t)n * G.par_col;
             const uint32_t np = s_npairs[g];
             double e_pair = 0.0;
-            for(uint32_t k = tid; k < np; k += bs)
+            for(uint32_t base = 0; base < np; base += BS * kPairBatch)
             {
-                const uint32_t pk = G.pairs[k];
-                rep_pair<Real, ENERGY>(G, pk >> 16, pk & 0xffffu, R.box, P, F, e_pair);
+                uint32_t pk[kPairBatch];
+#pragma unroll
+                for(int m = 0; m < kPairBatch; ++m)
+                {
+                    const uint32_t k = base + m * BS + tid;
+                    pk[m] = (k < np) ? G.pairs[k] : 0xffffffffu;
+                }
+#pragma unroll
+                for(int m = 0; m < kPairBatch; ++m)
+                {
+                    if(pk[m] == 0xffffffffu) {continue;}
+                    rep_pair<Real, ENERGY>(G, par, pk[m] >> 16, pk[m] & 0xffffu, R.box, P, F, e_pair);
+                }
             }
             if(ENERGY) {atomicAdd(&s_energy[R.n_local + g], e_pair);}
         }
@@ -537,35 +623,38 @@ k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
         if(!eval_only)
         {
 #pragma unroll
-            for(int k = 0; k < kRepPPT; ++k)
+            for(int k = 0; k < PPT; ++k)
             {
-                const uint32_t p = tid + k * bs;
+                const uint32_t p = tid + k * BS;
                 if(p >= n) {continue;}
                 const double hrm = R.halfdt * R.rmass[p];
-                vx[k] += hrm * ((double)(long long)F[3 * p + 0] * kFixInv);
-                vy[k] += hrm * ((double)(long long)F[3 * p + 1] * kFixInv);
-                vz[k] += hrm * ((double)(long long)F[3 * p + 2] * kFixInv);
+                vx[k] += hrm * from_fix(F[3 * p + 0]);
+                vy[k] += hrm * from_fix(F[3 * p + 1]);
+                vz[k] += hrm * from_fix(F[3 * p + 2]);
             }
+            ++step;
         }
-        if(!eval_only) {++step;}
-        __syncthreads();
     }
 #pragma unroll
-    for(int k = 0; k < kRepPPT; ++k)
+    for(int k = 0; k < PPT; ++k)
     {
-        const uint32_t p = tid + k * bs;
+        const uint32_t p = tid + k * BS;
         if(p < n)
         {
             R.pos[p] = make_double4(P.x[p], P.y[p], P.z[p], 0.0);
             R.vel[p] = make_double4(vx[k], vy[k], vz[k], 0.0);
-            R.force[p] = make_double4((double)(long long)F[3 * p + 0] * kFixInv, (double)(long long)F[3 * p + 1] * kFixInv,
-                                      (double)(long long)F[3 * p + 2] * kFixInv, 0.0);
+            R.force[p] = make_double4(from_fix(F[3 * p + 0]), from_fix(F[3 * p + 1]), from_fix(F[3 * p + 2]), 0.0);
         }
     }
     if(tid == 0)
     {
         if(!eval_only) {st.step = step; st.remaining -= steps_to_do;}
-        if(ENERGY) {for(int q = 0; q < R.n_local + R.ng; ++q) {st.energy[q] = s_energy[q];}}
+        st.rebuilds = s_rebuilds;
+        st.force_rebuild = 0;
+        if(s_flags & 1) {st.overflow = 1;}
+        if(s_flags & 2) {st.nonfinite = 1;}
+        for(int g = 0; g < ng; ++g) {st.margin[g] = s_margin[g]; st.npairs[g] = s_npairs[g]; st.maxpairs[g] = s_maxpairs[g];}
+        if(ENERGY) {for(int q = 0; q < R.n_local + ng; ++q) {st.energy[q] = s_energy[q];}}
     }
 }
 

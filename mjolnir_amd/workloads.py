@@ -19,7 +19,8 @@ every backend, because every backend receives the same ``SystemSpec``):
 from __future__ import annotations
 
 import math
-from typing import List, Tuple
+import os
+from typing import List, Optional, Tuple
 
 import numpy as np
 
@@ -75,19 +76,29 @@ def lj_fluid(n_side: int = 16, rho_reduced: float = 0.6, sigma: float = 4.0, eps
 # ----------------------------------------------------------------------------------------
 
 def self_avoiding_walk(n: int, rng: np.random.Generator, bond: float = 3.8,
-                       min_sep: float = 4.0, max_tries: int = 200) -> np.ndarray:
+                       min_sep: float = 4.0, max_tries: int = 200,
+                       max_bend_deg: Optional[float] = None) -> np.ndarray:
     """Seeded self-avoiding random walk: consecutive beads ``bond`` apart, every other
-    pair (|i-j| >= 2) at least ``min_sep`` apart.  Backtracks when a bead cannot be placed."""
+    pair (|i-j| >= 2) at least ``min_sep`` apart.  Backtracks when a bead cannot be placed.
+
+    ``max_bend_deg`` bounds the bond angle at each bead: angles near 180 degrees make the
+    angle force (1/sin theta) and the dihedral of the next quadruplet singular, and a chain
+    built with them can blow up within a few hundred steps, on the reference as here."""
     pos = np.zeros((n, 3))
     k = 1
     fails = np.zeros(n, dtype=np.int64)
     min_sep2 = min_sep * min_sep
+    cos_max = None if max_bend_deg is None else math.cos(math.radians(max_bend_deg))
     while k < n:
         placed = False
         for _ in range(max_tries):
             d = rng.standard_normal(3)
             d *= bond / np.linalg.norm(d)
             cand = pos[k - 1] + d
+            if k >= 2 and cos_max is not None:
+                a = pos[k - 2] - pos[k - 1]
+                if np.dot(a, d) / (np.linalg.norm(a) * bond) < cos_max:
+                    continue
             if k >= 2:
                 dd = pos[:k - 1] - cand
                 if np.min(np.einsum("ij,ij->i", dd, dd)) < min_sep2:
@@ -128,10 +139,10 @@ def cg_chain(n: int = 1000, seed: int = 2, T: float = 300.0, ionic_strength: flo
              bond_k: float = 110.0, angle_k: float = -0.5, angle_sigma: float = 0.15,
              dihedral_k1: float = 1.0, dihedral_k3: float = 0.5, go_eps: float = 0.3,
              go_cutoff: float = 6.5, go_min_seq: int = 4, ev_radius: float = 2.0,
-             ev_eps: float = 0.2, margin: float = 0.5) -> SystemSpec:
+             ev_eps: float = 0.2, margin: float = 0.5, max_bend_deg: Optional[float] = None) -> SystemSpec:
     """Protein-like CG chain in an unbounded box (UnlimitedGridCellList)."""
     rng = np.random.default_rng(seed)
-    pos = self_avoiding_walk(n, rng)
+    pos = self_avoiding_walk(n, rng, max_bend_deg=max_bend_deg)
     mass = np.ones(n)
     vel = maxwell_boltzmann(rng, mass, T)
 
@@ -175,13 +186,27 @@ def cg_chain(n: int = 1000, seed: int = 2, T: float = 300.0, ionic_strength: flo
 
 
 def replica_ensemble(n_replicas: int = 512, n_min: int = 50, n_max: int = 2000,
-                     T_min: float = 280.0, T_max: float = 400.0, seed: int = 4) -> List[SystemSpec]:
-    """Independent CG chains with log-uniform lengths and uniform temperatures."""
+                     T_min: float = 280.0, T_max: float = 400.0, seed: int = 4,
+                     max_bend_deg: float = 150.0) -> List[SystemSpec]:
+    """Independent CG chains with log-uniform lengths and uniform temperatures (bond angles
+    of the initial walks at most ``max_bend_deg``)."""
     rng = np.random.default_rng(seed)
     lens = np.exp(rng.uniform(math.log(n_min), math.log(n_max), size=n_replicas)).astype(np.int64)
     temps = rng.uniform(T_min, T_max, size=n_replicas)
     seeds = rng.integers(1, 2 ** 31 - 1, size=n_replicas)
-    return [cg_chain(int(n), seed=int(s), T=float(t)) for n, t, s in zip(lens, temps, seeds)]
+    jobs = [(int(n), int(s), float(t), max_bend_deg) for n, t, s in zip(lens, temps, seeds)]
+    workers = min(16, os.cpu_count() or 1)
+    if workers > 1 and n_replicas >= 32:
+        # every replica has its own seed, so the result does not depend on the pool
+        import multiprocessing as mp
+        with mp.get_context("fork").Pool(workers) as pool:
+            return pool.map(_ensemble_member, jobs, chunksize=4)
+    return [_ensemble_member(j) for j in jobs]
+
+
+def _ensemble_member(job) -> SystemSpec:
+    n, seed, T, max_bend_deg = job
+    return cg_chain(n, seed=seed, T=T, max_bend_deg=max_bend_deg)
 
 
 def zero_temperature(spec: SystemSpec) -> SystemSpec:

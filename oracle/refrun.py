@@ -100,3 +100,21 @@ def bench(spec, nsteps: int, threads: int = 0, timeout: float = 600):
                            text=True, env=env, timeout=timeout)
     line = [l for l in r.stdout.splitlines() if l.startswith("{")][-1]
     return json.loads(line)
+
+
+def bench_many(specs, nsteps, procs: int, timeout: float = 600):
+    """Time independent systems on the reference OpenMP build, one single-threaded process per
+    system and `procs` processes at a time (how a CPU runs a replica ensemble best).  `nsteps` is
+    one step count per system.  Returns (particle-steps, wall seconds)."""
+    from concurrent.futures import ThreadPoolExecutor
+    import time
+
+    def one(k):
+        return bench(specs[k], int(nsteps[k]), threads=1, timeout=timeout)
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=procs) as ex:
+        res = list(ex.map(one, range(len(specs))))
+    wall = time.perf_counter() - t0
+    ps = sum(int(nsteps[k]) * specs[k].n for k in range(len(specs)))
+    return ps, wall, res
