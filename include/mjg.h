@@ -139,6 +139,11 @@ mjg_status mjg_neighbor_pairs(mjg_engine* e, int id, uint32_t* pairs, size_t cap
 /* Integrator: type, delta_t, per-particle gammas (BAOAB) or alphas (G-JF), and the seed of
  * the counter-based noise stream (read_integrator.hpp:18-307). */
 mjg_status mjg_integrator_set(mjg_engine* e, int type, double dt, const double* coefs, uint64_t seed);
+/* SystemMotionRemover(translation, rotation, rescale) of the integrator (SystemMotionRemover.hpp:
+ * 21-148; TOML integrator.remove.{translation, rotation, rescale}, read_integrator.hpp:18-35):
+ * after every step, remove the centre-of-mass velocity and the rigid rotation about the centre of
+ * mass, then optionally rescale to the kinetic energy before the removal.  Default: all off. */
+mjg_status mjg_integrator_set_remover(mjg_engine* e, int translation, int rotation, int rescale);
 /* Integrator::initialize: coefficients from the temperature, then the first calc_force
  * (BAOABLangevinIntegrator.hpp:135-168, GJFNVTLangevinIntegrator.hpp:44-75). */
 mjg_status mjg_integrator_initialize(mjg_engine* e);
@@ -149,6 +154,19 @@ mjg_status mjg_step(mjg_engine* e, size_t n_steps);
 mjg_status mjg_counters(mjg_engine* e, uint64_t* steps, uint64_t* rebuilds);
 /* Kinetic energy sum_i m_i v_i^2 / 2 (EnergyObserver.hpp:89-150 prints it beside the terms). */
 mjg_status mjg_kinetic_energy(mjg_engine* e, double* ke);
+
+/* EnergyObserver (EnergyObserver.hpp:27-150): create prefix.ene (truncating it) and write the
+ * unit line and the column header: "# timestep", one column per interaction named as the
+ * reference names it, "kinetic_energy" and one "attribute:<key>" column per system attribute. */
+mjg_status mjg_observer_energy_open(mjg_engine* e, const char* prefix);
+/* EnergyObserver::output: append the line of step `step` (energies by ForceField::format_energy,
+ * the kinetic energy, the attributes) in the reference's fixed-width format.  A non-finite
+ * energy gives MJG_NAN_DETECTED after the line is written. */
+mjg_status mjg_observer_energy_output(mjg_engine* e, uint64_t step);
+/* MolecularDynamicsSimulator::run + finalize (MolecularDynamicsSimulator.hpp:84-123): take
+ * total_step integrator steps; when an energy observer is open, output at every multiple of
+ * save_step and after the last step. */
+mjg_status mjg_simulate(mjg_engine* e, uint64_t total_step, uint64_t save_step);
 
 /* Timing hook with no counterpart in the reference (it has no GPU kernels to time): launch
  * the force kernel of interaction `id` `reps` times back to back on the engine's stream into a

@@ -53,7 +53,8 @@ EXPORTED = [
     "mjg_system_set_state", "mjg_system_get_state", "mjg_system_set_attribute",
     "mjg_add_local", "mjg_add_global", "mjg_forcefield_initialize", "mjg_num_interactions",
     "mjg_calc_force", "mjg_calc_force_and_energy", "mjg_calc_energies", "mjg_calc_force_of",
-    "mjg_reduce_margin", "mjg_neighbor_pairs", "mjg_integrator_set", "mjg_integrator_initialize",
+    "mjg_reduce_margin", "mjg_neighbor_pairs", "mjg_integrator_set", "mjg_integrator_set_remover", "mjg_observer_energy_open",
+    "mjg_observer_energy_output", "mjg_simulate", "mjg_integrator_initialize",
     "mjg_step", "mjg_counters", "mjg_kinetic_energy", "mjg_profile_interaction",
     "mjg_batch_create", "mjg_batch_destroy", "mjg_batch_last_error", "mjg_batch_add_replica",
     "mjg_batch_set_attribute", "mjg_batch_add_local", "mjg_batch_add_global", "mjg_batch_set_integrator",
@@ -93,6 +94,10 @@ def lib():
     L.mjg_reduce_margin.argtypes = [vp, dbl]
     L.mjg_neighbor_pairs.argtypes = [vp, i32, vp, sz, C.POINTER(sz)]
     L.mjg_integrator_set.argtypes = [vp, i32, dbl, _dp, u64]
+    L.mjg_integrator_set_remover.argtypes = [vp, i32, i32, i32]
+    L.mjg_observer_energy_open.argtypes = [vp, C.c_char_p]
+    L.mjg_observer_energy_output.argtypes = [vp, u64]
+    L.mjg_simulate.argtypes = [vp, u64, u64]
     L.mjg_integrator_initialize.argtypes = [vp]
     L.mjg_step.argtypes = [vp, sz]
     L.mjg_counters.argtypes = [vp, C.POINTER(u64), C.POINTER(u64)]
@@ -163,6 +168,7 @@ class Engine:
         if with_integrator and spec.integrator is not None:
             it = spec.integrator
             self._check(L.mjg_integrator_set(self.h, INTEG_CODE[it.type], float(it.dt), it.coefs, int(it.seed)))
+            self._check(L.mjg_integrator_set_remover(self.h, *[int(bool(x)) for x in it.remove]))
             self._check(L.mjg_integrator_initialize(self.h))
 
     def _check(self, st: int):
@@ -236,6 +242,15 @@ class Engine:
         return ms.value, hp.value
 
     # -- System --------------------------------------------------------------------------
+    def open_energy_observer(self, prefix: str):
+        self._check(lib().mjg_observer_energy_open(self.h, prefix.encode()))
+
+    def output_energy(self, step: int):
+        self._check(lib().mjg_observer_energy_output(self.h, int(step)))
+
+    def simulate(self, total_step: int, save_step: int):
+        self._check(lib().mjg_simulate(self.h, int(total_step), int(save_step)))
+
     def state(self):
         x = np.zeros(3 * self.n); v = np.zeros(3 * self.n); f = np.zeros(3 * self.n)
         self._check(lib().mjg_system_get_state(self.h, x.ctypes.data, v.ctypes.data, f.ctypes.data))

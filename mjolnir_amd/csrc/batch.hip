@@ -460,7 +460,8 @@ class Batch
         struct GHost
         {
             std::vector<uint16_t> part;
-            std::vector<uint32_t> pslot, ex_start, ex;
+            std::vector<uint32_t> far_start, far;
+            std::vector<unsigned long long> ex_near;
             std::vector<double> ppar;
         };
         std::vector<GHost> gh(r.globals.size());
@@ -469,12 +470,29 @@ class Batch
             RepGlobalIn& G = r.globals[g];
             GHost& h = gh[g];
             h.part.assign(G.part.begin(), G.part.end());
-            h.pslot.assign(n, 0xffffffffu);
-            for(size_t k = 0; k < G.part.size(); ++k) {h.pslot[G.part[k]] = (uint32_t)k;}
             h.ppar.assign(n * G.npar, 0.0);
             for(size_t k = 0; k < G.part.size(); ++k)
                 for(int c = 0; c < G.npar; ++c) {h.ppar[(size_t)G.part[k] * G.npar + c] = G.par[k * G.npar + c];}
-            r.topo.exclusions(G.within, h.ex_start, h.ex);
+            {
+                // exclusions of each particle against higher indices: a 64-bit mask for the
+                // next 64 indices, a CSR list for the rest
+                std::vector<uint32_t> ex_start, ex;
+                r.topo.exclusions(G.within, ex_start, ex);
+                h.ex_near.assign(n, 0ull);
+                h.far_start.assign(n + 1, 0u);
+                for(size_t p = 0; p < n; ++p)
+                {
+                    h.far_start[p] = (uint32_t)h.far.size();
+                    for(uint32_t k = ex_start[p]; k < ex_start[p + 1]; ++k)
+                    {
+                        const uint32_t q = ex[k];
+                        if(q <= p) {continue;}
+                        if(q - p <= 64) {h.ex_near[p] |= 1ull << (q - p - 1);}
+                        else {h.far.push_back(q);}
+                    }
+                }
+                h.far_start[n] = (uint32_t)h.far.size();
+            }
             if(G.cap == 0)
             {
                 // first guess: a tenth of all participant pairs, at least 1024; grown on overflow.
@@ -495,15 +513,15 @@ class Batch
         const size_t o_tpar = take(n_terms * 24);
         const size_t o_bst = take(sizeof(RepState));
         if(r.slot < 0) {r.slot = take_slot();}
-        std::vector<size_t> o_part(gh.size()), o_pslot(gh.size()), o_ppar(gh.size()), o_exs(gh.size()), o_ex(gh.size()),
+        std::vector<size_t> o_part(gh.size()), o_exnear(gh.size()), o_ppar(gh.size()), o_exs(gh.size()), o_ex(gh.size()),
             o_pairs(gh.size());
         for(size_t g = 0; g < gh.size(); ++g)
         {
             o_part[g] = take(gh[g].part.size() * 2);
-            o_pslot[g] = take(n * 4);
+            o_exnear[g] = take(n * 8);
             o_ppar[g] = take(gh[g].ppar.size() * 8);
             o_exs[g] = take((n + 1) * 4);
-            o_ex[g] = take(std::max<size_t>(1, gh[g].ex.size()) * 4);
+            o_ex[g] = take(std::max<size_t>(1, gh[g].far.size()) * 4);
             o_pairs[g] = take((size_t)r.globals[g].cap * 4);
         }
         if(r.arena) {hip_check(hipFree(r.arena), "hipFree");}
@@ -585,10 +603,10 @@ class Batch
             RepGlobalIn& G = r.globals[g];
             GHost& h = gh[g];
             up(o_part[g], h.part.data(), h.part.size() * 2);
-            up(o_pslot[g], h.pslot.data(), n * 4);
+            up(o_exnear[g], h.ex_near.data(), n * 8);
             up(o_ppar[g], h.ppar.data(), h.ppar.size() * 8);
-            up(o_exs[g], h.ex_start.data(), (n + 1) * 4);
-            up(o_ex[g], h.ex.data(), h.ex.size() * 4);
+            up(o_exs[g], h.far_start.data(), (n + 1) * 4);
+            up(o_ex[g], h.far.data(), h.far.size() * 4);
             RepGlobal& RG = D.g[g];
             RG = RepGlobal{};
             RG.kind = G.pot;
@@ -604,12 +622,13 @@ class Batch
             RG.part_off = D.part_total;
             D.par_cols += (uint32_t)G.npar;
             D.part_total += (RG.np + 7u) & ~7u;
+            if(RG.use_distance && RG.np > kGridMinNp) {D.grid_np = std::max(D.grid_np, RG.np);}
             RG.cap = G.cap;
             RG.part = reinterpret_cast<const uint16_t*>(A + o_part[g]);
-            RG.pslot = reinterpret_cast<const uint32_t*>(A + o_pslot[g]);
+            RG.ex_near = reinterpret_cast<const unsigned long long*>(A + o_exnear[g]);
             RG.ppar = reinterpret_cast<const double*>(A + o_ppar[g]);
-            RG.ex_start = reinterpret_cast<const uint32_t*>(A + o_exs[g]);
-            RG.ex = reinterpret_cast<const uint32_t*>(A + o_ex[g]);
+            RG.far_start = reinterpret_cast<const uint32_t*>(A + o_exs[g]);
+            RG.far = reinterpret_cast<const uint32_t*>(A + o_ex[g]);
             RG.pairs = reinterpret_cast<uint32_t*>(A + o_pairs[g]);
             double max_cutoff = 0;
             if(G.pot == kPairLJ)
@@ -641,7 +660,7 @@ class Batch
             RG.r_list2 = r_list * r_list;
             RG.abs_margin = max_cutoff * G.margin;
         }
-        r.lds = rep_lds_bytes(D.n, D.par_cols, D.part_total, real_size());
+        r.lds = rep_lds_bytes(D.n, D.par_cols, D.part_total, real_size(), D.grid_np);
         if(r.lds > kRepLdsLimit)
         {
             throw std::invalid_argument("replica needs " + std::to_string(r.lds) + " B of LDS (limit " +

@@ -25,13 +25,16 @@ namespace mjg
 constexpr uint32_t kRepMaxN = 2048;
 constexpr int kRepMaxGlobals = 3;
 constexpr int kRepMaxLocals = 8;
-constexpr double kFixScale = 4294967296.0;         // 2^32
-constexpr double kFixInv = 2.3283064365386963e-10; // 2^-32
 // dynamic LDS available to one workgroup (160 KiB minus the kernel's static arrays)
 constexpr size_t kRepLdsLimit = 160 * 1024 - 1024;
 
 // size classes: a replica runs in the smallest class that holds it
 constexpr int kRepClasses = 3;
+
+// list rebuilds of interactions with more participants than kGridMinNp go through a cell grid
+// in LDS (at most kGridCells cells, 8 per dimension); smaller ones scan all participant pairs
+constexpr uint32_t kGridMinNp = 512;
+constexpr uint32_t kGridCells = 512;
 constexpr uint32_t kRepClassMaxN[kRepClasses] = {256, 1024, 2048};
 
 // terms and list entries per thread per load round (fewer at 1024 threads, where a thread
@@ -62,10 +65,12 @@ struct RepGlobal
     uint32_t last_slot;
     uint32_t cap;
     const uint16_t* part;      // participants (replica-local), input order
-    const uint32_t* pslot;     // per particle: participant slot or 0xffffffff
     const double* ppar;        // per particle parameters (npar columns)
-    const uint32_t* ex_start;  // exclusion CSR per particle
-    const uint32_t* ex;
+    // exclusions of particle p against higher indices q: bit q - p - 1 of ex_near[p] for
+    // q - p <= 64, the rest in the CSR list far[far_start[p] .. far_start[p + 1])
+    const unsigned long long* ex_near;
+    const uint32_t* far_start;
+    const uint32_t* far;
     uint32_t* pairs;           // packed (i << 16 | j), capacity cap
     double r_list2, abs_margin;
     double cutoff_ratio2, coef_at_cutoff, eps;           // LJ / EV
@@ -95,6 +100,7 @@ struct RepDesc
     int integ;                 // 1 BAOAB, 2 G-JF
     uint32_t par_cols;         // total LDS parameter columns
     uint32_t part_total;       // LDS participant entries (uint16, padded per interaction)
+    uint32_t grid_np;          // largest participant count of a grid-built list (0: none)
     double dt, halfdt;
     uint64_t seed;
     Box box;
@@ -121,27 +127,74 @@ struct RepDesc
 };
 
 // dynamic LDS bytes of one replica (layout in k_replica_segment)
-__host__ __device__ inline size_t rep_lds_bytes(uint32_t n, uint32_t par_cols, uint32_t part_total, size_t real_size)
+__host__ __device__ inline size_t rep_lds_bytes(uint32_t n, uint32_t par_cols, uint32_t part_total, size_t real_size,
+                                                uint32_t grid_np)
 {
-    return (size_t)n * 48 + (size_t)n * par_cols * real_size + (size_t)part_total * 2;
+    const size_t grid = grid_np ? (size_t)kGridCells * 4 + (size_t)((grid_np + 7u) & ~7u) * 4 : 0;
+    return (size_t)n * 48 + (size_t)n * par_cols * real_size + (size_t)part_total * 2 + grid;
 }
 
 // ------------------------------------------------------------------------------------
-// fixed-point accumulation
+// fixed-point accumulation: a force contribution v becomes the integer nearest v * 2^S
+// (round half to even, so conv(-v) == -conv(v)); S = 32 in the FP64 kernel, 24 in the FP32
+// kernel, where a contribution carries about 24 significant bits anyway.
 // ------------------------------------------------------------------------------------
-__device__ __forceinline__ long long to_fix(double v)
+template<typename Real> struct Fix;
+template<> struct Fix<double>
 {
-    return __double2ll_rn(v * kFixScale);
-}
-__device__ __forceinline__ void acc3(unsigned long long* F, uint32_t i, double x, double y, double z)
+    static constexpr double scale = 4294967296.0;          // 2^32
+    static constexpr double inv = 2.3283064365386963e-10;  // 2^-32
+    __device__ static __forceinline__ long long conv(double v) {return __double2ll_rn(v * scale);}
+};
+template<> struct Fix<float>
 {
-    atomicAdd(&F[3 * i + 0], (unsigned long long)to_fix(x));
-    atomicAdd(&F[3 * i + 1], (unsigned long long)to_fix(y));
-    atomicAdd(&F[3 * i + 2], (unsigned long long)to_fix(z));
+    static constexpr double scale = 16777216.0;            // 2^24
+    static constexpr double inv = 5.9604644775390625e-08;  // 2^-24
+    __device__ static __forceinline__ long long conv(double v) {return __double2ll_rn(v * scale);}
+    __device__ static __forceinline__ long long conv(float v) {return __double2ll_rn((double)v * scale);}
+};
+
+__device__ __forceinline__ void add3(unsigned long long* F, uint32_t i, long long x, long long y, long long z)
+{
+    atomicAdd(&F[3 * i + 0], (unsigned long long)x);
+    atomicAdd(&F[3 * i + 1], (unsigned long long)y);
+    atomicAdd(&F[3 * i + 2], (unsigned long long)z);
 }
+template<typename Real, typename V>
+__device__ __forceinline__ void acc3(unsigned long long* F, uint32_t i, V x, V y, V z)
+{
+    add3(F, i, Fix<Real>::conv(x), Fix<Real>::conv(y), Fix<Real>::conv(z));
+}
+template<typename Real>
 __device__ __forceinline__ double from_fix(unsigned long long v)
 {
-    return (double)(long long)v * kFixInv;
+    return (double)(long long)v * Fix<Real>::inv;
+}
+
+// Gaussian noise of the integrator: the Philox stream of noise3 (device_common.hpp); the FP32
+// kernel forms its Box-Muller transforms in FP32 from the top 24 bits of each draw.
+template<typename Real>
+__device__ __forceinline__ void rep_noise(uint64_t seed, uint64_t step, uint32_t particle, double& z0, double& z1, double& z2);
+template<>
+__device__ __forceinline__ void rep_noise<double>(uint64_t seed, uint64_t step, uint32_t particle,
+                                                  double& z0, double& z1, double& z2)
+{
+    noise3(seed, step, particle, z0, z1, z2);
+}
+template<>
+__device__ __forceinline__ void rep_noise<float>(uint64_t seed, uint64_t step, uint32_t particle,
+                                                 double& z0, double& z1, double& z2)
+{
+    uint32_t r[4];
+    philox4x32_10(particle, (uint32_t)step, (uint32_t)(step >> 32), 0x4D4A4721u, (uint32_t)seed, (uint32_t)(seed >> 32), r);
+    const float s = 5.9604644775390625e-08f;   // 2^-24
+    const float two_pi = 6.28318530717958648f;
+    const float u0 = ((float)(r[0] >> 8) + 0.5f) * s, u1 = ((float)(r[1] >> 8) + 0.5f) * s;
+    const float u2 = ((float)(r[2] >> 8) + 0.5f) * s, u3 = ((float)(r[3] >> 8) + 0.5f) * s;
+    const float a = sqrtf(-2.0f * logf(u0)), b = sqrtf(-2.0f * logf(u2));
+    z0 = (double)(a * cosf(two_pi * u1));
+    z1 = (double)(a * sinf(two_pi * u1));
+    z2 = (double)(b * cosf(two_pi * u3));
 }
 
 struct PosL
@@ -175,8 +228,9 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         const double rlen = 1.0 / sqrt(len2);
         const double len = len2 * rlen;
         const double c = (-1 * (2 * p0 * (len - p1))) * rlen;
-        acc3(F, i0, -(dx * c), -(dy * c), -(dz * c));
-        acc3(F, i1, dx * c, dy * c, dz * c);
+        const long long fx = Fix<Real>::conv(dx * c), fy = Fix<Real>::conv(dy * c), fz = Fix<Real>::conv(dz * c);
+        add3(F, i0, -fx, -fy, -fz);
+        add3(F, i1, fx, fy, fz);
         if(ENERGY) {const double dv = len - p1; e = p0 * dv * dv;}
     }
     else if(kind == kRBondGo || kind == kRContactGo)
@@ -195,8 +249,9 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         const Real v0r_10 = v0r_6 * v0r_2 * v0r_2;
         const Real v0r_12 = v0r_10 * v0r_2;
         const Real coef = (Real)(-60) * k * r2 * (v0r_10 - v0r_12);
-        acc3(F, i0, -(double)(coef * dx), -(double)(coef * dy), -(double)(coef * dz));
-        acc3(F, i1, (double)(coef * dx), (double)(coef * dy), (double)(coef * dz));
+        const long long fx = Fix<Real>::conv(coef * dx), fy = Fix<Real>::conv(coef * dy), fz = Fix<Real>::conv(coef * dz);
+        add3(F, i0, -fx, -fy, -fz);
+        add3(F, i1, fx, fy, fz);
         if(ENERGY) {e = (double)(k * ((Real)5 * v0r_12 - (Real)6 * v0r_10));}
     }
     else if(kind == kRAngleHarmonic || kind == kRAngleGaussian)
@@ -233,11 +288,14 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         const Real sin_t = sin(theta);
         const Real cis = coef / fmax(sin_t, (Real)1e-8);
         const Real ci = cis * inv_ij, ck = cis * inv_kj;
-        const Real Fix = ci * (cos_t * eix - ekx), Fiy = ci * (cos_t * eiy - eky), Fiz = ci * (cos_t * eiz - ekz);
+        const Real Fix_ = ci * (cos_t * eix - ekx), Fiy = ci * (cos_t * eiy - eky), Fiz = ci * (cos_t * eiz - ekz);
         const Real Fkx = ck * (cos_t * ekx - eix), Fky = ck * (cos_t * eky - eiy), Fkz = ck * (cos_t * ekz - eiz);
-        acc3(F, i0, Fix, Fiy, Fiz);
-        acc3(F, i1, -(double)Fix - (double)Fkx, -(double)Fiy - (double)Fky, -(double)Fiz - (double)Fkz);
-        acc3(F, i2, Fkx, Fky, Fkz);
+        // the middle body takes minus the sum of the ends, in integers (exact momentum balance)
+        const long long ix = Fix<Real>::conv(Fix_), iy = Fix<Real>::conv(Fiy), iz = Fix<Real>::conv(Fiz);
+        const long long kx = Fix<Real>::conv(Fkx), ky = Fix<Real>::conv(Fky), kz = Fix<Real>::conv(Fkz);
+        add3(F, i0, ix, iy, iz);
+        add3(F, i1, -(ix + kx), -(iy + ky), -(iz + kz));
+        add3(F, i2, kx, ky, kz);
         if(ENERGY) {e = (double)pot;}
     }
     else if(kind == kRDihedralClementi)
@@ -277,12 +335,12 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         const Real Flx = cl * nx, Fly = cl * ny, Flz = cl * nz;
         const Real cijk = (ijx * kjx + ijy * kjy + ijz * kjz) * r_kj_rlensq;
         const Real cjkl = (klx * kjx + kly * kjy + klz * kjz) * r_kj_rlensq;
-        acc3(F, i0, Fix, Fiy, Fiz);
-        acc3(F, i1, (cijk - (Real)1) * Fix - cjkl * Flx, (cijk - (Real)1) * Fiy - cjkl * Fly,
-             (cijk - (Real)1) * Fiz - cjkl * Flz);
-        acc3(F, i2, (cjkl - (Real)1) * Flx - cijk * Fix, (cjkl - (Real)1) * Fly - cijk * Fiy,
-             (cjkl - (Real)1) * Flz - cijk * Fiz);
-        acc3(F, i3, Flx, Fly, Flz);
+        acc3<Real>(F, i0, Fix, Fiy, Fiz);
+        acc3<Real>(F, i1, (cijk - (Real)1) * Fix - cjkl * Flx, (cijk - (Real)1) * Fiy - cjkl * Fly,
+                   (cijk - (Real)1) * Fiz - cjkl * Flz);
+        acc3<Real>(F, i2, (cjkl - (Real)1) * Flx - cijk * Fix, (cjkl - (Real)1) * Fly - cijk * Fiy,
+                   (cjkl - (Real)1) * Flz - cijk * Fiz);
+        acc3<Real>(F, i3, Flx, Fly, Flz);
     }
 }
 
@@ -337,26 +395,73 @@ __device__ __forceinline__ void rep_pair(const RepGlobal& G, const Real* par, ui
         c = f_mag * rl;
         if(ENERGY) {e += (double)(qq * (ex / l - (Real)G.coef_at_cutoff));}
     }
-    const double fx = (double)(dx * c), fy = (double)(dy * c), fz = (double)(dz * c);
-    acc3(F, i, fx, fy, fz);
-    acc3(F, j, -fx, -fy, -fz);
+    const long long fx = Fix<Real>::conv(dx * c), fy = Fix<Real>::conv(dy * c), fz = Fix<Real>::conv(dz * c);
+    add3(F, i, fx, fy, fz);
+    add3(F, j, -fx, -fy, -fz);
 }
 
-// pair allowed by has_interaction + the leading rule (lo < hi), replica-local
-__device__ __forceinline__ bool rep_allowed(const RepGlobal& G, const uint32_t* __restrict__ mol, uint32_t lo, uint32_t hi)
+// Exclusions of one list row (particle pa against higher indices), held in registers while
+// the row's candidates are scanned: has_interaction's exclusion test (ExclusionList.hpp:56-84)
+// without a memory access per candidate.
+struct RowRules
 {
-    const uint32_t slo = G.pslot[lo];
-    if(G.pairwise) {if(slo > G.pslot[hi]) {return false;}}
-    else {if(slo == G.last_slot) {return false;}}
-    if(G.ignore_mol == 1 && mol[lo] == mol[hi]) {return false;}
-    if(G.ignore_mol == 2 && mol[lo] != mol[hi]) {return false;}
-    for(uint32_t k = G.ex_start[lo], e = G.ex_start[lo + 1]; k < e; ++k)
+    unsigned long long near;
+    uint32_t far[4];
+    uint32_t f0, f1;
+    uint32_t mol;
+};
+
+__device__ __forceinline__ RowRules row_rules(const RepGlobal& G, const uint32_t* __restrict__ mol, uint32_t pa)
+{
+    RowRules r;
+    r.near = G.ex_near[pa];
+    r.f0 = G.far_start[pa];
+    r.f1 = G.far_start[pa + 1];
+#pragma unroll
+    for(int k = 0; k < 4; ++k) {r.far[k] = (r.f0 + k < r.f1) ? G.far[r.f0 + k] : 0xffffffffu;}
+    r.mol = G.ignore_mol ? mol[pa] : 0u;
+    return r;
+}
+
+// has_interaction(pa, pb) for pa < pb plus the partition's leading rule on participant slots
+// a (of pa) and b (of pb): Verlet / Naive pair b after a in input order, cell list any pair
+// whose lower particle is not the last participant (LorentzBerthelotRule, ParameterList.hpp:
+// 240-252,285-289; VerletList.hpp:99-134)
+__device__ __forceinline__ bool rep_allowed(const RepGlobal& G, const RowRules& r, const uint32_t* __restrict__ mol,
+                                            uint32_t a, uint32_t b, uint32_t pa, uint32_t pb)
+{
+    if(G.pairwise) {if(a > b) {return false;}}
+    else {if(a == G.last_slot) {return false;}}
+    if(G.ignore_mol == 1 && r.mol == mol[pb]) {return false;}
+    if(G.ignore_mol == 2 && r.mol != mol[pb]) {return false;}
+    const uint32_t d = pb - pa;
+    if(d <= 64) {return ((r.near >> (d - 1)) & 1ull) == 0ull;}
+    if(pb == r.far[0] || pb == r.far[1] || pb == r.far[2] || pb == r.far[3]) {return false;}
+    for(uint32_t k = r.f0 + 4; k < r.f1; ++k)
     {
-        const uint32_t x = G.ex[k];
-        if(x > hi) {break;}
-        if(x == hi) {return false;}
+        const uint32_t x = G.far[k];
+        if(x > pb) {break;}
+        if(x == pb) {return false;}
     }
     return true;
+}
+
+// Append (pa << 16 | pb) for the lanes that want it: one LDS atomic per wave, lane offsets
+// from the ballot.  Every lane active at the call site must make the call.
+__device__ __forceinline__ void wave_append(bool want, uint32_t val, uint32_t* count, uint32_t cap, uint32_t* pairs)
+{
+    const unsigned long long m = __ballot(want);
+    if(m == 0ull) {return;}
+    const int leader = __ffsll((long long)m) - 1;
+    const int lane = (int)__lane_id();
+    uint32_t base = 0;
+    if(lane == leader) {base = atomicAdd(count, (uint32_t)__popcll(m));}
+    base = (uint32_t)__shfl((int)base, leader, kWave);
+    if(want)
+    {
+        const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if(k < cap) {pairs[k] = val;}
+    }
 }
 
 // List build of one interaction inside the workgroup: every participant pair (each once, owned
@@ -374,20 +479,146 @@ __device__ void rep_build_rows(const RepGlobal& G, const RepDesc& R, const PosL&
         const uint32_t a = (m & 1) ? m * BS + (BS - 1 - tid) : m * BS + tid;
         if(a >= np) {continue;}
         const uint32_t pa = part[a];
+        const RowRules rr = row_rules(G, R.mol, pa);
         const double xa = P.x[pa], ya = P.y[pa], za = P.z[pa];
         for(uint32_t b = G.sorted ? a + 1 : 0; b < np; ++b)
         {
             const uint32_t pb = part[b];
-            if(pb <= pa) {continue;}
-            if(G.use_distance)
+            bool want = pb > pa;
+            if(want && G.use_distance)
             {
                 double dx, dy, dz;
                 adjust_direction(R.box, xa, ya, za, P.x[pb], P.y[pb], P.z[pb], dx, dy, dz);
-                if(!(length_sq(dx, dy, dz) < G.r_list2)) {continue;}
+                want = length_sq(dx, dy, dz) < G.r_list2;
             }
-            if(!rep_allowed(G, R.mol, pa, pb)) {continue;}
-            const uint32_t k = atomicAdd(count, 1u);
-            if(k < G.cap) {G.pairs[k] = (pa << 16) | pb;}
+            want = want && rep_allowed(G, rr, R.mol, a, b, pa, pb);
+            wave_append(want, (pa << 16) | pb, count, G.cap, G.pairs);
+        }
+    }
+}
+
+// Cell-grid list build (same pair set as rep_build_rows).  Cells are at least r_list wide,
+// so every pair within r_list sits in the same or adjacent cells: a periodic box gets
+// min(8, max(3, floor(L / r_list))) cells per dimension with wrap-around; an unbounded system
+// hashes floor(x / r_list) modulo 8 per dimension (UnlimitedGridCellList.hpp:105-125 uses the
+// same kind of hash), where far-apart particles can share a cell and the distance test
+// rejects them.  Participants are counting-sorted by cell in LDS.
+struct GridShape
+{
+    int dims[3];
+    double lo[3], inv_w[3];
+    int periodic;
+};
+
+__device__ __forceinline__ GridShape grid_shape(const Box& box, double r_list)
+{
+    GridShape g;
+    g.periodic = box.periodic;
+    for(int d = 0; d < 3; ++d)
+    {
+        if(box.periodic)
+        {
+            const double f = floor(box.width[d] / r_list);
+            g.dims[d] = (int)fmin(8.0, fmax(3.0, f));
+            g.lo[d] = box.lower[d];
+            g.inv_w[d] = (double)g.dims[d] / box.width[d];
+        }
+        else
+        {
+            g.dims[d] = 8;
+            g.lo[d] = 0.0;
+            g.inv_w[d] = 1.0 / r_list;
+        }
+    }
+    return g;
+}
+
+__device__ __forceinline__ int grid_coord(const GridShape& g, int d, double x)
+{
+    double c = floor((x - g.lo[d]) * g.inv_w[d]);
+    c = fmin(fmax(c, -1.0e9), 1.0e9);
+    const int ic = (int)c;
+    if(g.periodic) {return ic < 0 ? 0 : (ic >= g.dims[d] ? g.dims[d] - 1 : ic);}
+    return ic & 7;
+}
+
+__device__ __forceinline__ int grid_wrap(const GridShape& g, int d, int c)
+{
+    if(g.periodic) {return c < 0 ? c + g.dims[d] : (c >= g.dims[d] ? c - g.dims[d] : c);}
+    return c & 7;
+}
+
+template<int BS>
+__device__ void rep_build_grid(const RepGlobal& G, const RepDesc& R, const PosL& P, const uint16_t* part,
+                               uint32_t* cell_end, uint16_t* key, uint16_t* sorted, uint32_t* count)
+{
+    const uint32_t np = G.np;
+    const uint32_t tid = threadIdx.x;
+    const GridShape gs = grid_shape(R.box, sqrt(G.r_list2));
+    for(uint32_t c = tid; c < kGridCells; c += BS) {cell_end[c] = 0;}
+    __syncthreads();
+    for(uint32_t a = tid; a < np; a += BS)
+    {
+        const uint32_t pa = part[a];
+        const int k = grid_coord(gs, 0, P.x[pa]) + 8 * grid_coord(gs, 1, P.y[pa]) + 64 * grid_coord(gs, 2, P.z[pa]);
+        key[a] = (uint16_t)k;
+        atomicAdd(&cell_end[k], 1u);
+    }
+    __syncthreads();
+    // exclusive scan of the 512 counts in the first wave: 8 consecutive cells per lane
+    if(tid < (uint32_t)kWave)
+    {
+        uint32_t v[8], sum = 0;
+#pragma unroll
+        for(int q = 0; q < 8; ++q) {v[q] = cell_end[tid * 8 + q]; sum += v[q];}
+        uint32_t incl = sum;
+#pragma unroll
+        for(int o = 1; o < kWave; o <<= 1)
+        {
+            const uint32_t t = __shfl_up(incl, o, kWave);
+            if((int)tid >= o) {incl += t;}
+        }
+        uint32_t run = incl - sum;
+#pragma unroll
+        for(int q = 0; q < 8; ++q) {cell_end[tid * 8 + q] = run; run += v[q];}
+    }
+    __syncthreads();
+    // scatter: afterwards cell_end[k] is the end of cell k (the start of k + 1)
+    for(uint32_t a = tid; a < np; a += BS)
+    {
+        const uint32_t slot = atomicAdd(&cell_end[key[a]], 1u);
+        sorted[slot] = (uint16_t)a;
+    }
+    __syncthreads();
+    const uint32_t nblk = (np + BS - 1) / BS;
+    for(uint32_t m = 0; m < nblk; ++m)
+    {
+        const uint32_t a = (m & 1) ? m * BS + (BS - 1 - tid) : m * BS + tid;
+        if(a >= np) {continue;}
+        const uint32_t pa = part[a];
+        const RowRules rr = row_rules(G, R.mol, pa);
+        const double xa = P.x[pa], ya = P.y[pa], za = P.z[pa];
+        const int cx = grid_coord(gs, 0, xa), cy = grid_coord(gs, 1, ya), cz = grid_coord(gs, 2, za);
+        for(int dz = -1; dz <= 1; ++dz)
+        for(int dy = -1; dy <= 1; ++dy)
+        for(int dx = -1; dx <= 1; ++dx)
+        {
+            const int c = grid_wrap(gs, 0, cx + dx) + 8 * grid_wrap(gs, 1, cy + dy) + 64 * grid_wrap(gs, 2, cz + dz);
+            const uint32_t beg = c ? cell_end[c - 1] : 0u, end = cell_end[c];
+            for(uint32_t q = beg; q < end; ++q)
+            {
+                const uint32_t b = sorted[q];
+                const uint32_t pb = part[b];
+                bool want = pb > pa;
+                if(want)
+                {
+                    double ddx, ddy, ddz;
+                    adjust_direction(R.box, xa, ya, za, P.x[pb], P.y[pb], P.z[pb], ddx, ddy, ddz);
+                    want = length_sq(ddx, ddy, ddz) < G.r_list2;
+                }
+                want = want && rep_allowed(G, rr, R.mol, a, b, pa, pb);
+                wave_append(want, (pa << 16) | pb, count, G.cap, G.pairs);
+            }
         }
     }
 }
@@ -415,6 +646,9 @@ k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
     unsigned long long* F = reinterpret_cast<unsigned long long*>(P.z + n);
     Real* par_lds = reinterpret_cast<Real*>(F + 3 * n);
     uint16_t* part_lds = reinterpret_cast<uint16_t*>(par_lds + (size_t)n * R.par_cols);
+    uint32_t* grid_end = reinterpret_cast<uint32_t*>(part_lds + R.part_total);
+    uint16_t* grid_key = reinterpret_cast<uint16_t*>(grid_end + kGridCells);
+    uint16_t* grid_sorted = grid_key + ((R.grid_np + 7u) & ~7u);
 
     __shared__ double s_red[BS / kWave];
     __shared__ double s_margin[kRepMaxGlobals];
@@ -441,9 +675,9 @@ k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
             const double4 a = R.pos[p], v = R.vel[p], f = R.force[p];
             vx[k] = v.x; vy[k] = v.y; vz[k] = v.z;
             P.x[p] = a.x; P.y[p] = a.y; P.z[p] = a.z;
-            F[3 * p + 0] = (unsigned long long)to_fix(f.x);
-            F[3 * p + 1] = (unsigned long long)to_fix(f.y);
-            F[3 * p + 2] = (unsigned long long)to_fix(f.z);
+            F[3 * p + 0] = (unsigned long long)Fix<Real>::conv(f.x);
+            F[3 * p + 1] = (unsigned long long)Fix<Real>::conv(f.y);
+            F[3 * p + 2] = (unsigned long long)Fix<Real>::conv(f.z);
             if(!eval_only) {R.bak_pos[p] = a; R.bak_vel[p] = v; R.bak_force[p] = f;}
         }
     }
@@ -480,9 +714,10 @@ k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
                 const uint32_t p = tid + k * BS;
                 if(p >= n) {continue;}
                 const double rmk = R.rmass[p], c1k = R.c1[p], c2k = R.c2[p];
-                const double fxk = from_fix(F[3 * p + 0]), fyk = from_fix(F[3 * p + 1]), fzk = from_fix(F[3 * p + 2]);
+                const double fxk = from_fix<Real>(F[3 * p + 0]), fyk = from_fix<Real>(F[3 * p + 1]);
+                const double fzk = from_fix<Real>(F[3 * p + 2]);
                 double Rx, Ry, Rz;
-                noise3(R.seed, step, p, Rx, Ry, Rz);
+                rep_noise<Real>(R.seed, step, p, Rx, Ry, Rz);
                 double dpx, dpy, dpz;
                 if(R.integ == 1)
                 {
@@ -547,7 +782,15 @@ k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
         {
             for(int g = 0; g < ng; ++g)
             {
-                if(s_rebuild[g]) {rep_build_rows<BS>(R.g[g], R, P, part_lds + R.g[g].part_off, &s_count[g]);}
+                if(!s_rebuild[g]) {continue;}
+                const RepGlobal& G = R.g[g];
+                if(G.use_distance && G.np > kGridMinNp)
+                {
+                    // the grid build synchronizes internally; the condition is uniform
+                    rep_build_grid<BS>(G, R, P, part_lds + G.part_off, grid_end, grid_key, grid_sorted, &s_count[g]);
+                    __syncthreads();
+                }
+                else {rep_build_rows<BS>(G, R, P, part_lds + G.part_off, &s_count[g]);}
             }
             __syncthreads();
             if(tid == 0)
@@ -628,9 +871,9 @@ k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
                 const uint32_t p = tid + k * BS;
                 if(p >= n) {continue;}
                 const double hrm = R.halfdt * R.rmass[p];
-                vx[k] += hrm * from_fix(F[3 * p + 0]);
-                vy[k] += hrm * from_fix(F[3 * p + 1]);
-                vz[k] += hrm * from_fix(F[3 * p + 2]);
+                vx[k] += hrm * from_fix<Real>(F[3 * p + 0]);
+                vy[k] += hrm * from_fix<Real>(F[3 * p + 1]);
+                vz[k] += hrm * from_fix<Real>(F[3 * p + 2]);
             }
             ++step;
         }
@@ -643,7 +886,8 @@ k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
         {
             R.pos[p] = make_double4(P.x[p], P.y[p], P.z[p], 0.0);
             R.vel[p] = make_double4(vx[k], vy[k], vz[k], 0.0);
-            R.force[p] = make_double4(from_fix(F[3 * p + 0]), from_fix(F[3 * p + 1]), from_fix(F[3 * p + 2]), 0.0);
+            R.force[p] = make_double4(from_fix<Real>(F[3 * p + 0]), from_fix<Real>(F[3 * p + 1]),
+                                      from_fix<Real>(F[3 * p + 2]), 0.0);
         }
     }
     if(tid == 0)

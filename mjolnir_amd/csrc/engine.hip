@@ -6,7 +6,10 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <iomanip>
 #include <new>
+#include <sstream>
 
 namespace mjg
 {
@@ -852,6 +855,16 @@ void Engine::set_integrator(int type, double dt, const double* coefs, uint64_t s
     if(graph_exec_) {(void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr;}
 }
 
+// SystemMotionRemover(translation, rotation, rescale) (SystemMotionRemover.hpp:21-31), applied
+// after every step as the integrators do (BAOABLangevinIntegrator.hpp:236-237).
+void Engine::set_remover(int translation, int rotation, int rescale)
+{
+    rm_translation_ = translation != 0;
+    rm_rotation_ = rotation != 0;
+    rm_rescale_ = rescale != 0;
+    if(graph_exec_) {(void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr;}
+}
+
 // reset_parameters (BAOABLangevinIntegrator.hpp:73-99; GJFNVTLangevinIntegrator.hpp:106-134)
 // and the first force call (:151-166 / :58-73).
 void Engine::integrator_initialize()
@@ -902,6 +915,11 @@ void Engine::launch_step()
     }
     if(!globals_.empty()) {launch_rebuilds(0);}
     forces(d_force_, false, -1, true);
+    if(rm_translation_ || rm_rotation_)
+    {
+        hipLaunchKernelGGL(k_remove_motion, dim3(1), dim3(kRemoverThreads), 0, stream_, (uint32_t)n_, d_pos_, d_vel_,
+                           d_mass_, rm_translation_, rm_rotation_, rm_rescale_);
+    }
 }
 
 void Engine::run_segment(size_t n)
@@ -992,6 +1010,107 @@ void Engine::counters(uint64_t* steps, uint64_t* rebuilds)
     download(&sc, d_sctl_, 1);
     if(steps) {*steps = sc.step;}
     if(rebuilds) {*rebuilds = sc.rebuilds;}
+}
+
+// ------------------------------------------------------------------------------------
+// observers and the simulator loop
+// ------------------------------------------------------------------------------------
+
+// the energy-column name each interaction prints (LocalInteractionBase::name / GlobalInteraction
+// name(): BondLengthInteraction.hpp:88, BondLengthGoContactInteraction.hpp:102,
+// GoContactInteraction.hpp:134, BondAngleInteraction.hpp:100, DihedralAngleInteraction.hpp:95,
+// GlobalPairLennardJonesInteraction.hpp:134, GlobalPairExcludedVolumeInteraction.hpp:134,
+// GlobalPairInteraction.hpp:112)
+std::string Engine::interaction_name(size_t id) const
+{
+    if(id < locals_.size())
+    {
+        switch(locals_[id].kind)
+        {
+            case MJG_BOND_HARMONIC: return "BondLength:Harmonic";
+            case MJG_BOND_GOCONTACT: return "BondLengthGoContact";
+            case MJG_CONTACT_GOCONTACT: return "Contact:GoContact";
+            case MJG_ANGLE_HARMONIC: return "BondAngle:Harmonic";
+            case MJG_ANGLE_GAUSSIAN: return "BondAngle:Gaussian";
+            case MJG_DIHEDRAL_CLEMENTI: return "DihedralAngle:ClementiDihedral";
+            default: return "Unknown";
+        }
+    }
+    switch(globals_.at(id - locals_.size()).pot)
+    {
+        case MJG_PAIR_LJ: return "GlobalPairLennardJones";
+        case MJG_PAIR_EV: return "GlobalPairExcludedVolume";
+        case MJG_PAIR_DH: return "Pair:DebyeHuckel";
+        default: return "Unknown";
+    }
+}
+
+// EnergyObserver constructor + initialize (EnergyObserver.hpp:27-63): truncate prefix.ene and
+// write the unit line and the column header.  System attributes print in key order, as the
+// reference's std::map iterates them.
+void Engine::energy_observer_open(const std::string& prefix)
+{
+    if(!initialized_) {throw std::logic_error("force field not initialized");}
+    const std::string path = prefix + ".ene";
+    std::ofstream ofs(path);
+    if(!ofs.good()) {throw std::runtime_error("[error] EnergyObserver: file open error: " + path);}
+    ofs << "# unit of length : angstrom, unit of energy : kcal/mol\n";
+    ofs << "# timestep  ";
+    for(size_t k = 0; k < num_interactions(); ++k) {ofs << interaction_name(k) << ' ';}
+    ofs << " kinetic_energy";
+    if(has_I_) {ofs << " attribute:ionic_strength";}
+    if(has_T_) {ofs << " attribute:temperature";}
+    ofs << '\n';
+    ene_path_ = path;
+}
+
+// EnergyObserver::output (EnergyObserver.hpp:87-150) with ForceField::format_energy
+// (LocalForceField.hpp:173-200, GlobalForceField.hpp:170-191): step, one fixed-format column per
+// interaction (width max(name length, 10)), the kinetic energy, the attributes.  Returns false
+// when a value is not finite (the reference writes the line, then throws).
+bool Engine::energy_observer_output(uint64_t step)
+{
+    if(ene_path_.empty()) {throw std::logic_error("no energy observer is open");}
+    const size_t ni = num_interactions();
+    std::vector<double> e(std::max<size_t>(1, ni));
+    double total = 0;
+    calc_energies(e.data(), &total);
+    const double ek = kinetic_energy();
+    std::ostringstream line;
+    line << std::setw(11) << std::left << std::to_string(step) << ' ';
+    bool ok = std::isfinite(total) && std::isfinite(ek);
+    for(size_t k = 0; k < ni; ++k)
+    {
+        const size_t w = std::max<size_t>(interaction_name(k).size(), 10);
+        line << std::setw((int)w) << std::fixed << std::right << e[k] << ' ';
+    }
+    line << std::setw(14) << std::right << std::fixed << ek;
+    if(has_I_) {line << ' ' << std::setw(10 + 14) << std::right << std::fixed << I_;}
+    if(has_T_) {line << ' ' << std::setw(10 + 11) << std::right << std::fixed << T_;}
+    std::ofstream ofs(ene_path_, std::ios::app);
+    ofs << line.str() << std::endl;
+    return ok;
+}
+
+// MolecularDynamicsSimulator::run + finalize (MolecularDynamicsSimulator.hpp:84-123): output at
+// every step that is a multiple of save_step, then after the last step; the steps between
+// outputs run on the device in one call.
+void Engine::simulate(uint64_t total_step, uint64_t save_step)
+{
+    if(!integ_ready_) {throw std::logic_error("integrator not initialized");}
+    if(save_step == 0) {throw std::invalid_argument("save_step must be positive");}
+    uint64_t done = 0;
+    while(done < total_step)
+    {
+        if(done % save_step == 0 && !ene_path_.empty())
+        {
+            if(!energy_observer_output(done)) {throw NonFinite("Energy value becomes NaN");}
+        }
+        const uint64_t next = std::min<uint64_t>(total_step, (done / save_step + 1) * save_step);
+        step((size_t)(next - done));
+        done = next;
+    }
+    if(!ene_path_.empty() && !energy_observer_output(total_step)) {throw NonFinite("Energy value becomes NaN");}
 }
 
 double Engine::kinetic_energy()
@@ -1186,6 +1305,25 @@ mjg_status mjg_neighbor_pairs(mjg_engine* e, int id, uint32_t* pairs, size_t cap
 mjg_status mjg_integrator_set(mjg_engine* e, int type, double dt, const double* coefs, uint64_t seed)
 {
     return guarded(e, [&](mjg::Engine& x) {x.set_integrator(type, dt, coefs, seed);});
+}
+mjg_status mjg_observer_energy_open(mjg_engine* e, const char* prefix)
+{
+    if(!prefix) {return MJG_INVALID_ARG;}
+    return guarded(e, [&](mjg::Engine& x) {x.energy_observer_open(prefix);});
+}
+mjg_status mjg_observer_energy_output(mjg_engine* e, uint64_t step)
+{
+    return guarded(e, [&](mjg::Engine& x) {
+        if(!x.energy_observer_output(step)) {throw mjg::NonFinite("Energy value becomes NaN");}
+    });
+}
+mjg_status mjg_simulate(mjg_engine* e, uint64_t total_step, uint64_t save_step)
+{
+    return guarded(e, [&](mjg::Engine& x) {x.simulate(total_step, save_step);});
+}
+mjg_status mjg_integrator_set_remover(mjg_engine* e, int translation, int rotation, int rescale)
+{
+    return guarded(e, [&](mjg::Engine& x) {x.set_remover(translation, rotation, rescale);});
 }
 mjg_status mjg_integrator_initialize(mjg_engine* e)
 {

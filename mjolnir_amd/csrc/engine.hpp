@@ -91,10 +91,16 @@ class Engine
     size_t neighbor_pairs(int id, uint32_t* pairs, size_t cap);
 
     void set_integrator(int type, double dt, const double* coefs, uint64_t seed);
+    void set_remover(int translation, int rotation, int rescale);
     void integrator_initialize();
     void step(size_t n);
     void counters(uint64_t* steps, uint64_t* rebuilds);
     double kinetic_energy();
+    // observers and the simulator loop
+    std::string interaction_name(size_t id) const;
+    void energy_observer_open(const std::string& prefix);
+    bool energy_observer_output(uint64_t step);
+    void simulate(uint64_t total_step, uint64_t save_step);
     double profile_interaction(int id, int reps, uint64_t* half_pairs);
 
     std::string last_error;
@@ -127,6 +133,7 @@ class Engine
     std::vector<double> mass_;
     bool has_T_ = false, has_I_ = false, initialized_ = false, integ_ready_ = false, state_set_ = false;
     double T_ = 0, I_ = 0;
+    std::string ene_path_;          // open energy observer ("" if none)
 
     double4 *d_pos_ = nullptr, *d_vel_ = nullptr, *d_force_ = nullptr, *d_force_tmp_ = nullptr;
     double4 *d_bak_pos_ = nullptr, *d_bak_vel_ = nullptr, *d_bak_force_ = nullptr;
@@ -149,6 +156,7 @@ class Engine
 
     // integrator
     int integ_ = 0;
+    int rm_translation_ = 0, rm_rotation_ = 0, rm_rescale_ = 0;   // SystemMotionRemover
     double dt_ = 0;
     uint64_t seed_ = 0;
     std::vector<double> coefs_;

@@ -112,4 +112,156 @@ k_kinetic(uint32_t n, const double4* __restrict__ vel, const double* __restrict_
     }
 }
 
+
+// ------------------------------------------------------------------------------------
+// SystemMotionRemover::remove (SystemMotionRemover.hpp:42-148) after the closing kick: remove
+// the centre-of-mass velocity, then the rigid rotation about the centre of mass (inertia tensor
+// inverted as math::inverse, math/Matrix.hpp:340-375), then optionally rescale to the kinetic
+// energy before removal.  One workgroup; every sum is a fixed-order block reduction, so the
+// result is deterministic (it differs from the reference's sequential sums in the last bits).
+// ------------------------------------------------------------------------------------
+constexpr int kRemoverThreads = 1024;
+
+// sums of K doubles over the block, broadcast to every thread
+template<int K>
+__device__ __forceinline__ void block_sums(double (&v)[K], double* lds /* >= 16*K + K */)
+{
+    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave, nw = blockDim.x / kWave;
+#pragma unroll
+    for(int k = 0; k < K; ++k)
+    {
+        const double s = wave_sum(v[k]);
+        if(l == 0) {lds[w * K + k] = s;}
+    }
+    __syncthreads();
+    if(threadIdx.x < (unsigned)K)
+    {
+        double t = 0.0;
+        for(int q = 0; q < nw; ++q) {t += lds[q * K + threadIdx.x];}
+        lds[16 * K + threadIdx.x] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for(int k = 0; k < K; ++k) {v[k] = lds[16 * K + k];}
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(kRemoverThreads)
+k_remove_motion(uint32_t n, const double4* __restrict__ pos, double4* __restrict__ vel, const double* __restrict__ mass,
+                int translation, int rotation, int rescale)
+{
+    __shared__ double lds[16 * 10 + 10];
+    __shared__ double s_w[3];
+    __shared__ int s_ok;
+    const uint32_t tid = threadIdx.x, bs = blockDim.x;
+    double E_pre = 0.0;
+    if(rescale)
+    {
+        double a[1] = {0.0};
+        for(uint32_t i = tid; i < n; i += bs) {const double4 v = vel[i]; a[0] += mass[i] * length_sq(v.x, v.y, v.z);}
+        block_sums<1>(a, lds);
+        E_pre = a[0] * 0.5;
+    }
+    if(translation)
+    {
+        double a[4] = {0.0, 0.0, 0.0, 0.0};
+        for(uint32_t i = tid; i < n; i += bs)
+        {
+            const double m = mass[i];
+            const double4 v = vel[i];
+            a[0] += m; a[1] += m * v.x; a[2] += m * v.y; a[3] += m * v.z;
+        }
+        block_sums<4>(a, lds);
+        const double inv = 1.0 / a[0];
+        const double tx = a[1] * inv, ty = a[2] * inv, tz = a[3] * inv;
+        for(uint32_t i = tid; i < n; i += bs)
+        {
+            double4 v = vel[i];
+            v.x -= tx; v.y -= ty; v.z -= tz;
+            vel[i] = v;
+        }
+    }
+    if(rotation)
+    {
+        double a[4] = {0.0, 0.0, 0.0, 0.0};
+        for(uint32_t i = tid; i < n; i += bs)
+        {
+            const double m = mass[i];
+            const double4 r = pos[i];
+            a[0] += m; a[1] += m * r.x; a[2] += m * r.y; a[3] += m * r.z;
+        }
+        block_sums<4>(a, lds);
+        const double inv = 1.0 / a[0];
+        const double cx = a[1] * inv, cy = a[2] * inv, cz = a[3] * inv;
+        // L (3) and the upper triangle of I (6)
+        double b[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for(uint32_t i = tid; i < n; i += bs)
+        {
+            const double m = mass[i];
+            const double4 p = pos[i];
+            const double4 v = vel[i];
+            const double rx = p.x - cx, ry = p.y - cy, rz = p.z - cz;
+            b[0] += m * (ry * v.z - rz * v.y);
+            b[1] += m * (rz * v.x - rx * v.z);
+            b[2] += m * (rx * v.y - ry * v.x);
+            const double Ixx = m * rx * rx, Iyy = m * ry * ry, Izz = m * rz * rz;
+            b[3] += Iyy + Izz;              // I00
+            b[4] += Izz + Ixx;              // I11
+            b[5] += Ixx + Iyy;              // I22
+            b[6] -= m * rx * ry;            // I01
+            b[7] -= m * rz * rx;            // I02
+            b[8] -= m * ry * rz;            // I12
+        }
+        block_sums<9>(b, lds);
+        if(tid == 0)
+        {
+            const double I00 = b[3], I11 = b[4], I22 = b[5], I01 = b[6], I02 = b[7], I12 = b[8];
+            const double I10 = I01, I20 = I02, I21 = I12;
+            const double det = I00 * I11 * I22 + I10 * I21 * I02 + I20 * I01 * I12 -
+                               I00 * I21 * I12 - I20 * I11 * I02 - I10 * I01 * I22;
+            s_ok = det != 0.0;
+            if(s_ok)
+            {
+                const double di = 1.0 / det;
+                const double J00 = di * (I11 * I22 - I12 * I21), J11 = di * (I00 * I22 - I02 * I20);
+                const double J22 = di * (I00 * I11 - I01 * I10), J01 = di * (I02 * I21 - I01 * I22);
+                const double J02 = di * (I01 * I12 - I02 * I11), J12 = di * (I02 * I10 - I00 * I12);
+                const double J10 = di * (I12 * I20 - I10 * I22), J20 = di * (I10 * I21 - I20 * I11);
+                const double J21 = di * (I20 * I01 - I00 * I21);
+                s_w[0] = J00 * b[0] + J01 * b[1] + J02 * b[2];
+                s_w[1] = J10 * b[0] + J11 * b[1] + J12 * b[2];
+                s_w[2] = J20 * b[0] + J21 * b[1] + J22 * b[2];
+            }
+        }
+        __syncthreads();
+        if(s_ok)
+        {
+            const double wx = s_w[0], wy = s_w[1], wz = s_w[2];
+            for(uint32_t i = tid; i < n; i += bs)
+            {
+                const double4 p = pos[i];
+                double4 v = vel[i];
+                const double rx = p.x - cx, ry = p.y - cy, rz = p.z - cz;
+                v.x -= wy * rz - wz * ry;
+                v.y -= wz * rx - wx * rz;
+                v.z -= wx * ry - wy * rx;
+                vel[i] = v;
+            }
+        }
+    }
+    if(rescale)
+    {
+        double a[1] = {0.0};
+        for(uint32_t i = tid; i < n; i += bs) {const double4 v = vel[i]; a[0] += mass[i] * length_sq(v.x, v.y, v.z);}
+        block_sums<1>(a, lds);
+        const double scale = sqrt(E_pre / (a[0] * 0.5));
+        for(uint32_t i = tid; i < n; i += bs)
+        {
+            double4 v = vel[i];
+            v.x *= scale; v.y *= scale; v.z *= scale;
+            vel[i] = v;
+        }
+    }
+}
+
 } // mjg

@@ -22,7 +22,7 @@ that runs the reference's own classes.
 from __future__ import annotations
 
 from dataclasses import dataclass, field
-from typing import Dict, List, Optional
+from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
@@ -116,6 +116,8 @@ class IntegratorSpec:
     dt: float
     coefs: np.ndarray          # gammas (BAOAB) or alphas (G-JF), one per particle
     seed: int = 12345
+    # SystemMotionRemover flags remove.{translation, rotation, rescale} (read_integrator.hpp:18-35)
+    remove: Tuple[bool, bool, bool] = (False, False, False)
 
     def __post_init__(self):
         self.coefs = np.ascontiguousarray(self.coefs, dtype=np.float64).reshape(-1)
@@ -186,6 +188,8 @@ def write_mjspec(spec: SystemSpec, path: str) -> None:
         it = spec.integrator
         out.append(f"integrator {it.type} {_fmt(it.dt)} {int(it.seed)}")
         out.append(" ".join(_fmt(c) for c in it.coefs))
+        if any(it.remove):
+            out.append("remover " + " ".join(str(int(bool(x))) for x in it.remove))
     out.append("end")
     with open(path, "w") as f:
         f.write("\n".join(out) + "\n")

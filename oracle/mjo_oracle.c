@@ -160,6 +160,7 @@ struct orc_engine {
     size_t* mol; size_t n_mol;
     /* integrator */
     int integ; double dt, halfdt; uint64_t seed; uint64_t step;
+    int rm_translation, rm_rotation, rm_rescale;   /* SystemMotionRemover flags */
     double *coef, *c1, *c2, *c3; /* BAOAB: exp(-g dt), noise; GJF: b, beta, a */
     size_t rebuilds;
 };
@@ -1364,6 +1365,107 @@ static void gjf_step(orc_engine* e)
     }
 }
 
+/* SystemMotionRemover::remove (SystemMotionRemover.hpp:42-148): sequential sums in particle
+ * order; the inertia tensor is inverted by math::determinant / math::inverse
+ * (math/Matrix.hpp:340-375). */
+static void remove_motion(orc_engine* e)
+{
+    const size_t n = e->n;
+    double E_pre = 0;
+    if(e->rm_rescale)
+    {
+        for(size_t i = 0; i < n; ++i) E_pre += e->mass[i] * length_sq(e->vel + 3 * i);
+        E_pre *= 0.5;
+    }
+    if(e->rm_translation)
+    {
+        double m_tot = 0, t[3] = {0, 0, 0};
+        for(size_t i = 0; i < n; ++i)
+        {
+            m_tot += e->mass[i];
+            for(int a = 0; a < 3; ++a) t[a] += e->mass[i] * e->vel[3 * i + a];
+        }
+        const double inv = 1.0 / m_tot;
+        for(int a = 0; a < 3; ++a) t[a] *= inv;
+        for(size_t i = 0; i < n; ++i)
+            for(int a = 0; a < 3; ++a) e->vel[3 * i + a] -= t[a];
+    }
+    if(e->rm_rotation)
+    {
+        double m_tot = 0, com[3] = {0, 0, 0};
+        for(size_t i = 0; i < n; ++i)
+        {
+            m_tot += e->mass[i];
+            for(int a = 0; a < 3; ++a) com[a] += e->mass[i] * e->pos[3 * i + a];
+        }
+        const double inv = 1.0 / m_tot;
+        for(int a = 0; a < 3; ++a) com[a] *= inv;
+        double L[3] = {0, 0, 0};
+        double I[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for(size_t i = 0; i < n; ++i)
+        {
+            const double m = e->mass[i];
+            const double r[3] = {e->pos[3 * i] - com[0], e->pos[3 * i + 1] - com[1], e->pos[3 * i + 2] - com[2]};
+            const double* v = e->vel + 3 * i;
+            const double c[3] = {r[1] * v[2] - r[2] * v[1], r[2] * v[0] - r[0] * v[2], r[0] * v[1] - r[1] * v[0]};
+            for(int a = 0; a < 3; ++a) L[a] += m * c[a];
+            const double Ixx = m * r[0] * r[0], Iyy = m * r[1] * r[1], Izz = m * r[2] * r[2];
+            const double Ixy = m * r[0] * r[1], Iyz = m * r[1] * r[2], Izx = m * r[2] * r[0];
+            I[0][0] += Iyy + Izz;
+            I[1][1] += Izz + Ixx;
+            I[2][2] += Ixx + Iyy;
+            I[0][1] -= Ixy;
+            I[0][2] -= Izx;
+            I[1][2] -= Iyz;
+        }
+        I[1][0] = I[0][1];
+        I[2][0] = I[0][2];
+        I[2][1] = I[1][2];
+        const double det = I[0][0] * I[1][1] * I[2][2] + I[1][0] * I[2][1] * I[0][2] + I[2][0] * I[0][1] * I[1][2] -
+                           I[0][0] * I[2][1] * I[1][2] - I[2][0] * I[1][1] * I[0][2] - I[1][0] * I[0][1] * I[2][2];
+        if(det != 0.0)
+        {
+            const double di = 1.0 / det;
+            double J[3][3];
+            J[0][0] = di * (I[1][1] * I[2][2] - I[1][2] * I[2][1]);
+            J[1][1] = di * (I[0][0] * I[2][2] - I[0][2] * I[2][0]);
+            J[2][2] = di * (I[0][0] * I[1][1] - I[0][1] * I[1][0]);
+            J[0][1] = di * (I[0][2] * I[2][1] - I[0][1] * I[2][2]);
+            J[0][2] = di * (I[0][1] * I[1][2] - I[0][2] * I[1][1]);
+            J[1][2] = di * (I[0][2] * I[1][0] - I[0][0] * I[1][2]);
+            J[1][0] = di * (I[1][2] * I[2][0] - I[1][0] * I[2][2]);
+            J[2][0] = di * (I[1][0] * I[2][1] - I[2][0] * I[1][1]);
+            J[2][1] = di * (I[2][0] * I[0][1] - I[0][0] * I[2][1]);
+            /* Matrix * Vector (math/Matrix.hpp operator*): row-wise dot products */
+            double w[3];
+            for(int a = 0; a < 3; ++a) w[a] = J[a][0] * L[0] + J[a][1] * L[1] + J[a][2] * L[2];
+            for(size_t i = 0; i < n; ++i)
+            {
+                const double r[3] = {e->pos[3 * i] - com[0], e->pos[3 * i + 1] - com[1], e->pos[3 * i + 2] - com[2]};
+                e->vel[3 * i + 0] -= w[1] * r[2] - w[2] * r[1];
+                e->vel[3 * i + 1] -= w[2] * r[0] - w[0] * r[2];
+                e->vel[3 * i + 2] -= w[0] * r[1] - w[1] * r[0];
+            }
+        }
+    }
+    if(e->rm_rescale)
+    {
+        double E_post = 0;
+        for(size_t i = 0; i < n; ++i) E_post += e->mass[i] * length_sq(e->vel + 3 * i);
+        E_post *= 0.5;
+        const double scale = sqrt(E_pre / E_post);
+        for(size_t i = 0; i < 3 * n; ++i) e->vel[i] *= scale;
+    }
+}
+
+int orc_set_remover(orc_engine* e, int translation, int rotation, int rescale)
+{
+    e->rm_translation = translation != 0;
+    e->rm_rotation = rotation != 0;
+    e->rm_rescale = rescale != 0;
+    return 0;
+}
+
 int orc_step(orc_engine* e, size_t nsteps)
 {
     for(size_t s = 0; s < nsteps; ++s)
@@ -1371,6 +1473,7 @@ int orc_step(orc_engine* e, size_t nsteps)
         if(e->integ == ORC_INTEG_BAOAB) baoab_step(e);
         else if(e->integ == ORC_INTEG_GJF) gjf_step(e);
         else return -1;
+        if(e->rm_translation || e->rm_rotation) remove_motion(e);
         e->step++;
     }
     return 0;

@@ -79,6 +79,8 @@ size_t orc_pairs_of(const orc_engine* e, int gid, uint32_t* out, size_t cap);
 int orc_reduce_margin(orc_engine* e, double dmargin);
 
 int orc_set_integrator(orc_engine* e, int type, double dt, const double* coefs, uint64_t seed);
+/* SystemMotionRemover(translation, rotation, rescale), applied after every step */
+int orc_set_remover(orc_engine* e, int translation, int rotation, int rescale);
 int orc_integrator_initialize(orc_engine* e);
 int orc_step(orc_engine* e, size_t nsteps);
 uint64_t orc_step_count(const orc_engine* e);

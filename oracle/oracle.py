@@ -64,6 +64,7 @@ def lib():
         L.orc_reduce_margin.argtypes = [C.c_void_p, C.c_double]
         L.orc_set_integrator.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp, C.c_uint64]
         L.orc_integrator_initialize.argtypes = [C.c_void_p]
+        L.orc_set_remover.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.orc_step.argtypes = [C.c_void_p, C.c_size_t]
         L.orc_step_count.restype = C.c_uint64
         L.orc_step_count.argtypes = [C.c_void_p]
@@ -119,6 +120,7 @@ class OracleEngine:
         if with_integrator and spec.integrator is not None:
             it = spec.integrator
             L.orc_set_integrator(self.h, INTEG_CODE[it.type], float(it.dt), it.coefs, int(it.seed))
+            L.orc_set_remover(self.h, *[int(bool(x)) for x in getattr(it, "remove", (False, False, False))])
             if L.orc_integrator_initialize(self.h) != 0:
                 raise ValueError("oracle integrator initialize failed")
 

@@ -27,6 +27,7 @@
 #include <mjolnir/core/BAOABLangevinIntegrator.hpp>
 #include <mjolnir/core/GJFNVTLangevinIntegrator.hpp>
 #include <mjolnir/core/SystemMotionRemover.hpp>
+#include <mjolnir/core/EnergyObserver.hpp>
 #include <mjolnir/core/PeriodicGridCellList.hpp>
 #include <mjolnir/core/UnlimitedGridCellList.hpp>
 #include <mjolnir/core/VerletList.hpp>
@@ -87,6 +88,7 @@ struct SpecIn
     std::vector<LocalIn> local;
     std::vector<GlobalIn> global;
     bool has_integrator = false;
+    bool rm_translation = false, rm_rotation = false, rm_rescale = false;
     std::string integrator;
     double dt = 0;
     std::uint32_t seed = 0;
@@ -173,6 +175,12 @@ static SpecIn read_spec(const std::string& path)
             ifs >> s.integrator >> s.dt >> s.seed;
             s.coefs.resize(s.mass.size());
             for(auto& c : s.coefs) {ifs >> c;}
+        }
+        else if(tok == "remover")
+        {
+            int t = 0, r = 0, c = 0;
+            ifs >> t >> r >> c;
+            s.rm_translation = t != 0; s.rm_rotation = r != 0; s.rm_rescale = c != 0;
         }
         else {throw std::runtime_error("unknown token " + tok);}
     }
@@ -515,7 +523,7 @@ static int run_impl(const SpecIn& s, std::size_t nsteps, const std::string& out,
     sys.initialize(rng);
     b.ff->initialize(sys);
     std::vector<real_type> coefs(s.coefs.begin(), s.coefs.end());
-    integT integ(s.dt, std::move(coefs), SystemMotionRemover<traitsT>(false, false, false));
+    integT integ(s.dt, std::move(coefs), SystemMotionRemover<traitsT>(s.rm_translation, s.rm_rotation, s.rm_rescale));
     integ.initialize(sys, b.ff, rng);
 
     std::ofstream os;
@@ -556,6 +564,40 @@ static int run_impl(const SpecIn& s, std::size_t nsteps, const std::string& out,
     return 0;
 }
 
+// MolecularDynamicsSimulator::initialize/step/finalize (MolecularDynamicsSimulator.hpp:72-123)
+// with only the EnergyObserver attached: prefix.ene after `total` steps, output every `save`
+template<typename traitsT, typename integT>
+static int sim_impl(const SpecIn& s, std::size_t total, std::size_t save, const std::string& prefix)
+{
+    using real_type = typename traitsT::real_type;
+    auto b = build<traitsT>(s);
+    auto& sys = *b.sys;
+    RandomNumberGenerator<traitsT> rng(s.seed);
+    sys.initialize(rng);
+    b.ff->initialize(sys);
+    std::vector<real_type> coefs(s.coefs.begin(), s.coefs.end());
+    integT integ(s.dt, std::move(coefs), SystemMotionRemover<traitsT>(s.rm_translation, s.rm_rotation, s.rm_rescale));
+    integ.initialize(sys, b.ff, rng);
+    EnergyObserver<traitsT> obs(prefix);
+    obs.initialize(total, save, integ.delta_t(), sys, b.ff);
+    real_type t = 0;
+    for(std::size_t step = 0; step < total; ++step)
+    {
+        if(step % save == 0) {obs.output(step, integ.delta_t(), sys, b.ff);}
+        t = integ.step(t, sys, b.ff, rng);
+    }
+    obs.output(total, integ.delta_t(), sys, b.ff);
+    return 0;
+}
+
+template<typename traitsT>
+static int cmd_sim(const SpecIn& s, std::size_t total, std::size_t save, const std::string& prefix)
+{
+    if(s.integrator == "BAOABLangevin") {return sim_impl<traitsT, BAOABLangevinIntegrator<traitsT>>(s, total, save, prefix);}
+    if(s.integrator == "GJFNVTLangevin") {return sim_impl<traitsT, GJFNVTLangevinIntegrator<traitsT>>(s, total, save, prefix);}
+    throw std::runtime_error("unsupported integrator " + s.integrator);
+}
+
 template<typename traitsT>
 static int cmd_run(const SpecIn& s, std::size_t nsteps, const std::string& out, std::size_t every, bool bench)
 {
@@ -590,6 +632,7 @@ static int dispatch(const std::string& cmd, const SpecIn& s, int argc, char** ar
         return cmd_run<traits>(s, std::stoul(argv[3]), argv[4], every, false);
     }
     if(cmd == "bench") {return cmd_run<traits>(s, std::stoul(argv[3]), "", 0, true);}
+    if(cmd == "sim") {return cmd_sim<traits>(s, std::stoul(argv[3]), std::stoul(argv[4]), argv[5]);}
     throw std::runtime_error("unknown command " + cmd);
 }
 } // drv

@@ -118,3 +118,13 @@ def bench_many(specs, nsteps, procs: int, timeout: float = 600):
     wall = time.perf_counter() - t0
     ps = sum(int(nsteps[k]) * specs[k].n for k in range(len(specs)))
     return ps, wall, res
+
+
+def sim(spec, total: int, save: int) -> str:
+    """Text of the .ene file the reference's EnergyObserver writes over a `total`-step run."""
+    with tempfile.TemporaryDirectory() as d:
+        p = _write(spec, d)
+        prefix = os.path.join(d, "sim")
+        subprocess.run([REF_DRIVER, "sim", p, str(total), str(save), prefix], check=True, stdout=subprocess.DEVNULL)
+        with open(prefix + ".ene") as f:
+            return f.read()

@@ -131,7 +131,17 @@ def cases():
         "two_chains_others": (lambda: two_chains(ignore="Others"), True, 0),
         "lj_pair_scan": (lj_pair_scan, True, 0),
         "lj_boundary": (lj_boundary_cases, True, 0),
+        "chain200_remover": (chain_with_remover, True, 1000),
     }
+
+
+def chain_with_remover() -> SystemSpec:
+    """CG chain whose integrator removes net translation and rotation and rescales the kinetic
+    energy after every step (SystemMotionRemover.hpp:42-148)."""
+    s = perturb(workloads.cg_chain(n=200, seed=3), 0.3, 10)
+    s.integrator.remove = (True, True, True)
+    s.name = "chain200_remover"
+    return s
 
 
 def sensitivity(spec: SystemSpec, steps: int) -> np.ndarray:
@@ -189,11 +199,25 @@ def make(name: str, factory, full_pairs: bool, zt_steps: int):
     print(f"{name}: {len(names)} interactions, {os.path.getsize(path) / 1024:.1f} KiB")
 
 
+def make_sim():
+    """The reference's EnergyObserver file for a 300-step zero-temperature run of chain200,
+    output every 100 steps (MolecularDynamicsSimulator cadence)."""
+    import refrun
+    spec = workloads.zero_temperature(cases()["chain200"][0]())
+    text = refrun.sim(spec, 300, 100)
+    path = os.path.join(HERE, "sim_chain200.npz")
+    np.savez_compressed(path, spec_hash=np.array(spec_hash(spec)), ene=np.array(text), total=np.array(300),
+                        save=np.array(100))
+    print(f"sim_chain200: {len(text.splitlines())} lines")
+
+
 if __name__ == "__main__":
     import refrun
     if not refrun.available():
         sys.exit("oracle/_ref/ref_driver is missing: run `make -C oracle ref` where the reference sources exist")
     only = sys.argv[1:]
+    if not only or "sim_chain200" in only:
+        make_sim()
     for name, (factory, full, zt) in cases().items():
         if only and name not in only:
             continue

@@ -155,3 +155,35 @@ def test_thermostat_mixed_precision_kinetic_temperature():
     for T, x in ratio.items():
         print(f"[gpu] T={T}: <KE>/(3/2 N kB T) = {np.mean(x):.4f}", flush=True)
         assert abs(np.mean(x) - 1.0) < 0.03, (T, np.mean(x))
+
+
+def test_periodic_lj_batch_vs_oracle():
+    """Periodic boundary (minimum image, wrapped positions, grid with wrap-around) in both
+    precisions: FP64 forces at 1e-9, FP32 RMS error at 1e-5, FP64 zero-T trajectory."""
+    import oracle
+    from mjolnir_amd import capi, workloads
+    specs = [make_golden.perturb(workloads.lj_fluid(n_side=k, seed=20 + k), 0.2, 20 + k) for k in (6, 12)]
+    for prec in (capi.FP64, capi.FP32_FIXED):
+        b, ids = _batch(specs, prec)
+        for spec, r in zip(specs, ids):
+            o = oracle.OracleEngine(spec, with_integrator=False)
+            f_ref, _ = o.force_and_energy()
+            _, _, f = b.evaluate(r)
+            if prec == capi.FP64:
+                err = np.linalg.norm(f - f_ref, axis=1)
+                assert np.all(err <= force_tolerance(f_ref, 1e-9)), err.max()
+            else:
+                rms_ref = np.sqrt(np.mean(np.sum(f_ref ** 2, axis=1)))
+                rms_err = np.sqrt(np.mean(np.sum((f - f_ref) ** 2, axis=1)))
+                assert rms_err <= 1e-5 * rms_ref, rms_err / rms_ref
+    zt = [workloads.zero_temperature(s) for s in specs]
+    b, ids = _batch(zt, capi.FP64)
+    b.step(200)
+    for spec, r in zip(zt, ids):
+        o = oracle.OracleEngine(spec)
+        o.step(200)
+        x, _, _ = b.state(r)
+        xo, _, _ = o.state()
+        d = np.abs(x - xo)
+        d = np.minimum(d, (spec.upper - spec.lower) - d)   # wrapped coordinates
+        assert d.max() < 1e-8, d.max()

@@ -170,3 +170,55 @@ def test_neighbor_sets_after_rebuilds_vs_oracle():
         inside = want[np.einsum("ij,ij->i", d, d) < rc * rc]
         held = set(map(tuple, e.pairs(k).tolist()))
         assert set(map(tuple, inside.tolist())) <= held
+
+
+@pytest.mark.parametrize("flags", [(True, False, False), (True, True, False), (True, True, True)])
+def test_motion_remover_vs_oracle_same_noise(flags):
+    """SystemMotionRemover after every Langevin step: the trajectory follows the oracle with the
+    same noise, and the net momentum and angular momentum are zero (to rounding)."""
+    import copy
+    import oracle
+    from mjolnir_amd import workloads
+    spec = copy.deepcopy(make_golden.perturb(workloads.cg_chain(n=150, seed=11), 0.2, 3))
+    spec.integrator.remove = flags
+    e = _engine(spec)
+    o = oracle.OracleEngine(spec)
+    e.step(100)
+    o.step(100)
+    x, v, _ = e.state()
+    xo, vo, _ = o.state()
+    assert np.abs(x - xo).max() < 1e-7
+    assert np.abs(v - vo).max() < 1e-7
+    m = spec.mass[:, None]
+    p = (m * v).sum(0)
+    assert np.abs(p).max() < 1e-10
+    if flags[1]:
+        com = (m * x).sum(0) / m.sum()
+        L = (m * np.cross(x - com, v)).sum(0)
+        assert np.abs(L).max() < 1e-8
+
+
+def test_energy_observer_and_simulator_vs_reference(tmp_path):
+    """mjg_simulate with an open energy observer writes the reference's .ene file: identical
+    header lines, and every value of the 300-step zero-temperature run within the printed
+    precision (plus the trajectory's own rounding)."""
+    from mjolnir_amd import workloads
+    from conftest import GOLDEN
+    import os
+    g = np.load(os.path.join(GOLDEN, "sim_chain200.npz"))
+    spec = workloads.zero_temperature(make_golden.cases()["chain200"][0]())
+    assert str(g["spec_hash"]) == make_golden.spec_hash(spec)
+    e = _engine(spec)
+    prefix = str(tmp_path / "run")
+    e.open_energy_observer(prefix)
+    e.simulate(int(g["total"]), int(g["save"]))
+    text = open(prefix + ".ene").read()
+    ref = str(g["ene"])
+    got_lines, ref_lines = text.splitlines(), ref.splitlines()
+    assert len(got_lines) == len(ref_lines)
+    assert got_lines[:2] == ref_lines[:2]
+    for a, b in zip(got_lines[2:], ref_lines[2:]):
+        x = np.array([float(t) for t in a.split()])
+        y = np.array([float(t) for t in b.split()])
+        assert np.allclose(x, y, rtol=1e-9, atol=2e-6), (a, b)
+        assert len(a) == len(b)          # fixed-width columns

@@ -8,10 +8,12 @@
   method, test/util/check_force.hpp:52-166), Newton's third law, closed-form potential
   minima, brute-force neighbour sets, and the Debye-Hueckel constants quoted for 0.1 M/300 K.
 """
+import os
+
 import numpy as np
 import pytest
 
-from conftest import golden_case
+from conftest import GOLDEN, golden_case
 import make_golden
 import oracle
 from mjolnir_amd import workloads
@@ -200,3 +202,31 @@ def test_velocity_verlet_harmonic_energy_drift():
     t = 0.001 * np.arange(1, 2001)
     analytic = r0 + 0.2 * np.cos(omega * t)
     assert np.abs(xs - analytic).max() < 2e-3
+
+
+def _parse_ene(text):
+    rows = [l.split() for l in text.splitlines() if l and not l.startswith("#")]
+    return np.array([[float(x) for x in r] for r in rows])
+
+
+def test_energy_observer_golden_matches_oracle():
+    """The reference's .ene lines (EnergyObserver.hpp:87-150) hold the oracle's per-interaction
+    energies (calc_energy path), kinetic energy and attributes at steps 0, 100, 200, 300."""
+    import make_golden
+    import oracle
+    from mjolnir_amd import workloads
+    g = np.load(os.path.join(GOLDEN, "sim_chain200.npz"))
+    spec = workloads.zero_temperature(make_golden.cases()["chain200"][0]())
+    assert str(g["spec_hash"]) == make_golden.spec_hash(spec)
+    rows = _parse_ene(str(g["ene"]))
+    assert rows.shape[0] == 4
+    o = oracle.OracleEngine(spec)
+    ni = len(spec.interaction_names())
+    for r, step in enumerate((0, 100, 200, 300)):
+        if step:
+            o.step(100)
+        _, v, _ = o.state()
+        e = [o.energy_of(k) for k in range(ni)]
+        ek = 0.5 * np.sum(spec.mass * np.sum(v * v, axis=1))
+        want = np.array([step] + e + [ek, spec.attributes["ionic_strength"], spec.attributes["temperature"]])
+        assert np.allclose(rows[r], want, rtol=0, atol=1.5e-6), (step, rows[r] - want)
