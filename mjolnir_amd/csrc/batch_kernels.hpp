@@ -836,21 +836,30 @@ k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
                 if(ENERGY) {atomicAdd(&s_energy[kd[m] >> 8], e);}
             }
         }
-        // pair terms, kPairBatch list entries per thread per load round
+        // pair terms, kPairBatch list entries per thread per load round; the next round's
+        // entries are loaded before the current round is computed (the round's LDS atomics
+        // would otherwise hold the loads back)
         for(int g = 0; g < ng; ++g)
         {
             const RepGlobal& G = R.g[g];
             const Real* par = par_lds + (size_t)n * G.par_col;
             const uint32_t np = s_npairs[g];
             double e_pair = 0.0;
+            uint32_t pk[kPairBatch];
+#pragma unroll
+            for(int m = 0; m < kPairBatch; ++m)
+            {
+                const uint32_t k = m * BS + tid;
+                pk[m] = (k < np) ? G.pairs[k] : 0xffffffffu;
+            }
             for(uint32_t base = 0; base < np; base += BS * kPairBatch)
             {
-                uint32_t pk[kPairBatch];
+                uint32_t nx[kPairBatch];
 #pragma unroll
                 for(int m = 0; m < kPairBatch; ++m)
                 {
-                    const uint32_t k = base + m * BS + tid;
-                    pk[m] = (k < np) ? G.pairs[k] : 0xffffffffu;
+                    const uint32_t k = base + BS * kPairBatch + m * BS + tid;
+                    nx[m] = (k < np) ? G.pairs[k] : 0xffffffffu;
                 }
 #pragma unroll
                 for(int m = 0; m < kPairBatch; ++m)
@@ -858,6 +867,8 @@ k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
                     if(pk[m] == 0xffffffffu) {continue;}
                     rep_pair<Real, ENERGY>(G, par, pk[m] >> 16, pk[m] & 0xffffu, R.box, P, F, e_pair);
                 }
+#pragma unroll
+                for(int m = 0; m < kPairBatch; ++m) {pk[m] = nx[m];}
             }
             if(ENERGY) {atomicAdd(&s_energy[R.n_local + g], e_pair);}
         }

@@ -42,19 +42,31 @@ LOCAL_KINDS = {
     ("DihedralAngle", "ClementiDihedral"): (4, 3),
     ("DihedralAngle", "Gaussian"): (4, 3),
     ("BondLength", "3SPN2Bond"): (2, 2),
+    # 3SPN2BaseStacking: (S5, B5, B3, P5, S3, P3), absent phosphate NIL; {stack kind 4*b5 + b3}
+    ("3SPN2BaseStacking", "3SPN2"): (6, 1),
 }
+
+NIL = 0xFFFFFFFF   # absent index (the 5' phosphate of a strand's first nucleotide)
 
 # Per-participant parameter columns of each global pair potential.
 #   LennardJones: {sigma, epsilon}           LennardJonesPotential.hpp:21-25
 #   ExcludedVolume: {radius}                 ExcludedVolumePotential.hpp:32-35
 #   DebyeHuckel: {charge}                    DebyeHuckelPotential.hpp:212
+#   3SPN2ExcludedVolume: {bead kind P 0, S 1, A 2, T 3, G 4, C 5}   ThreeSPN2ExcludedVolumePotential.hpp
+#   3SPN2BasePair: {base A 0, T 1, G 2, C 3, sugar index}        ThreeSPN2BasePairPotential.hpp:207-211
+#   3SPN2CrossStacking: {base, sugar, 5' base, 3' base (-1: none)} ThreeSPN2CrossStackingPotential.hpp:203-207
 GLOBAL_KINDS = {
     "LennardJones": 2,
     "ExcludedVolume": 1,
     "DebyeHuckel": 1,
+    "3SPN2ExcludedVolume": 1,
+    "3SPN2BasePair": 2,
+    "3SPN2CrossStacking": 4,
 }
 
-DEFAULT_CUTOFF = {"LennardJones": 2.5, "ExcludedVolume": 2.0, "DebyeHuckel": 5.5}
+# relative cutoffs; the 3SPN2 interactions carry absolute cutoffs in their parameter sets
+DEFAULT_CUTOFF = {"LennardJones": 2.5, "ExcludedVolume": 2.0, "DebyeHuckel": 5.5, "3SPN2ExcludedVolume": 1.0,
+                  "3SPN2BasePair": 1.0, "3SPN2CrossStacking": 1.0}
 
 
 @dataclass
@@ -79,6 +91,15 @@ class LocalSpec:
         BondLengthGoContactInteraction.hpp:102)."""
         if (self.interaction, self.potential) == ("BondLength", "GoContact"):
             return "BondLengthGoContact"
+        # BondLengthInteraction.hpp:88 with ThreeSPN2BondPotential::name(), a Gaussian dihedral
+        # is a PeriodicGaussianPotential (read_local_interaction.hpp:282-290),
+        # ThreeSPN2BaseStackingInteraction.hpp:106
+        if (self.interaction, self.potential) == ("BondLength", "3SPN2Bond"):
+            return "BondLength:ThreeSPN2BondPotential"
+        if (self.interaction, self.potential) == ("DihedralAngle", "Gaussian"):
+            return "DihedralAngle:PeriodicGaussian"
+        if self.interaction == "3SPN2BaseStacking":
+            return "3SPN2BaseStacking"
         return f"{self.interaction}:{self.potential}"
 
 
@@ -107,6 +128,8 @@ class GlobalSpec:
         # GlobalPairExcludedVolumeInteraction.hpp:134; generic: GlobalPairInteraction.hpp:112
         if self.potential in ("LennardJones", "ExcludedVolume"):
             return "GlobalPair" + self.potential
+        if self.potential in ("3SPN2BasePair", "3SPN2CrossStacking"):
+            return self.potential       # ThreeSPN2BasePairInteraction.hpp:106, ...CrossStacking...:106
         return "Pair:" + self.potential
 
 

@@ -24,7 +24,7 @@ from typing import List, Optional, Tuple
 
 import numpy as np
 
-from .spec import GlobalSpec, IntegratorSpec, LocalSpec, SystemSpec
+from .spec import NIL, GlobalSpec, IntegratorSpec, LocalSpec, SystemSpec
 
 # kB [J/K] -> [kcal/mol/K]: kB * (J_to_cal / 1000) * N_A  (read_units.hpp:35-36)
 _KB_SI = 1.38064852e-23
@@ -219,3 +219,153 @@ def zero_temperature(spec: SystemSpec) -> SystemSpec:
     if s.integrator is not None:
         s.integrator.coefs = np.zeros_like(s.integrator.coefs)
     return s
+
+
+# ------------------------------------------------------------------------------------------
+# 3SPN2 double-stranded DNA
+# ------------------------------------------------------------------------------------------
+DNA_MASS = {"P": 94.97, "S": 83.11, "A": 134.1, "T": 125.1, "G": 150.1, "C": 110.1}
+_COMP = {"A": "T", "T": "A", "G": "C", "C": "G"}
+_BASE_CODE = {"A": 0, "T": 1, "G": 2, "C": 3}
+_BEAD_CODE = {"P": 0, "S": 1, "A": 2, "T": 3, "G": 4, "C": 5}
+# the base-pairing angle S-B-B' of each base in the reference's parameter set
+# (ThreeSPN2BasePairPotential.hpp:530-560: theta1 of AT/TA/GC/CG is the angle at the first base)
+_SBB_ANGLE = {"A": 156.54, "T": 135.78, "G": 159.81, "C": 141.16}
+
+
+def dna_duplex(n_bp: int = 200, seed: int = 3, T: float = 300.0, ionic_strength: float = 0.15,
+               dt: float = 0.02, gamma: float = 0.1, rise: float = 3.6, twist_deg: float = 36.0,
+               r_base: float = 3.0, bp_dist: float = 6.0, r_phos: float = 8.9, sb_bond: float = 4.4,
+               margin: float = 0.4) -> SystemSpec:
+    """Ideal right-handed B-form duplex of the 3SPN2 model (3 beads per nucleotide, no 5' terminal
+    phosphate), random sequence, with the interactions of the reference's dna_3SPN2 example:
+    3SPN2 bonds, harmonic angles, Gaussian backbone dihedrals, base stacking, base pairing,
+    cross stacking, 3SPN2 excluded volume and Debye-Hueckel on the phosphates (charge -0.6).
+
+    Bases sit at radius r_base; the two bases of a pair lie in one plane, bp_dist apart (the
+    partner turned by 2 asin(bp_dist / 2 r_base) about the axis; 180 degrees by default).  The
+    rise (3.6 A rather than B-DNA's 3.38) puts consecutive bases of a strand near the
+    base-stacking distances of the 3SPN2 parameter set, which start the duplex with pairing,
+    stacking and cross stacking formed and no strained contacts.
+    Each sugar lies in its base's plane, sb_bond from the base, at the angle S-B-B' of its base in
+    the reference's pairing parameters, on the side away from the axis; phosphates sit at radius
+    r_phos midway (angle and height) between consecutive sugars of their strand.  Bond, angle
+    and dihedral minima are the initial values.  G-JF with alpha = gamma * mass."""
+    rng = np.random.default_rng(seed)
+    seq = [("A", "T", "G", "C")[k] for k in rng.integers(0, 4, size=n_bp)]
+    tw = math.radians(twist_deg)
+    groove = 2.0 * math.asin(min(1.0, bp_dist / (2.0 * r_base)))
+
+    beads = []        # (kind, position)
+    nucs = [[], []]   # per strand: dicts {P, S, B, base}
+
+    def base_pos(strand: int, k: int):
+        th = k * tw + (0.0 if strand == 0 else groove)
+        return np.array([r_base * math.cos(th), r_base * math.sin(th), k * rise])
+
+    def place(strand: int, k: int, base: str):
+        rB = base_pos(strand, k)
+        rP = base_pos(1 - strand, k)
+        v = (rP - rB) / np.linalg.norm(rP - rB)
+        a = math.radians(_SBB_ANGLE[base])
+        best = None
+        for sgn in (1.0, -1.0):
+            # rotate v by +-a about the helix axis (z): the sugar stays in the base plane
+            c, s_ = math.cos(sgn * a), math.sin(sgn * a)
+            u = np.array([c * v[0] - s_ * v[1], s_ * v[0] + c * v[1], 0.0])
+            rS = rB + sb_bond * u
+            if best is None or np.hypot(rS[0], rS[1]) > np.hypot(best[0], best[1]):
+                best = rS
+        return best, rB
+
+    pairs = [(seq[k], _COMP[seq[k]]) for k in range(n_bp)]
+    for strand in (0, 1):
+        order = range(n_bp) if strand == 0 else range(n_bp - 1, -1, -1)
+        prev_S = None
+        for j, k in enumerate(order):
+            base = pairs[k][strand]
+            rS, rB = place(strand, k, base)
+            nuc = {"base": base}
+            if j > 0:
+                # phosphate between the previous sugar and this one, half a rise towards the previous
+                mid = 0.5 * (prev_S + rS)
+                ang = math.atan2(mid[1], mid[0])
+                zP = mid[2]
+                nuc["P"] = len(beads)
+                beads.append(("P", np.array([r_phos * math.cos(ang), r_phos * math.sin(ang), zP])))
+            nuc["S"] = len(beads)
+            beads.append(("S", rS))
+            nuc["B"] = len(beads)
+            beads.append((base, rB))
+            nucs[strand].append(nuc)
+            prev_S = rS
+    n = len(beads)
+    pos = np.array([b[1] for b in beads])
+    pos -= pos.mean(axis=0)
+    mass = np.array([DNA_MASS[b[0]] for b in beads])
+    vel = maxwell_boltzmann(rng, mass, T)
+
+    bonds, angles, dihedrals, stacks, stack_kind = [], [], [], [], []
+    for strand in (0, 1):
+        ns = nucs[strand]
+        for j, nu in enumerate(ns):
+            if j > 0:
+                bonds.append((nu["P"], nu["S"]))
+            bonds.append((nu["S"], nu["B"]))
+            if j + 1 < len(ns):
+                bonds.append((nu["S"], ns[j + 1]["P"]))
+        for j, nu in enumerate(ns):
+            if j > 0:
+                angles.append((nu["P"], nu["S"], nu["B"]))
+                if j + 1 < len(ns):
+                    angles.append((nu["P"], nu["S"], ns[j + 1]["P"]))
+            if j + 1 < len(ns):
+                nx = ns[j + 1]
+                angles.append((nu["B"], nu["S"], nx["P"]))
+                angles.append((nu["S"], nx["P"], nx["S"]))
+                dihedrals.append((nu["S"], nx["P"], nx["S"], ns[j + 2]["P"]) if j + 2 < len(ns) else None)
+                if j > 0:
+                    dihedrals.append((nu["P"], nu["S"], nx["P"], nx["S"]))
+                stacks.append((nu["S"], nu["B"], nx["B"], nu.get("P", NIL), nx["S"], nx.get("P", NIL)))
+                stack_kind.append(4 * _BASE_CODE[nu["base"]] + _BASE_CODE[nx["base"]])
+    dihedrals = [d for d in dihedrals if d is not None]
+    bonds = np.array(bonds); angles = np.array(angles); dihedrals = np.array(dihedrals)
+    r0 = np.linalg.norm(pos[bonds[:, 1]] - pos[bonds[:, 0]], axis=1)
+    local = [
+        LocalSpec("BondLength", "3SPN2Bond", "bond", bonds, np.stack([np.full(len(bonds), 0.143), r0], axis=1)),
+        LocalSpec("BondAngle", "Harmonic", "angle", angles,
+                  np.stack([np.full(len(angles), 47.801), _angles(pos, angles)], axis=1)),
+        LocalSpec("DihedralAngle", "Gaussian", "dihedral", dihedrals,
+                  np.stack([np.full(len(dihedrals), -1.434), np.full(len(dihedrals), 0.3),
+                            _dihedrals(pos, dihedrals)], axis=1)),
+        LocalSpec("3SPN2BaseStacking", "3SPN2", "nucleotide", np.array(stacks, dtype=np.uint64).astype(np.uint32),
+                  np.array(stack_kind, dtype=np.float64)[:, None]),
+    ]
+    all_nucs = nucs[0] + nucs[1]
+    bases = np.array([nu["B"] for nu in all_nucs])
+    bp_par = np.array([[_BASE_CODE[nu["base"]], nu["S"]] for nu in all_nucs], dtype=np.float64)
+    cs_par = []
+    for strand in (0, 1):
+        ns = nucs[strand]
+        for j, nu in enumerate(ns):
+            b5 = ns[j - 1]["B"] if j > 0 else -1
+            b3 = ns[j + 1]["B"] if j + 1 < len(ns) else -1
+            cs_par.append([_BASE_CODE[nu["base"]], nu["S"], b5, b3])
+    cs_par = np.array(cs_par, dtype=np.float64)
+    phos = np.array([i for i, b in enumerate(beads) if b[0] == "P"])
+    kinds = np.array([_BEAD_CODE[b[0]] for b in beads], dtype=np.float64)
+    bp = GlobalSpec(potential="3SPN2BasePair", participants=bases, params=bp_par, partition="VerletList",
+                    margin=margin, ignore_molecule="Nothing", ignore_within={"nucleotide": 3})
+    cs = GlobalSpec(potential="3SPN2CrossStacking", participants=bases, params=cs_par, partition="VerletList",
+                    margin=margin, ignore_molecule="Nothing", ignore_within={"nucleotide": 3})
+    ev = GlobalSpec(potential="3SPN2ExcludedVolume", participants=np.arange(n), params=kinds[:, None],
+                    partition="CellList", margin=margin, ignore_molecule="Nothing",
+                    ignore_within={"bond": 1, "angle": 1, "dihedral": 1, "nucleotide": 1})
+    dh = GlobalSpec(potential="DebyeHuckel", participants=phos, params=np.full((len(phos), 1), -0.6),
+                    cutoff=5.5, partition="VerletList", margin=margin, ignore_molecule="Nothing",
+                    ignore_within={"nucleotide": 1})
+    return SystemSpec(boundary="unlimited", mass=mass, pos=pos, vel=vel,
+                      attributes={"temperature": T, "ionic_strength": ionic_strength},
+                      local=local, global_=[bp, cs, ev, dh],
+                      integrator=IntegratorSpec("GJFNVTLangevin", dt, gamma * mass, seed=seed),
+                      name=f"dna_duplex_{n_bp}bp")

@@ -290,9 +290,17 @@ int orc_add_local(orc_engine* e, int kind, const char* topology, size_t n_terms,
         case ORC_ANGLE_HARMONIC: arity = 3; npar = 2; break;
         case ORC_ANGLE_GAUSSIAN: arity = 3; npar = 3; break;
         case ORC_DIHEDRAL_CLEMENTI: arity = 4; npar = 3; break;
+        case ORC_BOND_3SPN2: arity = 2; npar = 2; break;
+        case ORC_DIHEDRAL_PGAUSSIAN: arity = 4; npar = 3; break;
+        case ORC_3SPN2_STACKING: arity = 6; npar = 1; break;
         default: return -1;
     }
-    for(size_t t = 0; t < n_terms * (size_t)arity; ++t) if(indices[t] >= e->n) return -1;
+    for(size_t t = 0; t < n_terms * (size_t)arity; ++t)
+    {
+        /* the phosphates of a stacking term may be absent */
+        const int optional = (kind == ORC_3SPN2_STACKING) && (t % 6 == 3 || t % 6 == 5);
+        if(indices[t] >= e->n && !(optional && indices[t] == ORC_NIL)) return -1;
+    }
     e->loc = (local_t*)realloc(e->loc, (e->n_loc + 1) * sizeof(local_t));
     local_t* L = &e->loc[e->n_loc];
     memset(L, 0, sizeof(*L));
@@ -314,7 +322,9 @@ int orc_add_global(orc_engine* e, int potential, double cutoff_ratio, double eps
     switch(potential)
     {
         case ORC_PAIR_LJ: npar = 2; break;
-        case ORC_PAIR_EV: case ORC_PAIR_DH: npar = 1; break;
+        case ORC_PAIR_EV: case ORC_PAIR_DH: case ORC_PAIR_3SPN2_EV: npar = 1; break;
+        case ORC_3SPN2_BASEPAIR: npar = 2; break;
+        case ORC_3SPN2_CROSSSTACK: npar = 4; break;
         default: return -1;
     }
     if(n_within > MAX_WITHIN || n_participants < 1) return -1;
@@ -378,6 +388,24 @@ static void write_topology(orc_engine* e)
     {
         local_t* L = &e->loc[l];
         if(L->topo_kind < 0) continue;
+        if(L->kind == ORC_3SPN2_STACKING)
+        {
+            /* ThreeSPN2BaseStackingInteraction::write_topology (:125-179): every bead of the 5'
+               nucleotide to every bead of the 3' one, phosphates when present */
+            for(size_t t = 0; t < L->n; ++t)
+            {
+                const uint32_t* ix = L->idx + 6 * t;
+                const uint32_t S5 = ix[0], B5 = ix[1], B3 = ix[2], P5 = ix[3], S3 = ix[4], P3 = ix[5];
+                add_connection(e, S5, S3, L->topo_kind);
+                add_connection(e, S5, B3, L->topo_kind);
+                add_connection(e, B5, S3, L->topo_kind);
+                add_connection(e, B5, B3, L->topo_kind);
+                if(P5 != ORC_NIL) { add_connection(e, P5, S3, L->topo_kind); add_connection(e, P5, B3, L->topo_kind); }
+                if(P3 != ORC_NIL) { add_connection(e, S5, P3, L->topo_kind); add_connection(e, B5, P3, L->topo_kind); }
+                if(P5 != ORC_NIL && P3 != ORC_NIL) add_connection(e, P5, P3, L->topo_kind);
+            }
+            continue;
+        }
         for(size_t t = 0; t < L->n; ++t)
         {
             const uint32_t* ix = L->idx + t * L->arity;
@@ -447,6 +475,102 @@ static void make_exclusions(orc_engine* e, global_t* G)
     memcpy(G->mol, e->mol, N * sizeof(size_t));
 }
 
+/* ------------------------------------------------------------------------------------- */
+/* 3SPN2 parameters (the reference's default tables, kJ/mol and degrees as published, converted */
+/* as the reference's initialize() does: epsilon * (1/4.184), degrees * (pi/180))            */
+/* ------------------------------------------------------------------------------------- */
+static const double TSP_PI = 3.14159265358979323846264338;       /* math/constants.hpp:13 */
+static const double TSP_TWO_PI = 6.28318530717958647692528677;   /* math/constants.hpp:15 */
+static const double TSP_TOL = 1e-8;                              /* math::abs_tolerance<double> */
+static double tsp_J_to_cal(void) { return 1.0 / 4.184; }        /* Unit.hpp:43-44 */
+
+/* ThreeSPN2BaseStackingPotentialParameter (ThreeSPN2BaseStackingPotential.hpp:298-326) */
+static const double TSP_BS_EPS[16] = {14.39, 14.34, 13.25, 14.51, 10.37, 13.36, 10.34, 12.89,
+                                      14.81, 15.57, 14.93, 15.39, 11.42, 12.79, 10.52, 13.24};
+static const double TSP_BS_R0[16] = {3.716, 3.675, 3.827, 3.744, 4.238, 3.984, 4.416, 4.141,
+                                     3.576, 3.598, 3.664, 3.635, 4.038, 3.798, 4.208, 3.935};
+static const double TSP_BS_TH0[16] = {101.15, 85.94, 105.26, 89.00, 101.59, 89.50, 104.31, 91.28,
+                                      100.89, 84.83, 105.48, 88.28, 106.49, 93.31, 109.54, 95.46};
+/* ThreeSPN2CrossStackingGlobalPotentialParameter (ThreeSPN2CrossStackingPotential.hpp:516-566) */
+static const double TSP_CS_EPS[32] = {2.186, 2.774, 2.833, 1.951, 2.774, 2.186, 2.539, 2.980,
+                                      2.833, 2.539, 3.774, 1.129, 1.951, 2.980, 1.129, 4.802,
+                                      2.186, 2.774, 2.980, 2.539, 2.774, 2.186, 1.951, 2.833,
+                                      2.980, 1.951, 4.802, 1.129, 2.539, 2.833, 1.129, 3.774};
+static const double TSP_CS_R0[32] = {6.208, 6.876, 6.072, 6.811, 6.876, 7.480, 6.771, 7.453,
+                                     6.072, 6.771, 5.921, 6.688, 6.811, 7.453, 6.688, 7.409,
+                                     5.435, 6.295, 5.183, 6.082, 6.295, 7.195, 6.028, 6.981,
+                                     5.183, 6.028, 4.934, 5.811, 6.082, 6.981, 5.811, 6.757};
+static const double TSP_CS_TH0[32] = {154.38, 159.10, 152.46, 158.38, 147.10, 153.79, 144.44, 151.48,
+                                      154.69, 157.83, 153.43, 158.04, 152.99, 159.08, 150.53, 157.17,
+                                      116.88, 121.74, 114.23, 119.06, 109.42, 112.95, 107.32, 110.56,
+                                      119.34, 124.72, 116.51, 121.98, 114.60, 118.26, 112.45, 115.88};
+/* ThreeSPN2ExcludedVolumePotentialParameter (ThreeSPN2ExcludedVolumePotential.hpp:437-446),
+   bead kinds P, S, A, T, G, C */
+static double tsp_ev_sigma(int kind)
+{
+    static const double sigma[6] = {4.5, 6.2, 5.4, 7.1, 4.9, 6.4};
+    return sigma[kind];
+}
+static double tsp_bs_eps(int k) { return TSP_BS_EPS[k] * tsp_J_to_cal(); }
+static double tsp_bs_theta0(int k) { return TSP_BS_TH0[k] * (TSP_PI / 180.0); }
+static double tsp_cs_eps(int k) { return TSP_CS_EPS[k] * tsp_J_to_cal(); }
+static double tsp_cs_theta0(int k) { return TSP_CS_TH0[k] * (TSP_PI / 180.0); }
+
+/* base-pair kinds AT 0, TA 1, GC 2, CG 3 from bases A 0, T 1, G 2, C 3
+   (ThreeSPN2BasePairParameterList::bp_kind, ThreeSPN2BasePairPotential.hpp:275-285) */
+static int tsp_bp_kind(int a)
+{
+    switch(a) { case 0: return 0; case 1: return 1; case 3: return 3; default: return 2; }
+}
+/* ThreeSPN2BasePairGlobalPotentialParameter (ThreeSPN2BasePairPotential.hpp:530-560) */
+static double tsp_bp_eps(int bp) { return (bp < 2 ? 16.73 : 21.18) * tsp_J_to_cal(); }
+static double tsp_bp_r0(int bp) { return bp < 2 ? 5.941 : 5.530; }
+static double tsp_bp_theta1(int bp)
+{
+    switch(bp)
+    {
+        case 0: return 156.54 / 180.0 * TSP_PI;
+        case 1: return 135.78 / 180.0 * TSP_PI;
+        case 2: return 159.81 / 180.0 * TSP_PI;
+        default: return 141.16 / 180.0 * TSP_PI;
+    }
+}
+static double tsp_bp_theta2(int bp)
+{
+    switch(bp)
+    {
+        case 0: return 135.78 / 180.0 * TSP_PI;
+        case 1: return 156.54 / 180.0 * TSP_PI;
+        case 2: return 141.16 / 180.0 * TSP_PI;
+        default: return 159.81 / 180.0 * TSP_PI;
+    }
+}
+static double tsp_bp_phi0(int bp) { return (bp < 2 ? -38.35 : -42.98) / 180.0 * TSP_PI; }
+static double tsp_cs_theta3(int bp) { return (bp < 2 ? 116.09 : 124.94) / 180.0 * TSP_PI; }
+
+/* f(theta) and df/dtheta of the 3SPN2 angular modulations (ThreeSPN2BasePairPotential.hpp:
+   60-95, identical in the base-stacking and cross-stacking potentials) */
+static double tsp_f(double K, double pi_over_K, double theta, double theta0)
+{
+    const double dtheta = theta - theta0;
+    const double abs_dtheta = fabs(dtheta);
+    if(abs_dtheta < pi_over_K * 0.5) return 1.0;
+    if(abs_dtheta < pi_over_K)
+    {
+        const double cos_Kdtheta = cos(K * dtheta);
+        return 1.0 - cos_Kdtheta * cos_Kdtheta;
+    }
+    return 0.0;
+}
+static double tsp_df(double K, double pi_over_K, double theta, double theta0)
+{
+    const double dtheta = theta - theta0;
+    const double abs_dtheta = fabs(dtheta);
+    if(abs_dtheta < pi_over_K * 0.5) return 0.0;
+    if(abs_dtheta < pi_over_K) return K * sin(2 * K * dtheta);
+    return 0.0;
+}
+
 /* ExclusionList::is_excluded (ExclusionList.hpp:56-84) */
 static int is_excluded(const global_t* G, size_t i, size_t j)
 {
@@ -460,10 +584,30 @@ static int is_excluded(const global_t* G, size_t i, size_t j)
     }
     return 0;
 }
-/* ParameterList.hpp:285-289 */
+/* complementary bases A-T, G-C (codes A 0, T 1, G 2, C 3) */
+static int complementary(int a, int b)
+{
+    return (a == 0 && b == 1) || (a == 1 && b == 0) || (a == 2 && b == 3) || (a == 3 && b == 2);
+}
+/* ParameterList.hpp:285-289; 3SPN2 EV: ThreeSPN2ExcludedVolumePotential.hpp:326-364 (base pairs that
+ * could pair never interact; its within-3-nucleotides exception is keyed to a connection kind
+ * no interaction writes, so it never applies); 3SPN2 BP / CS: ThreeSPN2BasePairPotential.hpp:
+ * 445-459, ThreeSPN2CrossStackingPotential.hpp:437-451 (complementary bases only) */
 static int has_interaction(const global_t* G, size_t i, size_t j)
 {
-    return (i < j) && !is_excluded(G, i, j);
+    if(!((i < j) && !is_excluded(G, i, j))) return 0;
+    if(G->pot == ORC_PAIR_3SPN2_EV)
+    {
+        const int ki = (int)G->ppar[i], kj = (int)G->ppar[j];
+        if(ki < 2 || kj < 2) return 1;   /* a phosphate or a sugar */
+        return !complementary(ki - 2, kj - 2);
+    }
+    if(G->pot == ORC_3SPN2_BASEPAIR || G->pot == ORC_3SPN2_CROSSSTACK)
+    {
+        const int w = G->npar;
+        return complementary((int)G->ppar[i * w], (int)G->ppar[j * w]);
+    }
+    return 1;
 }
 
 /* ------------------------------------------------------------------------------------- */
@@ -486,6 +630,17 @@ static int setup_potential(orc_engine* e, global_t* G)
         double mr = 0;
         for(size_t i = 0; i < e->n; ++i) mr = fmax(mr, G->ppar[i]);
         G->max_cutoff = mr * 2 * G->cutoff_ratio;
+    }
+    else if(G->pot == ORC_PAIR_3SPN2_EV)
+    {
+        /* ThreeSPN2ExcludedVolumePotential.hpp:120-134 (epsilon 1 kJ/mol in kcal/mol), the list
+           cutoff is the largest sigma of the table (:176-178) */
+        G->eps = 1.0 * tsp_J_to_cal();
+        G->max_cutoff = 7.1;
+    }
+    else if(G->pot == ORC_3SPN2_BASEPAIR || G->pot == ORC_3SPN2_CROSSSTACK)
+    {
+        G->max_cutoff = 18.0;   /* ThreeSPN2BasePairGlobalPotentialParameter::cutoff */
     }
     else
     {
@@ -521,6 +676,15 @@ static void prepare_params(const global_t* G, size_t i, size_t j, double* out)
     else if(G->pot == ORC_PAIR_EV)
     {
         out[0] = G->ppar[i] + G->ppar[j];
+    }
+    else if(G->pot == ORC_PAIR_3SPN2_EV)
+    {
+        /* ThreeSPN2ExcludedVolumeParameterList::prepare_params (:198-202) */
+        out[0] = (tsp_ev_sigma((int)G->ppar[i]) + tsp_ev_sigma((int)G->ppar[j])) * 0.5;
+    }
+    else if(G->pot == ORC_3SPN2_BASEPAIR || G->pot == ORC_3SPN2_CROSSSTACK)
+    {
+        out[0] = 0.0;   /* the pair's kinds are read from the per-base parameters */
     }
     else
     {
@@ -972,6 +1136,156 @@ double local_force_impl(orc_engine* e, const local_t* L, double* F, const int ne
             }
         }
     }
+    else if(L->kind == ORC_BOND_3SPN2)
+    {
+        /* BondLengthInteraction.hpp:126-148; ThreeSPN2BondPotential.hpp:22-35 */
+        for(size_t t = 0; t < L->n; ++t)
+        {
+            const size_t i0 = L->idx[2 * t], i1 = L->idx[2 * t + 1];
+            const double k = L->par[2 * t], v0 = L->par[2 * t + 1];
+            double d[3];
+            adjust_direction(e, e->pos + 3 * i0, e->pos + 3 * i1, d);
+            const double len2 = length_sq(d);
+            const double rlen = rsqrt(len2);
+            const double len = len2 * rlen;
+            const double dv = len - v0;
+            const double force = -1 * (2 * k * (dv + 200 * dv * dv * dv));
+            if(need_energy) { const double dv2 = dv * dv; energy += k * (dv2 + 100 * dv2 * dv2); }
+            const double c = force * rlen;
+            for(int a = 0; a < 3; ++a) { F[3 * i0 + a] -= d[a] * c; F[3 * i1 + a] += d[a] * c; }
+        }
+    }
+    else if(L->kind == ORC_DIHEDRAL_PGAUSSIAN)
+    {
+        /* DihedralAngleInteraction.hpp:157-227; PeriodicGaussianPotential.hpp:21-44 */
+        for(size_t t = 0; t < L->n; ++t)
+        {
+            const uint32_t* ix = L->idx + 4 * t;
+            const double* p = L->par + 3 * t;
+            const double *r_i = e->pos + 3 * ix[0], *r_j = e->pos + 3 * ix[1];
+            const double *r_k = e->pos + 3 * ix[2], *r_l = e->pos + 3 * ix[3];
+            double r_ij[3], r_kj[3], r_lk[3], r_kl[3], m[3], n[3];
+            adjust_direction(e, r_j, r_i, r_ij);
+            adjust_direction(e, r_j, r_k, r_kj);
+            adjust_direction(e, r_k, r_l, r_lk);
+            for(int a = 0; a < 3; ++a) r_kl[a] = -1.0 * r_lk[a];
+            const double r_kj_lensq = length_sq(r_kj);
+            const double r_kj_rlen = rsqrt(r_kj_lensq);
+            const double r_kj_rlensq = r_kj_rlen * r_kj_rlen;
+            const double r_kj_len = r_kj_rlen * r_kj_lensq;
+            cross(r_ij, r_kj, m);
+            cross(r_kj, r_kl, n);
+            const double m_lensq = length_sq(m), n_lensq = length_sq(n);
+            const double dot_mn = dot(m, n) * rsqrt(m_lensq * n_lensq);
+            const double cos_phi = clampd(dot_mn, -1, 1);
+            const double phi = copysign(acos(cos_phi), dot(r_ij, n));
+            const double k = p[0], sigma = p[1], v0 = p[2];
+            const double inv_sigma2 = -1. / (2 * sigma * sigma);
+            double dval = phi - v0;
+            if(dval < -TSP_PI) { dval += TSP_TWO_PI; } else if(TSP_PI < dval) { dval -= TSP_TWO_PI; }
+            const double coef = -(2 * inv_sigma2 * dval * k * exp(inv_sigma2 * dval * dval));
+            if(need_energy)
+            {
+                double dv = phi - v0;
+                if(dv < -TSP_PI) { dv += TSP_TWO_PI; } else if(TSP_PI < dv) { dv -= TSP_TWO_PI; }
+                energy += k * exp(inv_sigma2 * dv * dv);
+            }
+            double Fi[3], Fl[3], Fj[3], Fk[3];
+            const double ci = coef * r_kj_len / m_lensq;
+            const double cl = -coef * r_kj_len / n_lensq;
+            for(int a = 0; a < 3; ++a) { Fi[a] = ci * m[a]; Fl[a] = cl * n[a]; }
+            const double coef_ijk = dot(r_ij, r_kj) * r_kj_rlensq;
+            const double coef_jkl = dot(r_kl, r_kj) * r_kj_rlensq;
+            for(int a = 0; a < 3; ++a)
+            {
+                Fj[a] = (coef_ijk - 1.0) * Fi[a] - coef_jkl * Fl[a];
+                Fk[a] = (coef_jkl - 1.0) * Fl[a] - coef_ijk * Fi[a];
+            }
+            for(int a = 0; a < 3; ++a)
+            {
+                F[3 * ix[0] + a] += Fi[a];
+                F[3 * ix[1] + a] += Fj[a];
+                F[3 * ix[2] + a] += Fk[a];
+                F[3 * ix[3] + a] += Fl[a];
+            }
+        }
+    }
+    else if(L->kind == ORC_3SPN2_STACKING)
+    {
+        /* ThreeSPN2BaseStackingInteraction::calc_force_energy_virial_impl (:281-428) with
+           ThreeSPN2BaseStackingPotential (:172-294) */
+        const double K = 6.0, piK = TSP_PI / 6.0, alp = 3.0;
+        for(size_t t = 0; t < L->n; ++t)
+        {
+            const uint32_t* ix = L->idx + 6 * t;
+            const size_t Si = ix[0], Bi = ix[1], Bj = ix[2];
+            const int bs = (int)L->par[t];
+            const double eps = tsp_bs_eps(bs), r0 = TSP_BS_R0[bs], th0 = tsp_bs_theta0(bs);
+            double Bji[3], SBi[3], Bji_reg[3], SBi_reg[3];
+            adjust_direction(e, e->pos + 3 * Bj, e->pos + 3 * Bi, Bji);
+            adjust_direction(e, e->pos + 3 * Si, e->pos + 3 * Bi, SBi);
+            const double lBji_sq = length_sq(Bji);
+            const double rlBji = rsqrt(lBji_sq);
+            const double lBji = lBji_sq * rlBji;
+            for(int a = 0; a < 3; ++a) Bji_reg[a] = Bji[a] * rlBji;
+            double f_Bi[3] = {0, 0, 0}, f_Bj[3] = {0, 0, 0}, f_Si[3] = {0, 0, 0};
+            double dU_rep = 0.0;
+            if(!(r0 <= lBji)) { const double term = exp(-alp * (lBji - r0)); dU_rep = 2 * alp * eps * term * (1.0 - term); }
+            if(dU_rep != 0.0)
+            {
+                for(int a = 0; a < 3; ++a) { f_Bi[a] -= dU_rep * Bji_reg[a]; f_Bj[a] += dU_rep * Bji_reg[a]; }
+            }
+            if(need_energy)
+            {
+                if(!(r0 <= lBji)) { const double term = 1.0 - exp(-alp * (lBji - r0)); energy += eps * term * term; }
+            }
+            const double lSBi_sq = length_sq(SBi);
+            const double rlSBi = rsqrt(lSBi_sq);
+            for(int a = 0; a < 3; ++a) SBi_reg[a] = SBi[a] * rlSBi;
+            const double cos_theta = dot(SBi_reg, Bji_reg);
+            const double theta = acos(clampd(cos_theta, -1, 1));
+            const double f_theta = tsp_f(K, piK, theta, th0);
+            if(f_theta == 0.0)
+            {
+                for(int a = 0; a < 3; ++a) { F[3 * Bi + a] += f_Bi[a]; F[3 * Bj + a] += f_Bj[a]; }
+                continue;
+            }
+            const double df_theta = tsp_df(K, piK, theta, th0);
+            double U_attr, dU_attr;
+            if(lBji <= r0) { U_attr = -eps; dU_attr = 0.0; }
+            else
+            {
+                const double term = exp(-alp * (lBji - r0));
+                U_attr = eps * (term * term - 2 * term);
+                dU_attr = 2 * alp * eps * term * (1.0 - term);
+            }
+            if(need_energy) energy += U_attr * f_theta;
+            if(df_theta != 0.0)
+            {
+                const double coef = -df_theta * U_attr;
+                const double sin_theta = sin(theta);
+                const double coef_rsin = (sin_theta > TSP_TOL) ? coef / sin_theta : coef / TSP_TOL;
+                double fSi[3], fBj[3];
+                for(int a = 0; a < 3; ++a)
+                {
+                    fSi[a] = (coef_rsin * rlSBi) * (Bji_reg[a] - cos_theta * SBi_reg[a]);
+                    fBj[a] = (coef_rsin * rlBji) * (SBi_reg[a] - cos_theta * Bji_reg[a]);
+                }
+                for(int a = 0; a < 3; ++a) { f_Si[a] += fSi[a]; f_Bi[a] -= (fSi[a] + fBj[a]); f_Bj[a] += fBj[a]; }
+            }
+            if(dU_attr != 0.0)
+            {
+                const double coef = f_theta * dU_attr;
+                for(int a = 0; a < 3; ++a) { f_Bi[a] -= coef * Bji_reg[a]; f_Bj[a] += coef * Bji_reg[a]; }
+            }
+            for(int a = 0; a < 3; ++a)
+            {
+                F[3 * Bi + a] += f_Bi[a];
+                F[3 * Bj + a] += f_Bj[a];
+                F[3 * Si + a] += f_Si[a];
+            }
+        }
+    }
     return energy;
 }
 
@@ -1064,6 +1378,518 @@ static double local_energy(orc_engine* e, const local_t* L)
             E += p[0] * (1.0 - cos1) + p[1] * (1.0 - cos3);
         }
     }
+    else if(L->kind == ORC_BOND_3SPN2)
+    {
+        for(size_t t = 0; t < L->n; ++t)
+        {
+            double d[3];
+            adjust_direction(e, e->pos + 3 * L->idx[2 * t], e->pos + 3 * L->idx[2 * t + 1], d);
+            const double r = sqrt(length_sq(d));
+            const double dv = r - L->par[2 * t + 1];
+            const double dv2 = dv * dv;
+            E += L->par[2 * t] * (dv2 + 100 * dv2 * dv2);
+        }
+    }
+    else if(L->kind == ORC_DIHEDRAL_PGAUSSIAN)
+    {
+        for(size_t t = 0; t < L->n; ++t)
+        {
+            const uint32_t* ix = L->idx + 4 * t;
+            const double* p = L->par + 3 * t;
+            double r_ij[3], r_kj[3], r_lk[3], mr_lk[3], n[3], R[3], S[3];
+            adjust_direction(e, e->pos + 3 * ix[1], e->pos + 3 * ix[0], r_ij);
+            adjust_direction(e, e->pos + 3 * ix[1], e->pos + 3 * ix[2], r_kj);
+            adjust_direction(e, e->pos + 3 * ix[2], e->pos + 3 * ix[3], r_lk);
+            const double inv = 1.0 / length_sq(r_kj);
+            for(int a = 0; a < 3; ++a) mr_lk[a] = -1.0 * r_lk[a];
+            cross(r_kj, mr_lk, n);
+            const double cij = dot(r_ij, r_kj) * inv, clk = dot(r_lk, r_kj) * inv;
+            for(int a = 0; a < 3; ++a) { R[a] = r_ij[a] - cij * r_kj[a]; S[a] = r_lk[a] - clk * r_kj[a]; }
+            const double Rl = length_sq(R), Sl = length_sq(S);
+            const double dRS = dot(R, S) * rsqrt(Rl * Sl);
+            const double phi = copysign(acos(clampd(dRS, -1.0, 1.0)), dot(r_ij, n));
+            const double inv_sigma2 = -1. / (2 * p[1] * p[1]);
+            double dval = phi - p[2];
+            if(dval < -TSP_PI) { dval += TSP_TWO_PI; } else if(TSP_PI < dval) { dval -= TSP_TWO_PI; }
+            E += p[0] * exp(inv_sigma2 * dval * dval);
+        }
+    }
+    else if(L->kind == ORC_3SPN2_STACKING)
+    {
+        /* ThreeSPN2BaseStackingInteraction::calc_energy (:206-276) */
+        const double K = 6.0, piK = TSP_PI / 6.0, alp = 3.0;
+        for(size_t t = 0; t < L->n; ++t)
+        {
+            const uint32_t* ix = L->idx + 6 * t;
+            const int bs = (int)L->par[t];
+            const double eps = tsp_bs_eps(bs), r0 = TSP_BS_R0[bs];
+            double Bji[3], SBi[3];
+            adjust_direction(e, e->pos + 3 * ix[2], e->pos + 3 * ix[1], Bji);
+            adjust_direction(e, e->pos + 3 * ix[0], e->pos + 3 * ix[1], SBi);
+            const double lBji_sq = length_sq(Bji);
+            const double rlBji = rsqrt(lBji_sq);
+            const double lBji = lBji_sq * rlBji;
+            if(!(r0 <= lBji)) { const double term = 1.0 - exp(-alp * (lBji - r0)); E += eps * term * term; }
+            const double lSBi_sq = length_sq(SBi);
+            const double rlSBi = rsqrt(lSBi_sq);
+            const double dot_SBij = dot(SBi, Bji);
+            const double cos_SBij = dot_SBij * rlBji * rlSBi;
+            const double theta = acos(clampd(cos_SBij, -1, 1));
+            const double f_theta = tsp_f(K, piK, theta, tsp_bs_theta0(bs));
+            if(f_theta == 0.0) continue;
+            double U_attr;
+            if(lBji <= r0) U_attr = -eps;
+            else { const double term = exp(-alp * (lBji - r0)); U_attr = eps * (term * term - 2 * term); }
+            E += U_attr * f_theta;
+        }
+    }
+    return E;
+}
+
+
+/* ------------------------------------------------------------------------------------- */
+/* 3SPN2 base pairing and cross stacking                                                  */
+/* ------------------------------------------------------------------------------------- */
+static inline double tsp_U_rep(double eps, double alpha, double r, double r0)
+{
+    if(r0 < r) return 0.0;
+    const double term = 1.0 - exp(-alpha * (r - r0));
+    return eps * term * term;
+}
+static inline double tsp_dU_rep(double eps, double alpha, double r, double r0)
+{
+    if(r0 < r) return 0.0;
+    const double term = exp(-alpha * (r - r0));
+    return 2 * alpha * eps * term * (1.0 - term);
+}
+static inline void tsp_U_dU_attr(double eps, double alpha, double r, double r0, double* U, double* dU)
+{
+    if(r <= r0) { *U = -eps; *dU = 0.0; return; }
+    const double term1 = exp(-alpha * (r - r0));
+    const double term2 = 1.0 - term1;
+    *U = eps * (term2 * term2 - 1.0);
+    *dU = 2 * alpha * eps * term1 * term2;
+}
+static inline double tsp_U_attr(double eps, double alpha, double r, double r0)
+{
+    if(r <= r0) return -eps;
+    const double term = 1.0 - exp(-alpha * (r - r0));
+    return eps * (term * term - 1.0);
+}
+
+/* ThreeSPN2BasePairInteraction::calc_force_energy_virial_impl (ThreeSPN2BasePairInteraction.hpp:
+ * 289-514).  Per base: ppar {base, sugar index}. */
+static inline __attribute__((always_inline))
+double basepair_force_impl(orc_engine* e, const global_t* G, double* F, const int need_energy)
+{
+    const double alpha = 2.0, K = 12.0, piK = TSP_PI / 12.0, cutoff_sq = 18.0 * 18.0;
+    double energy = 0;
+    const size_t nlead = G->np - 1;
+    for(size_t k = 0; k < nlead; ++k)
+    {
+        const size_t Bi = G->part[k];
+        const double* rBi = e->pos + 3 * Bi;
+        for(size_t q = G->rng_start[k]; q < G->rng_start[k + 1]; ++q)
+        {
+            const size_t Bj = G->nbr.a[q];
+            const double* rBj = e->pos + 3 * Bj;
+            double Bij[3];
+            adjust_direction(e, rBi, rBj, Bij);
+            const double lBij_sq = length_sq(Bij);
+            if(lBij_sq > cutoff_sq) continue;
+            const double rlBij = rsqrt(lBij_sq);
+            const double lBij = lBij_sq * rlBij;
+            double Bij_reg[3], Bji_reg[3];
+            for(int a = 0; a < 3; ++a) { Bij_reg[a] = rlBij * Bij[a]; Bji_reg[a] = -rlBij * Bij[a]; }
+            const int bp = tsp_bp_kind((int)G->ppar[2 * Bi]);
+            const double eps = tsp_bp_eps(bp), r0 = tsp_bp_r0(bp);
+            if(need_energy) energy += tsp_U_rep(eps, alpha, lBij, r0);
+            const double dU_rep = tsp_dU_rep(eps, alpha, lBij, r0);
+            if(dU_rep != 0.0)
+            {
+                double f[3];
+                for(int a = 0; a < 3; ++a) f[a] = -dU_rep * Bji_reg[a];
+                for(int a = 0; a < 3; ++a) { F[3 * Bi + a] += f[a]; F[3 * Bj + a] -= f[a]; }
+            }
+            const size_t Si = (size_t)G->ppar[2 * Bi + 1], Sj = (size_t)G->ppar[2 * Bj + 1];
+            double SBi[3], SBj[3], BSi_reg[3], BSj_reg[3];
+            adjust_direction(e, e->pos + 3 * Si, rBi, SBi);
+            adjust_direction(e, e->pos + 3 * Sj, rBj, SBj);
+            const double lSBi_sq = length_sq(SBi), lSBj_sq = length_sq(SBj);
+            const double rlSBi = rsqrt(lSBi_sq), rlSBj = rsqrt(lSBj_sq);
+            for(int a = 0; a < 3; ++a) { BSi_reg[a] = -rlSBi * SBi[a]; BSj_reg[a] = -rlSBj * SBj[a]; }
+            const double dot_SBiBj = -dot(SBi, Bij);
+            const double dot_SBjBi = dot(SBj, Bij);
+            const double cos_theta1 = dot_SBiBj * rlSBi * rlBij;
+            const double cos_theta2 = dot_SBjBi * rlSBj * rlBij;
+            const double theta1 = acos(clampd(cos_theta1, -1, 1));
+            const double theta2 = acos(clampd(cos_theta2, -1, 1));
+            const double f1 = tsp_f(K, piK, theta1, tsp_bp_theta1(bp));
+            const double f2 = tsp_f(K, piK, theta2, tsp_bp_theta2(bp));
+            if(f1 == 0.0 || f2 == 0.0) continue;
+            const double df1 = tsp_df(K, piK, theta1, tsp_bp_theta1(bp));
+            const double df2 = tsp_df(K, piK, theta2, tsp_bp_theta2(bp));
+            double mSBi[3], m[3], n[3];
+            for(int a = 0; a < 3; ++a) mSBi[a] = -SBi[a];
+            cross(mSBi, Bij, m);
+            cross(Bij, SBj, n);
+            const double m_lsq = length_sq(m), n_lsq = length_sq(n);
+            const double dot_phi = dot(m, n) * rsqrt(m_lsq * n_lsq);
+            const double cos_phi = clampd(dot_phi, -1, 1);
+            const double phi = copysign(acos(cos_phi), -dot(SBi, n));
+            double dphi = phi - tsp_bp_phi0(bp);
+            if(dphi < -TSP_PI) { dphi += TSP_TWO_PI; }
+            if(TSP_PI <= dphi) { dphi -= TSP_TWO_PI; }
+            const double cos_dphi = cos(dphi);
+            const double sin_dphi = sin(dphi);
+            double U, dU;
+            tsp_U_dU_attr(eps, alpha, lBij, r0, &U, &dU);
+            if(need_energy) energy += 0.5 * (1 + cos_dphi) * f1 * f2 * U;
+            if(cos_dphi == -1.0) continue;
+            double f_Si[3] = {0, 0, 0}, f_Sj[3] = {0, 0, 0}, f_Bi[3] = {0, 0, 0}, f_Bj[3] = {0, 0, 0};
+            if(sin_dphi != 0.0)
+            {
+                const double coef = 0.5 * sin_dphi * f1 * f2 * U;
+                const double rlBij_sq = rlBij * rlBij;
+                double fSi[3], fSj[3];
+                for(int a = 0; a < 3; ++a)
+                {
+                    fSi[a] = (coef * lBij / m_lsq) * m[a];
+                    fSj[a] = (-coef * lBij / n_lsq) * n[a];
+                }
+                const double coef_Bi = dot_SBiBj * rlBij_sq;
+                const double coef_Bj = dot_SBjBi * rlBij_sq;
+                for(int a = 0; a < 3; ++a)
+                {
+                    f_Si[a] += fSi[a];
+                    f_Bi[a] += (coef_Bi - 1.0) * fSi[a] - coef_Bj * fSj[a];
+                    f_Bj[a] += (coef_Bj - 1.0) * fSj[a] - coef_Bi * fSi[a];
+                    f_Sj[a] += fSj[a];
+                }
+            }
+            const double dihd_term = 0.5 * (1.0 + cos_dphi);
+            if(df1 != 0.0)
+            {
+                const double coef = -dihd_term * df1 * f2 * U;
+                const double sin_theta1 = sin(theta1);
+                const double coef_rsin = (sin_theta1 > TSP_TOL) ? (coef / sin_theta1) : (coef / TSP_TOL);
+                double fSi[3], fBj[3];
+                for(int a = 0; a < 3; ++a)
+                {
+                    fSi[a] = (coef_rsin * rlSBi) * (cos_theta1 * BSi_reg[a] - Bij_reg[a]);
+                    fBj[a] = (coef_rsin * rlBij) * (cos_theta1 * Bij_reg[a] - BSi_reg[a]);
+                }
+                for(int a = 0; a < 3; ++a) { f_Si[a] += fSi[a]; f_Bi[a] -= (fSi[a] + fBj[a]); f_Bj[a] += fBj[a]; }
+            }
+            if(df2 != 0.0)
+            {
+                const double coef = -dihd_term * f1 * df2 * U;
+                const double sin_theta2 = sin(theta2);
+                const double coef_rsin = (sin_theta2 > TSP_TOL) ? (coef / sin_theta2) : (coef / TSP_TOL);
+                double fBi[3], fSj[3];
+                for(int a = 0; a < 3; ++a)
+                {
+                    fBi[a] = (coef_rsin * rlBij) * (cos_theta2 * Bji_reg[a] - BSj_reg[a]);
+                    fSj[a] = (coef_rsin * rlSBj) * (cos_theta2 * BSj_reg[a] - Bji_reg[a]);
+                }
+                for(int a = 0; a < 3; ++a) { f_Bi[a] += fBi[a]; f_Bj[a] -= (fBi[a] + fSj[a]); f_Sj[a] += fSj[a]; }
+            }
+            if(dU != 0.0)
+            {
+                const double coef = -dihd_term * f1 * f2 * dU;
+                for(int a = 0; a < 3; ++a) { f_Bi[a] += coef * Bji_reg[a]; f_Bj[a] += coef * Bij_reg[a]; }
+            }
+            for(int a = 0; a < 3; ++a)
+            {
+                F[3 * Si + a] += f_Si[a];
+                F[3 * Bi + a] += f_Bi[a];
+                F[3 * Bj + a] += f_Bj[a];
+                F[3 * Sj + a] += f_Sj[a];
+            }
+        }
+    }
+    return energy;
+}
+
+/* ThreeSPN2BasePairInteraction::calc_energy (ThreeSPN2BasePairInteraction.hpp:133-284) */
+static double basepair_energy(orc_engine* e, const global_t* G)
+{
+    const double alpha = 2.0, K = 12.0, piK = TSP_PI / 12.0, cutoff_sq = 18.0 * 18.0;
+    double E = 0;
+    const size_t nlead = G->np - 1;
+    for(size_t k = 0; k < nlead; ++k)
+    {
+        const size_t Bi = G->part[k];
+        const double* rBi = e->pos + 3 * Bi;
+        for(size_t q = G->rng_start[k]; q < G->rng_start[k + 1]; ++q)
+        {
+            const size_t Bj = G->nbr.a[q];
+            const double* rBj = e->pos + 3 * Bj;
+            const int bp = tsp_bp_kind((int)G->ppar[2 * Bi]);
+            double Bij[3];
+            adjust_direction(e, rBi, rBj, Bij);
+            const double lBij_sq = length_sq(Bij);
+            if(lBij_sq > cutoff_sq) continue;
+            const double rlBij = rsqrt(lBij_sq);
+            const double lBij = lBij_sq * rlBij;
+            const double eps = tsp_bp_eps(bp), r0 = tsp_bp_r0(bp);
+            E += tsp_U_rep(eps, alpha, lBij, r0);
+            const size_t Si = (size_t)G->ppar[2 * Bi + 1], Sj = (size_t)G->ppar[2 * Bj + 1];
+            double SBi[3], SBj[3];
+            adjust_direction(e, e->pos + 3 * Si, rBi, SBi);
+            adjust_direction(e, e->pos + 3 * Sj, rBj, SBj);
+            const double lSBi_sq = length_sq(SBi), lSBj_sq = length_sq(SBj);
+            const double rlSBi = rsqrt(lSBi_sq), rlSBj = rsqrt(lSBj_sq);
+            const double dot_SBiBj = -dot(SBi, Bij);
+            const double dot_SBjBi = dot(SBj, Bij);
+            const double cos_theta1 = dot_SBiBj * rlSBi * rlBij;
+            const double cos_theta2 = dot_SBjBi * rlSBj * rlBij;
+            const double theta1 = acos(clampd(cos_theta1, -1, 1));
+            const double theta2 = acos(clampd(cos_theta2, -1, 1));
+            const double f1 = tsp_f(K, piK, theta1, tsp_bp_theta1(bp));
+            const double f2 = tsp_f(K, piK, theta2, tsp_bp_theta2(bp));
+            if(f1 == 0.0 || f2 == 0.0) continue;
+            double Bij_reg[3], R[3], S[3], n[3];
+            for(int a = 0; a < 3; ++a) Bij_reg[a] = Bij[a] * rlBij;
+            const double dR = dot(SBi, Bij_reg), dS = dot(SBj, Bij_reg);
+            for(int a = 0; a < 3; ++a) { R[a] = dR * Bij_reg[a] - SBi[a]; S[a] = dS * Bij_reg[a] - SBj[a]; }
+            const double dot_RS = dot(R, S) * rsqrt(length_sq(R) * length_sq(S));
+            const double cos_phi = clampd(dot_RS, -1, 1);
+            cross(Bij, SBj, n);
+            const double sign = -dot(SBi, n);
+            const double phi = copysign(acos(cos_phi), sign);
+            double dphi = phi - tsp_bp_phi0(bp);
+            if(TSP_PI < dphi) { dphi -= TSP_TWO_PI; }
+            else if(dphi < -TSP_PI) { dphi += TSP_TWO_PI; }
+            const double cos_dphi = cos(dphi);
+            const double U = tsp_U_attr(eps, alpha, lBij, r0);
+            E += 0.5 * (1 + cos_dphi) * f1 * f2 * U;
+        }
+    }
+    return E;
+}
+
+/* one cross-stacking element of ThreeSPN2CrossStackingInteraction::calc_force_energy_virial_impl
+ * (ThreeSPN2CrossStackingInteraction.hpp:445-526) */
+static inline __attribute__((always_inline))
+double cs_force_element(orc_engine* e, int cs, double f3, double df3, const double* rB1, const double* rBn,
+                        const double* SB1, double rlSB1, const double* BS1_reg,
+                        const double* fS1_t3, const double* fB1_t3, const double* fS2_t3, const double* fB2_t3,
+                        double* f_S1, double* f_B1, double* f_S2, double* f_B2, double* f_Bn, const int need_energy)
+{
+    const double K_CS = 8.0, piK_CS = TSP_PI / 8.0, alpha = 4.0;
+    const double eps = tsp_cs_eps(cs), r0 = TSP_CS_R0[cs];
+    double Bn1[3], Bn1_reg[3];
+    adjust_direction(e, rBn, rB1, Bn1);
+    const double lBn1_sq = length_sq(Bn1);
+    const double rlBn1 = rsqrt(lBn1_sq);
+    const double dot_theta_CS = dot(SB1, Bn1);
+    const double cos_theta_CS = dot_theta_CS * rlSB1 * rlBn1;
+    const double theta_CS = acos(clampd(cos_theta_CS, -1, 1));
+    const double th0 = tsp_cs_theta0(cs);
+    const double fCS = tsp_f(K_CS, piK_CS, theta_CS, th0);
+    const double dfCS = tsp_df(K_CS, piK_CS, theta_CS, th0);
+    if(fCS == 0.0) return 0.0;
+    const double lBn1 = lBn1_sq * rlBn1;
+    double U, dU;
+    tsp_U_dU_attr(eps, alpha, lBn1, r0, &U, &dU);
+    for(int a = 0; a < 3; ++a) Bn1_reg[a] = rlBn1 * Bn1[a];
+    if(df3 != 0.0)
+    {
+        const double coef = -0.5 * df3 * fCS * U;
+        for(int a = 0; a < 3; ++a)
+        {
+            f_S1[a] += coef * fS1_t3[a];
+            f_S2[a] += coef * fS2_t3[a];
+            f_B1[a] += coef * fB1_t3[a];
+            f_B2[a] += coef * fB2_t3[a];
+        }
+    }
+    if(dfCS != 0.0)
+    {
+        const double coef = -0.5 * f3 * dfCS * U;
+        const double sin_theta_CS = sin(theta_CS);
+        const double coef_rsin = (sin_theta_CS > TSP_TOL) ? (coef / sin_theta_CS) : (coef / TSP_TOL);
+        double fS1[3], fBn[3];
+        for(int a = 0; a < 3; ++a)
+        {
+            fS1[a] = coef_rsin * rlSB1 * (cos_theta_CS * BS1_reg[a] + Bn1_reg[a]);
+            fBn[a] = -coef_rsin * rlBn1 * (cos_theta_CS * Bn1_reg[a] + BS1_reg[a]);
+        }
+        for(int a = 0; a < 3; ++a) { f_S1[a] += fS1[a]; f_B1[a] -= (fS1[a] + fBn[a]); f_Bn[a] += fBn[a]; }
+    }
+    if(dU != 0.0)
+    {
+        const double coef = -0.5 * f3 * fCS * dU;
+        for(int a = 0; a < 3; ++a) { f_B1[a] += coef * Bn1_reg[a]; f_Bn[a] -= coef * Bn1_reg[a]; }
+    }
+    return need_energy ? 0.5 * f3 * fCS * U : 0.0;
+}
+
+/* ThreeSPN2CrossStackingInteraction::calc_force_energy_virial_impl (:340-640).  Per base:
+ * ppar {base, sugar index, 5' base (-1: none), 3' base (-1: none)}; cross-stacking kinds
+ * cs5 = b1 << 2 | b2, cs3 = (b1 << 2 | b2) + 16 (ThreeSPN2CrossStackingPotential.hpp:290-307) */
+static inline __attribute__((always_inline))
+double crossstack_force_impl(orc_engine* e, const global_t* G, double* F, const int need_energy)
+{
+    const double K_BP = 12.0, piK_BP = TSP_PI / 12.0, cutoff_sq = 18.0 * 18.0;
+    double energy = 0;
+    const size_t nlead = G->np - 1;
+    for(size_t k = 0; k < nlead; ++k)
+    {
+        const size_t Bi = G->part[k];
+        const double* rBi = e->pos + 3 * Bi;
+        const double* pi_ = G->ppar + 4 * Bi;
+        const long Bi5 = (long)pi_[2], Bi3 = (long)pi_[3];
+        const size_t Si = (size_t)pi_[1];
+        const int Bi_base = (int)pi_[0];
+        const int Bi5_exists = Bi5 >= 0, Bi3_exists = Bi3 >= 0;
+        for(size_t q = G->rng_start[k]; q < G->rng_start[k + 1]; ++q)
+        {
+            const size_t Bj = G->nbr.a[q];
+            const double* rBj = e->pos + 3 * Bj;
+            const int bp = tsp_bp_kind(Bi_base);
+            double Bij[3];
+            adjust_direction(e, rBi, rBj, Bij);
+            const double lBij_sq = length_sq(Bij);
+            if(lBij_sq > cutoff_sq) continue;
+            const double* pj = G->ppar + 4 * Bj;
+            const long Bj5 = (long)pj[2], Bj3 = (long)pj[3];
+            const size_t Sj = (size_t)pj[1];
+            const int Bj5_exists = Bj5 >= 0, Bj3_exists = Bj3 >= 0;
+            if(!Bi3_exists && !Bi5_exists && !Bj3_exists && !Bj5_exists) continue;
+            const int Bj_base = (int)pj[0];
+            double SBi[3], SBj[3], BSi_reg[3], BSj_reg[3];
+            adjust_direction(e, e->pos + 3 * Si, rBi, SBi);
+            adjust_direction(e, e->pos + 3 * Sj, rBj, SBj);
+            const double lSBi_sq = length_sq(SBi), lSBj_sq = length_sq(SBj);
+            const double rlSBi = rsqrt(lSBi_sq), rlSBj = rsqrt(lSBj_sq);
+            for(int a = 0; a < 3; ++a) { BSi_reg[a] = -rlSBi * SBi[a]; BSj_reg[a] = -rlSBj * SBj[a]; }
+            const double cos_theta3 = dot(BSi_reg, BSj_reg);
+            const double theta3 = acos(clampd(cos_theta3, -1, 1));
+            const double th3_0 = tsp_cs_theta3(bp);
+            const double f3 = tsp_f(K_BP, piK_BP, theta3, th3_0);
+            if(f3 == 0.0) continue;
+            const double df3 = tsp_df(K_BP, piK_BP, theta3, th3_0);
+            const double sin_theta3 = sin(theta3);
+            const double rsin_theta3 = sin_theta3 > TSP_TOL ? 1.0 / sin_theta3 : 1.0 / TSP_TOL;
+            double fBi_t3[3], fBj_t3[3], fSi_t3[3], fSj_t3[3];
+            for(int a = 0; a < 3; ++a)
+            {
+                fBi_t3[a] = rsin_theta3 * rlSBi * (BSj_reg[a] - cos_theta3 * BSi_reg[a]);
+                fBj_t3[a] = rsin_theta3 * rlSBj * (BSi_reg[a] - cos_theta3 * BSj_reg[a]);
+            }
+            for(int a = 0; a < 3; ++a) { fSi_t3[a] = -1.0 * fBi_t3[a]; fSj_t3[a] = -1.0 * fBj_t3[a]; }
+            double f_Si[3] = {0, 0, 0}, f_Sj[3] = {0, 0, 0}, f_Bi[3] = {0, 0, 0}, f_Bi3[3] = {0, 0, 0};
+            double f_Bi5[3] = {0, 0, 0}, f_Bj[3] = {0, 0, 0}, f_Bj3[3] = {0, 0, 0}, f_Bj5[3] = {0, 0, 0};
+            if(Bj5_exists)
+            {
+                const int cs = (Bi_base << 2 | (int)G->ppar[4 * Bj5]);
+                energy += cs_force_element(e, cs, f3, df3, rBi, e->pos + 3 * Bj5, SBi, rlSBi, BSi_reg,
+                                           fSi_t3, fBi_t3, fSj_t3, fBj_t3, f_Si, f_Bi, f_Sj, f_Bj, f_Bj5, need_energy);
+            }
+            if(Bi5_exists)
+            {
+                const int cs = (Bj_base << 2 | (int)G->ppar[4 * Bi5]);
+                energy += cs_force_element(e, cs, f3, df3, rBj, e->pos + 3 * Bi5, SBj, rlSBj, BSj_reg,
+                                           fSj_t3, fBj_t3, fSi_t3, fBi_t3, f_Sj, f_Bj, f_Si, f_Bi, f_Bi5, need_energy);
+            }
+            if(Bi3_exists)
+            {
+                const int cs = (Bj_base << 2 | (int)G->ppar[4 * Bi3]) + 16;
+                energy += cs_force_element(e, cs, f3, df3, rBj, e->pos + 3 * Bi3, SBj, rlSBj, BSj_reg,
+                                           fSj_t3, fBj_t3, fSi_t3, fBi_t3, f_Sj, f_Bj, f_Si, f_Bi, f_Bi3, need_energy);
+            }
+            if(Bj3_exists)
+            {
+                const int cs = (Bi_base << 2 | (int)G->ppar[4 * Bj3]) + 16;
+                energy += cs_force_element(e, cs, f3, df3, rBi, e->pos + 3 * Bj3, SBi, rlSBi, BSi_reg,
+                                           fSi_t3, fBi_t3, fSj_t3, fBj_t3, f_Si, f_Bi, f_Sj, f_Bj, f_Bj3, need_energy);
+            }
+            for(int a = 0; a < 3; ++a)
+            {
+                F[3 * Si + a] += f_Si[a];
+                F[3 * Sj + a] += f_Sj[a];
+                F[3 * Bi + a] += f_Bi[a];
+                F[3 * Bj + a] += f_Bj[a];
+            }
+            if(Bi3_exists) for(int a = 0; a < 3; ++a) F[3 * Bi3 + a] += f_Bi3[a];
+            if(Bi5_exists) for(int a = 0; a < 3; ++a) F[3 * Bi5 + a] += f_Bi5[a];
+            if(Bj3_exists) for(int a = 0; a < 3; ++a) F[3 * Bj3 + a] += f_Bj3[a];
+            if(Bj5_exists) for(int a = 0; a < 3; ++a) F[3 * Bj5 + a] += f_Bj5[a];
+        }
+    }
+    return energy;
+}
+
+static double cs_energy_element(orc_engine* e, int cs, double f3, const double* rB1, const double* rB2,
+                                const double* SB1, double rlSB1)
+{
+    const double K_CS = 8.0, piK_CS = TSP_PI / 8.0, alpha = 4.0;
+    const double eps = tsp_cs_eps(cs), r0 = TSP_CS_R0[cs];
+    double B21[3];
+    adjust_direction(e, rB2, rB1, B21);
+    const double lB21_sq = length_sq(B21);
+    const double rlB21 = rsqrt(lB21_sq);
+    const double cos_theta_CS = dot(SB1, B21) * rlSB1 * rlB21;
+    const double theta_CS = acos(clampd(cos_theta_CS, -1, 1));
+    const double fCS = tsp_f(K_CS, piK_CS, theta_CS, tsp_cs_theta0(cs));
+    if(fCS == 0.0) return 0.0;
+    const double lB21 = lB21_sq * rlB21;
+    const double U = tsp_U_attr(eps, alpha, lB21, r0);
+    return 0.5 * f3 * fCS * U;
+}
+
+/* ThreeSPN2CrossStackingInteraction::calc_energy (:133-337) */
+static double crossstack_energy(orc_engine* e, const global_t* G)
+{
+    const double K_BP = 12.0, piK_BP = TSP_PI / 12.0, cutoff_sq = 18.0 * 18.0;
+    double E = 0;
+    const size_t nlead = G->np - 1;
+    for(size_t k = 0; k < nlead; ++k)
+    {
+        const size_t Bi = G->part[k];
+        const double* rBi = e->pos + 3 * Bi;
+        const double* pi_ = G->ppar + 4 * Bi;
+        const long Bi5 = (long)pi_[2], Bi3 = (long)pi_[3];
+        const size_t Si = (size_t)pi_[1];
+        const int Bi_base = (int)pi_[0];
+        const int Bi5_exists = Bi5 >= 0, Bi3_exists = Bi3 >= 0;
+        for(size_t q = G->rng_start[k]; q < G->rng_start[k + 1]; ++q)
+        {
+            const size_t Bj = G->nbr.a[q];
+            const double* rBj = e->pos + 3 * Bj;
+            const int bp = tsp_bp_kind(Bi_base);
+            const double* pj = G->ppar + 4 * Bj;
+            const int Bj_base = (int)pj[0];
+            double Bij[3];
+            adjust_direction(e, rBi, rBj, Bij);
+            const double lBij_sq = length_sq(Bij);
+            if(lBij_sq > cutoff_sq) continue;
+            const long Bj5 = (long)pj[2], Bj3 = (long)pj[3];
+            const size_t Sj = (size_t)pj[1];
+            const int Bj5_exists = Bj5 >= 0, Bj3_exists = Bj3 >= 0;
+            if(!Bi3_exists && !Bi5_exists && !Bj3_exists && !Bj5_exists) continue;
+            double SBi[3], SBj[3];
+            adjust_direction(e, e->pos + 3 * Si, rBi, SBi);
+            adjust_direction(e, e->pos + 3 * Sj, rBj, SBj);
+            const double lSBi_sq = length_sq(SBi), lSBj_sq = length_sq(SBj);
+            const double rlSBi = rsqrt(lSBi_sq), rlSBj = rsqrt(lSBj_sq);
+            const double dot_SBiSBj = dot(SBi, SBj);
+            const double cos_theta3 = dot_SBiSBj * rlSBi * rlSBj;
+            const double theta3 = acos(clampd(cos_theta3, -1, 1));
+            const double f3 = tsp_f(K_BP, piK_BP, theta3, tsp_cs_theta3(bp));
+            if(f3 == 0.0) continue;
+            if(Bj5_exists)
+                E += cs_energy_element(e, (Bi_base << 2 | (int)G->ppar[4 * Bj5]), f3, rBi, e->pos + 3 * Bj5, SBi, rlSBi);
+            if(Bi5_exists)
+                E += cs_energy_element(e, (Bj_base << 2 | (int)G->ppar[4 * Bi5]), f3, rBj, e->pos + 3 * Bi5, SBj, rlSBj);
+            if(Bi3_exists)
+                E += cs_energy_element(e, (Bj_base << 2 | (int)G->ppar[4 * Bi3]) + 16, f3, rBj, e->pos + 3 * Bi3, SBj, rlSBj);
+            if(Bj3_exists)
+                E += cs_energy_element(e, (Bi_base << 2 | (int)G->ppar[4 * Bj3]) + 16, f3, rBi, e->pos + 3 * Bj3, SBi, rlSBi);
+        }
+    }
     return E;
 }
 
@@ -1072,6 +1898,50 @@ double global_force_impl(orc_engine* e, const global_t* G, double* F, const int 
 {
     double energy = 0;
     const size_t nlead = G->np - 1;
+    if(G->pot == ORC_3SPN2_BASEPAIR) return basepair_force_impl(e, G, F, need_energy);
+    if(G->pot == ORC_3SPN2_CROSSSTACK) return crossstack_force_impl(e, G, F, need_energy);
+    if(G->pot == ORC_PAIR_3SPN2_EV)
+    {
+        /* GlobalPairInteraction.hpp:148-189 with ThreeSPN2ExcludedVolumePotential.hpp:47-64 */
+        const double eps = G->eps;
+        for(size_t k = 0; k < nlead; ++k)
+        {
+            const size_t i = G->part[k];
+            for(size_t q = G->rng_start[k]; q < G->rng_start[k + 1]; ++q)
+            {
+                const size_t j = G->nbr.a[q];
+                const double sigma = G->npp[q];
+                double d[3];
+                adjust_direction(e, e->pos + 3 * i, e->pos + 3 * j, d);
+                const double l2 = length_sq(d);
+                const double rl = rsqrt(l2);
+                const double l = l2 * rl;
+                double f_mag = 0.0;
+                if(!(sigma <= l))
+                {
+                    const double rinv = 1 / l;
+                    const double r1s1 = sigma * rinv;
+                    const double r3s3 = r1s1 * r1s1 * r1s1;
+                    const double r6s6 = r3s3 * r3s3;
+                    f_mag = 12.0 * eps * rinv * r6s6 * (1.0 - r6s6);
+                }
+                if(f_mag == 0.0) continue;
+                const double c = f_mag * rl;
+                for(int a = 0; a < 3; ++a) { F[3 * i + a] += d[a] * c; F[3 * j + a] -= d[a] * c; }
+                if(need_energy)
+                {
+                    if(!(sigma <= l))
+                    {
+                        const double r1s1 = sigma / l;
+                        const double r3s3 = r1s1 * r1s1 * r1s1;
+                        const double r6s6 = r3s3 * r3s3;
+                        energy += eps * (r6s6 * (r6s6 - 2.0) + 1.0);
+                    }
+                }
+            }
+        }
+        return energy;
+    }
     if(G->pot == ORC_PAIR_LJ)
     {
         /* GlobalPairLennardJonesInteraction.hpp:148-198 */
@@ -1163,6 +2033,8 @@ static double global_force(orc_engine* e, const global_t* G, double* F, int need
  * GlobalPairExcludedVolumeInteraction.hpp:100-132, GlobalPairInteraction.hpp:191-213 */
 static double global_energy(orc_engine* e, const global_t* G)
 {
+    if(G->pot == ORC_3SPN2_BASEPAIR) return basepair_energy(e, G);
+    if(G->pot == ORC_3SPN2_CROSSSTACK) return crossstack_energy(e, G);
     double E = 0;
     const size_t nlead = G->np - 1;
     for(size_t k = 0; k < nlead; ++k)
@@ -1193,6 +2065,16 @@ static double global_energy(orc_engine* e, const global_t* G)
                 const double s2l2 = sigma_sq / l_sq;
                 const double s6l6 = s2l2 * s2l2 * s2l2;
                 E += G->eps * (s6l6 * s6l6 - G->coef_at_cutoff);
+            }
+            else if(G->pot == ORC_PAIR_3SPN2_EV)
+            {
+                const double l = sqrt(length_sq(d));
+                const double sigma = G->npp[q];
+                if(sigma <= l) continue;
+                const double r1s1 = sigma / l;
+                const double r3s3 = r1s1 * r1s1 * r1s1;
+                const double r6s6 = r3s3 * r3s3;
+                E += G->eps * (r6s6 * (r6s6 - 2.0) + 1.0);
             }
             else
             {

@@ -26,15 +26,26 @@ enum {
     ORC_CONTACT_GOCONTACT  = 3, /* Contact:GoContact        GoContactInteraction.hpp:159-228 */
     ORC_ANGLE_HARMONIC     = 4, /* BondAngle:Harmonic       BondAngleInteraction.hpp:129-189 */
     ORC_ANGLE_GAUSSIAN     = 5, /* BondAngle:Gaussian       BondAngleInteraction.hpp:129-189 */
-    ORC_DIHEDRAL_CLEMENTI  = 6  /* DihedralAngle:ClementiDihedral DihedralAngleInteraction.hpp:157-227 */
+    ORC_DIHEDRAL_CLEMENTI  = 6, /* DihedralAngle:ClementiDihedral DihedralAngleInteraction.hpp:157-227 */
+    ORC_BOND_3SPN2         = 7, /* BondLength:3SPN2Bond {k, v0}  ThreeSPN2BondPotential.hpp:22-35 */
+    ORC_DIHEDRAL_PGAUSSIAN = 8, /* DihedralAngle:Gaussian {k, sigma, v0}, PeriodicGaussianPotential.hpp:21-44 */
+    ORC_3SPN2_STACKING     = 9  /* 3SPN2BaseStacking {stack kind}, indices (S5, B5, B3, P5, S3, P3), P may be
+                                   ORC_NIL; ThreeSPN2BaseStackingInteraction.hpp:127-428 */
 };
 /* global pair potentials */
 enum {
     ORC_PAIR_LJ = 1,  /* GlobalPairLennardJonesInteraction.hpp:148-198 */
     ORC_PAIR_EV = 2,  /* GlobalPairExcludedVolumeInteraction.hpp:148-195 */
-    ORC_PAIR_DH = 3   /* GlobalPairInteraction.hpp:148-213 + DebyeHuckelPotential.hpp:47-65 */
+    ORC_PAIR_DH = 3,  /* GlobalPairInteraction.hpp:148-213 + DebyeHuckelPotential.hpp:47-65 */
+    ORC_PAIR_3SPN2_EV = 4,   /* Pair:3SPN2ExcludedVolume {bead kind P,S,A,T,G,C = 0..5}
+                                ThreeSPN2ExcludedVolumePotential.hpp:27-446 */
+    ORC_3SPN2_BASEPAIR = 5,  /* 3SPN2BasePair {base A,T,G,C = 0..3, sugar index}
+                                ThreeSPN2BasePairInteraction.hpp:130-514 */
+    ORC_3SPN2_CROSSSTACK = 6 /* 3SPN2CrossStacking {base, sugar, 5' base, 3' base (ORC_NIL if none)}
+                                ThreeSPN2CrossStackingInteraction.hpp:133-640 */
 };
 enum { ORC_PART_CELLLIST = 1, ORC_PART_VERLET = 2, ORC_PART_NAIVE = 3 };
+#define ORC_NIL 0xffffffffu   /* absent index (a nucleotide without phosphate, a strand end) */
 enum { ORC_IGNORE_NOTHING = 0, ORC_IGNORE_SELF = 1, ORC_IGNORE_OTHERS = 2 };
 enum { ORC_INTEG_BAOAB = 1, ORC_INTEG_GJF = 2 };
 

@@ -47,6 +47,12 @@
 #include <mjolnir/forcefield/local/GaussianPotential.hpp>
 #include <mjolnir/forcefield/local/ClementiDihedralPotential.hpp>
 #include <mjolnir/forcefield/local/GoContactPotential.hpp>
+#include <mjolnir/forcefield/local/PeriodicGaussianPotential.hpp>
+#include <mjolnir/forcefield/3SPN2/ThreeSPN2BondPotential.hpp>
+#include <mjolnir/forcefield/3SPN2/ThreeSPN2BaseStackingInteraction.hpp>
+#include <mjolnir/forcefield/3SPN2/ThreeSPN2BasePairInteraction.hpp>
+#include <mjolnir/forcefield/3SPN2/ThreeSPN2CrossStackingInteraction.hpp>
+#include <mjolnir/forcefield/3SPN2/ThreeSPN2ExcludedVolumePotential.hpp>
 #ifdef MJOLNIR_WITH_OPENMP
 #include <mjolnir/omp/omp.hpp>
 #endif
@@ -137,6 +143,8 @@ static SpecIn read_spec(const std::string& path)
             if(l.interaction == "BondAngle") {arity = 3;}
             if(l.interaction == "DihedralAngle") {arity = 4;}
             if(l.potential == "Gaussian" || l.potential == "ClementiDihedral") {npar = 3;}
+            // 3SPN2 base stacking: S5 B5 B3 P5 S3 P3 (absent phosphate 4294967295), stack kind
+            if(l.interaction == "3SPN2BaseStacking") {arity = 6; npar = 1;}
             for(std::size_t t = 0; t < n; ++t)
             {
                 std::vector<std::size_t> ix(arity); std::vector<double> p(npar);
@@ -160,7 +168,9 @@ static SpecIn read_spec(const std::string& path)
                     g.within[item.substr(0, c)] = std::stoul(item.substr(c + 1));
                 }
             }
-            const std::size_t npar = (g.potential == "LennardJones") ? 2 : 1;
+            std::size_t npar = (g.potential == "LennardJones") ? 2 : 1;
+            if(g.potential == "3SPN2BasePair") {npar = 2;}        // base, sugar
+            if(g.potential == "3SPN2CrossStacking") {npar = 4;}   // base, sugar, 5' base, 3' base (-1: none)
             for(std::size_t t = 0; t < n; ++t)
             {
                 std::size_t i; std::vector<double> p(npar);
@@ -371,6 +381,51 @@ static Built<traitsT> build(const SpecIn& s)
             }
             local.emplace(std::unique_ptr<LocalInteractionBase<traitsT>>(new I(l.topology, std::move(c))));
         }
+        else if(l.interaction == "BondLength" && l.potential == "3SPN2Bond")
+        {
+            using pot = ThreeSPN2BondPotential<real_type>;
+            using I = BondLengthInteraction<traitsT, pot>;
+            typename I::container_type c;
+            for(std::size_t t = 0; t < l.idx.size(); ++t)
+            {
+                c.emplace_back(typename I::indices_type{{l.idx[t][0], l.idx[t][1]}}, pot(l.par[t][0], l.par[t][1]));
+            }
+            local.emplace(std::unique_ptr<LocalInteractionBase<traitsT>>(new I(l.topology, std::move(c))));
+        }
+        else if(l.interaction == "DihedralAngle" && l.potential == "Gaussian")
+        {
+            // read_local_interaction.hpp:282-290: a Gaussian dihedral is PeriodicGaussianPotential
+            using pot = PeriodicGaussianPotential<real_type>;
+            using I = DihedralAngleInteraction<traitsT, pot>;
+            typename I::container_type c;
+            for(std::size_t t = 0; t < l.idx.size(); ++t)
+            {
+                c.emplace_back(typename I::indices_type{{l.idx[t][0], l.idx[t][1], l.idx[t][2], l.idx[t][3]}},
+                               pot(l.par[t][0], l.par[t][1], l.par[t][2]));
+            }
+            local.emplace(std::unique_ptr<LocalInteractionBase<traitsT>>(new I(l.topology, std::move(c))));
+        }
+        else if(l.interaction == "3SPN2BaseStacking")
+        {
+            using I = ThreeSPN2BaseStackingInteraction<traitsT>;
+            using nuc_t = parameter_3SPN2::NucleotideInfo;
+            using bsk = parameter_3SPN2::base_stack_kind;
+            const std::size_t nil_in = 4294967295u;
+            typename I::container_type c;
+            std::vector<nuc_t> nucs;
+            for(std::size_t t = 0; t < l.idx.size(); ++t)
+            {
+                const auto& ix = l.idx[t];
+                c.emplace_back(typename I::indices_type{{ix[0], ix[1], ix[2]}}, static_cast<bsk>((int)l.par[t][0]));
+                nuc_t n5, n3;
+                n5.strand = 0; n5.S = ix[0]; n5.B = ix[1]; n5.P = (ix[3] == nil_in) ? nuc_t::nil() : ix[3];
+                n3.strand = 0; n3.S = ix[4]; n3.B = ix[2]; n3.P = (ix[5] == nil_in) ? nuc_t::nil() : ix[5];
+                nucs.push_back(n5); nucs.push_back(n3);
+            }
+            ThreeSPN2BaseStackingPotential<real_type> pot{ThreeSPN2BaseStackingPotentialParameter<real_type>{}};
+            local.emplace(std::unique_ptr<LocalInteractionBase<traitsT>>(
+                new I(l.topology, std::move(c), std::move(pot), std::move(nucs))));
+        }
         else
         {
             throw std::runtime_error("unsupported local " + l.interaction + ":" + l.potential);
@@ -426,6 +481,63 @@ static Built<traitsT> build(const SpecIn& s)
             SpatialPartition<traitsT, pot> part(make_partition<traitsT, pot, SpatialPartitionBase<traitsT, pot>>(g));
             auto* inter = new I(pot(g.cutoff), std::move(pl), std::move(part));
             b.dump_pairs.push_back(pair_dumper(inter, g.participants));
+            global.emplace(std::unique_ptr<GlobalInteractionBase<traitsT>>(inter));
+        }
+        else if(g.potential == "3SPN2ExcludedVolume")
+        {
+            using pot = ThreeSPN2ExcludedVolumePotential<real_type>;
+            using PL  = ThreeSPN2ExcludedVolumeParameterList<traitsT>;
+            using I = GlobalPairInteraction<traitsT, pot>;
+            std::vector<std::pair<std::size_t, typename PL::parameter_type>> ps;
+            for(std::size_t t = 0; t < g.participants.size(); ++t)
+            {
+                ps.emplace_back(g.participants[t], static_cast<parameter_3SPN2::bead_kind>((int)g.par[t][0]));
+            }
+            ParameterList<traitsT, pot> pl(make_unique<PL>(ThreeSPN2ExcludedVolumePotentialParameter<real_type>{},
+                                                           ps, g.within, ign_mol, ign_grp));
+            SpatialPartition<traitsT, pot> part(make_partition<traitsT, pot, SpatialPartitionBase<traitsT, pot>>(g));
+            auto* inter = new I(pot(), std::move(pl), std::move(part));
+            b.dump_pairs.push_back(pair_dumper(inter, g.participants));
+            global.emplace(std::unique_ptr<GlobalInteractionBase<traitsT>>(inter));
+        }
+        else if(g.potential == "3SPN2BasePair")
+        {
+            using I = ThreeSPN2BasePairInteraction<traitsT>;
+            using pot = typename I::potential_type;
+            using PL = typename I::parameter_list_type;
+            std::vector<std::pair<std::size_t, typename PL::parameter_type>> ps;
+            for(std::size_t t = 0; t < g.participants.size(); ++t)
+            {
+                typename PL::parameter_type p;
+                p.base = static_cast<parameter_3SPN2::base_kind>((int)g.par[t][0]);
+                p.S_idx = static_cast<std::size_t>(g.par[t][1]);
+                ps.emplace_back(g.participants[t], p);
+            }
+            PL pl(ThreeSPN2BasePairGlobalPotentialParameter<real_type>{}, ps, g.within, ign_mol, ign_grp);
+            typename I::partition_type part(make_partition<traitsT, pot, SpatialPartitionBase<traitsT, pot>>(g));
+            auto* inter = new I(pot{}, std::move(pl), std::move(part));
+            b.dump_pairs.push_back([](std::ostream&) {});   // no partition accessor on this class
+            global.emplace(std::unique_ptr<GlobalInteractionBase<traitsT>>(inter));
+        }
+        else if(g.potential == "3SPN2CrossStacking")
+        {
+            using I = ThreeSPN2CrossStackingInteraction<traitsT>;
+            using pot = typename I::potential_type;
+            using PL = typename I::parameter_list_type;
+            std::vector<std::pair<std::size_t, typename PL::parameter_type>> ps;
+            for(std::size_t t = 0; t < g.participants.size(); ++t)
+            {
+                typename PL::parameter_type p;
+                p.base = static_cast<parameter_3SPN2::base_kind>((int)g.par[t][0]);
+                p.S = static_cast<std::size_t>(g.par[t][1]);
+                p.B5 = (g.par[t][2] < 0) ? pot::invalid() : static_cast<std::size_t>(g.par[t][2]);
+                p.B3 = (g.par[t][3] < 0) ? pot::invalid() : static_cast<std::size_t>(g.par[t][3]);
+                ps.emplace_back(g.participants[t], p);
+            }
+            PL pl(ThreeSPN2CrossStackingGlobalPotentialParameter<real_type>{}, ps, g.within, ign_mol, ign_grp);
+            typename I::partition_type part(make_partition<traitsT, pot, SpatialPartitionBase<traitsT, pot>>(g));
+            auto* inter = new I(pot{}, std::move(pl), std::move(part));
+            b.dump_pairs.push_back([](std::ostream&) {});
             global.emplace(std::unique_ptr<GlobalInteractionBase<traitsT>>(inter));
         }
         else

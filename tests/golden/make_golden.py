@@ -132,6 +132,8 @@ def cases():
         "lj_pair_scan": (lj_pair_scan, True, 0),
         "lj_boundary": (lj_boundary_cases, True, 0),
         "chain200_remover": (chain_with_remover, True, 1000),
+        "dna12": (lambda: perturb(workloads.dna_duplex(12, seed=5), 0.2, 11), True, 1000),
+        "dna200": (lambda: perturb(workloads.dna_duplex(200, seed=3), 0.1, 12), False, 1000),
     }
 
 
