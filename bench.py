@@ -13,6 +13,7 @@ Workloads (seeded synthetic inputs, ``mjolnir_amd.workloads``):
                integrator, one persistent launch per 100-step segment
   chain1000    one CG chain, N = 1000, FP64 engine
   lj4096       LJ fluid, N = 4096, rho sigma^3 = 0.6, FP64 engine
+  dna200       3SPN2 double-stranded DNA, 200 bp (1198 beads), G-JF dt = 0.02, FP64 engine
 """
 from __future__ import annotations
 
@@ -86,7 +87,8 @@ def bench_single(args, spec):
     achieved = alg_bytes / (ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-                "kernel": f"k_pair_full<{g.potential}>", "avg_us": round(ms * 1e3, 3),
+                "kernel": ("k_tsp_pairs" if g.potential in ("3SPN2BasePair", "3SPN2CrossStacking") else "k_pairs")
+                + f" ({g.potential})", "avg_us": round(ms * 1e3, 3),
                 "half_pairs": hp, "participants": np_}
     cpu = None if args.no_cpu_baseline else cpu_baseline_single(spec)
     return {
@@ -212,7 +214,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=500)
-    ap.add_argument("--workload", default="replicas512", choices=["replicas512", "chain1000", "lj4096"])
+    ap.add_argument("--workload", default="replicas512", choices=["replicas512", "chain1000", "lj4096", "dna200"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-sample", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -237,7 +239,9 @@ def main():
         out = bench_batch(args)
     else:
         from mjolnir_amd import workloads
-        spec = workloads.cg_chain(n=1000, seed=2) if args.workload == "chain1000" else workloads.lj_fluid(n_side=16, seed=1)
+        spec = {"chain1000": lambda: workloads.cg_chain(n=1000, seed=2),
+                "lj4096": lambda: workloads.lj_fluid(n_side=16, seed=1),
+                "dna200": lambda: workloads.dna_duplex(n_bp=200)}[args.workload]()
         print(f"[bench] workload {spec.name} N={spec.n}", file=sys.stderr, flush=True)
         out = bench_single(args, spec)
     print(json.dumps(out), flush=True)

@@ -59,13 +59,24 @@ enum {
     MJG_CONTACT_GOCONTACT = 3, /* Contact:GoContact    {k, v0}        GoContactInteraction.hpp:159-228 */
     MJG_ANGLE_HARMONIC    = 4, /* BondAngle:Harmonic   {k, v0}        BondAngleInteraction.hpp:129-189 */
     MJG_ANGLE_GAUSSIAN    = 5, /* BondAngle:Gaussian   {k, sigma, v0} BondAngleInteraction.hpp:129-189 */
-    MJG_DIHEDRAL_CLEMENTI = 6  /* DihedralAngle:ClementiDihedral {k1, k3, v0} DihedralAngleInteraction.hpp:157-227 */
+    MJG_DIHEDRAL_CLEMENTI = 6, /* DihedralAngle:ClementiDihedral {k1, k3, v0} DihedralAngleInteraction.hpp:157-227 */
+    MJG_BOND_3SPN2        = 7, /* BondLength:3SPN2Bond {k, v0}        ThreeSPN2BondPotential.hpp:22-35 */
+    MJG_DIHEDRAL_GAUSSIAN = 8, /* DihedralAngle:Gaussian {k, sigma, v0}, periodic  PeriodicGaussianPotential.hpp:21-44 */
+    MJG_BASE_STACKING_3SPN2 = 9 /* 3SPN2BaseStacking, indices {S5, B5, B3, P5, S3, P3} (P5/P3 may be
+                                   0xFFFFFFFF: absent), {stack kind 4 b5 + b3}
+                                   ThreeSPN2BaseStackingInteraction.hpp:281-428 */
 };
 /* global pair potentials (read_global_potential.hpp:262-705) */
 enum {
     MJG_PAIR_LJ = 1, /* LennardJones {sigma, epsilon} per participant */
     MJG_PAIR_EV = 2, /* ExcludedVolume {radius} per participant, common epsilon */
-    MJG_PAIR_DH = 3  /* DebyeHuckel {charge} per participant */
+    MJG_PAIR_DH = 3, /* DebyeHuckel {charge} per participant */
+    MJG_PAIR_3SPN2_EV = 4,       /* 3SPN2ExcludedVolume {bead kind P 0, S 1, A 2, T 3, G 4, C 5}
+                                    ThreeSPN2ExcludedVolumePotential.hpp:47-64 */
+    MJG_3SPN2_BASEPAIR = 5,      /* 3SPN2BasePair {base A 0 T 1 G 2 C 3, sugar index} per base
+                                    ThreeSPN2BasePairInteraction.hpp:289-514 */
+    MJG_3SPN2_CROSSSTACKING = 6  /* 3SPN2CrossStacking {base, sugar, 5' base, 3' base; -1 absent}
+                                    ThreeSPN2CrossStackingInteraction.hpp:340-640 */
 };
 /* spatial_partition.type (read_spatial_partition.hpp:62-116) */
 enum { MJG_PART_CELLLIST = 1, MJG_PART_VERLET = 2, MJG_PART_NAIVE = 3 };

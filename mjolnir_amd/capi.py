@@ -31,8 +31,12 @@ LOCAL_CODE = {
     ("BondAngle", "Harmonic"): 4,
     ("BondAngle", "Gaussian"): 5,
     ("DihedralAngle", "ClementiDihedral"): 6,
+    ("BondLength", "3SPN2Bond"): 7,
+    ("DihedralAngle", "Gaussian"): 8,
+    ("3SPN2BaseStacking", "3SPN2"): 9,
 }
-GLOBAL_CODE = {"LennardJones": 1, "ExcludedVolume": 2, "DebyeHuckel": 3}
+GLOBAL_CODE = {"LennardJones": 1, "ExcludedVolume": 2, "DebyeHuckel": 3, "3SPN2ExcludedVolume": 4,
+               "3SPN2BasePair": 5, "3SPN2CrossStacking": 6}
 PARTITION_CODE = {"CellList": 1, "VerletList": 2, "Naive": 3}
 IGNORE_CODE = {"Nothing": 0, "Self": 1, "Intra": 1, "Others": 2, "Inter": 2}
 INTEG_CODE = {"BAOABLangevin": 1, "GJFNVTLangevin": 2}

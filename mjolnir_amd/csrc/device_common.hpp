@@ -17,9 +17,11 @@ constexpr int kWave = 64;
 // interaction kinds (the values of the C ABI's MJG_* enums)
 enum LocalKind : int
 {
-    kBondHarmonic = 1, kBondGo = 2, kContactGo = 3, kAngleHarmonic = 4, kAngleGaussian = 5, kDihedralClementi = 6
+    kBondHarmonic = 1, kBondGo = 2, kContactGo = 3, kAngleHarmonic = 4, kAngleGaussian = 5, kDihedralClementi = 6,
+    kBond3SPN2 = 7, kDihedralPGauss = 8, kBaseStacking = 9
 };
-enum PairKind : int {kPairLJ = 1, kPairEV = 2, kPairDH = 3};
+enum PairKind : int {kPairLJ = 1, kPairEV = 2, kPairDH = 3, kPair3SPN2EV = 4, kBasePair = 5, kCrossStack = 6};
+constexpr uint32_t kNil = 0xffffffffu;   // absent index (a 3SPN2 nucleotide without phosphate)
 
 struct Box
 {

@@ -185,12 +185,28 @@ int Engine::add_local(int kind, const char* topology, size_t n_terms, const uint
         case kAngleHarmonic: L.arity = 3; L.npar = 2; break;
         case kAngleGaussian: L.arity = 3; L.npar = 3; break;
         case kDihedralClementi: L.arity = 4; L.npar = 3; break;
+        case kBond3SPN2: L.arity = 2; L.npar = 2; break;
+        case kDihedralPGauss: L.arity = 4; L.npar = 3; break;
+        case kBaseStacking: L.arity = 6; L.npar = 1; break;
         default: throw std::invalid_argument("unknown local interaction kind");
     }
+    // base stacking reads {Si, Bi, Bj} and keeps the phosphates for its topology only
+    L.nbody = (kind == kBaseStacking) ? 3 : L.arity;
     if(n_terms > 0 && (!idx || !par)) {throw std::invalid_argument("null term arrays");}
     for(size_t t = 0; t < n_terms * L.arity; ++t)
     {
-        if(idx[t] >= n_) {throw std::out_of_range("local interaction index out of range");}
+        const bool optional = (kind == kBaseStacking) && (t % 6 == 3 || t % 6 == 5);
+        if(idx[t] >= n_ && !(optional && idx[t] == kNil))
+        {
+            throw std::out_of_range("local interaction index out of range");
+        }
+    }
+    if(kind == kBaseStacking)
+    {
+        for(size_t t = 0; t < n_terms; ++t)
+        {
+            if(!(par[t] >= 0 && par[t] < 16)) {throw std::invalid_argument("3SPN2 stacking kind out of range");}
+        }
     }
     if(locals_.size() + 1 >= (size_t)kMaxLocalSegs) {throw std::invalid_argument("too many local interactions");}
     const std::string topo = topology ? topology : "";
@@ -213,7 +229,9 @@ int Engine::add_global(int pot, double cutoff_ratio, double eps, int partition, 
     switch(pot)
     {
         case kPairLJ: G.npar = 2; break;
-        case kPairEV: case kPairDH: G.npar = 1; break;
+        case kPairEV: case kPairDH: case kPair3SPN2EV: G.npar = 1; break;
+        case kBasePair: G.npar = 2; break;
+        case kCrossStack: G.npar = 4; break;
         default: throw std::invalid_argument("unknown pair potential");
     }
     if(partition != MJG_PART_CELLLIST && partition != MJG_PART_VERLET && partition != MJG_PART_NAIVE)
@@ -232,6 +250,26 @@ int Engine::add_global(int pot, double cutoff_ratio, double eps, int partition, 
     }
     G.part.assign(part, part + np);
     G.par.assign(par, par + np * G.npar);
+    for(size_t k = 0; k < np; ++k)
+    {
+        const double* p = G.par.data() + k * G.npar;
+        if(pot == kPair3SPN2EV && !(p[0] >= 0 && p[0] < 6)) {throw std::invalid_argument("3SPN2 bead kind out of range");}
+        if(pot == kBasePair || pot == kCrossStack)
+        {
+            if(!(p[0] >= 0 && p[0] < 4)) {throw std::invalid_argument("3SPN2 base kind out of range");}
+            if(!(p[1] >= 0 && p[1] < (double)n_)) {throw std::out_of_range("3SPN2 sugar index out of range");}
+        }
+        if(pot == kCrossStack)
+        {
+            for(int c = 2; c < 4; ++c)
+            {
+                if(!(p[c] == -1.0 || (p[c] >= 0 && p[c] < (double)n_)))
+                {
+                    throw std::out_of_range("3SPN2 neighbouring base index out of range");
+                }
+            }
+        }
+    }
     globals_.push_back(std::move(G));
     return (int)globals_.size() - 1;
 }
@@ -253,6 +291,24 @@ void Engine::write_topology()
     for(const auto& L : locals_)
     {
         if(L.topo_kind < 0) {continue;}
+        if(L.kind == kBaseStacking)
+        {
+            // ThreeSPN2BaseStackingInteraction::write_topology (:125-179): every bead of the 5'
+            // nucleotide to every bead of the 3' one, phosphates when present
+            for(size_t t = 0; t < L.n_terms(); ++t)
+            {
+                const uint32_t* ix = L.idx.data() + t * 6;
+                const uint32_t S5 = ix[0], B5 = ix[1], B3 = ix[2], P5 = ix[3], S3 = ix[4], P3 = ix[5];
+                add(S5, S3, L.topo_kind);
+                add(S5, B3, L.topo_kind);
+                add(B5, S3, L.topo_kind);
+                add(B5, B3, L.topo_kind);
+                if(P5 != kNil) {add(P5, S3, L.topo_kind); add(P5, B3, L.topo_kind);}
+                if(P3 != kNil) {add(S5, P3, L.topo_kind); add(B5, P3, L.topo_kind);}
+                if(P5 != kNil && P3 != kNil) {add(P5, P3, L.topo_kind);}
+            }
+            continue;
+        }
         for(size_t t = 0; t < L.n_terms(); ++t)
         {
             const uint32_t* ix = L.idx.data() + t * L.arity;
@@ -365,6 +421,17 @@ void Engine::setup_global(GlobalIA& g)
         for(size_t i = 0; i < n_; ++i) {mr = std::max(mr, ppar[i]);}
         g.max_cutoff = mr * 2 * g.cutoff_ratio;
     }
+    else if(g.pot == kPair3SPN2EV)
+    {
+        // ThreeSPN2ExcludedVolumePotential.hpp:120-134: epsilon 1 kJ/mol in kcal/mol; the list
+        // cutoff is the largest sigma of the table (:176-178)
+        pp.eps = 1.0 * (1.0 / 4.184);
+        g.max_cutoff = 7.1;
+    }
+    else if(g.multibody())
+    {
+        g.max_cutoff = 18.0;   // ThreeSPN2BasePairGlobalPotentialParameter::cutoff
+    }
     else
     {
         // DebyeHuckelPotential.hpp:126-174
@@ -403,6 +470,9 @@ void Engine::setup_global(GlobalIA& g)
     upload(g.d_pslot, pslot.data(), n_);
     g.d_ppar = dalloc<double>(n_ * g.npar);
     upload(g.d_ppar, ppar.data(), n_ * g.npar);
+    g.rule.base_rule = (g.pot == kPair3SPN2EV) ? 1 : g.multibody() ? 2 : 0;
+    g.rule.base_stride = g.npar;
+    g.rule.base_par = g.d_ppar;
     make_exclusions(g);
 
     g.d_grid = dalloc<Grid>(1);
@@ -454,6 +524,7 @@ void Engine::setup_global(GlobalIA& g)
     g.d_epair = dalloc<double>(np);
     g.d_fg = dalloc<double4>(n_);
     check(hipMemsetAsync(g.d_fg, 0, n_ * sizeof(double4), stream_), "memset");
+    if(g.multibody()) {g.d_acc = dalloc<unsigned long long>(3 * n_);}
     sync();
     g.tpp = 8;
     alloc_list(g, 64);
@@ -501,6 +572,7 @@ PairSegTable Engine::pair_table(int only)
     {
         if(only >= 0 && only != nl + (int)gi) {continue;}
         GlobalIA& g = globals_[gi];
+        if(g.multibody()) {continue;}
         PairSeg& s = t.seg[t.n++];
         s.kind = g.pot;
         s.tpp = g.tpp;
@@ -638,13 +710,14 @@ void Engine::initialize()
         s.slot_off = slot;
         s.term_off = term;
         L.slot_off = slot; L.term_off = term;
+        s.nbody = L.nbody;
         lidx.insert(lidx.end(), L.idx.begin(), L.idx.end());
         lpar.insert(lpar.end(), L.par.begin(), L.par.end());
         for(uint32_t t = 0; t < s.n_terms; ++t)
-            for(int b = 0; b < L.arity; ++b) {per_particle[L.idx[(size_t)t * L.arity + b]].push_back(slot + t * L.arity + b);}
+            for(int b = 0; b < L.nbody; ++b) {per_particle[L.idx[(size_t)t * L.arity + b]].push_back(slot + t * L.nbody + b);}
         segtab_.seg[segtab_.n++] = s;
         block += blocks_for(s.n_terms, 256) * (s.n_terms > 0 ? 1 : 0);
-        slot += s.n_terms * L.arity;
+        slot += s.n_terms * L.nbody;
         term += s.n_terms;
     }
     local_blocks_ = block;
@@ -726,6 +799,10 @@ void Engine::forces(double4* f, bool energy, int only, bool kick)
         if(energy) {hipLaunchKernelGGL(k_pairs<true>, dim3(blocks), dim3(256), 0, stream_, d_pos_, box_, pt);}
         else {hipLaunchKernelGGL(k_pairs<false>, dim3(blocks), dim3(256), 0, stream_, d_pos_, box_, pt);}
     }
+    for(size_t gi = 0; gi < globals_.size(); ++gi)
+    {
+        if(globals_[gi].multibody() && (only < 0 || only == nl + (int)gi)) {launch_multibody(globals_[gi], energy);}
+    }
     if(energy)
     {
         for(int l = 0; l < nl; ++l)
@@ -747,12 +824,31 @@ void Engine::forces(double4* f, bool energy, int only, bool kick)
     fa.gstart = run_locals ? d_gstart_ : nullptr;
     fa.gslot = d_gslot_;
     fa.tf = d_tf_;
-    fa.ng = pt.n;
-    for(int k = 0; k < pt.n; ++k) {fa.fg[k] = pt.seg[k].fout;}
+    fa.ng = 0;
+    for(size_t gi = 0; gi < globals_.size(); ++gi)
+    {
+        if(only >= 0 && only != nl + (int)gi) {continue;}
+        fa.fg[fa.ng++] = globals_[gi].d_fg;
+    }
     fa.halfdt = dt_ / 2;
     fa.kick = kick ? 1 : 0;
     hipLaunchKernelGGL(k_finish, dim3(blocks_for(n_, 256)), dim3(256), 0, stream_, (uint32_t)n_, fa, f, d_vel_,
                        d_rmass_, &d_sctl_->step);
+}
+
+// 3SPN2 base pairing / cross stacking: clear the fixed-point sums, one thread per participant
+// row, then convert the sums into the interaction's force array.
+void Engine::launch_multibody(const GlobalIA& g, bool energy)
+{
+    const uint32_t np = (uint32_t)g.part.size();
+    hipLaunchKernelGGL(k_zero_u64, dim3(blocks_for(3 * n_, 256)), dim3(256), 0, stream_, g.d_acc, (uint32_t)(3 * n_));
+    TspSeg s{};
+    s.kind = g.pot; s.np = np; s.cap = g.cap;
+    s.part = g.d_part; s.order = g.d_order; s.nl = g.d_nl; s.nn = g.d_nn;
+    s.ppar = g.d_ppar; s.acc = g.d_acc; s.eout = g.d_epair;
+    if(energy) {hipLaunchKernelGGL(k_tsp_pairs<true>, dim3(blocks_for(np, 256)), dim3(256), 0, stream_, d_pos_, box_, s);}
+    else {hipLaunchKernelGGL(k_tsp_pairs<false>, dim3(blocks_for(np, 256)), dim3(256), 0, stream_, d_pos_, box_, s);}
+    hipLaunchKernelGGL(k_fix_to_force, dim3(blocks_for(n_, 256)), dim3(256), 0, stream_, g.d_acc, g.d_fg, (uint32_t)n_);
 }
 
 void Engine::calc_force()
@@ -1033,6 +1129,9 @@ std::string Engine::interaction_name(size_t id) const
             case MJG_ANGLE_HARMONIC: return "BondAngle:Harmonic";
             case MJG_ANGLE_GAUSSIAN: return "BondAngle:Gaussian";
             case MJG_DIHEDRAL_CLEMENTI: return "DihedralAngle:ClementiDihedral";
+            case MJG_BOND_3SPN2: return "BondLength:ThreeSPN2BondPotential";
+            case MJG_DIHEDRAL_GAUSSIAN: return "DihedralAngle:PeriodicGaussian";
+            case MJG_BASE_STACKING_3SPN2: return "3SPN2BaseStacking";
             default: return "Unknown";
         }
     }
@@ -1041,6 +1140,9 @@ std::string Engine::interaction_name(size_t id) const
         case MJG_PAIR_LJ: return "GlobalPairLennardJones";
         case MJG_PAIR_EV: return "GlobalPairExcludedVolume";
         case MJG_PAIR_DH: return "Pair:DebyeHuckel";
+        case MJG_PAIR_3SPN2_EV: return "Pair:3SPN2ExcludedVolume";
+        case MJG_3SPN2_BASEPAIR: return "3SPN2BasePair";
+        case MJG_3SPN2_CROSSSTACKING: return "3SPN2CrossStacking";
         default: return "Unknown";
     }
 }
@@ -1144,6 +1246,7 @@ double Engine::profile_interaction(int id, int reps, uint64_t* half_pairs)
             hipLaunchKernelGGL(k_local_terms<false>, dim3(local_blocks_), dim3(256), 0, stream_, d_pos_, box_, segtab_,
                                d_lidx_, d_lpar_, d_tf_, d_te_, id);
         }
+        else if(globals_[id - nl].multibody()) {launch_multibody(globals_[id - nl], false);}
         else
         {
             hipLaunchKernelGGL(k_pairs<false>, dim3(pblocks), dim3(256), 0, stream_, d_pos_, box_, pt);

@@ -27,7 +27,7 @@ struct NonFinite : std::runtime_error
 
 struct LocalIA
 {
-    int kind = 0, arity = 0, npar = 0;
+    int kind = 0, arity = 0, npar = 0, nbody = 0;
     int topo_kind = -1;           // interned connection kind, -1 for "none"
     std::vector<uint32_t> idx;    // n_terms * arity
     std::vector<double> par;      // n_terms * npar
@@ -64,6 +64,8 @@ struct GlobalIA
     bool small = false;           // one-workgroup rebuild (np <= kSmallMaxPart)
     double* d_epair = nullptr;
     double4* d_fg = nullptr;      // this interaction's force on every particle
+    unsigned long long* d_acc = nullptr;   // 3SPN2 base pairing / cross stacking: int64 fixed-point sums
+    bool multibody() const {return pot == kBasePair || pot == kCrossStack;}
 };
 
 class Engine
@@ -115,6 +117,7 @@ class Engine
     void build_lists_now(bool all);
     ListSegTable list_table();
     PairSegTable pair_table(int only);
+    void launch_multibody(const GlobalIA& g, bool energy);
     void forces(double4* f, bool energy, int only, bool kick);
     void launch_step();
     void run_segment(size_t n);

@@ -12,6 +12,7 @@
 // what prepare_params cached for the pair (i < j) in the reference.
 #pragma once
 #include "device_common.hpp"
+#include "tsp_kernels.hpp"
 
 namespace mjg
 {
@@ -22,6 +23,7 @@ struct LocalSeg
     int kind;
     int arity;
     int npar;
+    int nbody;              // force slots per term (3SPN2 base stacking: Si, Bi, Bj of six indices)
     uint32_t n_terms;
     uint32_t block_begin;   // first block of this segment in the launch
     uint32_t idx_off;       // into the flat index array
@@ -48,7 +50,7 @@ __device__ void local_term(const LocalSeg& s, uint32_t t, const double4* __restr
 {
     const uint32_t* ix = idx + s.idx_off + (size_t)t * s.arity;
     const double* p = par + s.par_off + (size_t)t * s.npar;
-    const uint32_t slot = s.slot_off + t * s.arity;
+    const uint32_t slot = s.slot_off + t * s.nbody;
     double energy = 0.0;
     if(s.kind == kBondHarmonic)
     {
@@ -183,6 +185,120 @@ __device__ void local_term(const LocalSeg& s, uint32_t t, const double4* __restr
                (coef_jkl - 1.0) * Flz - coef_ijk * Fiz);
         store3(tf, slot + 3, Flx, Fly, Flz);
     }
+    else if(s.kind == kBond3SPN2)
+    {
+        // BondLengthInteraction.hpp:126-148; ThreeSPN2BondPotential.hpp:22-35
+        const double4 a = pos[ix[0]], b = pos[ix[1]];
+        double dx, dy, dz;
+        adjust_direction(box, a.x, a.y, a.z, b.x, b.y, b.z, dx, dy, dz);
+        const double len2 = length_sq(dx, dy, dz);
+        const double rlen = 1.0 / sqrt(len2);
+        const double len = len2 * rlen;
+        const double k = p[0], v0 = p[1];
+        const double dv = len - v0;
+        const double force = -1 * (2 * k * (dv + 200 * dv * dv * dv));
+        const double c = force * rlen;
+        store3(tf, slot + 0, -(dx * c), -(dy * c), -(dz * c));
+        store3(tf, slot + 1, dx * c, dy * c, dz * c);
+        if(ENERGY) {const double ev = sqrt(len2) - v0; const double ev2 = ev * ev; energy = k * (ev2 + 100 * ev2 * ev2);}
+    }
+    else if(s.kind == kDihedralPGauss)
+    {
+        // DihedralAngleInteraction.hpp:157-227; PeriodicGaussianPotential.hpp:21-44
+        const double4 ri = pos[ix[0]], rj = pos[ix[1]], rk = pos[ix[2]], rl = pos[ix[3]];
+        double ijx, ijy, ijz, kjx, kjy, kjz, lkx, lky, lkz;
+        adjust_direction(box, rj.x, rj.y, rj.z, ri.x, ri.y, ri.z, ijx, ijy, ijz);
+        adjust_direction(box, rj.x, rj.y, rj.z, rk.x, rk.y, rk.z, kjx, kjy, kjz);
+        adjust_direction(box, rk.x, rk.y, rk.z, rl.x, rl.y, rl.z, lkx, lky, lkz);
+        const double klx = -1.0 * lkx, kly = -1.0 * lky, klz = -1.0 * lkz;
+        const double r_kj_lensq = length_sq(kjx, kjy, kjz);
+        const double r_kj_rlen = 1.0 / sqrt(r_kj_lensq);
+        const double r_kj_rlensq = r_kj_rlen * r_kj_rlen;
+        const double r_kj_len = r_kj_rlen * r_kj_lensq;
+        const double mx = ijy * kjz - ijz * kjy, my = ijz * kjx - ijx * kjz, mz = ijx * kjy - ijy * kjx;
+        const double nx = kjy * klz - kjz * kly, ny = kjz * klx - kjx * klz, nz = kjx * kly - kjy * klx;
+        const double m_lensq = length_sq(mx, my, mz), n_lensq = length_sq(nx, ny, nz);
+        const double dot_mn = (mx * nx + my * ny + mz * nz) * (1.0 / sqrt(m_lensq * n_lensq));
+        const double cos_phi = fmin(fmax(dot_mn, -1.0), 1.0);
+        const double phi = copysign(acos(cos_phi), ijx * nx + ijy * ny + ijz * nz);
+        const double k = p[0], sigma = p[1], v0 = p[2];
+        const double inv_sigma2 = -1. / (2 * sigma * sigma);
+        double dval = phi - v0;
+        if(dval < -tsp::kPi) {dval += tsp::kTwoPi;} else if(tsp::kPi < dval) {dval -= tsp::kTwoPi;}
+        const double ex = exp(inv_sigma2 * dval * dval);
+        const double coef = -(2 * inv_sigma2 * dval * k * ex);
+        if(ENERGY) {energy = k * ex;}
+        const double ci = coef * r_kj_len / m_lensq;
+        const double cl = -coef * r_kj_len / n_lensq;
+        const double Fix = ci * mx, Fiy = ci * my, Fiz = ci * mz;
+        const double Flx = cl * nx, Fly = cl * ny, Flz = cl * nz;
+        const double coef_ijk = (ijx * kjx + ijy * kjy + ijz * kjz) * r_kj_rlensq;
+        const double coef_jkl = (klx * kjx + kly * kjy + klz * kjz) * r_kj_rlensq;
+        store3(tf, slot + 0, Fix, Fiy, Fiz);
+        store3(tf, slot + 1, (coef_ijk - 1.0) * Fix - coef_jkl * Flx, (coef_ijk - 1.0) * Fiy - coef_jkl * Fly,
+               (coef_ijk - 1.0) * Fiz - coef_jkl * Flz);
+        store3(tf, slot + 2, (coef_jkl - 1.0) * Flx - coef_ijk * Fix, (coef_jkl - 1.0) * Fly - coef_ijk * Fiy,
+               (coef_jkl - 1.0) * Flz - coef_ijk * Fiz);
+        store3(tf, slot + 3, Flx, Fly, Flz);
+    }
+    else if(s.kind == kBaseStacking)
+    {
+        // ThreeSPN2BaseStackingInteraction.hpp:281-428 with ThreeSPN2BaseStackingPotential.hpp:
+        // 172-294; indices {Si, Bi, Bj, Pi, Sj, Pj}, force on Si, Bi, Bj (slots 0, 1, 2)
+        const double K = 6.0, piK = tsp::kPi / 6.0, alp = 3.0;
+        const uint32_t Si = ix[0], Bi = ix[1], Bj = ix[2];
+        const int bs = (int)p[0];
+        const double eps = tsp::bs_eps(bs), r0 = tsp::bs_r0(bs), th0 = tsp::bs_theta0(bs);
+        const tsp::V3 Bji = tsp::disp(box, pos, Bj, Bi);
+        const tsp::V3 SBi = tsp::disp(box, pos, Si, Bi);
+        const double lBji_sq = tsp::lsq(Bji);
+        const double rlBji = 1.0 / sqrt(lBji_sq);
+        const double lBji = lBji_sq * rlBji;
+        const tsp::V3 Bji_reg = rlBji * Bji;
+        tsp::V3 f_Bi{0, 0, 0}, f_Bj{0, 0, 0}, f_Si{0, 0, 0};
+        double dU_rep = 0.0;
+        if(!(r0 <= lBji)) {const double term = exp(-alp * (lBji - r0)); dU_rep = 2 * alp * eps * term * (1.0 - term);}
+        if(dU_rep != 0.0) {f_Bi = f_Bi - dU_rep * Bji_reg; f_Bj = f_Bj + dU_rep * Bji_reg;}
+        if(ENERGY && !(r0 <= lBji)) {const double term = 1.0 - exp(-alp * (lBji - r0)); energy += eps * term * term;}
+        const double rlSBi = 1.0 / sqrt(tsp::lsq(SBi));
+        const tsp::V3 SBi_reg = rlSBi * SBi;
+        const double cos_theta = tsp::dot(SBi_reg, Bji_reg);
+        const double theta = acos(fmin(fmax(cos_theta, -1.0), 1.0));
+        const double f_theta = tsp::f_mod(K, piK, theta, th0);
+        if(f_theta != 0.0)
+        {
+            const double df_theta = tsp::df_mod(K, piK, theta, th0);
+            double U_attr, dU_attr;
+            if(lBji <= r0) {U_attr = -eps; dU_attr = 0.0;}
+            else
+            {
+                const double term = exp(-alp * (lBji - r0));
+                U_attr = eps * (term * term - 2 * term);
+                dU_attr = 2 * alp * eps * term * (1.0 - term);
+            }
+            if(ENERGY) {energy += U_attr * f_theta;}
+            if(df_theta != 0.0)
+            {
+                const double coef = -df_theta * U_attr;
+                const double sn = sin(theta);
+                const double coef_rsin = (sn > tsp::kTol) ? coef / sn : coef / tsp::kTol;
+                const tsp::V3 fSi = (coef_rsin * rlSBi) * (Bji_reg - cos_theta * SBi_reg);
+                const tsp::V3 fBj = (coef_rsin * rlBji) * (SBi_reg - cos_theta * Bji_reg);
+                f_Si = f_Si + fSi;
+                f_Bi = f_Bi - (fSi + fBj);
+                f_Bj = f_Bj + fBj;
+            }
+            if(dU_attr != 0.0)
+            {
+                const double coef = f_theta * dU_attr;
+                f_Bi = f_Bi - coef * Bji_reg;
+                f_Bj = f_Bj + coef * Bji_reg;
+            }
+        }
+        store3(tf, slot + 0, f_Si.x, f_Si.y, f_Si.z);
+        store3(tf, slot + 1, f_Bi.x, f_Bi.y, f_Bi.z);
+        store3(tf, slot + 2, f_Bj.x, f_Bj.y, f_Bj.z);
+    }
     if(ENERGY) {te[s.term_off + t] = energy;}
 }
 
@@ -251,6 +367,30 @@ __device__ __forceinline__ void pair_term(const PairParams& pp, uint32_t i, uint
         fx += dx * c; fy += dy * c; fz += dz * c;
         if(ENERGY && i < j) {e += pp.eps * (s6l6 * s6l6 - pp.coef_at_cutoff);}
     }
+    else if(KIND == kPair3SPN2EV)
+    {
+        // GlobalPairInteraction.hpp:148-189 with ThreeSPN2ExcludedVolumePotential.hpp:47-64;
+        // sigma of the pair (ThreeSPN2ExcludedVolumeParameterList::prepare_params, :198-202)
+        const double sigma = (tsp::ev_sigma((int)ppar[i]) + tsp::ev_sigma((int)ppar[j])) * 0.5;
+        const double rl = 1.0 / sqrt(l_sq);
+        const double l = l_sq * rl;
+        if(sigma <= l) {return;}
+        const double rinv = 1 / l;
+        const double r1s1 = sigma * rinv;
+        const double r3s3 = r1s1 * r1s1 * r1s1;
+        const double r6s6 = r3s3 * r3s3;
+        const double f_mag = 12.0 * pp.eps * rinv * r6s6 * (1.0 - r6s6);
+        if(f_mag == 0.0) {return;}
+        const double c = f_mag * rl;
+        fx += dx * c; fy += dy * c; fz += dz * c;
+        if(ENERGY && i < j)
+        {
+            const double q1 = sigma / l;
+            const double q3 = q1 * q1 * q1;
+            const double q6 = q3 * q3;
+            e += pp.eps * (q6 * (q6 - 2.0) + 1.0);
+        }
+    }
     else
     {
         // GlobalPairInteraction.hpp:164-185 with DebyeHuckelPotential.hpp:47-65; charge
@@ -288,7 +428,7 @@ struct PairSeg
     double* eout;
     PairParams pp;
 };
-constexpr int kMaxPairSegs = 8;
+constexpr int kMaxPairSegs = 16;
 struct PairSegTable
 {
     PairSeg seg[kMaxPairSegs];
@@ -327,6 +467,7 @@ k_pairs(const double4* __restrict__ pos, Box box, PairSegTable tab)
     uint32_t i = 0;
     if(s.kind == kPairLJ) {pair_row<kPairLJ, ENERGY>(s, t, l, pos, box, fx, fy, fz, e, i);}
     else if(s.kind == kPairEV) {pair_row<kPairEV, ENERGY>(s, t, l, pos, box, fx, fy, fz, e, i);}
+    else if(s.kind == kPair3SPN2EV) {pair_row<kPair3SPN2EV, ENERGY>(s, t, l, pos, box, fx, fy, fz, e, i);}
     else {pair_row<kPairDH, ENERGY>(s, t, l, pos, box, fx, fy, fz, e, i);}
     for(uint32_t o = s.tpp / 2; o > 0; o >>= 1)
     {

@@ -172,7 +172,16 @@ struct ListRule
     int use_distance;    // 0 for Naive
     int ignore_mol;      // 0 nothing, 1 same molecule, 2 different molecules
     double r_list2;      // (r_c (1 + margin))^2
+    int base_rule;       // 0 none; 1 no complementary base pair (3SPN2 EV); 2 complementary pairs only
+    int base_stride;     // columns of base_par per particle (the base code is column 0)
+    const double* base_par;
 };
+
+// complementary bases A-T, G-C (codes A 0, T 1, G 2, C 3)
+__device__ __forceinline__ bool complementary_bases(int a, int b)
+{
+    return (a == 0 && b == 1) || (a == 1 && b == 0) || (a == 2 && b == 3) || (a == 3 && b == 2);
+}
 
 // has_interaction(lo, hi) plus the leading-participant rule, for lo < hi.
 __device__ __forceinline__ bool pair_allowed(uint32_t lo, uint32_t hi, const ListRule& rule,
@@ -184,6 +193,19 @@ __device__ __forceinline__ bool pair_allowed(uint32_t lo, uint32_t hi, const Lis
     else {if(slo == rule.last_slot) {return false;}}
     if(rule.ignore_mol == 1 && mol[lo] == mol[hi]) {return false;}
     if(rule.ignore_mol == 2 && mol[lo] != mol[hi]) {return false;}
+    if(rule.base_rule == 1)
+    {
+        // ThreeSPN2ExcludedVolumePotential.hpp:326-364: bead kinds P 0, S 1, bases 2 + code;
+        // bases that could pair never interact
+        const int ki = (int)rule.base_par[lo], kj = (int)rule.base_par[hi];
+        if(ki >= 2 && kj >= 2 && complementary_bases(ki - 2, kj - 2)) {return false;}
+    }
+    else if(rule.base_rule == 2)
+    {
+        // ThreeSPN2BasePairPotential.hpp:445-459, ThreeSPN2CrossStackingPotential.hpp:437-451
+        if(!complementary_bases((int)rule.base_par[(size_t)lo * rule.base_stride],
+                                (int)rule.base_par[(size_t)hi * rule.base_stride])) {return false;}
+    }
     const uint32_t b = ex_start[lo], e = ex_start[lo + 1];
     for(uint32_t k = b; k < e; ++k)
     {
