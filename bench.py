@@ -14,6 +14,12 @@ Workloads (seeded synthetic inputs, ``mjolnir_amd.workloads``):
   chain1000    one CG chain, N = 1000, FP64 engine
   lj4096       LJ fluid, N = 4096, rho sigma^3 = 0.6, FP64 engine
   dna200       3SPN2 double-stranded DNA, 200 bp (1198 beads), G-JF dt = 0.02, FP64 engine
+  serve        a seeded Poisson stream (50 jobs/s) of LJ fluids (1e3..2.6e5 particles), CG chains
+               (50..5000 beads) and 3SPN2 DNA (20..400 bp), 1e3..1e5 steps each, energy trace every
+               100 steps, through the continuous-batching server (``mjolnir_amd.serve``)
+
+The default run also serves a short stream (``--serve-jobs``, 0 to skip) and reports its
+p50/p99 job latency under "serving".
 """
 from __future__ import annotations
 
@@ -55,7 +61,7 @@ def cpu_baseline_single(spec, target_s: float = 12.0):
     r = refrun.bench(spec, steps, threads=threads, timeout=300)
     return {"value": r["particle_steps_per_s"], "unit": "particle-steps/s", "cores": r["threads"],
             "kind": "reference",
-            "sample": f"{spec.name}: {steps} BAOAB steps of the reference OpenMP build "
+            "sample": f"{spec.name}: {steps} {spec.integrator.type} steps of the reference OpenMP build "
                       f"(OMP_NUM_THREADS={r['threads']}), step loop only, {r['seconds']:.1f} s"}
 
 
@@ -209,14 +215,46 @@ def bench_batch(args):
     }
 
 
+# ------------------------------------------------------------------------------------------
+# serving stream
+# ------------------------------------------------------------------------------------------
+def run_serving(n_jobs: int, seed: int = 7):
+    from mjolnir_amd import serve
+    t = time.perf_counter()
+    jobs = serve.make_jobs(serve.job_trace(n_jobs, rate=50.0, seed=seed), procs=host_cores())
+    print(f"[bench] serving: {n_jobs} jobs built in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+    out = serve.Server(workers=4).run(jobs)
+    out.update({"rate_jobs_per_s": 50.0, "seed": seed,
+                "mix": "LJ fluid N log-uniform [1e3, 2.6e5], CG chain 50..5000 beads, 3SPN2 DNA 20..400 bp; "
+                       "steps log-uniform [1e3, 1e5]; energy trace every 100 steps",
+                "lanes": "batch: replicas <= 2048 particles (FP32 + int64 fixed-point sums); "
+                         "engine: FP64 engine per job, 4 at a time"})
+    print(f"[bench] serving: {json.dumps(out)}", file=sys.stderr, flush=True)
+    return out
+
+
+def bench_serve(args):
+    s = run_serving(args.serve_jobs)
+    return {
+        "metric": "particle-steps/s (serving)", "value": s["particle_steps_per_s"], "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": None, "warmup": 0, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+i64fixed+f64 (batch lane), f64 (engine lane)",
+        "data": "synthetic",
+        "config": {"workload": "serve", "jobs": args.serve_jobs, "rate_jobs_per_s": 50.0},
+        "serving": s, "roofline": None, "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=500)
-    ap.add_argument("--workload", default="replicas512", choices=["replicas512", "chain1000", "lj4096", "dna200"])
+    ap.add_argument("--workload", default="replicas512", choices=["replicas512", "chain1000", "lj4096", "dna200", "serve"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-sample", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=40,
+                    help="jobs of the serving stream (workload serve, and the default run's serving entry)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy kernel of this size (the reference "
                          "kernel that scales hardware byte counters in a profiler pass)")
@@ -237,6 +275,9 @@ def main():
         del src, dst
     if args.workload == "replicas512":
         out = bench_batch(args)
+        out["serving"] = run_serving(args.serve_jobs) if args.serve_jobs > 0 else None
+    elif args.workload == "serve":
+        out = bench_serve(args)
     else:
         from mjolnir_amd import workloads
         spec = {"chain1000": lambda: workloads.cg_chain(n=1000, seed=2),

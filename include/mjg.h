@@ -237,6 +237,13 @@ mjg_status mjg_batch_list_sizes(mjg_batch* b, int replica, uint64_t* entries, in
 /* Forces and per-interaction energies of one replica at its current positions
  * (calc_force_and_energy / format_energy, ForceField.hpp:126-161). */
 mjg_status mjg_batch_evaluate(mjg_batch* b, int replica, double* per_interaction, double* total, double* force);
+/* EnergyObserver::output (EnergyObserver.hpp:89-150) for every resident replica in one launch:
+ * per-interaction energies and the kinetic energy at the current state.  The forces are recomputed
+ * from the same lists, so the trajectories are unchanged. */
+mjg_status mjg_batch_observe(mjg_batch* b);
+/* The last observation of one replica: per-interaction energies (locals, then globals, in the
+ * order they were added) and the kinetic energy sum of m |v|^2 / 2. */
+mjg_status mjg_batch_energies(mjg_batch* b, int replica, double* per_interaction, double* kinetic);
 /* Sum over replicas of particles x steps taken, and list rebuilds. */
 mjg_status mjg_batch_counters(mjg_batch* b, uint64_t* particle_steps, uint64_t* rebuilds);
 /* Timing hook with no counterpart in the reference: `reps` launches of a `segment_steps` segment

@@ -64,6 +64,7 @@ EXPORTED = [
     "mjg_batch_set_attribute", "mjg_batch_add_local", "mjg_batch_add_global", "mjg_batch_set_integrator",
     "mjg_batch_initialize", "mjg_batch_step", "mjg_batch_set_steps", "mjg_batch_advance", "mjg_batch_retire",
     "mjg_batch_get_state", "mjg_batch_replica_info", "mjg_batch_list_sizes", "mjg_batch_evaluate", "mjg_batch_counters",
+    "mjg_batch_observe", "mjg_batch_energies",
     "mjg_batch_profile",
 ]
 
@@ -125,6 +126,8 @@ def lib():
     L.mjg_batch_get_state.argtypes = [vp, i32, vp, vp, vp]
     L.mjg_batch_replica_info.argtypes = [vp, i32, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), C.POINTER(sz)]
     L.mjg_batch_evaluate.argtypes = [vp, i32, vp, C.POINTER(dbl), vp]
+    L.mjg_batch_observe.argtypes = [vp]
+    L.mjg_batch_energies.argtypes = [vp, i32, vp, C.POINTER(dbl)]
     L.mjg_batch_counters.argtypes = [vp, C.POINTER(u64), C.POINTER(u64)]
     L.mjg_batch_list_sizes.argtypes = [vp, i32, vp, i32]
     L.mjg_batch_profile.argtypes = [vp, i32, i32, C.POINTER(dbl), C.POINTER(u64)]
@@ -361,6 +364,18 @@ class Batch:
         f = np.zeros(3 * spec.n)
         self._check(lib().mjg_batch_evaluate(self.h, r, per.ctypes.data, C.byref(tot), f.ctypes.data))
         return per[:ne], tot.value, f.reshape(-1, 3)
+
+    def observe(self):
+        """Energies of every resident replica at the current state (one launch)."""
+        self._check(lib().mjg_batch_observe(self.h))
+
+    def energies(self, r: int):
+        """(per-interaction energies, kinetic energy) of replica r at the last observe()."""
+        spec = self.specs[r]
+        per = np.zeros(max(1, len(spec.local) + len(spec.global_)))
+        ke = C.c_double()
+        self._check(lib().mjg_batch_energies(self.h, r, per.ctypes.data, C.byref(ke)))
+        return per[:len(spec.local) + len(spec.global_)], ke.value
 
     def counters(self):
         ps, rb = C.c_uint64(), C.c_uint64()

@@ -63,6 +63,7 @@ struct Replica
     size_t lds = 0;                // dynamic LDS of its workgroup
     int cls = 0;                   // size class (block shape)
     bool ready = false, active = false;
+    bool observed = false;          // an energy observation is held for this replica
 };
 
 static void hip_check(hipError_t e, const char* what)
@@ -318,6 +319,28 @@ class Batch
         if(force) {get_state(id, nullptr, nullptr, force);}
     }
 
+    // EnergyObserver::output for every active replica at once: one evaluation launch (forces
+    // recomputed from the same lists, so the trajectory is unchanged), then the states
+    void observe()
+    {
+        std::vector<int> act = active_ids();
+        if(act.empty()) {return;}
+        launch(act, 0, 2);
+        download_states();
+        for(int id : act) {reps_[id]->observed = true;}
+    }
+
+    // the last observation of one replica: per-interaction energies and the kinetic energy
+    void energies(int id, double* per, double* kinetic)
+    {
+        Replica& r = rep(id);
+        if(!r.observed) {throw std::logic_error("replica has no energy observation yet");}
+        const RepState& s = h_states_[r.slot];
+        const int ne = (int)r.locals.size() + (int)r.globals.size();
+        for(int k = 0; k < ne; ++k) {if(per) {per[k] = s.energy[k];}}
+        if(kinetic) {*kinetic = s.kinetic;}
+    }
+
     size_t footprint(int id) {return rep(id).footprint;}
     size_t num_interactions(int id) {Replica& r = rep(id); return r.locals.size() + r.globals.size();}
     int num_globals(int id) {return (int)rep(id).globals.size();}
@@ -509,6 +532,7 @@ class Batch
         const size_t o_pos = take(n * sizeof(double4)), o_vel = take(n * sizeof(double4)), o_force = take(n * sizeof(double4));
         const size_t o_bpos = take(n * sizeof(double4)), o_bvel = take(n * sizeof(double4)), o_bforce = take(n * sizeof(double4));
         const size_t o_rm = take(n * 8), o_c1 = take(n * 8), o_c2 = take(n * 8), o_c3 = take(n * 8);
+        const size_t o_mass = take(n * 8);
         const size_t o_mol = take(n * 4), o_tidx = take(n_terms * 8), o_tkind = take(n_terms * 4);
         const size_t o_tpar = take(n_terms * 24);
         const size_t o_bst = take(sizeof(RepState));
@@ -564,6 +588,7 @@ class Batch
         up(o_pos, p.data(), n * sizeof(double4));
         up(o_vel, v.data(), n * sizeof(double4));
         up(o_force, z.data(), n * sizeof(double4));
+        up(o_mass, r.mass.data(), n * 8);
         up(o_rm, rm.data(), n * 8); up(o_c1, c1.data(), n * 8); up(o_c2, c2.data(), n * 8); up(o_c3, c3.data(), n * 8);
         up(o_mol, r.topo.mol.data(), n * 4);
         up(o_tidx, t_idx.data(), n_terms * 8);
@@ -589,6 +614,7 @@ class Batch
         D.bak_vel = reinterpret_cast<double4*>(A + o_bvel);
         D.bak_force = reinterpret_cast<double4*>(A + o_bforce);
         D.rmass = reinterpret_cast<const double*>(A + o_rm);
+        D.mass = reinterpret_cast<const double*>(A + o_mass);
         D.c1 = reinterpret_cast<const double*>(A + o_c1);
         D.c2 = reinterpret_cast<const double*>(A + o_c2);
         D.c3 = reinterpret_cast<const double*>(A + o_c3);
@@ -1020,6 +1046,11 @@ mjg_status mjg_batch_list_sizes(mjg_batch* b, int replica, uint64_t* entries, in
         const int ng = std::min(max_globals, x.num_globals(replica));
         for(int g = 0; g < ng; ++g) {entries[g] = s.npairs[g];}
     });
+}
+mjg_status mjg_batch_observe(mjg_batch* b) {return bguard(b, [&](mjg::Batch& x) {x.observe();});}
+mjg_status mjg_batch_energies(mjg_batch* b, int replica, double* per_interaction, double* kinetic)
+{
+    return bguard(b, [&](mjg::Batch& x) {x.energies(replica, per_interaction, kinetic);});
 }
 mjg_status mjg_batch_evaluate(mjg_batch* b, int replica, double* per_interaction, double* total, double* force)
 {

@@ -89,6 +89,7 @@ struct RepState
     int nonfinite;
     int force_rebuild;              // rebuild every list at the first step of the next launch
     double energy[kRepMaxLocals + kRepMaxGlobals];
+    double kinetic;                 // sum of m v^2 / 2 at the last energy evaluation
 };
 
 struct RepDesc
@@ -108,6 +109,7 @@ struct RepDesc
     double4* vel;
     double4* force;
     const double* rmass;
+    const double* mass;
     const double* c1;
     const double* c2;
     const double* c3;
@@ -901,8 +903,22 @@ k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
                                       from_fix<Real>(F[3 * p + 2]), 0.0);
         }
     }
+    double ke = 0.0;
+    if(ENERGY)
+    {
+        // kinetic energy as the reference's EnergyObserver reports it: sum of m |v|^2 / 2
+#pragma unroll
+        for(int k = 0; k < PPT; ++k)
+        {
+            const uint32_t p = tid + k * BS;
+            if(p < n) {ke += 0.5 * R.mass[p] * length_sq(vx[k], vy[k], vz[k]);}
+        }
+        __syncthreads();
+        ke = block_sum(ke, s_red);
+    }
     if(tid == 0)
     {
+        if(ENERGY) {st.kinetic = ke;}
         if(!eval_only) {st.step = step; st.remaining -= steps_to_do;}
         st.rebuilds = s_rebuilds;
         st.force_rebuild = 0;

@@ -675,6 +675,9 @@ void Engine::build_lists_now(bool)
             }
             const uint32_t t = lanes_for(ctl[g].maxcount);
             if(t != G.tpp) {G.tpp = t; again = again || false;}
+            uint32_t mb = 1;
+            while(mb < 64 && mb < ctl[g].maxcount) {mb <<= 1;}
+            G.mb_tpp = mb;
         }
         if(graph_exec_) {(void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr;}
         if(!again) {break;}
@@ -843,11 +846,12 @@ void Engine::launch_multibody(const GlobalIA& g, bool energy)
     const uint32_t np = (uint32_t)g.part.size();
     hipLaunchKernelGGL(k_zero_u64, dim3(blocks_for(3 * n_, 256)), dim3(256), 0, stream_, g.d_acc, (uint32_t)(3 * n_));
     TspSeg s{};
-    s.kind = g.pot; s.np = np; s.cap = g.cap;
+    s.kind = g.pot; s.np = np; s.cap = g.cap; s.tpp = g.mb_tpp;
     s.part = g.d_part; s.order = g.d_order; s.nl = g.d_nl; s.nn = g.d_nn;
     s.ppar = g.d_ppar; s.acc = g.d_acc; s.eout = g.d_epair;
-    if(energy) {hipLaunchKernelGGL(k_tsp_pairs<true>, dim3(blocks_for(np, 256)), dim3(256), 0, stream_, d_pos_, box_, s);}
-    else {hipLaunchKernelGGL(k_tsp_pairs<false>, dim3(blocks_for(np, 256)), dim3(256), 0, stream_, d_pos_, box_, s);}
+    const uint32_t blocks = blocks_for((size_t)np * s.tpp, 256);
+    if(energy) {hipLaunchKernelGGL(k_tsp_pairs<true>, dim3(blocks), dim3(256), 0, stream_, d_pos_, box_, s);}
+    else {hipLaunchKernelGGL(k_tsp_pairs<false>, dim3(blocks), dim3(256), 0, stream_, d_pos_, box_, s);}
     hipLaunchKernelGGL(k_fix_to_force, dim3(blocks_for(n_, 256)), dim3(256), 0, stream_, g.d_acc, g.d_fg, (uint32_t)n_);
 }
 

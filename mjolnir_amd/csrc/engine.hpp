@@ -61,6 +61,7 @@ struct GlobalIA
     uint32_t *d_nl = nullptr, *d_nn = nullptr, *d_order = nullptr;
     uint32_t cap = 0;             // row length of the list: nl[t * cap + k]
     uint32_t tpp = 8;             // lanes per participant in k_verlet / k_pairs
+    uint32_t mb_tpp = 16;         // lanes per participant in k_tsp_pairs (one entry per lane)
     bool small = false;           // one-workgroup rebuild (np <= kSmallMaxPart)
     double* d_epair = nullptr;
     double4* d_fg = nullptr;      // this interaction's force on every particle

@@ -371,12 +371,14 @@ __device__ void cross_stack(uint32_t Bi, uint32_t Bj, const double4* __restrict_
 
 } // tsp
 
-// Base pairing or cross stacking over a full list: one thread per participant row takes the
-// row's partners with a higher particle index (each pair once, the lower base leading).
+// Base pairing or cross stacking over a full list: `tpp` lanes per participant row split the
+// row's entries and take those with a higher particle index (each pair once, the lower base
+// leading); each pair is a long chain of FP64 transcendentals, so one pair per lane keeps the
+// latency of a row short.
 struct TspSeg
 {
     int kind;
-    uint32_t np, cap;
+    uint32_t np, cap, tpp;
     const uint32_t* part;
     const uint32_t* order;
     const uint32_t* nl;
@@ -390,20 +392,25 @@ template<bool ENERGY>
 __global__ void __launch_bounds__(256)
 k_tsp_pairs(const double4* __restrict__ pos, Box box, TspSeg s)
 {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if(t >= s.np) {return;}
+    const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = gt / s.tpp, l = gt % s.tpp;
+    if(t >= s.np) {return;}   // whole groups leave together
     const uint32_t i = s.part[s.order[t]];
     const uint32_t* row = s.nl + (size_t)t * s.cap;
     const uint32_t cnt = s.nn[t];
     double e = 0.0;
-    for(uint32_t k = 0; k < cnt; ++k)
+    for(uint32_t k = l; k < cnt; k += s.tpp)
     {
         const uint32_t j = row[k];
         if(j <= i) {continue;}
         if(s.kind == kBasePair) {tsp::base_pair<ENERGY>(i, j, pos, box, s.ppar, s.acc, e);}
         else {tsp::cross_stack<ENERGY>(i, j, pos, box, s.ppar, s.acc, e);}
     }
-    if(ENERGY) {s.eout[t] = e;}
+    if(ENERGY)
+    {
+        for(uint32_t o = s.tpp / 2; o > 0; o >>= 1) {e += __shfl_xor(e, o, kWave);}
+        if(l == 0) {s.eout[t] = e;}
+    }
 }
 
 __global__ void k_zero_u64(unsigned long long* __restrict__ p, uint32_t n)

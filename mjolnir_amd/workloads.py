@@ -43,19 +43,28 @@ def maxwell_boltzmann(rng: np.random.Generator, mass: np.ndarray, T: float) -> n
 
 def lj_fluid(n_side: int = 16, rho_reduced: float = 0.6, sigma: float = 4.0, epsilon: float = 0.2,
              cutoff: float = 2.5, margin: float = 0.5, T: float = 300.0, gamma: float = 0.1,
-             dt: float = 0.01, seed: int = 1, jitter: float = 0.1) -> SystemSpec:
+             dt: float = 0.01, seed: int = 1, jitter: float = 0.1, n: Optional[int] = None) -> SystemSpec:
     """N = n_side**3 identical LJ particles on a jittered simple-cubic lattice in a periodic cube.
 
-    ``n_side=16`` is the 4096-particle LJ fluid (rho sigma^3 = 0.6, L = 75.880 A).
+    ``n_side=16`` is the 4096-particle LJ fluid (rho sigma^3 = 0.6, L = 75.880 A).  With ``n``
+    the fluid has exactly n particles: a seeded random subset of the sites of the smallest
+    lattice with at least n sites, in the cube that holds n particles at the same density.
     """
     rng = np.random.default_rng(seed)
-    n = n_side ** 3
+    if n is None:
+        n = n_side ** 3
+    else:
+        n_side = int(np.ceil(round(n ** (1.0 / 3.0), 9)))
+        while n_side ** 3 < n:
+            n_side += 1
     rho = rho_reduced / sigma ** 3
     L = (n / rho) ** (1.0 / 3.0)
     a = L / n_side
     g = np.arange(n_side, dtype=np.float64)
     gx, gy, gz = np.meshgrid(g, g, g, indexing="ij")
     lattice = np.stack([gx.ravel(), gy.ravel(), gz.ravel()], axis=1) * a + 0.5 * a
+    if len(lattice) > n:
+        lattice = lattice[np.sort(rng.choice(len(lattice), size=n, replace=False))]
     pos = lattice + rng.uniform(-jitter * sigma, jitter * sigma, size=(n, 3))
     pos = np.mod(pos, L)
     mass = np.ones(n)
