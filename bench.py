@@ -115,6 +115,20 @@ def bench_single(args, spec):
 SEGMENT = 100
 
 
+def measured_traffic(workload: str, segment_steps: int):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this
+    workload (scaled by the copy-kernel calibration), or None if there is none."""
+    p = os.path.join(ROOT, "profiles", "k_replica_segment_replicas512_traffic.json")
+    try:
+        with open(p) as f:
+            t = json.load(f)
+    except OSError:
+        return None
+    if t.get("workload") != workload or t.get("segment_steps") != segment_steps:
+        return None
+    return t["traffic_bytes_per_launch"]
+
+
 def batch_alg_bytes(b, ids, specs, steps):
     """Algorithmic bytes of one segment launch of `steps` steps over the batch.
 
@@ -191,8 +205,8 @@ def bench_batch(args):
     alg = batch_alg_bytes(b, ids, specs, SEGMENT)
     achieved = alg / (ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-                "kernel": "k_replica_segment<float> (3 size classes, one launch each per segment)",
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic("replicas512", SEGMENT),
+                "kernel": "k_replica_segment<float> (3 size classes, one launch each per segment, concurrent)",
                 "avg_ms": round(ms, 4), "segment_steps": SEGMENT, "alg_bytes_per_launch": int(alg),
                 "kernel_particle_steps_per_s": round(ps_launch / (ms * 1e-3), 1)}
     fp64 = None if args.no_fp64_sample else fp64_engine_rate(specs)

@@ -793,7 +793,9 @@ class Batch
     }
 
     // mode 0: step segment, 1: evaluate after a forced rebuild (set-up), 2: evaluate with energies.
-    // Each size class runs on its own stream; the classes fork from and join into stream_.
+    // Each size class runs on its own stream: the classes fork from stream_ (class 0's stream),
+    // all three are launched, and only then does stream_ wait for the other two, so the classes
+    // run concurrently and a segment takes as long as its longest class.
     void launch_descs(int nsteps, int mode)
     {
         const bool fp32 = (precision_ == MJG_FP32_FIXED);
@@ -805,11 +807,11 @@ class Batch
             const int ns = (mode == 0) ? nsteps : 0, ev = (mode == 0) ? 0 : 1;
             if(mode == 2) {if(fp32) {launch_class<float, true>(c, ns, ev);} else {launch_class<double, true>(c, ns, ev);}}
             else {if(fp32) {launch_class<float, false>(c, ns, ev);} else {launch_class<double, false>(c, ns, ev);}}
-            if(c != 0)
-            {
-                hip_check(hipEventRecord(ev_join_[c], cls_stream_[c]), "record");
-                hip_check(hipStreamWaitEvent(stream_, ev_join_[c], 0), "wait");
-            }
+            if(c != 0) {hip_check(hipEventRecord(ev_join_[c], cls_stream_[c]), "record");}
+        }
+        for(int c = 1; c < kRepClasses; ++c)
+        {
+            if(grp_count_[c] != 0) {hip_check(hipStreamWaitEvent(stream_, ev_join_[c], 0), "wait");}
         }
     }
 
