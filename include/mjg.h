@@ -83,7 +83,7 @@ enum { MJG_PART_CELLLIST = 1, MJG_PART_VERLET = 2, MJG_PART_NAIVE = 3 };
 /* ignore.molecule (IgnoreMolecule.hpp:101-121) */
 enum { MJG_IGNORE_NOTHING = 0, MJG_IGNORE_SELF = 1, MJG_IGNORE_OTHERS = 2 };
 /* integrator.type (read_integrator.hpp) */
-enum { MJG_INTEG_BAOAB = 1, MJG_INTEG_GJF = 2 };
+enum { MJG_INTEG_BAOAB = 1, MJG_INTEG_GJF = 2, MJG_INTEG_GBAOAB = 3 };
 
 /* Library version string. */
 const char* mjg_version(void);
@@ -125,6 +125,14 @@ mjg_status mjg_add_global(mjg_engine* e, int potential, double cutoff_ratio, dou
                           size_t n_participants, const uint32_t* participants, const double* params,
                           int* id);
 
+/* ConstraintForceField ([[forcefields.constraint]], read_constraint.hpp:12-46,
+ * ConstraintForceField.hpp:15-81): n bond constraints, indices[2k], indices[2k+1] held at
+ * v0[k], written into the topology with connection kind `topology` ("none": not written), with
+ * the Gauss-Seidel sweep cap and tolerance the gBAOABLangevin integrator uses.  At most one set,
+ * added before mjg_forcefield_initialize. */
+mjg_status mjg_add_constraint(mjg_engine* e, const char* topology, size_t n, const uint32_t* indices,
+                              const double* v0, size_t max_iteration, double tolerance);
+
 /* ForceField::initialize (ForceField.hpp:47-63): topology and molecules from the local
  * interactions, exclusion lists, potential parameters, first neighbour lists. */
 mjg_status mjg_forcefield_initialize(mjg_engine* e);
@@ -147,8 +155,9 @@ mjg_status mjg_reduce_margin(mjg_engine* e, double dmargin);
  * pairs (2 uint32 each) and the total count to *n_pairs. */
 mjg_status mjg_neighbor_pairs(mjg_engine* e, int id, uint32_t* pairs, size_t capacity, size_t* n_pairs);
 
-/* Integrator: type, delta_t, per-particle gammas (BAOAB) or alphas (G-JF), and the seed of
- * the counter-based noise stream (read_integrator.hpp:18-307). */
+/* Integrator: type, delta_t, per-particle gammas (BAOAB, g-BAOAB) or alphas (G-JF), and the
+ * seed of the counter-based noise stream (read_integrator.hpp:18-307).  g-BAOAB
+ * (gBAOABLangevinIntegrator.hpp:348-433) applies the constraints of mjg_add_constraint. */
 mjg_status mjg_integrator_set(mjg_engine* e, int type, double dt, const double* coefs, uint64_t seed);
 /* SystemMotionRemover(translation, rotation, rescale) of the integrator (SystemMotionRemover.hpp:
  * 21-148; TOML integrator.remove.{translation, rotation, rescale}, read_integrator.hpp:18-35):

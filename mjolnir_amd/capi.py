@@ -39,7 +39,7 @@ GLOBAL_CODE = {"LennardJones": 1, "ExcludedVolume": 2, "DebyeHuckel": 3, "3SPN2E
                "3SPN2BasePair": 5, "3SPN2CrossStacking": 6}
 PARTITION_CODE = {"CellList": 1, "VerletList": 2, "Naive": 3}
 IGNORE_CODE = {"Nothing": 0, "Self": 1, "Intra": 1, "Others": 2, "Inter": 2}
-INTEG_CODE = {"BAOABLangevin": 1, "GJFNVTLangevin": 2}
+INTEG_CODE = {"BAOABLangevin": 1, "GJFNVTLangevin": 2, "gBAOABLangevin": 3}
 
 
 class MjgError(RuntimeError):
@@ -55,7 +55,7 @@ _up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 EXPORTED = [
     "mjg_version", "mjg_engine_create", "mjg_engine_destroy", "mjg_last_error",
     "mjg_system_set_state", "mjg_system_get_state", "mjg_system_set_attribute",
-    "mjg_add_local", "mjg_add_global", "mjg_forcefield_initialize", "mjg_num_interactions",
+    "mjg_add_local", "mjg_add_global", "mjg_add_constraint", "mjg_forcefield_initialize", "mjg_num_interactions",
     "mjg_calc_force", "mjg_calc_force_and_energy", "mjg_calc_energies", "mjg_calc_force_of",
     "mjg_reduce_margin", "mjg_neighbor_pairs", "mjg_integrator_set", "mjg_integrator_set_remover", "mjg_observer_energy_open",
     "mjg_observer_energy_output", "mjg_simulate", "mjg_integrator_initialize",
@@ -87,6 +87,7 @@ def lib():
     L.mjg_system_get_state.argtypes = [vp, vp, vp, vp]
     L.mjg_system_set_attribute.argtypes = [vp, C.c_char_p, dbl]
     L.mjg_add_local.argtypes = [vp, i32, C.c_char_p, sz, _up, _dp, dbl, C.POINTER(i32)]
+    L.mjg_add_constraint.argtypes = [vp, C.c_char_p, sz, _up, _dp, sz, dbl]
     L.mjg_add_global.argtypes = [vp, i32, dbl, dbl, i32, dbl, i32, sz, C.POINTER(C.c_char_p), _up, sz, _up, _dp,
                                  C.POINTER(i32)]
     L.mjg_forcefield_initialize.argtypes = [vp]
@@ -170,6 +171,11 @@ class Engine:
                                          IGNORE_CODE[g.ignore_molecule], len(kinds), arr, dists,
                                          len(g.participants), np.ascontiguousarray(g.participants),
                                          np.ascontiguousarray(g.params.ravel()), C.byref(iid)))
+        if spec.constraint is not None:
+            c = spec.constraint
+            self._check(L.mjg_add_constraint(self.h, c.topology.encode(), len(c.v0),
+                                             np.ascontiguousarray(c.indices.ravel()), c.v0, int(c.max_iteration),
+                                             float(c.tolerance)))
         self._check(L.mjg_forcefield_initialize(self.h))
         self.n_local = len(spec.local)
         if with_integrator and spec.integrator is not None:

@@ -137,6 +137,7 @@ void Engine::set_state(const double* mass, const double* pos, const double* vel)
     upload(d_rmass_, rm.data(), n_);
     upload(d_pos_, p.data(), n_);
     upload(d_vel_, v.data(), n_);
+    if(d_old_pos_) {upload(d_old_pos_, p.data(), n_);}   // g-BAOAB: the last step end is now
     std::vector<double4> zf(n_, make_double4(0, 0, 0, 0));
     upload(d_force_, zf.data(), n_);
     state_set_ = true;
@@ -274,6 +275,29 @@ int Engine::add_global(int pot, double cutoff_ratio, double eps, int partition, 
     return (int)globals_.size() - 1;
 }
 
+// ConstraintForceField (read_constraint.hpp:12-46): one set of bond constraints per engine
+void Engine::add_constraint(const char* topology, size_t n, const uint32_t* idx, const double* v0, size_t max_it,
+                            double tol)
+{
+    if(initialized_) {throw std::invalid_argument("constraints must be added before initialize");}
+    if(con_.set) {throw std::invalid_argument("only one constraint set is supported");}
+    if(n > 0 && (!idx || !v0)) {throw std::invalid_argument("null constraint arrays");}
+    for(size_t k = 0; k < n; ++k)
+    {
+        if(idx[2 * k] >= n_ || idx[2 * k + 1] >= n_) {throw std::out_of_range("constraint index out of range");}
+        if(idx[2 * k] == idx[2 * k + 1]) {throw std::invalid_argument("a constraint needs two particles");}
+        if(!(v0[k] > 0)) {throw std::invalid_argument("constraint length must be positive");}
+    }
+    if(!(tol >= 0)) {throw std::invalid_argument("constraint tolerance must be non-negative");}
+    const std::string topo = topology ? topology : "";
+    con_.set = true;
+    con_.topo_kind = (topo.empty() || topo == "none") ? -1 : kind_id(topo);
+    con_.idx.assign(idx, idx + 2 * n);
+    con_.v0.assign(v0, v0 + n);
+    con_.max_it = max_it;
+    con_.tol = tol;
+}
+
 // Topology::add_connection (Topology.hpp:161-191) for every local interaction with a
 // connection kind, in the order the force field writes them (ForceField.hpp:53-56), then
 // construct_molecules (Topology.hpp:280-317): only "bond" edges join molecules.
@@ -315,6 +339,12 @@ void Engine::write_topology()
             for(int a = 0; a < L.arity; ++a)
                 for(int b = a + 1; b < L.arity; ++b) {add(ix[a], ix[b], L.topo_kind);}
         }
+    }
+    // ConstraintForceField::write_topology (ConstraintForceField.hpp:49-60), after the locals
+    // (ForceField.hpp:53-56)
+    if(con_.topo_kind >= 0)
+    {
+        for(size_t k = 0; k < con_.v0.size(); ++k) {add(con_.idx[2 * k], con_.idx[2 * k + 1], con_.topo_kind);}
     }
     const int bond = kind_id("bond");
     const uint32_t UNINIT = 0xffffffffu;
@@ -947,7 +977,10 @@ size_t Engine::neighbor_pairs(int id, uint32_t* pairs, size_t cap)
 
 void Engine::set_integrator(int type, double dt, const double* coefs, uint64_t seed)
 {
-    if(type != MJG_INTEG_BAOAB && type != MJG_INTEG_GJF) {throw std::invalid_argument("unknown integrator");}
+    if(type != MJG_INTEG_BAOAB && type != MJG_INTEG_GJF && type != MJG_INTEG_GBAOAB)
+    {
+        throw std::invalid_argument("unknown integrator");
+    }
     if(!(dt > 0) || !coefs) {throw std::invalid_argument("invalid integrator parameters");}
     integ_ = type; dt_ = dt; seed_ = seed;
     coefs_.assign(coefs, coefs + n_);
@@ -976,8 +1009,9 @@ void Engine::integrator_initialize()
     std::vector<double> c1(n_), c2(n_), c3(n_);
     for(size_t i = 0; i < n_; ++i)
     {
-        if(integ_ == MJG_INTEG_BAOAB)
+        if(integ_ == MJG_INTEG_BAOAB || integ_ == MJG_INTEG_GBAOAB)
         {
+            // gBAOABLangevinIntegrator::reset_parameter (:82-107) is BAOAB's
             const double gamma_dt = -1 * coefs_[i] * dt_;
             c1[i] = std::exp(gamma_dt);
             c2[i] = std::sqrt(kBT * (1 - std::exp(2 * gamma_dt)) * (1.0 / mass_[i]));
@@ -994,15 +1028,118 @@ void Engine::integrator_initialize()
     upload(d_c1_, c1.data(), n_);
     upload(d_c2_, c2.data(), n_);
     upload(d_c3_, c3.data(), n_);
+    if(integ_ == MJG_INTEG_GBAOAB) {setup_constraints();}
     forces(d_force_, false, -1, false);
+    if(integ_ == MJG_INTEG_GBAOAB)
+    {
+        hipLaunchKernelGGL(k_copy_double4, dim3(blocks_for(n_, 256)), dim3(256), 0, stream_, (uint32_t)n_, d_pos_,
+                           d_old_pos_);
+    }
     check(hipGetLastError(), "integrator_initialize launch");
     sync();
     integ_ready_ = true;
 }
 
+// gBAOABLangevinIntegrator::initialize (:290-345): v0^2, 1/v0^2 and the reduced masses of the
+// constraints, the correction time step (one correction per A step), and the connected
+// components of the constraint graph (union-find), each listing its constraints in input order.
+void Engine::setup_constraints()
+{
+    const size_t m = con_.v0.size();
+    dt_corr_ = dt_ * 0.5 / 1;
+    if(!d_old_pos_) {d_old_pos_ = dalloc<double4>(n_); d_old_rattle_ = dalloc<double4>(n_);}
+    ctab_ = ConTable{};
+    ctab_.max_it = con_.max_it;
+    ctab_.tol = con_.tol;
+    ctab_.r_dt_corr = 1. / dt_corr_;
+    if(m == 0) {return;}
+    std::vector<double> sq(m), rsq(m), redm(m);
+    for(size_t k = 0; k < m; ++k)
+    {
+        sq[k] = std::pow(con_.v0[k], 2);
+        rsq[k] = 1.0 / sq[k];
+        redm[k] = 1.0 / ((1.0 / mass_[con_.idx[2 * k]]) + (1.0 / mass_[con_.idx[2 * k + 1]]));
+    }
+    std::vector<uint32_t> root(n_);
+    for(size_t i = 0; i < n_; ++i) {root[i] = (uint32_t)i;}
+    auto find = [&](uint32_t a) {
+        while(root[a] != a) {root[a] = root[root[a]]; a = root[a];}
+        return a;
+    };
+    for(size_t k = 0; k < m; ++k)
+    {
+        const uint32_t a = find(con_.idx[2 * k]), b = find(con_.idx[2 * k + 1]);
+        if(a != b) {root[std::max(a, b)] = std::min(a, b);}
+    }
+    std::map<uint32_t, std::vector<uint32_t>> comps;
+    for(size_t k = 0; k < m; ++k) {comps[find(con_.idx[2 * k])].push_back((uint32_t)k);}
+    std::vector<uint32_t> start, list;
+    for(const auto& kv : comps)
+    {
+        start.push_back((uint32_t)list.size());
+        list.insert(list.end(), kv.second.begin(), kv.second.end());
+    }
+    start.push_back((uint32_t)list.size());
+    uint32_t* d_start = dalloc<uint32_t>(start.size());
+    uint32_t* d_list = dalloc<uint32_t>(list.size());
+    uint32_t* d_pair = dalloc<uint32_t>(2 * m);
+    double* d_sq = dalloc<double>(m);
+    double* d_rsq = dalloc<double>(m);
+    double* d_redm = dalloc<double>(m);
+    upload(d_start, start.data(), start.size());
+    upload(d_list, list.data(), list.size());
+    upload(d_pair, con_.idx.data(), 2 * m);
+    upload(d_sq, sq.data(), m);
+    upload(d_rsq, rsq.data(), m);
+    upload(d_redm, redm.data(), m);
+    ctab_.ncomp = (uint32_t)comps.size();
+    ctab_.comp_start = d_start;
+    ctab_.comp_con = d_list;
+    ctab_.pair = d_pair;
+    ctab_.sq_v0 = d_sq;
+    ctab_.r_sq_v0 = d_rsq;
+    ctab_.redm = d_redm;
+}
+
 void Engine::launch_step()
 {
     const uint32_t nb = blocks_for(n_, 256);
+    if(integ_ == MJG_INTEG_GBAOAB)
+    {
+        // gBAOABLangevinIntegrator::step (:348-433)
+        const uint32_t nc = blocks_for(ctab_.ncomp, 64);
+        const bool con = ctab_.ncomp > 0 && ctab_.max_it > 0;
+        auto cvel = [&]() {
+            if(con) {hipLaunchKernelGGL(k_con_velocity, dim3(nc), dim3(64), 0, stream_, ctab_, d_pos_, d_vel_, d_rmass_, box_);}
+        };
+        auto drift = [&]() {
+            hipLaunchKernelGGL(k_gb_drift, dim3(nb), dim3(256), 0, stream_, (uint32_t)n_, d_pos_, d_vel_, d_old_rattle_,
+                               dt_corr_);
+            if(con)
+            {
+                hipLaunchKernelGGL(k_con_coordinate, dim3(nc), dim3(64), 0, stream_, ctab_, d_pos_, d_vel_,
+                                   d_old_rattle_, d_rmass_, box_);
+            }
+            cvel();
+        };
+        hipLaunchKernelGGL(k_gb_kick, dim3(nb), dim3(256), 0, stream_, (uint32_t)n_, d_vel_, d_force_, d_rmass_, dt_ / 2);
+        cvel();
+        drift();
+        hipLaunchKernelGGL(k_gb_o, dim3(nb), dim3(256), 0, stream_, (uint32_t)n_, d_vel_, d_c1_, d_c2_, seed_, d_sctl_);
+        cvel();
+        drift();
+        hipLaunchKernelGGL(k_gb_wrap, dim3(nb), dim3(256), 0, stream_, (uint32_t)n_, d_pos_, d_old_pos_, box_, d_sctl_);
+        if(!globals_.empty()) {launch_rebuilds(0);}
+        forces(d_force_, false, -1, true);
+        cvel();
+        if(rm_translation_ || rm_rotation_)
+        {
+            hipLaunchKernelGGL(k_remove_motion, dim3(1), dim3(kRemoverThreads), 0, stream_, (uint32_t)n_, d_pos_,
+                               d_vel_, d_mass_, rm_translation_, rm_rotation_, rm_rescale_);
+        }
+        hipLaunchKernelGGL(k_copy_double4, dim3(nb), dim3(256), 0, stream_, (uint32_t)n_, d_pos_, d_old_pos_);
+        return;
+    }
     if(integ_ == MJG_INTEG_BAOAB)
     {
         hipLaunchKernelGGL(k_baoab_a, dim3(nb), dim3(256), 0, stream_, (uint32_t)n_, d_pos_, d_vel_, d_force_,
@@ -1084,6 +1221,11 @@ void Engine::run_segment(size_t n)
         check(hipMemcpyAsync(d_force_, d_bak_force_, n_ * sizeof(double4), hipMemcpyDeviceToDevice, stream_), "restore");
         check(hipMemcpyAsync(d_sctl_, d_bak_sctl_, sizeof(StepCtl), hipMemcpyDeviceToDevice, stream_), "restore");
         check(hipMemcpyAsync(d_lctl_, d_bak_lctl_, ng * sizeof(ListCtl), hipMemcpyDeviceToDevice, stream_), "restore");
+        if(integ_ == MJG_INTEG_GBAOAB)
+        {
+            hipLaunchKernelGGL(k_copy_double4, dim3(blocks_for(n_, 256)), dim3(256), 0, stream_, (uint32_t)n_, d_pos_,
+                               d_old_pos_);
+        }
         sync();
         build_lists_now(true);
         forces(d_force_, false, -1, false);
@@ -1378,6 +1520,11 @@ mjg_status mjg_add_global(mjg_engine* e, int potential, double cutoff_ratio, dou
         // interaction ids follow calc_energy order: locals first, then globals
         if(id) {*id = (int)x.num_locals() + k;}
     });
+}
+mjg_status mjg_add_constraint(mjg_engine* e, const char* topology, size_t n, const uint32_t* indices,
+                              const double* v0, size_t max_iteration, double tolerance)
+{
+    return guarded(e, [&](mjg::Engine& x) {x.add_constraint(topology, n, indices, v0, max_iteration, tolerance);});
 }
 mjg_status mjg_forcefield_initialize(mjg_engine* e)
 {

@@ -5,6 +5,7 @@
 #include "device_common.hpp"
 #include "force_kernels.hpp"
 #include "integrate_kernels.hpp"
+#include "constraint_kernels.hpp"
 #include "nlist_kernels.hpp"
 #include "sort_kernels.hpp"
 
@@ -69,6 +70,17 @@ struct GlobalIA
     bool multibody() const {return pot == kBasePair || pot == kCrossStack;}
 };
 
+// ConstraintForceField (ConstraintForceField.hpp:15-81)
+struct ConstraintSet
+{
+    bool set = false;
+    int topo_kind = -1;
+    std::vector<uint32_t> idx;    // 2 per constraint
+    std::vector<double> v0;
+    unsigned long long max_it = 0;
+    double tol = 0;
+};
+
 class Engine
 {
   public:
@@ -82,6 +94,8 @@ class Engine
     int add_global(int pot, double cutoff_ratio, double eps, int partition, double margin, int ignore_mol,
                    size_t n_within, const char* const* kinds, const uint32_t* dists,
                    size_t np, const uint32_t* part, const double* par);
+    void add_constraint(const char* topology, size_t n, const uint32_t* idx, const double* v0, size_t max_it,
+                        double tol);
     void initialize();
     size_t num_interactions() const {return locals_.size() + globals_.size();}
     size_t num_locals() const {return locals_.size();}
@@ -121,6 +135,7 @@ class Engine
     void launch_multibody(const GlobalIA& g, bool energy);
     void forces(double4* f, bool energy, int only, bool kick);
     void launch_step();
+    void setup_constraints();
     void run_segment(size_t n);
     void sync();
     void check(hipError_t e, const char* what);
@@ -165,6 +180,12 @@ class Engine
     uint64_t seed_ = 0;
     std::vector<double> coefs_;
     double *d_c1_ = nullptr, *d_c2_ = nullptr, *d_c3_ = nullptr;
+
+    // g-BAOAB constraints
+    ConstraintSet con_;
+    ConTable ctab_{};
+    double4 *d_old_pos_ = nullptr, *d_old_rattle_ = nullptr;
+    double dt_corr_ = 0;
 
     // control
     StepCtl* d_sctl_ = nullptr;

@@ -134,8 +134,24 @@ class GlobalSpec:
 
 
 @dataclass
+class ConstraintSpec:
+    """Bond constraints for gBAOABLangevin (``[[forcefields.constraint]]``,
+    read_constraint.hpp:12-46): pairs (i, j) held at distance v0, the connection kind they
+    write into the topology, the Gauss-Seidel sweep cap and the tolerance."""
+    indices: np.ndarray                 # (m, 2) uint32
+    v0: np.ndarray                      # (m,)
+    topology: str = "bond"
+    max_iteration: int = 500
+    tolerance: float = 1e-6
+
+    def __post_init__(self):
+        self.indices = np.ascontiguousarray(self.indices, dtype=np.uint32).reshape(-1, 2)
+        self.v0 = np.ascontiguousarray(self.v0, dtype=np.float64).reshape(-1)
+
+
+@dataclass
 class IntegratorSpec:
-    type: str                  # "BAOABLangevin" or "GJFNVTLangevin"
+    type: str                  # "BAOABLangevin", "GJFNVTLangevin" or "gBAOABLangevin"
     dt: float
     coefs: np.ndarray          # gammas (BAOAB) or alphas (G-JF), one per particle
     seed: int = 12345
@@ -158,6 +174,7 @@ class SystemSpec:
     local: List[LocalSpec] = field(default_factory=list)
     global_: List[GlobalSpec] = field(default_factory=list)
     integrator: Optional[IntegratorSpec] = None
+    constraint: Optional[ConstraintSpec] = None
     name: str = "system"
 
     def __post_init__(self):
@@ -207,6 +224,11 @@ def write_mjspec(spec: SystemSpec, path: str) -> None:
                    f"{g.partition} {_fmt(g.margin)} {g.ignore_molecule} {within}")
         for idx, par in zip(g.participants, g.params):
             out.append(str(int(idx)) + " " + " ".join(_fmt(p) for p in par))
+    if spec.constraint is not None:
+        c = spec.constraint
+        out.append(f"constraint {c.topology} {len(c.v0)} {int(c.max_iteration)} {_fmt(c.tolerance)}")
+        for (i, j), v0 in zip(c.indices, c.v0):
+            out.append(f"{int(i)} {int(j)} {_fmt(v0)}")
     if spec.integrator is not None:
         it = spec.integrator
         out.append(f"integrator {it.type} {_fmt(it.dt)} {int(it.seed)}")

@@ -24,7 +24,7 @@ from typing import List, Optional, Tuple
 
 import numpy as np
 
-from .spec import NIL, GlobalSpec, IntegratorSpec, LocalSpec, SystemSpec
+from .spec import NIL, ConstraintSpec, GlobalSpec, IntegratorSpec, LocalSpec, SystemSpec
 
 # kB [J/K] -> [kcal/mol/K]: kB * (J_to_cal / 1000) * N_A  (read_units.hpp:35-36)
 _KB_SI = 1.38064852e-23
@@ -192,6 +192,22 @@ def cg_chain(n: int = 1000, seed: int = 2, T: float = 300.0, ionic_strength: flo
                       local=local, global_=[ev, dh],
                       integrator=IntegratorSpec("BAOABLangevin", dt, np.full(n, gamma), seed=seed),
                       name=f"cg_chain_N{n}")
+
+
+def constrained_chain(n: int = 100, seed: int = 5, gamma: float = 0.1, dt: float = 0.01,
+                      max_iteration: int = 500, tolerance: float = 1e-6) -> SystemSpec:
+    """The CG chain with its bonds held rigid instead of harmonic: bond constraints at the
+    initial lengths (connection kind "bond", so exclusions and molecules are unchanged) and the
+    g-BAOAB integrator (gBAOABLangevinIntegrator.hpp, read_constraint.hpp)."""
+    s = cg_chain(n=n, seed=seed, gamma=gamma, dt=dt, max_bend_deg=150.0)
+    bonds = s.local[0]
+    assert (bonds.interaction, bonds.potential) == ("BondLength", "Harmonic")
+    s.local = s.local[1:]
+    s.constraint = ConstraintSpec(indices=bonds.indices, v0=bonds.params[:, 1], topology="bond",
+                                  max_iteration=max_iteration, tolerance=tolerance)
+    s.integrator = IntegratorSpec("gBAOABLangevin", dt, np.full(n, gamma), seed=seed)
+    s.name = f"constrained_chain_N{n}"
+    return s
 
 
 def replica_ensemble(n_replicas: int = 512, n_min: int = 50, n_max: int = 2000,

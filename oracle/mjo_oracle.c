@@ -163,6 +163,11 @@ struct orc_engine {
     int rm_translation, rm_rotation, rm_rescale;   /* SystemMotionRemover flags */
     double *coef, *c1, *c2, *c3; /* BAOAB: exp(-g dt), noise; GJF: b, beta, a */
     size_t rebuilds;
+    /* constraints (g-BAOAB) */
+    size_t n_con; uint32_t* con_idx; double* con_v0; int con_kind; size_t con_maxit; double con_tol;
+    double *con_sq_v0, *con_r_sq_v0, *con_redm;   /* v0^2, 1 / v0^2, 1 / (1/m1 + 1/m2) */
+    double *old_pos, *old_rattle;                  /* position at the last step end / before an A step */
+    double dt_corr, r_dt_corr;
 };
 
 static int kind_id(orc_engine* e, const char* name)
@@ -247,6 +252,8 @@ void orc_destroy(orc_engine* e)
     free(e->adjs); free(e->mol);
     free(e->mass); free(e->rmass); free(e->pos); free(e->vel); free(e->force);
     free(e->coef); free(e->c1); free(e->c2); free(e->c3);
+    free(e->con_idx); free(e->con_v0); free(e->con_sq_v0); free(e->con_r_sq_v0); free(e->con_redm);
+    free(e->old_pos); free(e->old_rattle);
     free(e);
 }
 
@@ -414,6 +421,10 @@ static void write_topology(orc_engine* e)
                     add_connection(e, ix[a], ix[b], L->topo_kind);
         }
     }
+    /* ConstraintForceField::write_topology (ConstraintForceField.hpp:49-60), after the locals
+       (ForceField.hpp:53-56) */
+    if(e->con_kind >= 0)
+        for(size_t k = 0; k < e->n_con; ++k) add_connection(e, e->con_idx[2 * k], e->con_idx[2 * k + 1], e->con_kind);
     /* construct_molecules: only "bond" edges join molecules */
     const int bond = kind_id(e, "bond");
     const size_t UNINIT = (size_t)-1;
@@ -910,6 +921,22 @@ static void contact_make_list(orc_engine* e, local_t* L)
         if(len2 < rc * rc) zpush(&L->active, t);
     }
     L->current_margin = L->cutoff * L->margin;
+}
+
+int orc_add_constraint(orc_engine* e, const char* topology, size_t n, const uint32_t* indices, const double* v0,
+                       size_t max_iteration, double tolerance)
+{
+    if(e->con_idx != NULL) return -1;
+    for(size_t k = 0; k < 2 * n; ++k) if(indices[k] >= e->n) return -1;
+    e->n_con = n;
+    e->con_idx = (uint32_t*)malloc(2 * n * sizeof(uint32_t) + 1);
+    e->con_v0 = (double*)malloc(n * sizeof(double) + 1);
+    memcpy(e->con_idx, indices, 2 * n * sizeof(uint32_t));
+    memcpy(e->con_v0, v0, n * sizeof(double));
+    e->con_kind = (topology == NULL || topology[0] == '\0' || strcmp(topology, "none") == 0) ? -1 : kind_id(e, topology);
+    e->con_maxit = max_iteration;
+    e->con_tol = tolerance;
+    return 0;
 }
 
 int orc_initialize(orc_engine* e)
@@ -2144,7 +2171,7 @@ int orc_reduce_margin(orc_engine* e, double dmargin)
 /* ------------------------------------------------------------------------------------- */
 int orc_set_integrator(orc_engine* e, int type, double dt, const double* coefs, uint64_t seed)
 {
-    if(type != ORC_INTEG_BAOAB && type != ORC_INTEG_GJF) return -1;
+    if(type != ORC_INTEG_BAOAB && type != ORC_INTEG_GJF && type != ORC_INTEG_GBAOAB) return -1;
     e->integ = type; e->dt = dt; e->halfdt = dt / 2; e->seed = seed; e->step = 0;
     free(e->coef); free(e->c1); free(e->c2); free(e->c3);
     e->coef = (double*)malloc(e->n * sizeof(double) + 1);
@@ -2163,8 +2190,9 @@ int orc_integrator_initialize(orc_engine* e)
     const double kBT = orc_kB() * e->temperature;
     for(size_t i = 0; i < e->n; ++i)
     {
-        if(e->integ == ORC_INTEG_BAOAB)
+        if(e->integ == ORC_INTEG_BAOAB || e->integ == ORC_INTEG_GBAOAB)
         {
+            /* gBAOABLangevinIntegrator::reset_parameter (:82-107) is BAOAB's */
             const double gamma_dt = -1 * e->coef[i] * e->dt;
             e->c1[i] = exp(gamma_dt);
             e->c2[i] = sqrt(kBT * (1 - exp(2 * gamma_dt)) * e->rmass[i]);
@@ -2179,7 +2207,147 @@ int orc_integrator_initialize(orc_engine* e)
     }
     memset(e->force, 0, 3 * e->n * sizeof(double));
     calc_force_impl(e, 0);
+    if(e->integ == ORC_INTEG_GBAOAB)
+    {
+        /* gBAOABLangevinIntegrator::initialize (:290-345), one correction step per A step */
+        const size_t m = e->n_con;
+        free(e->con_sq_v0); free(e->con_r_sq_v0); free(e->con_redm); free(e->old_pos); free(e->old_rattle);
+        e->con_sq_v0 = (double*)malloc(m * sizeof(double) + 1);
+        e->con_r_sq_v0 = (double*)malloc(m * sizeof(double) + 1);
+        e->con_redm = (double*)malloc(m * sizeof(double) + 1);
+        for(size_t k = 0; k < m; ++k)
+        {
+            e->con_sq_v0[k] = pow(e->con_v0[k], 2);
+            e->con_r_sq_v0[k] = 1.0 / e->con_sq_v0[k];
+            e->con_redm[k] = 1.0 / (e->rmass[e->con_idx[2 * k]] + e->rmass[e->con_idx[2 * k + 1]]);
+        }
+        if(e->con_idx == NULL) e->con_tol = INFINITY;   /* ConstraintForceField() default */
+        e->dt_corr = e->dt * 0.5 / 1;
+        e->r_dt_corr = 1. / e->dt_corr;
+        e->old_pos = (double*)malloc(3 * e->n * sizeof(double) + 1);
+        e->old_rattle = (double*)malloc(3 * e->n * sizeof(double) + 1);
+        memcpy(e->old_pos, e->pos, 3 * e->n * sizeof(double));
+    }
     return 0;
+}
+
+/* gBAOABLangevinIntegrator::correct_velocity (:177-225): Gauss-Seidel sweeps over the
+ * constraints in input order until no correction exceeds the tolerance */
+static void gb_correct_velocity(orc_engine* e)
+{
+    for(size_t it = 0; it < e->con_maxit; ++it)
+    {
+        int corrected = 0;
+        for(size_t k = 0; k < e->n_con; ++k)
+        {
+            const uint32_t a = e->con_idx[2 * k], b = e->con_idx[2 * k + 1];
+            double* v1 = e->vel + 3 * a; double* v2 = e->vel + 3 * b;
+            double pd[3], vd[3];
+            adjust_direction(e, e->pos + 3 * a, e->pos + 3 * b, pd);
+            for(int c = 0; c < 3; ++c) vd[c] = v2[c] - v1[c];
+            const double lambda = dot(pd, vd) * e->con_redm[k] * e->con_r_sq_v0[k];
+            if(e->con_tol < fabs(lambda))
+            {
+                const double rm1 = e->rmass[a], rm2 = e->rmass[b];
+                double cv[3];
+                for(int c = 0; c < 3; ++c) cv[c] = lambda * pd[c];
+                for(int c = 0; c < 3; ++c) { v1[c] += cv[c] * rm1; v2[c] -= cv[c] * rm2; }
+                corrected = 1;
+            }
+        }
+        if(!corrected) break;
+    }
+}
+
+/* gBAOABLangevinIntegrator::correct_coordinate (:115-175) */
+static void gb_correct_coordinate(orc_engine* e)
+{
+    for(size_t it = 0; it < e->con_maxit; ++it)
+    {
+        int corrected = 0;
+        for(size_t k = 0; k < e->n_con; ++k)
+        {
+            const uint32_t a = e->con_idx[2 * k], b = e->con_idx[2 * k + 1];
+            double* p1 = e->pos + 3 * a; double* p2 = e->pos + 3 * b;
+            double* v1 = e->vel + 3 * a; double* v2 = e->vel + 3 * b;
+            double dp[3];
+            adjust_direction(e, p1, p2, dp);
+            const double dp2 = length_sq(dp);
+            const double miss = e->con_sq_v0[k] - dp2;
+            if(e->con_tol < fabs(miss))
+            {
+                double odp[3];
+                adjust_direction(e, e->old_rattle + 3 * a, e->old_rattle + 3 * b, odp);
+                const double dot_on = dot(odp, dp);
+                const double lambda = 0.5 * miss * e->con_redm[k] / dot_on;
+                const double rm1 = e->rmass[a], rm2 = e->rmass[b];
+                double cf[3], c1[3], c2[3];
+                for(int c = 0; c < 3; ++c) cf[c] = lambda * odp[c];
+                for(int c = 0; c < 3; ++c) { c1[c] = cf[c] * rm1; c2[c] = cf[c] * rm2; }
+                for(int c = 0; c < 3; ++c)
+                {
+                    p1[c] -= c1[c];
+                    p2[c] += c2[c];
+                    v1[c] -= c1[c] * e->r_dt_corr;
+                    v2[c] += c2[c] * e->r_dt_corr;
+                }
+                corrected = 1;
+            }
+        }
+        if(!corrected) break;
+    }
+}
+
+/* the A step with its corrections (gBAOABLangevinIntegrator.hpp:363-372) */
+static void gb_drift(orc_engine* e)
+{
+    for(size_t i = 0; i < e->n; ++i)
+        for(int c = 0; c < 3; ++c)
+        {
+            e->old_rattle[3 * i + c] = e->pos[3 * i + c];
+            e->pos[3 * i + c] += e->dt_corr * e->vel[3 * i + c];
+        }
+    gb_correct_coordinate(e);
+    gb_correct_velocity(e);
+}
+
+static void gbaoab_step(orc_engine* e)
+{
+    /* gBAOABLangevinIntegrator::step (:348-433) with the engine's noise stream */
+    const double h = e->halfdt;
+    for(size_t i = 0; i < e->n; ++i)
+    {
+        const double rm = e->rmass[i];
+        for(int c = 0; c < 3; ++c) e->vel[3 * i + c] += h * rm * e->force[3 * i + c];
+    }
+    gb_correct_velocity(e);
+    gb_drift(e);
+    for(size_t i = 0; i < e->n; ++i)
+    {
+        double R[3];
+        orc_noise3(e->seed, e->step, (uint32_t)i, R);
+        for(int c = 0; c < 3; ++c) e->vel[3 * i + c] *= e->c1[i];
+        for(int c = 0; c < 3; ++c) e->vel[3 * i + c] += e->c2[i] * R[c];
+    }
+    gb_correct_velocity(e);
+    gb_drift(e);
+    double largest = 0.0;
+    for(size_t i = 0; i < e->n; ++i)
+    {
+        double d[3];
+        for(int c = 0; c < 3; ++c) d[c] = e->pos[3 * i + c] - e->old_pos[3 * i + c];
+        largest = fmax(largest, length_sq(d));
+        adjust_position(e, e->pos + 3 * i);
+        for(int c = 0; c < 3; ++c) e->force[3 * i + c] = 0;
+    }
+    orc_reduce_margin(e, 2 * sqrt(largest));
+    calc_force_impl(e, 0);
+    for(size_t i = 0; i < e->n; ++i)
+    {
+        const double rm = e->rmass[i];
+        for(int c = 0; c < 3; ++c) e->vel[3 * i + c] += h * rm * e->force[3 * i + c];
+    }
+    gb_correct_velocity(e);
 }
 
 static void baoab_step(orc_engine* e)
@@ -2354,8 +2522,10 @@ int orc_step(orc_engine* e, size_t nsteps)
     {
         if(e->integ == ORC_INTEG_BAOAB) baoab_step(e);
         else if(e->integ == ORC_INTEG_GJF) gjf_step(e);
+        else if(e->integ == ORC_INTEG_GBAOAB) gbaoab_step(e);
         else return -1;
         if(e->rm_translation || e->rm_rotation) remove_motion(e);
+        if(e->integ == ORC_INTEG_GBAOAB) memcpy(e->old_pos, e->pos, 3 * e->n * sizeof(double));
         e->step++;
     }
     return 0;

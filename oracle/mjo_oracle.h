@@ -47,7 +47,7 @@ enum {
 enum { ORC_PART_CELLLIST = 1, ORC_PART_VERLET = 2, ORC_PART_NAIVE = 3 };
 #define ORC_NIL 0xffffffffu   /* absent index (a nucleotide without phosphate, a strand end) */
 enum { ORC_IGNORE_NOTHING = 0, ORC_IGNORE_SELF = 1, ORC_IGNORE_OTHERS = 2 };
-enum { ORC_INTEG_BAOAB = 1, ORC_INTEG_GJF = 2 };
+enum { ORC_INTEG_BAOAB = 1, ORC_INTEG_GJF = 2, ORC_INTEG_GBAOAB = 3 };
 
 typedef struct orc_engine orc_engine;
 
@@ -73,6 +73,12 @@ int orc_add_global(orc_engine* e, int potential, double cutoff_ratio, double eps
                    int partition, double margin, int ignore_molecule,
                    size_t n_within, const char* const* within_kinds, const uint32_t* within_dist,
                    size_t n_participants, const uint32_t* participants, const double* params);
+
+/* ConstraintForceField (ConstraintForceField.hpp:15-81, read_constraint.hpp:12-46): bond
+ * constraints (i, j, v0) for gBAOABLangevin, with the connection kind they write into the
+ * topology ("none": none), the iteration cap and the tolerance; one set per engine */
+int orc_add_constraint(orc_engine* e, const char* topology, size_t n, const uint32_t* indices, const double* v0,
+                       size_t max_iteration, double tolerance);
 
 /* ForceField::initialize (ForceField.hpp:47-63): topology, molecules, exclusions, lists */
 int orc_initialize(orc_engine* e);

@@ -28,7 +28,7 @@ GLOBAL_CODE = {"LennardJones": 1, "ExcludedVolume": 2, "DebyeHuckel": 3, "3SPN2E
                "3SPN2BasePair": 5, "3SPN2CrossStacking": 6}
 PARTITION_CODE = {"CellList": 1, "VerletList": 2, "Naive": 3}
 IGNORE_CODE = {"Nothing": 0, "Self": 1, "Intra": 1, "Others": 2, "Inter": 2}
-INTEG_CODE = {"BAOABLangevin": 1, "GJFNVTLangevin": 2}
+INTEG_CODE = {"BAOABLangevin": 1, "GJFNVTLangevin": 2, "gBAOABLangevin": 3}
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
@@ -56,6 +56,7 @@ def lib():
         L.orc_add_global.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int,
                                      C.c_size_t, C.POINTER(C.c_char_p), _up, C.c_size_t, _up, _dp]
         L.orc_initialize.argtypes = [C.c_void_p]
+        L.orc_add_constraint.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t, _up, _dp, C.c_size_t, C.c_double]
         L.orc_num_interactions.restype = C.c_size_t
         L.orc_num_interactions.argtypes = [C.c_void_p]
         L.orc_calc_energy_of.restype = C.c_double
@@ -118,6 +119,12 @@ class OracleEngine:
                                   np.ascontiguousarray(g.participants), np.ascontiguousarray(g.params.ravel()))
             if rc < 0:
                 raise ValueError(f"oracle rejected global {g.name}")
+        c = getattr(spec, "constraint", None)
+        if c is not None:
+            rc = L.orc_add_constraint(self.h, c.topology.encode(), len(c.v0), np.ascontiguousarray(c.indices.ravel()),
+                                      c.v0, int(c.max_iteration), float(c.tolerance))
+            if rc < 0:
+                raise ValueError("oracle rejected the constraints")
         if L.orc_initialize(self.h) != 0:
             raise ValueError("oracle initialize failed")
         self.n_local = len(spec.local)

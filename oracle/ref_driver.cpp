@@ -10,7 +10,8 @@
 // DebyeHuckelParameterList and a CellList / VerletList / Naive partition
 // (mjolnir/input/read_global_interaction.hpp, read_spatial_partition.hpp:62-116), the local
 // BondLength / BondAngle / DihedralAngle / Contact interactions
-// (mjolnir/input/read_local_interaction.hpp), and BAOABLangevinIntegrator /
+// (mjolnir/input/read_local_interaction.hpp), a ConstraintForceField
+// (mjolnir/input/read_constraint.hpp), and BAOABLangevinIntegrator / gBAOABLangevinIntegrator /
 // GJFNVTLangevinIntegrator (mjolnir/input/read_integrator.hpp).  Units are set exactly as
 // read_units does for "kcal/mol" and "angstrom" (mjolnir/input/read_units.hpp:32-92).
 //
@@ -26,6 +27,7 @@
 #include <mjolnir/core/RandomNumberGenerator.hpp>
 #include <mjolnir/core/BAOABLangevinIntegrator.hpp>
 #include <mjolnir/core/GJFNVTLangevinIntegrator.hpp>
+#include <mjolnir/core/gBAOABLangevinIntegrator.hpp>
 #include <mjolnir/core/SystemMotionRemover.hpp>
 #include <mjolnir/core/EnergyObserver.hpp>
 #include <mjolnir/core/PeriodicGridCellList.hpp>
@@ -96,6 +98,13 @@ struct SpecIn
     bool has_integrator = false;
     bool rm_translation = false, rm_rotation = false, rm_rescale = false;
     std::string integrator;
+    // constraint block (read_constraint.hpp:12-46)
+    bool has_constraint = false;
+    std::string con_topology;
+    std::size_t con_max_iteration = 0;
+    double con_tolerance = 0;
+    std::vector<std::array<std::size_t, 2>> con_idx;
+    std::vector<double> con_v0;
     double dt = 0;
     std::uint32_t seed = 0;
     std::vector<double> coefs;
@@ -185,6 +194,18 @@ static SpecIn read_spec(const std::string& path)
             ifs >> s.integrator >> s.dt >> s.seed;
             s.coefs.resize(s.mass.size());
             for(auto& c : s.coefs) {ifs >> c;}
+        }
+        else if(tok == "constraint")
+        {
+            std::size_t n = 0;
+            s.has_constraint = true;
+            ifs >> s.con_topology >> n >> s.con_max_iteration >> s.con_tolerance;
+            for(std::size_t k = 0; k < n; ++k)
+            {
+                std::array<std::size_t, 2> ij; double v0;
+                ifs >> ij[0] >> ij[1] >> v0;
+                s.con_idx.push_back(ij); s.con_v0.push_back(v0);
+            }
         }
         else if(tok == "remover")
         {
@@ -545,8 +566,16 @@ static Built<traitsT> build(const SpecIn& s)
             throw std::runtime_error("unsupported global " + g.potential);
         }
     }
+    ConstraintForceField<traitsT> constraint;
+    if(s.has_constraint)
+    {
+        typename ConstraintForceField<traitsT>::container_type cs;
+        for(std::size_t k = 0; k < s.con_idx.size(); ++k) {cs.emplace_back(s.con_idx[k], s.con_v0[k]);}
+        constraint = ConstraintForceField<traitsT>(s.con_topology, std::move(cs), s.con_max_iteration,
+                                                   s.con_tolerance);
+    }
     b.ff.reset(new ForceField<traitsT>(std::move(local), std::move(global),
-                                        ExternalForceField<traitsT>{}, ConstraintForceField<traitsT>{}));
+                                        ExternalForceField<traitsT>{}, std::move(constraint)));
     return b;
 }
 
@@ -707,6 +736,7 @@ static int cmd_sim(const SpecIn& s, std::size_t total, std::size_t save, const s
 {
     if(s.integrator == "BAOABLangevin") {return sim_impl<traitsT, BAOABLangevinIntegrator<traitsT>>(s, total, save, prefix);}
     if(s.integrator == "GJFNVTLangevin") {return sim_impl<traitsT, GJFNVTLangevinIntegrator<traitsT>>(s, total, save, prefix);}
+    if(s.integrator == "gBAOABLangevin") {return sim_impl<traitsT, gBAOABLangevinIntegrator<traitsT>>(s, total, save, prefix);}
     throw std::runtime_error("unsupported integrator " + s.integrator);
 }
 
@@ -721,6 +751,10 @@ static int cmd_run(const SpecIn& s, std::size_t nsteps, const std::string& out, 
     if(s.integrator == "GJFNVTLangevin")
     {
         return run_impl<traitsT, GJFNVTLangevinIntegrator<traitsT>>(s, nsteps, out, every, bench);
+    }
+    if(s.integrator == "gBAOABLangevin")
+    {
+        return run_impl<traitsT, gBAOABLangevinIntegrator<traitsT>>(s, nsteps, out, every, bench);
     }
     throw std::runtime_error("unsupported integrator " + s.integrator);
 }

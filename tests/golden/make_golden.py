@@ -36,6 +36,8 @@ def spec_hash(spec: SystemSpec) -> str:
         h.update(g.participants.tobytes()); h.update(g.params.tobytes())
     if spec.integrator is not None:
         h.update(spec.integrator.coefs.tobytes())
+    if spec.constraint is not None:
+        h.update(spec.constraint.indices.tobytes()); h.update(spec.constraint.v0.tobytes())
     return h.hexdigest()
 
 
@@ -134,6 +136,7 @@ def cases():
         "chain200_remover": (chain_with_remover, True, 1000),
         "dna12": (lambda: perturb(workloads.dna_duplex(12, seed=5), 0.2, 11), True, 1000),
         "dna200": (lambda: perturb(workloads.dna_duplex(200, seed=3), 0.1, 12), False, 1000),
+        "chain100_gbaoab": (lambda: perturb(workloads.constrained_chain(100, seed=5), 0.1, 13), True, 1000),
     }
 
 

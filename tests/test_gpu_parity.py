@@ -125,6 +125,25 @@ def test_gjf_trajectory_vs_oracle_same_noise():
     assert np.abs(v - vo).max() < 1e-7
 
 
+def test_gbaoab_trajectory_vs_oracle_same_noise():
+    """g-BAOAB with bond constraints at T > 0: the engine's per-component constraint sweeps
+    against the oracle's sequential ones, same noise stream."""
+    import oracle
+    from mjolnir_amd import workloads
+    spec = make_golden.perturb(workloads.constrained_chain(150, seed=9, gamma=0.5), 0.05, 4)
+    e = _engine(spec)
+    o = oracle.OracleEngine(spec)
+    e.step(150)
+    o.step(150)
+    x, v, _ = e.state()
+    xo, vo, _ = o.state()
+    assert np.abs(x - xo).max() < 1e-7
+    assert np.abs(v - vo).max() < 1e-7
+    c = spec.constraint
+    d = x[c.indices[:, 1]] - x[c.indices[:, 0]]
+    assert np.abs(np.sum(d * d, axis=1) - c.v0 ** 2).max() <= c.tolerance
+
+
 def test_kinetic_temperature_lj():
     """<KE> = 3/2 N kB T for the thermostatted LJ fluid (equipartition)."""
     from mjolnir_amd import workloads

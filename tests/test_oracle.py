@@ -257,3 +257,42 @@ def test_energy_observer_golden_matches_oracle():
         ek = 0.5 * np.sum(spec.mass * np.sum(v * v, axis=1))
         want = np.array([step] + e + [ek, spec.attributes["ionic_strength"], spec.attributes["temperature"]])
         assert np.allclose(rows[r], want, rtol=0, atol=1.5e-6), (step, rows[r] - want)
+
+
+# ----------------------------------------------------------------------------------------
+# g-BAOAB with bond constraints (gBAOABLangevinIntegrator.hpp)
+# ----------------------------------------------------------------------------------------
+def test_gbaoab_constraints_hold_at_finite_temperature():
+    """After every step the constrained bonds sit at their lengths to the position tolerance
+    (|r^2 - v0^2| <= tol, what correct_coordinate enforces) and the relative velocity along each
+    bond is below the velocity tolerance (|lambda| <= tol, correct_velocity)."""
+    spec = workloads.constrained_chain(80, seed=7, gamma=0.5)
+    c = spec.constraint
+    o = oracle.OracleEngine(spec)
+    redm = 1.0 / (1.0 / spec.mass[c.indices[:, 0]] + 1.0 / spec.mass[c.indices[:, 1]])
+    for _ in range(5):
+        o.step(40)
+        x, v, _ = o.state()
+        d = x[c.indices[:, 1]] - x[c.indices[:, 0]]
+        assert np.abs(np.sum(d * d, axis=1) - c.v0 ** 2).max() <= c.tolerance
+        dv = v[c.indices[:, 1]] - v[c.indices[:, 0]]
+        lam = np.sum(d * dv, axis=1) * redm / c.v0 ** 2
+        assert np.abs(lam).max() <= c.tolerance * 1.0001
+
+
+def test_gbaoab_without_constraints_is_baoab():
+    """With no constraint every correction is empty and g-BAOAB is BAOAB: the two split the
+    A step as x + h v + h v' against x + (h v + h v'), so they agree to rounding."""
+    import copy
+    spec = workloads.zero_temperature(make_golden.perturb(workloads.cg_chain(n=120, seed=4), 0.1, 3))
+    a = oracle.OracleEngine(spec)
+    s2 = copy.deepcopy(spec)
+    s2.integrator.type = "gBAOABLangevin"
+    b = oracle.OracleEngine(s2)
+    a.step(200)
+    b.step(200)
+    xa, va, _ = a.state()
+    xb, vb, _ = b.state()
+    assert np.abs(xa - xb).max() < 1e-9
+    assert np.abs(va - vb).max() < 1e-9
+    assert not np.array_equal(xa, xb)     # the step really is the g-BAOAB one
