@@ -153,7 +153,33 @@ template<> struct Fix<float>
     static constexpr double scale = 16777216.0;            // 2^24
     static constexpr double inv = 5.9604644775390625e-08;  // 2^-24
     __device__ static __forceinline__ long long conv(double v) {return __double2ll_rn(v * scale);}
-    __device__ static __forceinline__ long long conv(float v) {return __double2ll_rn((double)v * scale);}
+    // v 2^24 is exact in FP32, and rounding it to an integer in FP32 gives the integer the FP64
+    // path gives; below 2^31 that is a 32-bit conversion (two instructions instead of a dozen)
+    __device__ static __forceinline__ long long conv(float v)
+    {
+        const float s = v * 16777216.0f;
+        if(fabsf(s) < 2147483520.0f) {return (long long)__float2int_rn(s);}
+        return __double2ll_rn((double)v * scale);
+    }
+};
+
+// Reciprocal, reciprocal square root, square root and exp of the pair and local arithmetic.
+// The FP32 (mixed) mode uses the hardware instructions (about 1 ulp, within the mode's force
+// error budget); the FP64 mode keeps correctly rounded operations and the library exp.
+template<typename Real> struct RMath;
+template<> struct RMath<float>
+{
+    __device__ static __forceinline__ float rcp(float x) {return __builtin_amdgcn_rcpf(x);}
+    __device__ static __forceinline__ float rsqrt(float x) {return __builtin_amdgcn_rsqf(x);}
+    __device__ static __forceinline__ float sqrt(float x) {return __builtin_amdgcn_sqrtf(x);}
+    __device__ static __forceinline__ float exp(float x) {return __expf(x);}
+};
+template<> struct RMath<double>
+{
+    __device__ static __forceinline__ double rcp(double x) {return 1.0 / x;}
+    __device__ static __forceinline__ double rsqrt(double x) {return 1.0 / ::sqrt(x);}
+    __device__ static __forceinline__ double sqrt(double x) {return ::sqrt(x);}
+    __device__ static __forceinline__ double exp(double x) {return ::exp(x);}
 };
 
 __device__ __forceinline__ void add3(unsigned long long* F, uint32_t i, long long x, long long y, long long z)
@@ -245,7 +271,7 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         const Real k = (Real)p0, v0 = (Real)p1;
         const Real cut = v0 * (Real)2.5;
         if(cut * cut <= len2) {return;}
-        const Real r2 = (Real)1 / len2;
+        const Real r2 = RMath<Real>::rcp(len2);
         const Real v0r_2 = v0 * v0 * r2;
         const Real v0r_6 = v0r_2 * v0r_2 * v0r_2;
         const Real v0r_10 = v0r_6 * v0r_2 * v0r_2;
@@ -264,8 +290,8 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         const Real ijx = (Real)a, ijy = (Real)b, ijz = (Real)c;
         disp(box, P, i1, i2, a, b, c);
         const Real kjx = (Real)a, kjy = (Real)b, kjz = (Real)c;
-        const Real inv_ij = (Real)1 / sqrt(ijx * ijx + ijy * ijy + ijz * ijz);
-        const Real inv_kj = (Real)1 / sqrt(kjx * kjx + kjy * kjy + kjz * kjz);
+        const Real inv_ij = RMath<Real>::rsqrt(ijx * ijx + ijy * ijy + ijz * ijz);
+        const Real inv_kj = RMath<Real>::rsqrt(kjx * kjx + kjy * kjy + kjz * kjz);
         const Real eix = ijx * inv_ij, eiy = ijy * inv_ij, eiz = ijz * inv_ij;
         const Real ekx = kjx * inv_kj, eky = kjy * inv_kj, ekz = kjz * inv_kj;
         Real cos_t = eix * ekx + eiy * eky + eiz * ekz;
@@ -281,14 +307,14 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         else
         {
             const Real k = (Real)p0, sg = (Real)p1, v0 = (Real)p2;
-            const Real inv_s2 = (Real)-1 / ((Real)2 * sg * sg);
+            const Real inv_s2 = -RMath<Real>::rcp((Real)2 * sg * sg);
             const Real dv = theta - v0;
-            const Real ex = exp(inv_s2 * dv * dv);
+            const Real ex = RMath<Real>::exp(inv_s2 * dv * dv);
             coef = -((Real)2 * inv_s2 * dv * k * ex);
             if(ENERGY) {pot = k * ex;}
         }
         const Real sin_t = sin(theta);
-        const Real cis = coef / fmax(sin_t, (Real)1e-8);
+        const Real cis = coef * RMath<Real>::rcp(fmax(sin_t, (Real)1e-8));
         const Real ci = cis * inv_ij, ck = cis * inv_kj;
         const Real Fix_ = ci * (cos_t * eix - ekx), Fiy = ci * (cos_t * eiy - eky), Fiz = ci * (cos_t * eiz - ekz);
         const Real Fkx = ck * (cos_t * ekx - eix), Fky = ck * (cos_t * eky - eiy), Fkz = ck * (cos_t * ekz - eiz);
@@ -311,13 +337,13 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         disp(box, P, i2, i3, a, b, c);
         const Real klx = -(Real)a, kly = -(Real)b, klz = -(Real)c;
         const Real r_kj_lensq = kjx * kjx + kjy * kjy + kjz * kjz;
-        const Real r_kj_rlen = (Real)1 / sqrt(r_kj_lensq);
+        const Real r_kj_rlen = RMath<Real>::rsqrt(r_kj_lensq);
         const Real r_kj_rlensq = r_kj_rlen * r_kj_rlen;
         const Real r_kj_len = r_kj_rlen * r_kj_lensq;
         const Real mx = ijy * kjz - ijz * kjy, my = ijz * kjx - ijx * kjz, mz = ijx * kjy - ijy * kjx;
         const Real nx = kjy * klz - kjz * kly, ny = kjz * klx - kjx * klz, nz = kjx * kly - kjy * klx;
         const Real m_lensq = mx * mx + my * my + mz * mz, n_lensq = nx * nx + ny * ny + nz * nz;
-        Real cos_p = (mx * nx + my * ny + mz * nz) / sqrt(m_lensq * n_lensq);
+        Real cos_p = (mx * nx + my * ny + mz * nz) * RMath<Real>::rsqrt(m_lensq * n_lensq);
         cos_p = fmin(fmax(cos_p, (Real)-1), (Real)1);
         const Real phi = copysign(acos(cos_p), ijx * nx + ijy * ny + ijz * nz);
         const Real k1 = (Real)p0, k3 = (Real)p1;
@@ -331,8 +357,8 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
             const Real cos3 = cos1 * ((Real)4 * cos1 * cos1 - (Real)3);
             e = (double)(k1 * ((Real)1 - cos1) + k3 * ((Real)1 - cos3));
         }
-        const Real ci = coef * r_kj_len / m_lensq;
-        const Real cl = -coef * r_kj_len / n_lensq;
+        const Real ci = coef * r_kj_len * RMath<Real>::rcp(m_lensq);
+        const Real cl = -coef * r_kj_len * RMath<Real>::rcp(n_lensq);
         const Real Fix = ci * mx, Fiy = ci * my, Fiz = ci * mz;
         const Real Flx = cl * nx, Fly = cl * ny, Flz = cl * nz;
         const Real cijk = (ijx * kjx + ijy * kjy + ijz * kjz) * r_kj_rlensq;
@@ -364,10 +390,10 @@ __device__ __forceinline__ void rep_pair(const RepGlobal& G, const Real* par, ui
         // LorentzBerthelotRule (ParameterList.hpp:240-252), LennardJonesPotential.hpp:34-63
         const Real s1 = par[2 * i], e1 = par[2 * i + 1], s2 = par[2 * j], e2 = par[2 * j + 1];
         const Real sigma = (s1 + s2) / (Real)2;
-        const Real eps = (e1 == e2) ? e1 : sqrt(e1 * e2);
+        const Real eps = (e1 == e2) ? e1 : RMath<Real>::sqrt(e1 * e2);
         const Real sigma_sq = sigma * sigma;
         if(sigma_sq * (Real)G.cutoff_ratio2 < l_sq) {return;}
-        const Real rcp = (Real)1 / l_sq;
+        const Real rcp = RMath<Real>::rcp(l_sq);
         const Real s2l2 = sigma_sq * rcp;
         const Real s6 = s2l2 * s2l2 * s2l2;
         c = (Real)24 * eps * (s6 - (Real)2 * s6 * s6) * rcp;
@@ -379,7 +405,7 @@ __device__ __forceinline__ void rep_pair(const RepGlobal& G, const Real* par, ui
         const Real radius = par[i] + par[j];
         const Real sigma_sq = radius * radius;
         if(sigma_sq * (Real)G.cutoff_ratio2 < l_sq) {return;}
-        const Real rcp = (Real)1 / l_sq;
+        const Real rcp = RMath<Real>::rcp(l_sq);
         const Real s2l2 = sigma_sq * rcp;
         const Real s6 = s2l2 * s2l2 * s2l2;
         c = -((Real)G.eps * (Real)12) * s6 * s6 * rcp;
@@ -389,13 +415,13 @@ __device__ __forceinline__ void rep_pair(const RepGlobal& G, const Real* par, ui
     {
         // DebyeHuckelPotential.hpp:47-65
         const Real qq = (Real)G.inv4pieps * par[i] * par[j];
-        const Real rl = (Real)1 / sqrt(l_sq);
+        const Real rl = RMath<Real>::rsqrt(l_sq);
         const Real l = l_sq * rl;
         if((Real)G.abs_cutoff <= l) {return;}
-        const Real ex = exp(-l * (Real)G.inv_debye);
-        const Real f_mag = -qq * ((Real)G.debye + l) * (Real)G.inv_debye / (l * l) * ex;
+        const Real ex = RMath<Real>::exp(-l * (Real)G.inv_debye);
+        const Real f_mag = -qq * ((Real)G.debye + l) * (Real)G.inv_debye * (rl * rl) * ex;
         c = f_mag * rl;
-        if(ENERGY) {e += (double)(qq * (ex / l - (Real)G.coef_at_cutoff));}
+        if(ENERGY) {e += (double)(qq * (ex * rl - (Real)G.coef_at_cutoff));}
     }
     const long long fx = Fix<Real>::conv(dx * c), fy = Fix<Real>::conv(dy * c), fz = Fix<Real>::conv(dz * c);
     add3(F, i, fx, fy, fz);

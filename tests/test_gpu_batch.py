@@ -135,18 +135,20 @@ def test_list_overflow_is_recovered(monkeypatch):
 
 
 def test_thermostat_mixed_precision_kinetic_temperature():
-    """Kinetic temperature of 8 + 8 replicas at two temperatures, pooled per temperature.
-
-    The oracle's own per-replica means for this set-up spread by about 2.5 % (correlated
-    samples: the thermostat relaxes over 1/gamma = 1000 steps), so the pooled mean of eight
-    replicas has a spread near 1 %; the bar is 3 %."""
+    """Kinetic temperature of 8 + 8 LJ-fluid replicas (216 particles, gamma = 0.5) at two
+    temperatures, pooled per temperature.  The FP64 oracle gives 0.998 for this set-up (BAOAB's
+    kinetic-temperature bias at dt = 0.01 is below 0.5 % here; a CG chain with stiff bonds shows
+    several per cent at the same dt in either precision, so it cannot test the thermostat).
+    The kinetic energy of 216 particles fluctuates by sqrt(2 / (3 N)) = 5.5 % and decorrelates
+    within about 100 steps; 160 samples 50 steps apart from 8 replicas give a pooled mean with a
+    spread near 0.3 %; the bar is 2 %."""
     from mjolnir_amd import capi, workloads
     temps = [300.0] * 8 + [350.0] * 8
-    specs = _chains([400] * 16, seed0=800, temps=temps)
+    specs = [workloads.lj_fluid(n_side=6, seed=900 + k, T=t, gamma=0.5) for k, t in enumerate(temps)]
     b, ids = _batch(specs, capi.FP32_FIXED)
-    b.step(2000)
+    b.step(4000)
     ratio = {300.0: [], 350.0: []}
-    for _ in range(80):
+    for _ in range(160):
         b.step(50)
         for r, s in zip(ids, specs):
             _, v, _ = b.state(r)
@@ -154,7 +156,7 @@ def test_thermostat_mixed_precision_kinetic_temperature():
             ratio[T].append(0.5 * np.sum(s.mass[:, None] * v * v) / (1.5 * s.n * workloads.KB_KCAL * T))
     for T, x in ratio.items():
         print(f"[gpu] T={T}: <KE>/(3/2 N kB T) = {np.mean(x):.4f}", flush=True)
-        assert abs(np.mean(x) - 1.0) < 0.03, (T, np.mean(x))
+        assert abs(np.mean(x) - 1.0) < 0.02, (T, np.mean(x))
 
 
 def test_periodic_lj_batch_vs_oracle():
