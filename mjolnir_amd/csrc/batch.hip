@@ -64,6 +64,7 @@ struct Replica
     int cls = 0;                   // size class (block shape)
     bool ready = false, active = false;
     bool observed = false;          // an energy observation is held for this replica
+    bool tsp = false;               // has 3SPN2 terms: runs in the kernel variant that holds them
 };
 
 static void hip_check(hipError_t e, const char* what)
@@ -159,10 +160,25 @@ class Batch
             case kAngleHarmonic: L.arity = 3; L.npar = 2; break;
             case kAngleGaussian: L.arity = 3; L.npar = 3; break;
             case kDihedralClementi: L.arity = 4; L.npar = 3; break;
+            case kBond3SPN2: L.arity = 2; L.npar = 2; break;
+            case kDihedralPGauss: L.arity = 4; L.npar = 3; break;
+            case kBaseStacking: L.arity = 6; L.npar = 1; break;
             default: throw std::invalid_argument("unknown local interaction kind");
         }
         if(nt > 0 && (!idx || !par)) {throw std::invalid_argument("null term arrays");}
-        for(size_t t = 0; t < nt * L.arity; ++t) {if(idx[t] >= r.n) {throw std::out_of_range("term index out of range");}}
+        for(size_t t = 0; t < nt * L.arity; ++t)
+        {
+            // a base-stacking term may lack its phosphates (positions 3 and 5)
+            const bool optional = (kind == kBaseStacking) && (t % 6 == 3 || t % 6 == 5);
+            if(idx[t] >= r.n && !(optional && idx[t] == kNil)) {throw std::out_of_range("term index out of range");}
+        }
+        if(kind == kBaseStacking)
+        {
+            for(size_t t = 0; t < nt; ++t)
+            {
+                if(!(par[t] >= 0 && par[t] < 16)) {throw std::invalid_argument("3SPN2 stacking kind out of range");}
+            }
+        }
         const std::string topo = topology ? topology : "";
         L.topo = (topo.empty() || topo == "none") ? -1 : r.topo.kind_id(topo);
         L.idx.assign(idx, idx + nt * L.arity);
@@ -182,7 +198,9 @@ class Batch
         switch(pot)
         {
             case kPairLJ: G.npar = 2; break;
-            case kPairEV: case kPairDH: G.npar = 1; break;
+            case kPairEV: case kPairDH: case kPair3SPN2EV: G.npar = 1; break;
+            case kBasePair: G.npar = 2; break;
+            case kCrossStack: G.npar = 4; break;
             default: throw std::invalid_argument("unknown pair potential");
         }
         if(partition != MJG_PART_CELLLIST && partition != MJG_PART_VERLET && partition != MJG_PART_NAIVE)
@@ -200,6 +218,26 @@ class Batch
         }
         G.part.assign(part, part + np);
         G.par.assign(par, par + np * G.npar);
+        for(size_t k = 0; k < np; ++k)
+        {
+            const double* q = G.par.data() + k * G.npar;
+            if(pot == kPair3SPN2EV && !(q[0] >= 0 && q[0] < 6)) {throw std::invalid_argument("3SPN2 bead kind out of range");}
+            if(pot == kBasePair || pot == kCrossStack)
+            {
+                if(!(q[0] >= 0 && q[0] < 4)) {throw std::invalid_argument("3SPN2 base kind out of range");}
+                if(!(q[1] >= 0 && q[1] < (double)r.n)) {throw std::out_of_range("3SPN2 sugar index out of range");}
+            }
+            if(pot == kCrossStack)
+            {
+                for(int c = 2; c < 4; ++c)
+                {
+                    if(!(q[c] == -1.0 || (q[c] >= 0 && q[c] < (double)r.n)))
+                    {
+                        throw std::out_of_range("3SPN2 neighbouring base index out of range");
+                    }
+                }
+            }
+        }
         r.globals.push_back(std::move(G));
     }
 
@@ -452,7 +490,25 @@ class Batch
         if(r.integ == 0) {throw std::invalid_argument("replica " + std::to_string(id) + " has no integrator");}
         if(!r.has_T) {throw std::invalid_argument("Langevin integrator requires the `temperature` attribute");}
         // topology, molecules
-        for(auto& L : r.locals) {r.topo.add_terms(L.idx, L.arity, L.topo);}
+        for(auto& L : r.locals)
+        {
+            if(L.kind != kBaseStacking) {r.topo.add_terms(L.idx, L.arity, L.topo); continue;}
+            if(L.topo < 0) {continue;}
+            // ThreeSPN2BaseStackingInteraction::write_topology (:125-179): every bead of the 5'
+            // nucleotide to every bead of the 3' one, phosphates when present
+            for(size_t t = 0; t + 6 <= L.idx.size(); t += 6)
+            {
+                const uint32_t S5 = L.idx[t], B5 = L.idx[t + 1], B3 = L.idx[t + 2];
+                const uint32_t P5 = L.idx[t + 3], S3 = L.idx[t + 4], P3 = L.idx[t + 5];
+                r.topo.add_connection(S5, S3, L.topo);
+                r.topo.add_connection(S5, B3, L.topo);
+                r.topo.add_connection(B5, S3, L.topo);
+                r.topo.add_connection(B5, B3, L.topo);
+                if(P5 != kNil) {r.topo.add_connection(P5, S3, L.topo); r.topo.add_connection(P5, B3, L.topo);}
+                if(P3 != kNil) {r.topo.add_connection(S5, P3, L.topo); r.topo.add_connection(B5, P3, L.topo);}
+                if(P5 != kNil && P3 != kNil) {r.topo.add_connection(P5, P3, L.topo);}
+            }
+        }
         r.topo.construct_molecules();
         const size_t n = r.n;
 
@@ -470,8 +526,10 @@ class Batch
                 const size_t nt = L.idx.size() / L.arity;
                 for(size_t q = 0; q < nt; ++q, ++t)
                 {
+                    // a base-stacking term keeps {Si, Bi, Bj}: its phosphates only enter the topology
+                    const int packed = (L.kind == kBaseStacking) ? 3 : L.arity;
                     unsigned long long ix = 0;
-                    for(int a = 0; a < L.arity; ++a) {ix |= (unsigned long long)L.idx[q * L.arity + a] << (16 * a);}
+                    for(int a = 0; a < packed; ++a) {ix |= (unsigned long long)L.idx[q * L.arity + a] << (16 * a);}
                     t_idx[t] = ix;
                     t_kind[t] = (uint32_t)L.kind | ((uint32_t)l << 8);
                     for(int c = 0; c < L.npar; ++c) {t_par[c * n_terms + t] = L.par[q * L.npar + c];}
@@ -636,6 +694,7 @@ class Batch
             RepGlobal& RG = D.g[g];
             RG = RepGlobal{};
             RG.kind = G.pot;
+            RG.base_rule = (G.pot == kPair3SPN2EV) ? 1 : (G.pot == kBasePair || G.pot == kCrossStack) ? 2 : 0;
             RG.pairwise = (G.partition != MJG_PART_CELLLIST);
             RG.use_distance = (G.partition != MJG_PART_NAIVE);
             RG.never_rebuild = (G.partition == MJG_PART_NAIVE);
@@ -674,6 +733,17 @@ class Batch
                 for(size_t i = 0; i < n; ++i) {mr = std::max(mr, h.ppar[i]);}
                 max_cutoff = mr * 2 * G.cutoff_ratio;
             }
+            else if(G.pot == kPair3SPN2EV)
+            {
+                // ThreeSPN2ExcludedVolumePotential.hpp:120-134 (1 kJ/mol), list cutoff: the largest
+                // sigma of the table (:176-178)
+                RG.eps = 1.0 * (1.0 / 4.184);
+                max_cutoff = 7.1;
+            }
+            else if(G.pot == kBasePair || G.pot == kCrossStack)
+            {
+                max_cutoff = 18.0;   // ThreeSPN2BasePairGlobalPotentialParameter::cutoff
+            }
             else
             {
                 if(!r.has_I) {throw std::invalid_argument("DebyeHuckel requires `temperature` and `ionic_strength`");}
@@ -694,6 +764,9 @@ class Batch
         }
         r.cls = 0;
         while(r.n > kRepClassMaxN[r.cls]) {++r.cls;}
+        r.tsp = false;
+        for(const auto& L : r.locals) {r.tsp = r.tsp || L.kind == kBond3SPN2 || L.kind == kDihedralPGauss || L.kind == kBaseStacking;}
+        for(const auto& G : r.globals) {r.tsp = r.tsp || G.pot == kPair3SPN2EV || G.pot == kBasePair || G.pot == kCrossStack;}
         hip_check(hipStreamSynchronize(stream_), "upload sync");
         r.ready = true;
         descs_dirty_ = true;
@@ -719,15 +792,17 @@ class Batch
             const Replica& x = *reps_[a];
             const Replica& y = *reps_[b];
             if(x.cls != y.cls) {return x.cls > y.cls;}
+            if(x.tsp != y.tsp) {return y.tsp;}
             return x.n > y.n;
         });
         std::vector<RepDesc> h;
-        for(int c = 0; c < kRepClasses; ++c) {grp_count_[c] = 0; grp_lds_[c] = 0;}
+        for(int c = 0; c < kRepClasses; ++c) {grp_count_[c] = 0; grp_tsp_[c] = 0; grp_lds_[c] = 0;}
         for(int id : order)
         {
             const Replica& r = *reps_[id];
             h.push_back(r.desc);
             grp_count_[r.cls] += 1;
+            grp_tsp_[r.cls] += r.tsp ? 1 : 0;
             grp_lds_[r.cls] = std::max(grp_lds_[r.cls], r.lds);
         }
         // groups are stored class 2, 1, 0
@@ -751,7 +826,10 @@ class Batch
     template<typename Real, bool ENERGY, int BS, int PPT>
     void set_lds_limit()
     {
-        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_replica_segment<Real, ENERGY, BS, PPT>),
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_replica_segment<Real, ENERGY, BS, PPT, false>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRepLdsLimit),
+                  "LDS attribute");
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_replica_segment<Real, ENERGY, BS, PPT, true>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRepLdsLimit),
                   "LDS attribute");
     }
@@ -781,8 +859,18 @@ class Batch
     {
         constexpr int BS = Shape<Real, C>::bs, PPT = Shape<Real, C>::ppt;
         static_assert(BS * PPT == (int)kRepClassMaxN[C], "class shape must cover its size");
-        hipLaunchKernelGGL((k_replica_segment<Real, ENERGY, BS, PPT>), dim3((uint32_t)grp_count_[C]), dim3(BS),
-                           grp_lds_[C], cls_stream_[C], d_descs_ + grp_first_[C], nsteps, eval_only);
+        // the class's replicas without 3SPN2 terms, then (same stream) those with them
+        const size_t plain = grp_count_[C] - grp_tsp_[C];
+        if(plain > 0)
+        {
+            hipLaunchKernelGGL((k_replica_segment<Real, ENERGY, BS, PPT, false>), dim3((uint32_t)plain), dim3(BS),
+                               grp_lds_[C], cls_stream_[C], d_descs_ + grp_first_[C], nsteps, eval_only);
+        }
+        if(grp_tsp_[C] > 0)
+        {
+            hipLaunchKernelGGL((k_replica_segment<Real, ENERGY, BS, PPT, true>), dim3((uint32_t)grp_tsp_[C]), dim3(BS),
+                               grp_lds_[C], cls_stream_[C], d_descs_ + grp_first_[C] + plain, nsteps, eval_only);
+        }
     }
     template<typename Real, bool ENERGY>
     void launch_class(int c, int nsteps, int eval_only)
@@ -910,6 +998,7 @@ class Batch
     hipStream_t cls_stream_[kRepClasses] = {};
     hipEvent_t ev_fork_ = nullptr, ev_join_[kRepClasses] = {};
     size_t grp_count_[kRepClasses] = {}, grp_first_[kRepClasses] = {}, grp_lds_[kRepClasses] = {};
+    size_t grp_tsp_[kRepClasses] = {};             // replicas with 3SPN2 terms, last in their class
     std::vector<std::unique_ptr<Replica>> reps_;
     RepDesc* d_descs_ = nullptr;
     size_t desc_cap_ = 0;

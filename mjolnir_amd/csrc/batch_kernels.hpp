@@ -19,11 +19,12 @@
 //   round), since one workgroup per CU cannot hide a dependent load per item.
 #pragma once
 #include "device_common.hpp"
+#include "tsp_model.hpp"
 
 namespace mjg
 {
 constexpr uint32_t kRepMaxN = 2048;
-constexpr int kRepMaxGlobals = 3;
+constexpr int kRepMaxGlobals = 4;
 constexpr int kRepMaxLocals = 8;
 // dynamic LDS available to one workgroup (160 KiB minus the kernel's static arrays)
 constexpr size_t kRepLdsLimit = 160 * 1024 - 1024;
@@ -47,12 +48,14 @@ template<int BS> struct RepBatching
 
 enum RepTermKind : int
 {
-    kRBondHarmonic = 1, kRBondGo = 2, kRContactGo = 3, kRAngleHarmonic = 4, kRAngleGaussian = 5, kRDihedralClementi = 6
+    kRBondHarmonic = 1, kRBondGo = 2, kRContactGo = 3, kRAngleHarmonic = 4, kRAngleGaussian = 5, kRDihedralClementi = 6,
+    kRBond3SPN2 = 7, kRDihedralPGauss = 8, kRBaseStacking = 9
 };
 
 struct RepGlobal
 {
-    int kind;            // kPairLJ / kPairEV / kPairDH
+    int kind;            // kPairLJ / kPairEV / kPairDH / kPair3SPN2EV / kBasePair / kCrossStack
+    int base_rule;       // 0 none; 1 no complementary base pair (3SPN2 EV); 2 complementary pairs only
     int pairwise;        // Verlet / Naive rule (partner after i in input order)
     int use_distance;
     int never_rebuild;
@@ -238,10 +241,57 @@ __device__ __forceinline__ void disp(const Box& b, const PosL& P, uint32_t from,
     adjust_direction(b, P.x[from], P.y[from], P.z[from], P.x[to], P.y[to], P.z[to], dx, dy, dz);
 }
 
+// 3SPN2 accessors over the LDS state: displacements in FP64, then Real; forces into the
+// fixed-point accumulators
+template<typename Real> struct LdsPos
+{
+    PosL P;
+    Box box;
+    __device__ __forceinline__ tsp::V3<Real> disp(uint32_t from, uint32_t to) const
+    {
+        double dx, dy, dz;
+        ::mjg::disp(box, P, from, to, dx, dy, dz);
+        return {(Real)dx, (Real)dy, (Real)dz};
+    }
+};
+template<typename Real> struct LdsAcc : LdsPos<Real>
+{
+    unsigned long long* F;
+    __device__ __forceinline__ void add(uint32_t i, tsp::V3<Real> f) const {acc3<Real>(F, i, f.x, f.y, f.z);}
+};
+
+// The 3SPN2 terms are long chains of transcendentals: the kernel is instantiated with and
+// without them (TSP), so the registers they need do not weigh on the replicas that have none.
+template<typename Real, bool ENERGY>
+__device__ __forceinline__ double rep_base_stack(uint32_t Si, uint32_t Bi, uint32_t Bj, int bs, Box box,
+                                                           PosL P, unsigned long long* F)
+{
+    const LdsPos<Real> acc{P, box};
+    tsp::V3<Real> fSi, fBi, fBj;
+    const double en = tsp::base_stack<Real, ENERGY>(Si, Bi, Bj, bs, acc, fSi, fBi, fBj);
+    acc3<Real>(F, Si, fSi.x, fSi.y, fSi.z);
+    acc3<Real>(F, Bi, fBi.x, fBi.y, fBi.z);
+    acc3<Real>(F, Bj, fBj.x, fBj.y, fBj.z);
+    return en;
+}
+template<typename Real, bool ENERGY>
+__device__ __forceinline__ double rep_tsp_pair(int kind, uint32_t i, uint32_t j, const double* ppar, Box box,
+                                                         PosL P, unsigned long long* F)
+{
+    LdsAcc<Real> acc;
+    acc.P = P;
+    acc.box = box;
+    acc.F = F;
+    double e = 0.0;
+    if(kind == kBasePair) {tsp::base_pair<Real, ENERGY>(i, j, acc, ppar, e);}
+    else {tsp::cross_stack<Real, ENERGY>(i, j, acc, ppar, e);}
+    return e;
+}
+
 // ------------------------------------------------------------------------------------
 // local terms (Real arithmetic except the harmonic bond)
 // ------------------------------------------------------------------------------------
-template<typename Real, bool ENERGY>
+template<typename Real, bool ENERGY, bool TSP>
 __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, double p0, double p1, double p2,
                                           const Box& box, const PosL& P, unsigned long long* F, double& e)
 {
@@ -370,16 +420,89 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
                    (cjkl - (Real)1) * Flz - cijk * Fiz);
         acc3<Real>(F, i3, Flx, Fly, Flz);
     }
+    else if(!TSP) {}
+    else if(kind == kRBond3SPN2)
+    {
+        // BondLengthInteraction.hpp:126-148, ThreeSPN2BondPotential.hpp:22-35 (FP64, as the
+        // harmonic bond)
+        double dx, dy, dz;
+        disp(box, P, i0, i1, dx, dy, dz);
+        const double len2 = length_sq(dx, dy, dz);
+        const double rlen = 1.0 / sqrt(len2);
+        const double len = len2 * rlen;
+        const double dv = len - p1;
+        const double c = (-1 * (2 * p0 * (dv + 200 * dv * dv * dv))) * rlen;
+        const long long fx = Fix<Real>::conv(dx * c), fy = Fix<Real>::conv(dy * c), fz = Fix<Real>::conv(dz * c);
+        add3(F, i0, -fx, -fy, -fz);
+        add3(F, i1, fx, fy, fz);
+        if(ENERGY) {const double dv2 = dv * dv; e = p0 * (dv2 + 100 * dv2 * dv2);}
+    }
+    else if(kind == kRDihedralPGauss)
+    {
+        // DihedralAngleInteraction.hpp:157-227; PeriodicGaussianPotential.hpp:21-44
+        double a, b, c;
+        disp(box, P, i1, i0, a, b, c);
+        const Real ijx = (Real)a, ijy = (Real)b, ijz = (Real)c;
+        disp(box, P, i1, i2, a, b, c);
+        const Real kjx = (Real)a, kjy = (Real)b, kjz = (Real)c;
+        disp(box, P, i2, i3, a, b, c);
+        const Real klx = -(Real)a, kly = -(Real)b, klz = -(Real)c;
+        const Real r_kj_lensq = kjx * kjx + kjy * kjy + kjz * kjz;
+        const Real r_kj_rlen = RMath<Real>::rsqrt(r_kj_lensq);
+        const Real r_kj_rlensq = r_kj_rlen * r_kj_rlen;
+        const Real r_kj_len = r_kj_rlen * r_kj_lensq;
+        const Real mx = ijy * kjz - ijz * kjy, my = ijz * kjx - ijx * kjz, mz = ijx * kjy - ijy * kjx;
+        const Real nx = kjy * klz - kjz * kly, ny = kjz * klx - kjx * klz, nz = kjx * kly - kjy * klx;
+        const Real m_lensq = mx * mx + my * my + mz * mz, n_lensq = nx * nx + ny * ny + nz * nz;
+        Real cos_p = (mx * nx + my * ny + mz * nz) * RMath<Real>::rsqrt(m_lensq * n_lensq);
+        cos_p = fmin(fmax(cos_p, (Real)-1), (Real)1);
+        const Real phi = copysign(acos(cos_p), ijx * nx + ijy * ny + ijz * nz);
+        const Real k = (Real)p0, sg = (Real)p1;
+        const Real inv_s2 = -RMath<Real>::rcp((Real)2 * sg * sg);
+        Real dval = phi - (Real)p2;
+        if(dval < (Real)(-tsp::kPi)) {dval += (Real)tsp::kTwoPi;} else if((Real)tsp::kPi < dval) {dval -= (Real)tsp::kTwoPi;}
+        const Real ex = RMath<Real>::exp(inv_s2 * dval * dval);
+        const Real coef = -((Real)2 * inv_s2 * dval * k * ex);
+        if(ENERGY) {e = (double)(k * ex);}
+        const Real ci = coef * r_kj_len * RMath<Real>::rcp(m_lensq);
+        const Real cl = -coef * r_kj_len * RMath<Real>::rcp(n_lensq);
+        const Real Fix = ci * mx, Fiy = ci * my, Fiz = ci * mz;
+        const Real Flx = cl * nx, Fly = cl * ny, Flz = cl * nz;
+        const Real cijk = (ijx * kjx + ijy * kjy + ijz * kjz) * r_kj_rlensq;
+        const Real cjkl = (klx * kjx + kly * kjy + klz * kjz) * r_kj_rlensq;
+        acc3<Real>(F, i0, Fix, Fiy, Fiz);
+        acc3<Real>(F, i1, (cijk - (Real)1) * Fix - cjkl * Flx, (cijk - (Real)1) * Fiy - cjkl * Fly,
+                   (cijk - (Real)1) * Fiz - cjkl * Flz);
+        acc3<Real>(F, i2, (cjkl - (Real)1) * Flx - cijk * Fix, (cjkl - (Real)1) * Fly - cijk * Fiy,
+                   (cjkl - (Real)1) * Flz - cijk * Fiz);
+        acc3<Real>(F, i3, Flx, Fly, Flz);
+    }
+    else if(kind == kRBaseStacking)
+    {
+        // ThreeSPN2BaseStackingInteraction.hpp:281-428: packed indices {Si, Bi, Bj}
+        const double en = rep_base_stack<Real, ENERGY>(i0, i1, i2, (int)p0, box, P, F);
+        if(ENERGY) {e = en;}
+    }
 }
 
 // ------------------------------------------------------------------------------------
 // pair term (i < j): force on i is +f, on j is -f (GlobalPair*Interaction loops); the
 // per-particle parameters come from LDS (`par`, npar columns)
 // ------------------------------------------------------------------------------------
-template<typename Real, bool ENERGY>
+template<typename Real, bool ENERGY, bool TSP>
 __device__ __forceinline__ void rep_pair(const RepGlobal& G, const Real* par, uint32_t i, uint32_t j, const Box& box,
                                          const PosL& P, unsigned long long* F, double& e)
 {
+    if constexpr(TSP)
+    {
+        if(G.kind == kBasePair || G.kind == kCrossStack)
+        {
+            // 3SPN2 base pairing / cross stacking of the list pair (i < j, both bases)
+            const double ep = rep_tsp_pair<Real, ENERGY>(G.kind, i, j, G.ppar, box, P, F);
+            if(ENERGY) {e += ep;}
+            return;
+        }
+    }
     double ddx, ddy, ddz;
     disp(box, P, i, j, ddx, ddy, ddz);
     const Real dx = (Real)ddx, dy = (Real)ddy, dz = (Real)ddz;
@@ -410,6 +533,19 @@ __device__ __forceinline__ void rep_pair(const RepGlobal& G, const Real* par, ui
         const Real s6 = s2l2 * s2l2 * s2l2;
         c = -((Real)G.eps * (Real)12) * s6 * s6 * rcp;
         if(ENERGY) {e += (double)((Real)G.eps * (s6 * s6 - (Real)G.coef_at_cutoff));}
+    }
+    else if(TSP && G.kind == kPair3SPN2EV)
+    {
+        // ThreeSPN2ExcludedVolumePotential.hpp:47-64, sigma of the pair (:198-202)
+        const Real sigma = (Real)((tsp::ev_sigma((int)par[i]) + tsp::ev_sigma((int)par[j])) * 0.5);
+        const Real rl = RMath<Real>::rsqrt(l_sq);
+        const Real l = l_sq * rl;
+        if(sigma <= l) {return;}
+        const Real r1s1 = sigma * rl;
+        const Real r3s3 = r1s1 * r1s1 * r1s1;
+        const Real r6s6 = r3s3 * r3s3;
+        c = (Real)12 * (Real)G.eps * rl * r6s6 * ((Real)1 - r6s6) * rl;
+        if(ENERGY) {e += (double)((Real)G.eps * (r6s6 * (r6s6 - (Real)2) + (Real)1));}
     }
     else
     {
@@ -462,6 +598,17 @@ __device__ __forceinline__ bool rep_allowed(const RepGlobal& G, const RowRules& 
     else {if(a == G.last_slot) {return false;}}
     if(G.ignore_mol == 1 && r.mol == mol[pb]) {return false;}
     if(G.ignore_mol == 2 && r.mol != mol[pb]) {return false;}
+    if(G.base_rule == 1)
+    {
+        // ThreeSPN2ExcludedVolumePotential.hpp:326-364: bases that could pair never interact
+        const int ki = (int)G.ppar[pa], kj = (int)G.ppar[pb];
+        if(ki >= 2 && kj >= 2 && complementary_bases(ki - 2, kj - 2)) {return false;}
+    }
+    else if(G.base_rule == 2)
+    {
+        // ThreeSPN2BasePairPotential.hpp:445-459 / ThreeSPN2CrossStackingPotential.hpp:437-451
+        if(!complementary_bases((int)G.ppar[(size_t)pa * G.npar], (int)G.ppar[(size_t)pb * G.npar])) {return false;}
+    }
     const uint32_t d = pb - pa;
     if(d <= 64) {return ((r.near >> (d - 1)) & 1ull) == 0ull;}
     if(pb == r.far[0] || pb == r.far[1] || pb == r.far[2] || pb == r.far[3]) {return false;}
@@ -658,7 +805,7 @@ __device__ void rep_build_grid(const RepGlobal& G, const RepDesc& R, const PosL&
 // LDS: x[n], y[n], z[n] (double); F[3n] (int64 fixed point); parameters[par_cols][n] as
 // npar-column rows per interaction (Real); participant lists (uint16)
 // ------------------------------------------------------------------------------------
-template<typename Real, bool ENERGY, int BS, int PPT>
+template<typename Real, bool ENERGY, int BS, int PPT, bool TSP>
 __global__ void __launch_bounds__(BS)
 k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
 {
@@ -860,7 +1007,7 @@ k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
             {
                 if(kd[m] == 0) {continue;}
                 double e = 0.0;
-                rep_local<Real, ENERGY>((int)(kd[m] & 0xff), ix[m], p0[m], p1[m], p2[m], R.box, P, F, e);
+                rep_local<Real, ENERGY, TSP>((int)(kd[m] & 0xff), ix[m], p0[m], p1[m], p2[m], R.box, P, F, e);
                 if(ENERGY) {atomicAdd(&s_energy[kd[m] >> 8], e);}
             }
         }
@@ -893,7 +1040,7 @@ k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
                 for(int m = 0; m < kPairBatch; ++m)
                 {
                     if(pk[m] == 0xffffffffu) {continue;}
-                    rep_pair<Real, ENERGY>(G, par, pk[m] >> 16, pk[m] & 0xffffu, R.box, P, F, e_pair);
+                    rep_pair<Real, ENERGY, TSP>(G, par, pk[m] >> 16, pk[m] & 0xffffu, R.box, P, F, e_pair);
                 }
 #pragma unroll
                 for(int m = 0; m < kPairBatch; ++m) {pk[m] = nx[m];}

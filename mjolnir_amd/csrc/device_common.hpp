@@ -23,6 +23,12 @@ enum LocalKind : int
 enum PairKind : int {kPairLJ = 1, kPairEV = 2, kPairDH = 3, kPair3SPN2EV = 4, kBasePair = 5, kCrossStack = 6};
 constexpr uint32_t kNil = 0xffffffffu;   // absent index (a 3SPN2 nucleotide without phosphate)
 
+// complementary bases A-T, G-C (codes A 0, T 1, G 2, C 3)
+__device__ __forceinline__ bool complementary_bases(int a, int b)
+{
+    return (a == 0 && b == 1) || (a == 1 && b == 0) || (a == 2 && b == 3) || (a == 3 && b == 2);
+}
+
 struct Box
 {
     double lower[3], upper[3], width[3], halfw[3];

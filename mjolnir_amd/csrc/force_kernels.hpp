@@ -243,61 +243,15 @@ __device__ void local_term(const LocalSeg& s, uint32_t t, const double4* __restr
     }
     else if(s.kind == kBaseStacking)
     {
-        // ThreeSPN2BaseStackingInteraction.hpp:281-428 with ThreeSPN2BaseStackingPotential.hpp:
-        // 172-294; indices {Si, Bi, Bj, Pi, Sj, Pj}, force on Si, Bi, Bj (slots 0, 1, 2)
-        const double K = 6.0, piK = tsp::kPi / 6.0, alp = 3.0;
-        const uint32_t Si = ix[0], Bi = ix[1], Bj = ix[2];
-        const int bs = (int)p[0];
-        const double eps = tsp::bs_eps(bs), r0 = tsp::bs_r0(bs), th0 = tsp::bs_theta0(bs);
-        const tsp::V3 Bji = tsp::disp(box, pos, Bj, Bi);
-        const tsp::V3 SBi = tsp::disp(box, pos, Si, Bi);
-        const double lBji_sq = tsp::lsq(Bji);
-        const double rlBji = 1.0 / sqrt(lBji_sq);
-        const double lBji = lBji_sq * rlBji;
-        const tsp::V3 Bji_reg = rlBji * Bji;
-        tsp::V3 f_Bi{0, 0, 0}, f_Bj{0, 0, 0}, f_Si{0, 0, 0};
-        double dU_rep = 0.0;
-        if(!(r0 <= lBji)) {const double term = exp(-alp * (lBji - r0)); dU_rep = 2 * alp * eps * term * (1.0 - term);}
-        if(dU_rep != 0.0) {f_Bi = f_Bi - dU_rep * Bji_reg; f_Bj = f_Bj + dU_rep * Bji_reg;}
-        if(ENERGY && !(r0 <= lBji)) {const double term = 1.0 - exp(-alp * (lBji - r0)); energy += eps * term * term;}
-        const double rlSBi = 1.0 / sqrt(tsp::lsq(SBi));
-        const tsp::V3 SBi_reg = rlSBi * SBi;
-        const double cos_theta = tsp::dot(SBi_reg, Bji_reg);
-        const double theta = acos(fmin(fmax(cos_theta, -1.0), 1.0));
-        const double f_theta = tsp::f_mod(K, piK, theta, th0);
-        if(f_theta != 0.0)
-        {
-            const double df_theta = tsp::df_mod(K, piK, theta, th0);
-            double U_attr, dU_attr;
-            if(lBji <= r0) {U_attr = -eps; dU_attr = 0.0;}
-            else
-            {
-                const double term = exp(-alp * (lBji - r0));
-                U_attr = eps * (term * term - 2 * term);
-                dU_attr = 2 * alp * eps * term * (1.0 - term);
-            }
-            if(ENERGY) {energy += U_attr * f_theta;}
-            if(df_theta != 0.0)
-            {
-                const double coef = -df_theta * U_attr;
-                const double sn = sin(theta);
-                const double coef_rsin = (sn > tsp::kTol) ? coef / sn : coef / tsp::kTol;
-                const tsp::V3 fSi = (coef_rsin * rlSBi) * (Bji_reg - cos_theta * SBi_reg);
-                const tsp::V3 fBj = (coef_rsin * rlBji) * (SBi_reg - cos_theta * Bji_reg);
-                f_Si = f_Si + fSi;
-                f_Bi = f_Bi - (fSi + fBj);
-                f_Bj = f_Bj + fBj;
-            }
-            if(dU_attr != 0.0)
-            {
-                const double coef = f_theta * dU_attr;
-                f_Bi = f_Bi - coef * Bji_reg;
-                f_Bj = f_Bj + coef * Bji_reg;
-            }
-        }
-        store3(tf, slot + 0, f_Si.x, f_Si.y, f_Si.z);
-        store3(tf, slot + 1, f_Bi.x, f_Bi.y, f_Bi.z);
-        store3(tf, slot + 2, f_Bj.x, f_Bj.y, f_Bj.z);
+        // ThreeSPN2BaseStackingInteraction.hpp:281-428; indices {Si, Bi, Bj, Pi, Sj, Pj}, force
+        // on Si, Bi, Bj (slots 0, 1, 2)
+        const tsp::GlobalPos acc{pos, box};
+        tsp::V3<double> fSi, fBi, fBj;
+        const double en = tsp::base_stack<double, ENERGY>(ix[0], ix[1], ix[2], (int)p[0], acc, fSi, fBi, fBj);
+        store3(tf, slot + 0, fSi.x, fSi.y, fSi.z);
+        store3(tf, slot + 1, fBi.x, fBi.y, fBi.z);
+        store3(tf, slot + 2, fBj.x, fBj.y, fBj.z);
+        if(ENERGY) {energy = en;}
     }
     if(ENERGY) {te[s.term_off + t] = energy;}
 }

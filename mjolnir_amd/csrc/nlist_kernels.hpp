@@ -177,11 +177,6 @@ struct ListRule
     const double* base_par;
 };
 
-// complementary bases A-T, G-C (codes A 0, T 1, G 2, C 3)
-__device__ __forceinline__ bool complementary_bases(int a, int b)
-{
-    return (a == 0 && b == 1) || (a == 1 && b == 0) || (a == 2 && b == 3) || (a == 3 && b == 2);
-}
 
 // has_interaction(lo, hi) plus the leading-participant rule, for lo < hi.
 __device__ __forceinline__ bool pair_allowed(uint32_t lo, uint32_t hi, const ListRule& rule,

@@ -12,9 +12,9 @@ Two lanes share the GPU:
   FP64 integrator).  Each segment advances every resident replica in one launch per size class
   and then records every replica's energies in one evaluation launch.  Between segments the
   finished replicas are retired and the waiting jobs admitted.
-* engine lane: larger systems and 3SPN2 DNA run on FP64 engines, one per job and ``workers`` at a
-  time.  Each engine has its own HIP stream and host thread (the library's C calls release the
-  GIL), so their segments overlap on the device with the batch lane's.
+* engine lane: larger systems (and constrained ones) run on FP64 engines, one per job and
+  ``workers`` at a time.  Each engine has its own HIP stream and host thread (the library's C
+  calls release the GIL), so their segments overlap on the device with the batch lane's.
 
 Admission checks the slot counts and a device-memory budget (``hbm_budget`` bytes against an
 estimated footprint per job).  Noise depends only on a job's seed, step and particle index, and
@@ -38,8 +38,10 @@ from .spec import SystemSpec
 
 # interactions the replica kernel covers
 _BATCH_LOCALS = {("BondLength", "Harmonic"), ("BondLength", "GoContact"), ("Contact", "GoContact"),
-                 ("BondAngle", "Harmonic"), ("BondAngle", "Gaussian"), ("DihedralAngle", "ClementiDihedral")}
-_BATCH_GLOBALS = {"LennardJones", "ExcludedVolume", "DebyeHuckel"}
+                 ("BondAngle", "Harmonic"), ("BondAngle", "Gaussian"), ("DihedralAngle", "ClementiDihedral"),
+                 ("BondLength", "3SPN2Bond"), ("DihedralAngle", "Gaussian"), ("3SPN2BaseStacking", "3SPN2")}
+_BATCH_GLOBALS = {"LennardJones", "ExcludedVolume", "DebyeHuckel", "3SPN2ExcludedVolume", "3SPN2BasePair",
+                  "3SPN2CrossStacking"}
 
 # footprint estimate per particle (state, backups, coefficients, lists) and per engine
 _BYTES_PER_PARTICLE = 1024
@@ -71,7 +73,7 @@ class Job:
 
 
 def batch_eligible(spec: SystemSpec, max_n: int = 2048) -> bool:
-    return (spec.n <= max_n and len(spec.local) <= 8 and len(spec.global_) <= 3
+    return (spec.n <= max_n and len(spec.local) <= 8 and len(spec.global_) <= 4 and spec.constraint is None
             and all((l.interaction, l.potential) in _BATCH_LOCALS for l in spec.local)
             and all(g.potential in _BATCH_GLOBALS for g in spec.global_))
 

@@ -189,3 +189,59 @@ def test_periodic_lj_batch_vs_oracle():
         d = np.abs(x - xo)
         d = np.minimum(d, (spec.upper - spec.lower) - d)   # wrapped coordinates
         assert d.max() < 1e-8, d.max()
+
+
+def _dna_specs():
+    from mjolnir_amd import workloads
+    return [make_golden.perturb(workloads.dna_duplex(12, seed=5), 0.2, 11),
+            make_golden.perturb(workloads.dna_duplex(200, seed=3), 0.1, 12)]
+
+
+def test_dna_batch_forces_vs_oracle():
+    """3SPN2 in the replica kernel: FP32 mode against the FP64 oracle at the mixed-precision bar
+    (RMS force error <= 1e-5 of the RMS force, energies 1e-5), FP64 mode at 1e-9 per particle."""
+    import oracle
+    from mjolnir_amd import capi
+    specs = _dna_specs()
+    b32, ids32 = _batch(specs, capi.FP32_FIXED)
+    b64, ids64 = _batch(specs, capi.FP64)
+    for spec, r32, r64 in zip(specs, ids32, ids64):
+        o = oracle.OracleEngine(spec, with_integrator=False)
+        f_ref, _ = o.force_and_energy()
+        e_ref = [o.energy_of(k) for k in range(len(spec.interaction_names()))]
+        per, _, f = b32.evaluate(r32)
+        rms_ref = np.sqrt(np.mean(np.sum(f_ref ** 2, axis=1)))
+        rms_err = np.sqrt(np.mean(np.sum((f - f_ref) ** 2, axis=1)))
+        print(f"[gpu] DNA N={spec.n}: FP32 RMS force error {rms_err / rms_ref:.2e}", flush=True)
+        assert rms_err <= 1e-5 * rms_ref
+        for k, (a, c) in enumerate(zip(per, e_ref)):
+            assert abs(a - c) <= 1e-5 * abs(c) + 1e-4, (spec.interaction_names()[k], a, c)
+        per64, _, f64 = b64.evaluate(r64)
+        err = np.linalg.norm(f64 - f_ref, axis=1)
+        assert np.all(err <= force_tolerance(f_ref, 1e-9)), err.max()
+        for k, (a, c) in enumerate(zip(per64, e_ref)):
+            assert abs(a - c) <= 1e-9 * max(abs(c), 1.0), (spec.interaction_names()[k], a, c)
+
+
+def test_dna_batch_fp64_trajectory_and_mix():
+    """FP64 replica kernel on the 12-bp duplex: 150 zero-temperature G-JF steps against the
+    oracle; and a DNA replica in a batch with chains (two kernel variants in one class) ends
+    bit-identical to the same replica alone."""
+    import oracle
+    from mjolnir_amd import capi, workloads
+    dna = workloads.zero_temperature(_dna_specs()[0])
+    b, (r,) = _batch([dna], capi.FP64)
+    b.step(150)
+    o = oracle.OracleEngine(dna)
+    o.step(150)
+    x, _, _ = b.state(r)
+    xo, _, _ = o.state()
+    assert np.abs(x - xo).max() < 1e-8, np.abs(x - xo).max()
+    target = _dna_specs()[0]
+    b1, (r1,) = _batch([target], capi.FP32_FIXED)
+    b2, ids = _batch(_chains([60, 90], seed0=900) + [target] + _chains([180], seed0=950), capi.FP32_FIXED)
+    b1.step(200)
+    b2.step(200)
+    x1, v1, _ = b1.state(r1)
+    x2, v2, _ = b2.state(ids[2])
+    assert np.array_equal(x1, x2) and np.array_equal(v1, v2)

@@ -31,7 +31,9 @@ def test_lane_choice():
     assert serve.batch_eligible(workloads.cg_chain(n=120, seed=3, max_bend_deg=150.0))
     assert serve.batch_eligible(workloads.lj_fluid(n=1500, seed=3))
     assert not serve.batch_eligible(workloads.lj_fluid(n=3000, seed=3))
-    assert not serve.batch_eligible(workloads.dna_duplex(n_bp=20, seed=3))
+    assert serve.batch_eligible(workloads.dna_duplex(n_bp=20, seed=3))
+    assert not serve.batch_eligible(workloads.dna_duplex(n_bp=400, seed=3))      # 2398 beads
+    assert not serve.batch_eligible(workloads.constrained_chain(60, seed=3))
 
 
 def test_summary():
