@@ -13,7 +13,9 @@ Workloads (seeded synthetic inputs, ``mjolnir_amd.workloads``):
                integrator, one persistent launch per 100-step segment
   chain1000    one CG chain, N = 1000, FP64 engine
   lj4096       LJ fluid, N = 4096, rho sigma^3 = 0.6, FP64 engine
+  lj262144     LJ fluid, N = 262144 (the largest LJ job of the serving stream), FP64 engine
   dna200       3SPN2 double-stranded DNA, 200 bp (1198 beads), G-JF dt = 0.02, FP64 engine
+  dna200x128   128 independent 200-bp duplexes (random sequences) in the replica kernel, FP32-bulk mode
   serve        a seeded Poisson stream (50 jobs/s) of LJ fluids (1e3..2.6e5 particles), CG chains
                (50..5000 beads) and 3SPN2 DNA (20..400 bp), 1e3..1e5 steps each, energy trace every
                100 steps, through the continuous-batching server (``mjolnir_amd.serve``)
@@ -162,7 +164,7 @@ def cpu_baseline_batch(specs, target_s: float = 15.0):
     ps, wall, _ = refrun.bench_many(sample, [steps] * len(sample), procs)
     return {"value": round(ps / wall, 1), "unit": "particle-steps/s", "cores": procs, "kind": "reference",
             "sample": f"{len(sample)} replicas of the ensemble (stratified by length, {tot_n} particles), "
-                      f"{steps} BAOAB steps each on the reference OpenMP build with OMP_NUM_THREADS=1, "
+                      f"{steps} {sample[0].integrator.type} steps each on the reference OpenMP build with OMP_NUM_THREADS=1, "
                       f"{procs} processes at a time, step loops only, {wall:.1f} s wall"}
 
 
@@ -230,6 +232,51 @@ def bench_batch(args):
 
 
 # ------------------------------------------------------------------------------------------
+# 3SPN2 duplex ensemble (config 3 in the FP32-bulk mode, batched)
+# ------------------------------------------------------------------------------------------
+def bench_dna_ensemble(args, n_rep: int = 128):
+    from mjolnir_amd import capi, workloads
+    specs = [workloads.dna_duplex(n_bp=200, seed=100 + k) for k in range(n_rep)]
+    b = capi.Batch(capi.FP32_FIXED)
+    ids = [b.add(s) for s in specs]
+    b.initialize()
+    b.step(args.warmup)
+    ps0, rb0 = b.counters()
+    t0 = time.perf_counter()
+    b.step(args.steps)
+    sec = time.perf_counter() - t0
+    ps1, rb1 = b.counters()
+    value = (ps1 - ps0) / sec
+    ms, ps_launch = b.profile(SEGMENT, reps=3)
+    alg = batch_alg_bytes(b, ids, specs, SEGMENT)
+    achieved = alg / (ms * 1e-3) / 1e9
+    e = capi.Engine(specs[0], capi.FP64)
+    e.step(100)
+    t = time.perf_counter()
+    e.step(500)
+    fp64 = specs[0].n * 500 / (time.perf_counter() - t)
+    e.close()
+    cpu = None if args.no_cpu_baseline else cpu_baseline_batch(specs)
+    return {
+        "metric": "particle-steps/s", "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec * 1e3 / args.steps, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+i64fixed+f64",
+        "data": "synthetic",
+        "config": {"workload": "dna200x128", "replicas": n_rep, "particles": int(sum(s.n for s in specs)),
+                   "model": "3SPN2 duplex, 200 bp, random sequences", "integrator": "GJFNVTLangevin", "dt": 0.02,
+                   "precision": "fp32 pair/local, int64 fixed-point sums, fp64 integrator",
+                   "rebuilds_in_timed_region": int(rb1 - rb0)},
+        "fp64_engine": {"value": round(fp64, 1), "unit": "particle-steps/s",
+                        "sample": "one duplex on the FP64 engine", "speedup_of_batch": round(value / fp64, 2)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                     "kernel": "k_replica_segment<float, false, 1024, 2, true>", "avg_ms": round(ms, 4),
+                     "segment_steps": SEGMENT, "alg_bytes_per_launch": int(alg)},
+        "cpu_baseline": cpu,
+    }
+
+
+# ------------------------------------------------------------------------------------------
 # serving stream
 # ------------------------------------------------------------------------------------------
 def run_serving(n_jobs: int, seed: int = 7):
@@ -264,7 +311,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=500)
-    ap.add_argument("--workload", default="replicas512", choices=["replicas512", "chain1000", "lj4096", "dna200", "serve"])
+    ap.add_argument("--workload", default="replicas512", choices=["replicas512", "chain1000", "lj4096", "lj262144", "dna200", "dna200x128", "serve"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-sample", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=40,
@@ -292,10 +339,13 @@ def main():
         out["serving"] = run_serving(args.serve_jobs) if args.serve_jobs > 0 else None
     elif args.workload == "serve":
         out = bench_serve(args)
+    elif args.workload == "dna200x128":
+        out = bench_dna_ensemble(args)
     else:
         from mjolnir_amd import workloads
         spec = {"chain1000": lambda: workloads.cg_chain(n=1000, seed=2),
                 "lj4096": lambda: workloads.lj_fluid(n_side=16, seed=1),
+                "lj262144": lambda: workloads.lj_fluid(n_side=64, seed=1),
                 "dna200": lambda: workloads.dna_duplex(n_bp=200)}[args.workload]()
         print(f"[bench] workload {spec.name} N={spec.n}", file=sys.stderr, flush=True)
         out = bench_single(args, spec)
