@@ -167,6 +167,10 @@ mjg_status mjg_integrator_set_remover(mjg_engine* e, int translation, int rotati
 /* Integrator::initialize: coefficients from the temperature, then the first calc_force
  * (BAOABLangevinIntegrator.hpp:135-168, GJFNVTLangevinIntegrator.hpp:44-75). */
 mjg_status mjg_integrator_initialize(mjg_engine* e);
+/* Position of the counter-based noise stream: the step count the next step draws its noise at.
+ * Restoring a checkpoint sets it (the reference restores its generator from the state string of
+ * <prefix>_rng.msg, MsgPackSaver.hpp:70-88). */
+mjg_status mjg_set_step_count(mjg_engine* e, uint64_t step);
 /* n_steps calls of Integrator::step (BAOABLangevinIntegrator.hpp:170-240) on the device,
  * including the skin check and list rebuilds; returns after the work is complete. */
 mjg_status mjg_step(mjg_engine* e, size_t n_steps);

@@ -55,7 +55,7 @@ _up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 EXPORTED = [
     "mjg_version", "mjg_engine_create", "mjg_engine_destroy", "mjg_last_error",
     "mjg_system_set_state", "mjg_system_get_state", "mjg_system_set_attribute",
-    "mjg_add_local", "mjg_add_global", "mjg_add_constraint", "mjg_forcefield_initialize", "mjg_num_interactions",
+    "mjg_add_local", "mjg_add_global", "mjg_add_constraint", "mjg_set_step_count", "mjg_forcefield_initialize", "mjg_num_interactions",
     "mjg_calc_force", "mjg_calc_force_and_energy", "mjg_calc_energies", "mjg_calc_force_of",
     "mjg_reduce_margin", "mjg_neighbor_pairs", "mjg_integrator_set", "mjg_integrator_set_remover", "mjg_observer_energy_open",
     "mjg_observer_energy_output", "mjg_simulate", "mjg_integrator_initialize",
@@ -88,6 +88,7 @@ def lib():
     L.mjg_system_set_attribute.argtypes = [vp, C.c_char_p, dbl]
     L.mjg_add_local.argtypes = [vp, i32, C.c_char_p, sz, _up, _dp, dbl, C.POINTER(i32)]
     L.mjg_add_constraint.argtypes = [vp, C.c_char_p, sz, _up, _dp, sz, dbl]
+    L.mjg_set_step_count.argtypes = [vp, C.c_uint64]
     L.mjg_add_global.argtypes = [vp, i32, dbl, dbl, i32, dbl, i32, sz, C.POINTER(C.c_char_p), _up, sz, _up, _dp,
                                  C.POINTER(i32)]
     L.mjg_forcefield_initialize.argtypes = [vp]
@@ -242,6 +243,10 @@ class Engine:
         s, r = C.c_uint64(), C.c_uint64()
         self._check(lib().mjg_counters(self.h, C.byref(s), C.byref(r)))
         return s.value, r.value
+
+    def set_step_count(self, step: int):
+        """Continue the noise stream at `step` (checkpoint restore)."""
+        self._check(lib().mjg_set_step_count(self.h, int(step)))
 
     def kinetic_energy(self) -> float:
         e = C.c_double()

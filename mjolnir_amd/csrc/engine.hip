@@ -1246,6 +1246,15 @@ void Engine::step(size_t n)
     }
 }
 
+// the position of the noise stream (a restored checkpoint continues the stream where it stopped)
+void Engine::set_step_count(uint64_t step)
+{
+    StepCtl sc{};
+    download(&sc, d_sctl_, 1);
+    sc.step = step;
+    upload(d_sctl_, &sc, 1);
+}
+
 void Engine::counters(uint64_t* steps, uint64_t* rebuilds)
 {
     StepCtl sc{};
@@ -1520,6 +1529,10 @@ mjg_status mjg_add_global(mjg_engine* e, int potential, double cutoff_ratio, dou
         // interaction ids follow calc_energy order: locals first, then globals
         if(id) {*id = (int)x.num_locals() + k;}
     });
+}
+mjg_status mjg_set_step_count(mjg_engine* e, uint64_t step)
+{
+    return guarded(e, [&](mjg::Engine& x) {x.set_step_count(step);});
 }
 mjg_status mjg_add_constraint(mjg_engine* e, const char* topology, size_t n, const uint32_t* indices,
                               const double* v0, size_t max_iteration, double tolerance)

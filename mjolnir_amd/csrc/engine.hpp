@@ -112,6 +112,7 @@ class Engine
     void integrator_initialize();
     void step(size_t n);
     void counters(uint64_t* steps, uint64_t* rebuilds);
+    void set_step_count(uint64_t step);
     double kinetic_energy();
     // observers and the simulator loop
     std::string interaction_name(size_t id) const;

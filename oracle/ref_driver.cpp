@@ -19,6 +19,7 @@
 //   ref_driver forces <spec> <out>         per-interaction energies, forces and pair lists
 //   ref_driver run    <spec> <nsteps> <out> [every]   integrate, dump x/v (every `every` steps)
 //   ref_driver bench  <spec> <nsteps>      time the step loop only, print one JSON line
+//   ref_driver save   <spec> <prefix>      MsgPack checkpoint of the initialized system
 #include <mjolnir/core/SimulatorTraits.hpp>
 #include <mjolnir/core/BoundaryCondition.hpp>
 #include <mjolnir/core/System.hpp>
@@ -30,6 +31,7 @@
 #include <mjolnir/core/gBAOABLangevinIntegrator.hpp>
 #include <mjolnir/core/SystemMotionRemover.hpp>
 #include <mjolnir/core/EnergyObserver.hpp>
+#include <mjolnir/core/MsgPackSaver.hpp>
 #include <mjolnir/core/PeriodicGridCellList.hpp>
 #include <mjolnir/core/UnlimitedGridCellList.hpp>
 #include <mjolnir/core/VerletList.hpp>
@@ -654,6 +656,22 @@ static int cmd_forces(const SpecIn& s, const std::string& out)
     return 0;
 }
 
+// save: the system after ForceField::initialize and one calc_force, written by MsgPackSaver
+// (MsgPackSaver.hpp:98-186) to <prefix>_system.msg
+template<typename traitsT>
+static int cmd_save(const SpecIn& s, const std::string& prefix)
+{
+    auto b = build<traitsT>(s);
+    auto& sys = *b.sys;
+    sys.initialize(*(new RandomNumberGenerator<traitsT>(1)));
+    b.ff->initialize(sys);
+    zero_forces(sys);
+    b.ff->calc_force(sys);
+    MsgPackSaver<traitsT> saver(prefix);
+    saver.save(sys);
+    return 0;
+}
+
 template<typename traitsT, typename integT>
 static int run_impl(const SpecIn& s, std::size_t nsteps, const std::string& out, std::size_t every, bool bench)
 {
@@ -779,6 +797,7 @@ static int dispatch(const std::string& cmd, const SpecIn& s, int argc, char** ar
     }
     if(cmd == "bench") {return cmd_run<traits>(s, std::stoul(argv[3]), "", 0, true);}
     if(cmd == "sim") {return cmd_sim<traits>(s, std::stoul(argv[3]), std::stoul(argv[4]), argv[5]);}
+    if(cmd == "save") {return cmd_save<traits>(s, argv[3]);}
     throw std::runtime_error("unknown command " + cmd);
 }
 } // drv

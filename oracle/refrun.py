@@ -128,3 +128,14 @@ def sim(spec, total: int, save: int) -> str:
         subprocess.run([REF_DRIVER, "sim", p, str(total), str(save), prefix], check=True, stdout=subprocess.DEVNULL)
         with open(prefix + ".ene") as f:
             return f.read()
+
+
+def save(spec) -> bytes:
+    """Bytes of the <prefix>_system.msg file the reference's MsgPackSaver writes for the
+    initialized system (forces from one calc_force)."""
+    with tempfile.TemporaryDirectory() as d:
+        p = _write(spec, d)
+        prefix = os.path.join(d, "ckpt")
+        subprocess.run([REF_DRIVER, "save", p, prefix], check=True, stdout=subprocess.DEVNULL)
+        with open(prefix + "_system.msg", "rb") as f:
+            return f.read()

@@ -216,6 +216,26 @@ def make_sim():
     print(f"sim_chain200: {len(text.splitlines())} lines")
 
 
+def checkpoint_cases():
+    """name -> spec factory for the MsgPack checkpoint fixtures (unlimited and periodic boxes)."""
+    return {
+        "ckpt_chain40": lambda: perturb(workloads.cg_chain(n=40, seed=21), 0.1, 21),
+        "ckpt_lj27": lambda: perturb(workloads.lj_fluid(n_side=3, seed=22), 0.2, 22),
+    }
+
+
+def make_checkpoints():
+    """The reference's MsgPackSaver output (<prefix>_system.msg) for each checkpoint case."""
+    import refrun
+    for name, factory in checkpoint_cases().items():
+        spec = factory()
+        data = refrun.save(spec)
+        path = os.path.join(HERE, name + "_system.msg")
+        with open(path, "wb") as f:
+            f.write(data)
+        print(f"{name}: {len(data)} bytes")
+
+
 if __name__ == "__main__":
     import refrun
     if not refrun.available():
@@ -223,6 +243,8 @@ if __name__ == "__main__":
     only = sys.argv[1:]
     if not only or "sim_chain200" in only:
         make_sim()
+    if not only or "checkpoints" in only:
+        make_checkpoints()
     for name, (factory, full, zt) in cases().items():
         if only and name not in only:
             continue
