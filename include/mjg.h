@@ -83,7 +83,7 @@ enum { MJG_PART_CELLLIST = 1, MJG_PART_VERLET = 2, MJG_PART_NAIVE = 3 };
 /* ignore.molecule (IgnoreMolecule.hpp:101-121) */
 enum { MJG_IGNORE_NOTHING = 0, MJG_IGNORE_SELF = 1, MJG_IGNORE_OTHERS = 2 };
 /* integrator.type (read_integrator.hpp) */
-enum { MJG_INTEG_BAOAB = 1, MJG_INTEG_GJF = 2, MJG_INTEG_GBAOAB = 3 };
+enum { MJG_INTEG_BAOAB = 1, MJG_INTEG_GJF = 2, MJG_INTEG_GBAOAB = 3, MJG_INTEG_UNDERDAMPED = 4 };
 
 /* Library version string. */
 const char* mjg_version(void);
@@ -155,9 +155,11 @@ mjg_status mjg_reduce_margin(mjg_engine* e, double dmargin);
  * pairs (2 uint32 each) and the total count to *n_pairs. */
 mjg_status mjg_neighbor_pairs(mjg_engine* e, int id, uint32_t* pairs, size_t capacity, size_t* n_pairs);
 
-/* Integrator: type, delta_t, per-particle gammas (BAOAB, g-BAOAB) or alphas (G-JF), and the
- * seed of the counter-based noise stream (read_integrator.hpp:18-307).  g-BAOAB
- * (gBAOABLangevinIntegrator.hpp:348-433) applies the constraints of mjg_add_constraint. */
+/* Integrator: type, delta_t, per-particle gammas (BAOAB, g-BAOAB, UnderdampedLangevin) or alphas
+ * (G-JF), and the seed of the counter-based noise stream (read_integrator.hpp:18-307).  g-BAOAB
+ * (gBAOABLangevinIntegrator.hpp:348-433) applies the constraints of mjg_add_constraint;
+ * UnderdampedLangevin (UnderdampedLangevinIntegrator.hpp:459-542) keeps the acceleration
+ * f/m + noise of the last force call on the device. */
 mjg_status mjg_integrator_set(mjg_engine* e, int type, double dt, const double* coefs, uint64_t seed);
 /* SystemMotionRemover(translation, rotation, rescale) of the integrator (SystemMotionRemover.hpp:
  * 21-148; TOML integrator.remove.{translation, rotation, rescale}, read_integrator.hpp:18-35):
@@ -169,7 +171,8 @@ mjg_status mjg_integrator_set_remover(mjg_engine* e, int translation, int rotati
 mjg_status mjg_integrator_initialize(mjg_engine* e);
 /* Position of the counter-based noise stream: the step count the next step draws its noise at.
  * Restoring a checkpoint sets it (the reference restores its generator from the state string of
- * <prefix>_rng.msg, MsgPackSaver.hpp:70-88). */
+ * <prefix>_rng.msg, MsgPackSaver.hpp:70-88).  UnderdampedLangevin then redraws the stored
+ * acceleration's noise at that count, as its initialize does after a restart (:404-457). */
 mjg_status mjg_set_step_count(mjg_engine* e, uint64_t step);
 /* n_steps calls of Integrator::step (BAOABLangevinIntegrator.hpp:170-240) on the device,
  * including the skin check and list rebuilds; returns after the work is complete. */

@@ -39,7 +39,7 @@ GLOBAL_CODE = {"LennardJones": 1, "ExcludedVolume": 2, "DebyeHuckel": 3, "3SPN2E
                "3SPN2BasePair": 5, "3SPN2CrossStacking": 6}
 PARTITION_CODE = {"CellList": 1, "VerletList": 2, "Naive": 3}
 IGNORE_CODE = {"Nothing": 0, "Self": 1, "Intra": 1, "Others": 2, "Inter": 2}
-INTEG_CODE = {"BAOABLangevin": 1, "GJFNVTLangevin": 2, "gBAOABLangevin": 3}
+INTEG_CODE = {"BAOABLangevin": 1, "GJFNVTLangevin": 2, "gBAOABLangevin": 3, "UnderdampedLangevin": 4}
 
 
 class MjgError(RuntimeError):

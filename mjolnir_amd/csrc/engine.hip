@@ -801,7 +801,7 @@ void Engine::initialize()
 // then k_finish sums them per particle (and with `kick` applies the closing half-kick of the
 // integrator step).  With `energy`, per-interaction energies land in d_energy_.  only >= 0
 // restricts the call to one interaction id.
-void Engine::forces(double4* f, bool energy, int only, bool kick)
+void Engine::forces(double4* f, bool energy, int only, bool kick, bool advance)
 {
     const int nl = (int)locals_.size();
     const bool run_locals = local_blocks_ > 0 && (only < 0 || only < nl);
@@ -865,6 +865,7 @@ void Engine::forces(double4* f, bool energy, int only, bool kick)
     }
     fa.halfdt = dt_ / 2;
     fa.kick = kick ? 1 : 0;
+    fa.advance = (kick || advance) ? 1 : 0;
     hipLaunchKernelGGL(k_finish, dim3(blocks_for(n_, 256)), dim3(256), 0, stream_, (uint32_t)n_, fa, f, d_vel_,
                        d_rmass_, &d_sctl_->step);
 }
@@ -977,7 +978,7 @@ size_t Engine::neighbor_pairs(int id, uint32_t* pairs, size_t cap)
 
 void Engine::set_integrator(int type, double dt, const double* coefs, uint64_t seed)
 {
-    if(type != MJG_INTEG_BAOAB && type != MJG_INTEG_GJF && type != MJG_INTEG_GBAOAB)
+    if(type != MJG_INTEG_BAOAB && type != MJG_INTEG_GJF && type != MJG_INTEG_GBAOAB && type != MJG_INTEG_UNDERDAMPED)
     {
         throw std::invalid_argument("unknown integrator");
     }
@@ -998,18 +999,25 @@ void Engine::set_remover(int translation, int rotation, int rescale)
     if(graph_exec_) {(void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr;}
 }
 
-// reset_parameters (BAOABLangevinIntegrator.hpp:73-99; GJFNVTLangevinIntegrator.hpp:106-134)
-// and the first force call (:151-166 / :58-73).
+// reset_parameters (BAOABLangevinIntegrator.hpp:73-99; GJFNVTLangevinIntegrator.hpp:106-134;
+// UnderdampedLangevinIntegrator.hpp:320-350) and the first force call (:151-166 / :58-73 /
+// :404-457).
 void Engine::integrator_initialize()
 {
     if(!initialized_) {throw std::logic_error("force field not initialized");}
     if(integ_ == 0) {throw std::logic_error("integrator not set");}
     if(!has_T_) {throw std::out_of_range("Langevin integrator requires the `temperature` attribute");}
     const double kBT = kB_kcal() * T_;
+    const double noise_coef = std::sqrt(2 * kB_kcal() * T_ / dt_);
     std::vector<double> c1(n_), c2(n_), c3(n_);
     for(size_t i = 0; i < n_; ++i)
     {
-        if(integ_ == MJG_INTEG_BAOAB || integ_ == MJG_INTEG_GBAOAB)
+        if(integ_ == MJG_INTEG_UNDERDAMPED)
+        {
+            c1[i] = coefs_[i];
+            c2[i] = noise_coef * std::sqrt(coefs_[i] * (1.0 / mass_[i]));
+        }
+        else if(integ_ == MJG_INTEG_BAOAB || integ_ == MJG_INTEG_GBAOAB)
         {
             // gBAOABLangevinIntegrator::reset_parameter (:82-107) is BAOAB's
             const double gamma_dt = -1 * coefs_[i] * dt_;
@@ -1034,6 +1042,11 @@ void Engine::integrator_initialize()
     {
         hipLaunchKernelGGL(k_copy_double4, dim3(blocks_for(n_, 256)), dim3(256), 0, stream_, (uint32_t)n_, d_pos_,
                            d_old_pos_);
+    }
+    if(integ_ == MJG_INTEG_UNDERDAMPED)
+    {
+        if(!d_accel_) {d_accel_ = dalloc<double4>(n_); d_bak_accel_ = dalloc<double4>(n_);}
+        launch_ul_accel(false);
     }
     check(hipGetLastError(), "integrator_initialize launch");
     sync();
@@ -1101,9 +1114,30 @@ void Engine::setup_constraints()
     ctab_.redm = d_redm;
 }
 
+void Engine::launch_ul_accel(bool kick)
+{
+    hipLaunchKernelGGL(k_ul_accel, dim3(blocks_for(n_, 256)), dim3(256), 0, stream_, (uint32_t)n_, d_vel_, d_accel_,
+                       d_force_, d_rmass_, d_c1_, d_c2_, dt_ / 2, seed_, d_sctl_, kick ? 1 : 0);
+}
+
 void Engine::launch_step()
 {
     const uint32_t nb = blocks_for(n_, 256);
+    if(integ_ == MJG_INTEG_UNDERDAMPED)
+    {
+        // UnderdampedLangevinIntegrator::step (:459-542)
+        hipLaunchKernelGGL(k_ul_a, dim3(nb), dim3(256), 0, stream_, (uint32_t)n_, d_pos_, d_vel_, d_accel_, d_c1_, dt_,
+                           dt_ / 2, dt_ * dt_ / 2, box_, d_sctl_);
+        if(!globals_.empty()) {launch_rebuilds(0);}
+        forces(d_force_, false, -1, false, true);
+        launch_ul_accel(true);
+        if(rm_translation_ || rm_rotation_)
+        {
+            hipLaunchKernelGGL(k_remove_motion, dim3(1), dim3(kRemoverThreads), 0, stream_, (uint32_t)n_, d_pos_,
+                               d_vel_, d_mass_, rm_translation_, rm_rotation_, rm_rescale_);
+        }
+        return;
+    }
     if(integ_ == MJG_INTEG_GBAOAB)
     {
         // gBAOABLangevinIntegrator::step (:348-433)
@@ -1168,6 +1202,11 @@ void Engine::run_segment(size_t n)
         check(hipMemcpyAsync(d_bak_vel_, d_vel_, n_ * sizeof(double4), hipMemcpyDeviceToDevice, stream_), "backup");
         check(hipMemcpyAsync(d_bak_force_, d_force_, n_ * sizeof(double4), hipMemcpyDeviceToDevice, stream_), "backup");
         check(hipMemcpyAsync(d_bak_sctl_, d_sctl_, sizeof(StepCtl), hipMemcpyDeviceToDevice, stream_), "backup");
+        if(integ_ == MJG_INTEG_UNDERDAMPED)
+        {
+            check(hipMemcpyAsync(d_bak_accel_, d_accel_, n_ * sizeof(double4), hipMemcpyDeviceToDevice, stream_),
+                  "backup");
+        }
         if(!globals_.empty())
         {
             check(hipMemcpyAsync(d_bak_lctl_, d_lctl_, ng * sizeof(ListCtl), hipMemcpyDeviceToDevice, stream_), "backup");
@@ -1220,6 +1259,11 @@ void Engine::run_segment(size_t n)
         check(hipMemcpyAsync(d_vel_, d_bak_vel_, n_ * sizeof(double4), hipMemcpyDeviceToDevice, stream_), "restore");
         check(hipMemcpyAsync(d_force_, d_bak_force_, n_ * sizeof(double4), hipMemcpyDeviceToDevice, stream_), "restore");
         check(hipMemcpyAsync(d_sctl_, d_bak_sctl_, sizeof(StepCtl), hipMemcpyDeviceToDevice, stream_), "restore");
+        if(integ_ == MJG_INTEG_UNDERDAMPED)
+        {
+            check(hipMemcpyAsync(d_accel_, d_bak_accel_, n_ * sizeof(double4), hipMemcpyDeviceToDevice, stream_),
+                  "restore");
+        }
         check(hipMemcpyAsync(d_lctl_, d_bak_lctl_, ng * sizeof(ListCtl), hipMemcpyDeviceToDevice, stream_), "restore");
         if(integ_ == MJG_INTEG_GBAOAB)
         {
@@ -1253,6 +1297,13 @@ void Engine::set_step_count(uint64_t step)
     download(&sc, d_sctl_, 1);
     sc.step = step;
     upload(d_sctl_, &sc, 1);
+    if(integ_ready_ && integ_ == MJG_INTEG_UNDERDAMPED)
+    {
+        // the stored acceleration carries the noise of the count it was drawn at
+        launch_ul_accel(false);
+        check(hipGetLastError(), "set_step_count launch");
+        sync();
+    }
 }
 
 void Engine::counters(uint64_t* steps, uint64_t* rebuilds)

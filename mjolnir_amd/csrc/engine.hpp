@@ -134,7 +134,8 @@ class Engine
     ListSegTable list_table();
     PairSegTable pair_table(int only);
     void launch_multibody(const GlobalIA& g, bool energy);
-    void forces(double4* f, bool energy, int only, bool kick);
+    void forces(double4* f, bool energy, int only, bool kick, bool advance = false);
+    void launch_ul_accel(bool kick);
     void launch_step();
     void setup_constraints();
     void run_segment(size_t n);
@@ -181,6 +182,7 @@ class Engine
     uint64_t seed_ = 0;
     std::vector<double> coefs_;
     double *d_c1_ = nullptr, *d_c2_ = nullptr, *d_c3_ = nullptr;
+    double4 *d_accel_ = nullptr, *d_bak_accel_ = nullptr;   // UnderdampedLangevin acceleration
 
     // g-BAOAB constraints
     ConstraintSet con_;

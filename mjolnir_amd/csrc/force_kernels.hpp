@@ -440,7 +440,8 @@ k_pairs(const double4* __restrict__ pos, Box box, PairSegTable tab)
 // Per particle: the local-term slots (in interaction, term order) plus every pair
 // interaction's force; stores the force and, with kick != 0, applies the velocity half-kick
 // v += (dt/2) (1/m) f of BAOAB (BAOABLangevinIntegrator.hpp:222-229) and G-JF
-// (GJFNVTLangevinIntegrator.hpp:222-229, (dt 0.5) rm == (dt/2) rm) and advances the step.
+// (GJFNVTLangevinIntegrator.hpp:222-229, (dt 0.5) rm == (dt/2) rm); with advance != 0 it advances
+// the step count.
 struct FinishArgs
 {
     const uint32_t* gstart;
@@ -449,7 +450,8 @@ struct FinishArgs
     const double4* fg[kMaxPairSegs];
     int ng;
     double halfdt;
-    int kick;
+    int kick;       // apply the half-kick
+    int advance;    // advance the step count (set with kick, or alone)
 };
 
 __global__ void __launch_bounds__(256)
@@ -483,7 +485,7 @@ k_finish(uint32_t n, FinishArgs a, double4* __restrict__ force, double4* __restr
             vel[i] = v;
         }
     }
-    if(a.kick && i == 0) {*step += 1;}
+    if(a.advance && i == 0) {*step += 1;}
 }
 
 __global__ void k_zero_doubles(double* __restrict__ p, uint32_t n)

@@ -5,8 +5,9 @@
 // largest-displacement reduction; k_margin is ForceField::reduce_margin on every partition
 // (ForceField.hpp:75-92, PeriodicGridCellList.hpp:100-110) decided on the device, so a step
 // never waits for the host; k_kick_b / k_gjf_b are the velocity updates after the force call
-// (BAOABLangevinIntegrator.hpp:222-229, GJFNVTLangevinIntegrator.hpp:222-229).  The noise is
-// the engine's Philox stream indexed by (seed, step, particle).
+// (BAOABLangevinIntegrator.hpp:222-229, GJFNVTLangevinIntegrator.hpp:222-229); k_ul_a /
+// k_ul_accel are UnderdampedLangevin's two per-particle loops.  The noise is the engine's
+// Philox stream indexed by (seed, step, particle).
 #pragma once
 #include "device_common.hpp"
 
@@ -96,6 +97,74 @@ k_gjf_a(uint32_t n, double4* __restrict__ pos, double4* __restrict__ vel, const 
         if(!isfinite(x) || !isfinite(y) || !isfinite(z)) {ctl->nonfinite = 1;}
     }
     block_max_to(d2, ctl, step);
+}
+
+// UnderdampedLangevinIntegrator::step (UnderdampedLangevinIntegrator.hpp:464-488): the drift
+// with the stored acceleration a (force/m + noise of the last force call) and the first
+// velocity update, fused with the largest-displacement reduction.  gamma holds the per-particle
+// friction.
+__global__ void __launch_bounds__(256)
+k_ul_a(uint32_t n, double4* __restrict__ pos, double4* __restrict__ vel, const double4* __restrict__ accel,
+       const double* __restrict__ gamma, double dt, double halfdt, double halfdt2, Box box, StepCtl* __restrict__ ctl)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double d2 = 0.0;
+    const uint64_t step = ctl->step;
+    if(i < n)
+    {
+        double4 p = pos[i], v = vel[i];
+        const double4 a = accel[i];
+        const double g2 = gamma[i] * halfdt;
+        const double omg = 1 - g2;
+        const double c = dt * omg;
+        const double dpx = c * v.x + halfdt2 * a.x;
+        const double dpy = c * v.y + halfdt2 * a.y;
+        const double dpz = c * v.z + halfdt2 * a.z;
+        double x = p.x + dpx, y = p.y + dpy, z = p.z + dpz;
+        adjust_position(box, x, y, z);
+        p.x = x; p.y = y; p.z = z;
+        const double sv = omg * (omg * omg + g2);
+        v.x *= sv; v.y *= sv; v.z *= sv;
+        const double h = halfdt * omg;
+        v.x += h * a.x; v.y += h * a.y; v.z += h * a.z;
+        pos[i] = p; vel[i] = v;
+        d2 = length_sq(dpx, dpy, dpz);
+        if(!isfinite(x) || !isfinite(y) || !isfinite(z)) {ctl->nonfinite = 1;}
+    }
+    block_max_to(d2, ctl, step);
+}
+
+// a = force/m + noise (UnderdampedLangevinIntegrator.hpp:439-446 at initialize, :514-523 after the
+// force call) with the noise of the current step count, and with `kick` the closing velocity
+// update v += (dt/2)(1 - gamma dt/2) a.  After a step's force call the count has already
+// advanced, so step s draws the noise of count s + 1; initialize at count s draws the same, and a
+// run restarted at s continues the same stream.  noise holds sqrt(2 kB T / dt) sqrt(gamma / m).
+__global__ void __launch_bounds__(256)
+k_ul_accel(uint32_t n, double4* __restrict__ vel, double4* __restrict__ accel, const double4* __restrict__ force,
+           const double* __restrict__ rmass, const double* __restrict__ gamma, const double* __restrict__ noise,
+           double halfdt, uint64_t seed, const StepCtl* __restrict__ ctl, int kick)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i < n)
+    {
+        double Rx, Ry, Rz;
+        noise3(seed, ctl->step, i, Rx, Ry, Rz);
+        const double rm = rmass[i], nc = noise[i];
+        const double4 f = force[i];
+        double4 a;
+        a.x = f.x * rm + nc * Rx;
+        a.y = f.y * rm + nc * Ry;
+        a.z = f.z * rm + nc * Rz;
+        a.w = 0.0;
+        accel[i] = a;
+        if(kick)
+        {
+            const double k = halfdt * (1 - gamma[i] * halfdt);
+            double4 v = vel[i];
+            v.x += k * a.x; v.y += k * a.y; v.z += k * a.z;
+            vel[i] = v;
+        }
+    }
 }
 
 // The velocity half-kick after the force call is part of k_finish (force_kernels.hpp), and

@@ -151,9 +151,9 @@ class ConstraintSpec:
 
 @dataclass
 class IntegratorSpec:
-    type: str                  # "BAOABLangevin", "GJFNVTLangevin" or "gBAOABLangevin"
+    type: str                  # "BAOABLangevin", "GJFNVTLangevin", "gBAOABLangevin", "UnderdampedLangevin"
     dt: float
-    coefs: np.ndarray          # gammas (BAOAB) or alphas (G-JF), one per particle
+    coefs: np.ndarray          # gammas (BAOAB, g-BAOAB, Underdamped) or alphas (G-JF), one per particle
     seed: int = 12345
     # SystemMotionRemover flags remove.{translation, rotation, rescale} (read_integrator.hpp:18-35)
     remove: Tuple[bool, bool, bool] = (False, False, False)

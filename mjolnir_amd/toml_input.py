@@ -11,7 +11,8 @@ Supported:
   rejected, not converted);
 * ``[simulator]`` type "MolecularDynamics", boundary_type "Unlimited" / "Periodic" /
   "PeriodicCuboid", delta_t, total_step, save_step, seed, integrator.type "BAOABLangevin" /
-  "g-BAOABLangevin" (gammas or parameters) and "G-JFLangevin" (alphas or parameters),
+  "g-BAOABLangevin" / "UnderdampedLangevin" (gammas or parameters) and "G-JFLangevin" (alphas or
+  parameters),
   integrator.remove.{translation, rotation, rescale} (read_simulator.hpp, read_integrator.hpp);
 * ``[[systems]]``: attributes, boundary_shape.{lower, upper}, particles with mass/m,
   position/pos, velocity/vel (read_system.hpp:100-299);
@@ -58,14 +59,20 @@ def _load_toml(path: str) -> Dict[str, Any]:
         return tomli.load(f)
 
 
-def _merge_include(table: Dict[str, Any], input_path: str) -> Dict[str, Any]:
-    """Merge the keys of ``include = "file"`` (or a list of files) into ``table``."""
+def _merge_include(table: Dict[str, Any], input_path: str, main_dir: str) -> Dict[str, Any]:
+    """Merge the keys of ``include = "file"`` (or a list of files) into ``table``.  The file is
+    ``input.path + name`` relative to the working directory, as the reference opens it
+    (utility.hpp:286-307); a relative path that does not exist there is tried relative to the
+    main input file's directory."""
     if "include" not in table:
         return table
     out = {k: v for k, v in table.items() if k != "include"}
     inc = table["include"]
     for name in ([inc] if isinstance(inc, str) else inc):
-        sub = _load_toml(os.path.join(input_path, name))
+        fname = os.path.join(input_path, name)
+        if not os.path.isabs(fname) and not os.path.exists(fname):
+            fname = os.path.join(main_dir, fname)
+        sub = _load_toml(fname)
         for k, v in sub.items():
             if k in out:
                 raise ValueError(f"key {k!r} defined both in a table and in its include {name}")
@@ -252,12 +259,12 @@ def _read_constraint(t: Dict[str, Any]) -> ConstraintSpec:
 def load(path: str) -> SimulatorInput:
     root = _load_toml(path)
     files = root.get("files", {})
-    input_path = os.path.join(os.path.dirname(os.path.abspath(path)), files.get("input", {}).get("path", "./")) \
-        if not os.path.isabs(files.get("input", {}).get("path", "./")) else files["input"]["path"]
+    input_path = files.get("input", {}).get("path", "./")
+    main_dir = os.path.dirname(os.path.abspath(path))
     units = root.get("units", {})
     if units.get("length", "angstrom") not in ("angstrom", "Å") or units.get("energy", "kcal/mol") != "kcal/mol":
         raise ValueError("only [units] length = \"angstrom\", energy = \"kcal/mol\" are supported")
-    sim = _merge_include(root["simulator"], input_path)
+    sim = _merge_include(root["simulator"], input_path, main_dir)
     if sim.get("type", "MolecularDynamics") != "MolecularDynamics":
         raise ValueError(f"simulator type {sim.get('type')!r} is not supported")
     btype = sim["boundary_type"]
@@ -266,12 +273,12 @@ def load(path: str) -> SimulatorInput:
     systems = root["systems"]
     if len(systems) != 1:
         raise ValueError("exactly one [[systems]] table is supported")
-    mass, pos, vel, attrs, lower, upper = _read_system(_merge_include(systems[0], input_path), boundary, seed)
+    mass, pos, vel, attrs, lower, upper = _read_system(_merge_include(systems[0], input_path, main_dir), boundary, seed)
     n = len(mass)
     ffs = root["forcefields"]
     if len(ffs) != 1:
         raise ValueError("exactly one [[forcefields]] table is supported")
-    ff = _merge_include(ffs[0], input_path)
+    ff = _merge_include(ffs[0], input_path, main_dir)
     local = [_read_local(t) for t in ff.get("local", [])]
     global_ = [_read_global(t, n) for t in ff.get("global", [])]
     constraints = ff.get("constraint", [])
@@ -281,7 +288,7 @@ def load(path: str) -> SimulatorInput:
     ienv = integ.get("env", {})
     itype = integ["type"]
     name = {"BAOABLangevin": "BAOABLangevin", "g-BAOABLangevin": "gBAOABLangevin",
-            "G-JFLangevin": "GJFNVTLangevin"}.get(itype)
+            "G-JFLangevin": "GJFNVTLangevin", "UnderdampedLangevin": "UnderdampedLangevin"}.get(itype)
     if name is None:
         raise ValueError(f"integrator {itype!r} is not supported")
     if name == "GJFNVTLangevin":
@@ -354,7 +361,7 @@ def dump(spec: SystemSpec, total_step: int, save_step: int, path: str, prefix: s
          f"delta_t = {_f(spec.integrator.dt)}", f"total_step = {total_step}", f"save_step = {save_step}",
          f"seed = {int(spec.integrator.seed)}"]
     itype = {"BAOABLangevin": "BAOABLangevin", "gBAOABLangevin": "g-BAOABLangevin",
-             "GJFNVTLangevin": "G-JFLangevin"}[spec.integrator.type]
+             "GJFNVTLangevin": "G-JFLangevin", "UnderdampedLangevin": "UnderdampedLangevin"}[spec.integrator.type]
     L.append(f'integrator.type = "{itype}"')
     if any(spec.integrator.remove):
         t, r, c = (str(bool(x)).lower() for x in spec.integrator.remove)

@@ -161,7 +161,9 @@ struct orc_engine {
     /* integrator */
     int integ; double dt, halfdt; uint64_t seed; uint64_t step;
     int rm_translation, rm_rotation, rm_rescale;   /* SystemMotionRemover flags */
-    double *coef, *c1, *c2, *c3; /* BAOAB: exp(-g dt), noise; GJF: b, beta, a */
+    double *coef, *c1, *c2, *c3; /* BAOAB: exp(-g dt), noise; GJF: b, beta, a;
+                                    UnderdampedLangevin: -, noise_coef sqrt(gamma / m) */
+    double* accel;                 /* UnderdampedLangevin: f/m + noise of the last force call */
     size_t rebuilds;
     /* constraints (g-BAOAB) */
     size_t n_con; uint32_t* con_idx; double* con_v0; int con_kind; size_t con_maxit; double con_tol;
@@ -251,7 +253,7 @@ void orc_destroy(orc_engine* e)
     for(size_t i = 0; i < e->n; ++i) free(e->adjs[i].a);
     free(e->adjs); free(e->mol);
     free(e->mass); free(e->rmass); free(e->pos); free(e->vel); free(e->force);
-    free(e->coef); free(e->c1); free(e->c2); free(e->c3);
+    free(e->coef); free(e->c1); free(e->c2); free(e->c3); free(e->accel);
     free(e->con_idx); free(e->con_v0); free(e->con_sq_v0); free(e->con_r_sq_v0); free(e->con_redm);
     free(e->old_pos); free(e->old_rattle);
     free(e);
@@ -2171,7 +2173,8 @@ int orc_reduce_margin(orc_engine* e, double dmargin)
 /* ------------------------------------------------------------------------------------- */
 int orc_set_integrator(orc_engine* e, int type, double dt, const double* coefs, uint64_t seed)
 {
-    if(type != ORC_INTEG_BAOAB && type != ORC_INTEG_GJF && type != ORC_INTEG_GBAOAB) return -1;
+    if(type != ORC_INTEG_BAOAB && type != ORC_INTEG_GJF && type != ORC_INTEG_GBAOAB &&
+       type != ORC_INTEG_UNDERDAMPED) return -1;
     e->integ = type; e->dt = dt; e->halfdt = dt / 2; e->seed = seed; e->step = 0;
     free(e->coef); free(e->c1); free(e->c2); free(e->c3);
     e->coef = (double*)malloc(e->n * sizeof(double) + 1);
@@ -2180,6 +2183,26 @@ int orc_set_integrator(orc_engine* e, int type, double dt, const double* coefs, 
     e->c3 = (double*)malloc(e->n * sizeof(double) + 1);
     memcpy(e->coef, coefs, e->n * sizeof(double));
     return 0;
+}
+
+/* UnderdampedLangevinIntegrator (:459-542): a = f/m + noise (counter `ctr`), and with `kick`
+ * the closing half-kick v += (dt/2)(1 - gamma dt/2) a */
+static void ul_accel(orc_engine* e, uint64_t ctr, int kick)
+{
+    const double h = e->halfdt;
+    for(size_t i = 0; i < e->n; ++i)
+    {
+        double R[3];
+        orc_noise3(e->seed, ctr, (uint32_t)i, R);
+        const double rm = e->rmass[i], nc = e->c2[i];
+        double* a = e->accel + 3 * i;
+        for(int c = 0; c < 3; ++c) a[c] = e->force[3 * i + c] * rm + nc * R[c];
+        if(kick)
+        {
+            const double k = h * (1 - e->coef[i] * h);
+            for(int c = 0; c < 3; ++c) e->vel[3 * i + c] += k * a[c];
+        }
+    }
 }
 
 /* BAOABLangevinIntegrator::initialize/reset_parameters (:73-99,136-168) and
@@ -2197,6 +2220,13 @@ int orc_integrator_initialize(orc_engine* e)
             e->c1[i] = exp(gamma_dt);
             e->c2[i] = sqrt(kBT * (1 - exp(2 * gamma_dt)) * e->rmass[i]);
         }
+        else if(e->integ == ORC_INTEG_UNDERDAMPED)
+        {
+            /* UnderdampedLangevinIntegrator::update (:320-350): noise_coef sqrt(gamma / m) */
+            const double noise_coef = sqrt(2 * orc_kB() * e->temperature / e->dt);
+            e->c1[i] = 0.0;
+            e->c2[i] = noise_coef * sqrt(e->coef[i] * e->rmass[i]);
+        }
         else
         {
             const double alpha = e->coef[i], m = e->mass[i];
@@ -2207,6 +2237,14 @@ int orc_integrator_initialize(orc_engine* e)
     }
     memset(e->force, 0, 3 * e->n * sizeof(double));
     calc_force_impl(e, 0);
+    if(e->integ == ORC_INTEG_UNDERDAMPED)
+    {
+        /* UnderdampedLangevinIntegrator::initialize (:404-457): a = f/m + noise, the noise of
+         * the current step count */
+        free(e->accel);
+        e->accel = (double*)malloc(3 * e->n * sizeof(double) + 1);
+        ul_accel(e, e->step, 0);
+    }
     if(e->integ == ORC_INTEG_GBAOAB)
     {
         /* gBAOABLangevinIntegrator::initialize (:290-345), one correction step per A step */
@@ -2415,6 +2453,35 @@ static void gjf_step(orc_engine* e)
     }
 }
 
+static void ul_step(orc_engine* e)
+{
+    /* UnderdampedLangevinIntegrator::step (:459-542); the noise of the closing acceleration
+     * is the one of step count + 1, so a restart at step s (initialize at s) continues the
+     * same stream */
+    double largest = 0;
+    const double dt = e->dt, h = e->halfdt, h2 = e->dt * e->dt / 2;
+    for(size_t i = 0; i < e->n; ++i)
+    {
+        double* p = e->pos + 3 * i; double* v = e->vel + 3 * i; double* f = e->force + 3 * i;
+        const double* a = e->accel + 3 * i;
+        const double g2 = e->coef[i] * h;
+        const double omg = 1 - g2;
+        double dp[3];
+        for(int c = 0; c < 3; ++c) dp[c] = (dt * omg) * v[c] + h2 * a[c];
+        double np[3] = {p[0] + dp[0], p[1] + dp[1], p[2] + dp[2]};
+        adjust_position(e, np);
+        p[0] = np[0]; p[1] = np[1]; p[2] = np[2];
+        const double sv = omg * (omg * omg + g2);
+        for(int c = 0; c < 3; ++c) v[c] *= sv;
+        for(int c = 0; c < 3; ++c) v[c] += (h * omg) * a[c];
+        f[0] = f[1] = f[2] = 0;
+        largest = fmax(largest, length_sq(dp));
+    }
+    orc_reduce_margin(e, 2 * sqrt(largest));
+    calc_force_impl(e, 0);
+    ul_accel(e, e->step + 1, 1);
+}
+
 /* SystemMotionRemover::remove (SystemMotionRemover.hpp:42-148): sequential sums in particle
  * order; the inertia tensor is inverted by math::determinant / math::inverse
  * (math/Matrix.hpp:340-375). */
@@ -2523,6 +2590,7 @@ int orc_step(orc_engine* e, size_t nsteps)
         if(e->integ == ORC_INTEG_BAOAB) baoab_step(e);
         else if(e->integ == ORC_INTEG_GJF) gjf_step(e);
         else if(e->integ == ORC_INTEG_GBAOAB) gbaoab_step(e);
+        else if(e->integ == ORC_INTEG_UNDERDAMPED) ul_step(e);
         else return -1;
         if(e->rm_translation || e->rm_rotation) remove_motion(e);
         if(e->integ == ORC_INTEG_GBAOAB) memcpy(e->old_pos, e->pos, 3 * e->n * sizeof(double));

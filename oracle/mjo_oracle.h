@@ -47,7 +47,7 @@ enum {
 enum { ORC_PART_CELLLIST = 1, ORC_PART_VERLET = 2, ORC_PART_NAIVE = 3 };
 #define ORC_NIL 0xffffffffu   /* absent index (a nucleotide without phosphate, a strand end) */
 enum { ORC_IGNORE_NOTHING = 0, ORC_IGNORE_SELF = 1, ORC_IGNORE_OTHERS = 2 };
-enum { ORC_INTEG_BAOAB = 1, ORC_INTEG_GJF = 2, ORC_INTEG_GBAOAB = 3 };
+enum { ORC_INTEG_BAOAB = 1, ORC_INTEG_GJF = 2, ORC_INTEG_GBAOAB = 3, ORC_INTEG_UNDERDAMPED = 4 };
 
 typedef struct orc_engine orc_engine;
 

@@ -28,7 +28,7 @@ GLOBAL_CODE = {"LennardJones": 1, "ExcludedVolume": 2, "DebyeHuckel": 3, "3SPN2E
                "3SPN2BasePair": 5, "3SPN2CrossStacking": 6}
 PARTITION_CODE = {"CellList": 1, "VerletList": 2, "Naive": 3}
 IGNORE_CODE = {"Nothing": 0, "Self": 1, "Intra": 1, "Others": 2, "Inter": 2}
-INTEG_CODE = {"BAOABLangevin": 1, "GJFNVTLangevin": 2, "gBAOABLangevin": 3}
+INTEG_CODE = {"BAOABLangevin": 1, "GJFNVTLangevin": 2, "gBAOABLangevin": 3, "UnderdampedLangevin": 4}
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
@@ -77,6 +77,11 @@ def lib():
         L.orc_rebuild_count.argtypes = [C.c_void_p]
         _LIB = L
     return _LIB
+
+
+def kB() -> float:
+    """Boltzmann's constant in kcal/mol/K after read_units (read_units.hpp:32-92)."""
+    return float(lib().orc_kB())
 
 
 def philox4x32_10(ctr, key):

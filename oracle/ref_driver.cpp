@@ -12,8 +12,9 @@
 // BondLength / BondAngle / DihedralAngle / Contact interactions
 // (mjolnir/input/read_local_interaction.hpp), a ConstraintForceField
 // (mjolnir/input/read_constraint.hpp), and BAOABLangevinIntegrator / gBAOABLangevinIntegrator /
-// GJFNVTLangevinIntegrator (mjolnir/input/read_integrator.hpp).  Units are set exactly as
-// read_units does for "kcal/mol" and "angstrom" (mjolnir/input/read_units.hpp:32-92).
+// GJFNVTLangevinIntegrator / UnderdampedLangevinIntegrator (mjolnir/input/read_integrator.hpp).
+// Units are set exactly as read_units does for "kcal/mol" and "angstrom"
+// (mjolnir/input/read_units.hpp:32-92).
 //
 // Commands:
 //   ref_driver forces <spec> <out>         per-interaction energies, forces and pair lists
@@ -28,6 +29,7 @@
 #include <mjolnir/core/RandomNumberGenerator.hpp>
 #include <mjolnir/core/BAOABLangevinIntegrator.hpp>
 #include <mjolnir/core/GJFNVTLangevinIntegrator.hpp>
+#include <mjolnir/core/UnderdampedLangevinIntegrator.hpp>
 #include <mjolnir/core/gBAOABLangevinIntegrator.hpp>
 #include <mjolnir/core/SystemMotionRemover.hpp>
 #include <mjolnir/core/EnergyObserver.hpp>
@@ -755,6 +757,10 @@ static int cmd_sim(const SpecIn& s, std::size_t total, std::size_t save, const s
     if(s.integrator == "BAOABLangevin") {return sim_impl<traitsT, BAOABLangevinIntegrator<traitsT>>(s, total, save, prefix);}
     if(s.integrator == "GJFNVTLangevin") {return sim_impl<traitsT, GJFNVTLangevinIntegrator<traitsT>>(s, total, save, prefix);}
     if(s.integrator == "gBAOABLangevin") {return sim_impl<traitsT, gBAOABLangevinIntegrator<traitsT>>(s, total, save, prefix);}
+    if(s.integrator == "UnderdampedLangevin")
+    {
+        return sim_impl<traitsT, UnderdampedLangevinIntegrator<traitsT>>(s, total, save, prefix);
+    }
     throw std::runtime_error("unsupported integrator " + s.integrator);
 }
 
@@ -773,6 +779,10 @@ static int cmd_run(const SpecIn& s, std::size_t nsteps, const std::string& out, 
     if(s.integrator == "gBAOABLangevin")
     {
         return run_impl<traitsT, gBAOABLangevinIntegrator<traitsT>>(s, nsteps, out, every, bench);
+    }
+    if(s.integrator == "UnderdampedLangevin")
+    {
+        return run_impl<traitsT, UnderdampedLangevinIntegrator<traitsT>>(s, nsteps, out, every, bench);
     }
     throw std::runtime_error("unsupported integrator " + s.integrator);
 }

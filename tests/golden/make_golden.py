@@ -137,7 +137,19 @@ def cases():
         "dna12": (lambda: perturb(workloads.dna_duplex(12, seed=5), 0.2, 11), True, 1000),
         "dna200": (lambda: perturb(workloads.dna_duplex(200, seed=3), 0.1, 12), False, 1000),
         "chain100_gbaoab": (lambda: perturb(workloads.constrained_chain(100, seed=5), 0.1, 13), True, 1000),
+        "lj512_ul": (lambda: underdamped(perturb(workloads.lj_fluid(n_side=8, seed=4), 0.2, 14), 1.0), True, 1000),
+        "chain200_ul": (lambda: underdamped(perturb(workloads.cg_chain(n=200, seed=6), 0.3, 15), 0.5), True, 1000),
     }
+
+
+def underdamped(spec: SystemSpec, gamma: float) -> SystemSpec:
+    """The same system under UnderdampedLangevin with one friction for every particle (the
+    reference's Lennard-Jones example uses gamma = 1)."""
+    it = spec.integrator
+    spec.integrator = IntegratorSpec("UnderdampedLangevin", it.dt, np.full(spec.n, gamma), seed=it.seed,
+                                     remove=it.remove)
+    spec.name = spec.name + "_ul"
+    return spec
 
 
 def chain_with_remover() -> SystemSpec:

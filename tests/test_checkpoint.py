@@ -71,3 +71,28 @@ def test_engine_checkpoint_restart_continues_trajectory(tmp_path):
     # changes the order of a few force sums
     assert np.abs(xa - xb).max() < 1e-8, np.abs(xa - xb).max()
     assert np.abs(va - vb).max() < 1e-6, np.abs(va - vb).max()
+
+
+@pytest.mark.gpu
+def test_underdamped_checkpoint_restart_continues_the_noise_stream(tmp_path):
+    """UnderdampedLangevin keeps an acceleration that carries the noise of the last force call.
+    A restored engine redraws it at the saved step count (as the reference's initialize does
+    after loading), which is the draw the uninterrupted run made there, so the two runs continue
+    together."""
+    from mjolnir_amd import capi, workloads
+    spec = make_golden.underdamped(make_golden.perturb(workloads.lj_fluid(n_side=6, seed=3), 0.1, 3), 1.0)
+    a = capi.Engine(spec)
+    print("[gpu] checkpoint restart, underdamped LJ N=216", flush=True)
+    a.step(300)
+    prefix = str(tmp_path / "ul")
+    checkpoint.save_engine(a, prefix)
+    b = checkpoint.load_engine(spec, prefix)
+    a.step(100)
+    b.step(100)
+    xa, va, _ = a.state()
+    xb, vb, _ = b.state()
+    w = spec.upper - spec.lower
+    d = xa - xb
+    d = d - w * np.round(d / w)
+    assert np.abs(d).max() < 1e-8, np.abs(d).max()
+    assert np.abs(va - vb).max() < 1e-6, np.abs(va - vb).max()

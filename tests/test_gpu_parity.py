@@ -144,6 +144,35 @@ def test_gbaoab_trajectory_vs_oracle_same_noise():
     assert np.abs(np.sum(d * d, axis=1) - c.v0 ** 2).max() <= c.tolerance
 
 
+@pytest.mark.parametrize("remove", [(False, False, False), (True, True, False)], ids=["plain", "remover"])
+def test_underdamped_trajectory_vs_oracle_same_noise(remove):
+    """UnderdampedLangevin at T > 0 (the reference's Lennard-Jones example integrator, gamma 1):
+    the engine's drift / acceleration kernels against the oracle's loops, same noise stream, with
+    list rebuilds in the periodic box; and a chain with the motion remover."""
+    import oracle
+    from mjolnir_amd import workloads
+    if remove[0]:
+        spec = make_golden.underdamped(make_golden.perturb(workloads.cg_chain(n=150, seed=8), 0.2, 5), 0.5)
+        spec.integrator.remove = remove
+    else:
+        spec = make_golden.underdamped(make_golden.perturb(workloads.lj_fluid(n_side=8, seed=6), 0.1, 6), 1.0)
+    print(f"[gpu] underdamped same-noise {spec.name} N={spec.n}", flush=True)
+    e = _engine(spec)
+    o = oracle.OracleEngine(spec)
+    for _ in range(4):
+        e.step(50)
+        o.step(50)
+    x, v, _ = e.state()
+    xo, vo, _ = o.state()
+    d = x - xo
+    if spec.boundary == "periodic":
+        w = spec.upper - spec.lower
+        d = d - w * np.round(d / w)
+    assert np.abs(d).max() < 1e-7
+    assert np.abs(v - vo).max() < 1e-7
+    assert e.counters()[0] == 200
+
+
 def test_kinetic_temperature_lj():
     """<KE> = 3/2 N kB T for the thermostatted LJ fluid (equipartition)."""
     from mjolnir_amd import workloads

@@ -296,3 +296,50 @@ def test_gbaoab_without_constraints_is_baoab():
     assert np.abs(xa - xb).max() < 1e-9
     assert np.abs(va - vb).max() < 1e-9
     assert not np.array_equal(xa, xb)     # the step really is the g-BAOAB one
+
+
+# ----------------------------------------------------------------------------------------
+# UnderdampedLangevin (UnderdampedLangevinIntegrator.hpp)
+# ----------------------------------------------------------------------------------------
+def test_underdamped_free_particles_velocity_variance():
+    """Free particles: a = noise with variance s2 = 2 kB T gamma / (m dt) per component, and one
+    step is v' = q v + c a + c a' (q = o (o^2 + g), o = 1 - g, g = gamma dt/2, c = (dt/2) o),
+    an ARMA(1,1) process whose stationary variance is 2 c^2 s2 / (1 - q).  A noise entering once,
+    or with a wrong amplitude, misses it by far more than the 1.5 % sampling bar."""
+    n, m, gamma, dt, T = 20000, 2.0, 0.3, 0.05, 300.0
+    spec = SystemSpec("unlimited", np.full(n, m), np.random.default_rng(1).normal(size=(n, 3)) * 100,
+                      np.zeros((n, 3)), attributes={"temperature": T},
+                      integrator=IntegratorSpec("UnderdampedLangevin", dt, np.full(n, gamma), seed=9))
+    o = oracle.OracleEngine(spec)
+    g = gamma * dt / 2
+    om = 1 - g
+    q = om * (om * om + g)
+    c = dt / 2 * om
+    s2 = 2 * oracle.kB() * T * gamma / (m * dt)
+    want = 2 * c * c * s2 / (1 - q)
+    o.step(400)
+    got = []
+    for _ in range(5):
+        o.step(60)
+        got.append(np.mean(o.state()[1] ** 2))
+    assert np.mean(got) == pytest.approx(want, rel=0.015)
+    assert want == pytest.approx(oracle.kB() * T / m, rel=0.02)   # equipartition to O(gamma dt)
+
+
+def test_underdamped_without_friction_is_velocity_verlet():
+    """gamma = 0, T = 0: UnderdampedLangevin moves x by dt v + dt^2/2 a and kicks v by dt/2 a
+    twice -- velocity Verlet, as BAOAB with gamma = 0; the two agree to rounding."""
+    import copy
+    spec = workloads.zero_temperature(make_golden.perturb(workloads.cg_chain(n=120, seed=4), 0.1, 3))
+    spec.integrator.coefs[:] = 0.0
+    a = oracle.OracleEngine(spec)
+    s2 = copy.deepcopy(spec)
+    s2.integrator.type = "UnderdampedLangevin"
+    b = oracle.OracleEngine(s2)
+    a.step(200)
+    b.step(200)
+    xa, va, _ = a.state()
+    xb, vb, _ = b.state()
+    assert np.abs(xa - xb).max() < 1e-9
+    assert np.abs(va - vb).max() < 1e-9
+    assert not np.array_equal(xa, xb)

@@ -43,7 +43,8 @@ def _same(a, b):
     lambda: workloads.dna_duplex(12, seed=5),
     lambda: workloads.constrained_chain(30, seed=3),
     lambda: make_golden.chain_with_remover(),
-], ids=["chain", "lj", "dna", "constrained", "remover"])
+    lambda: make_golden.cases()["lj512_ul"][0](),
+], ids=["chain", "lj", "dna", "constrained", "remover", "underdamped"])
 def test_dump_load_round_trip(tmp_path, make):
     spec = make()
     p = str(tmp_path / "in.toml")
@@ -81,6 +82,36 @@ def test_include_env_offset(tmp_path):
             '[[forcefields]]\ninclude = "ff.toml"\n'
     (tmp_path / "main.toml").write_text(main)
     _same(spec, toml_input.load(str(tmp_path / "main.toml")).spec)
+
+
+REF_LJ = "/root/reference/input/lennard-jones/lennard-jones.toml"
+
+
+@pytest.mark.skipif(not os.path.exists(REF_LJ), reason="the reference's example inputs are not present")
+def test_reference_lennard_jones_example(monkeypatch):
+    """The reference's own example input (four files joined by `include`, input.path relative to
+    the directory it is run from) reads as written: 4096 particles in a 32 A periodic box at
+    300 K, LJ sigma 2 / epsilon 1 on a cell list with margin 0.25, UnderdampedLangevin with
+    gamma 1, dt 0.01 and seed 2374."""
+    monkeypatch.chdir(os.path.dirname(os.path.dirname(os.path.dirname(REF_LJ))))
+    inp = toml_input.load("input/lennard-jones/lennard-jones.toml")
+    s = inp.spec
+    assert (inp.total_step, inp.save_step) == (10000, 100)
+    assert inp.output_prefix == os.path.join("./", "lennard-jones")
+    assert s.n == 4096 and s.boundary == "periodic"
+    assert np.array_equal(s.lower, [0, 0, 0]) and np.array_equal(s.upper, [32, 32, 32])
+    assert s.attributes["temperature"] == 300.0
+    assert np.all(s.mass == 1.0)
+    assert np.array_equal(s.pos[0], [1.0, 1.0, 1.0]) and np.array_equal(s.vel[0], [0.5101, 0.7734, -2.0929])
+    assert np.array_equal(s.pos[1], [3.0, 1.0, 1.0])
+    it = s.integrator
+    assert (it.type, it.dt, it.seed) == ("UnderdampedLangevin", 0.01, 2374)
+    assert np.all(it.coefs == 1.0) and len(it.coefs) == 4096
+    assert len(s.local) == 0 and len(s.global_) == 1
+    g = s.global_[0]
+    assert (g.potential, g.partition, g.margin, g.ignore_molecule) == ("LennardJones", "CellList", 0.25, "Nothing")
+    assert np.array_equal(g.participants, np.arange(4096))
+    assert np.all(g.params == [2.0, 1.0])
 
 
 def test_rejects_what_it_does_not_run(tmp_path):
