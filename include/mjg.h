@@ -190,9 +190,20 @@ mjg_status mjg_observer_energy_open(mjg_engine* e, const char* prefix);
  * the kinetic energy, the attributes) in the reference's fixed-width format.  A non-finite
  * energy gives MJG_NAN_DETECTED after the line is written. */
 mjg_status mjg_observer_energy_output(mjg_engine* e, uint64_t step);
-/* MolecularDynamicsSimulator::run + finalize (MolecularDynamicsSimulator.hpp:84-123): take
- * total_step integrator steps; when an energy observer is open, output at every multiple of
- * save_step and after the last step. */
+/* XYZObserver (XYZObserver.hpp:24-107; output.format = "xyz", read_observer.hpp:28-34): create
+ * (truncate) prefix_position.xyz and prefix_velocity.xyz; mjg_simulate appends one frame per
+ * output ("N", "step = S", then "name x y z" per particle, fixed with 8 decimals).  names: one
+ * string per particle, or NULL for "X" (read_system.hpp:277). */
+mjg_status mjg_observer_xyz_open(mjg_engine* e, const char* prefix, const char* const* names);
+/* DCDObserver (DCDObserver.hpp:27-341; output.format = "dcd", read_observer.hpp:35-41): create
+ * (truncate) prefix_position.dcd and prefix_velocity.dcd; mjg_simulate writes the header
+ * (save interval, total steps, number of molecules, dt, unit-cell flag), one float32 frame per
+ * output (with the unit-cell record in the position file of a periodic box), and the frame
+ * count at the end.  The 80-byte title record carries this library's own text. */
+mjg_status mjg_observer_dcd_open(mjg_engine* e, const char* prefix);
+/* MolecularDynamicsSimulator::initialize/run/finalize (MolecularDynamicsSimulator.hpp:72-123):
+ * take total_step integrator steps; the open observers (XYZ, DCD, then energy, the order of
+ * read_observer.hpp:84-98) output at every multiple of save_step and after the last step. */
 mjg_status mjg_simulate(mjg_engine* e, uint64_t total_step, uint64_t save_step);
 
 /* Timing hook with no counterpart in the reference (it has no GPU kernels to time): launch

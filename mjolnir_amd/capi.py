@@ -58,7 +58,7 @@ EXPORTED = [
     "mjg_add_local", "mjg_add_global", "mjg_add_constraint", "mjg_set_step_count", "mjg_forcefield_initialize", "mjg_num_interactions",
     "mjg_calc_force", "mjg_calc_force_and_energy", "mjg_calc_energies", "mjg_calc_force_of",
     "mjg_reduce_margin", "mjg_neighbor_pairs", "mjg_integrator_set", "mjg_integrator_set_remover", "mjg_observer_energy_open",
-    "mjg_observer_energy_output", "mjg_simulate", "mjg_integrator_initialize",
+    "mjg_observer_energy_output", "mjg_observer_xyz_open", "mjg_observer_dcd_open", "mjg_simulate", "mjg_integrator_initialize",
     "mjg_step", "mjg_counters", "mjg_kinetic_energy", "mjg_profile_interaction",
     "mjg_batch_create", "mjg_batch_destroy", "mjg_batch_last_error", "mjg_batch_add_replica",
     "mjg_batch_set_attribute", "mjg_batch_add_local", "mjg_batch_add_global", "mjg_batch_set_integrator",
@@ -104,6 +104,8 @@ def lib():
     L.mjg_integrator_set_remover.argtypes = [vp, i32, i32, i32]
     L.mjg_observer_energy_open.argtypes = [vp, C.c_char_p]
     L.mjg_observer_energy_output.argtypes = [vp, u64]
+    L.mjg_observer_xyz_open.argtypes = [vp, C.c_char_p, C.POINTER(C.c_char_p)]
+    L.mjg_observer_dcd_open.argtypes = [vp, C.c_char_p]
     L.mjg_simulate.argtypes = [vp, u64, u64]
     L.mjg_integrator_initialize.argtypes = [vp]
     L.mjg_step.argtypes = [vp, sz]
@@ -265,6 +267,17 @@ class Engine:
 
     def output_energy(self, step: int):
         self._check(lib().mjg_observer_energy_output(self.h, int(step)))
+
+    def open_xyz_observer(self, prefix: str, names=None):
+        arr = None
+        if names is not None:
+            if len(names) != self.n:
+                raise ValueError("one name per particle")
+            arr = (C.c_char_p * self.n)(*[str(x).encode() for x in names])
+        self._check(lib().mjg_observer_xyz_open(self.h, prefix.encode(), arr))
+
+    def open_dcd_observer(self, prefix: str):
+        self._check(lib().mjg_observer_dcd_open(self.h, prefix.encode()))
 
     def simulate(self, total_step: int, save_step: int):
         self._check(lib().mjg_simulate(self.h, int(total_step), int(save_step)))

@@ -144,7 +144,7 @@ def save_engine(engine, prefix: str):
     periodic = spec.boundary == "periodic"
     with open(prefix + "_system.msg", "wb") as fh:
         fh.write(encode_system(spec.mass, x, v, f, spec.lower if periodic else None, spec.upper if periodic else None,
-                               spec.attributes))
+                               spec.attributes, names=spec.particle_names))
     steps, _ = engine.counters()
     with open(prefix + "_rng.msg", "wb") as fh:
         fh.write(encode_rng(spec.integrator.seed, steps))

@@ -1,5 +1,6 @@
 // Host side of the engine (see engine.hpp) and the C ABI declared in include/mjg.h.
 #include "engine.hpp"
+#include "observers.hpp"
 #include "../../include/mjg.h"
 
 #include <algorithm>
@@ -359,6 +360,7 @@ void Engine::write_topology()
         }
         if(mol_[i] == UNINIT) {mol_[i] = next++;}
     }
+    n_molecules_ = next;
 }
 
 // ExclusionList::make (ExclusionList.hpp:86-234): per particle, itself plus every particle
@@ -1400,25 +1402,30 @@ bool Engine::energy_observer_output(uint64_t step)
     return ok;
 }
 
-// MolecularDynamicsSimulator::run + finalize (MolecularDynamicsSimulator.hpp:84-123): output at
-// every step that is a multiple of save_step, then after the last step; the steps between
-// outputs run on the device in one call.
+// MolecularDynamicsSimulator::initialize/run/finalize (MolecularDynamicsSimulator.hpp:72-123):
+// the observers' initialize, output at every step that is a multiple of save_step and after the
+// last step, then finalize; the steps between outputs run on the device in one call.  The
+// trajectory observers write before the energy observer, which read_observer appends last
+// (read_observer.hpp:84-98), so a step whose energy is not finite still gets its frame.
 void Engine::simulate(uint64_t total_step, uint64_t save_step)
 {
     if(!integ_ready_) {throw std::logic_error("integrator not initialized");}
     if(save_step == 0) {throw std::invalid_argument("save_step must be positive");}
+    if(!dcd_prefix_.empty()) {dcd_begin(total_step, save_step);}
+    auto output = [&](uint64_t step) {
+        trajectory_output(step);
+        if(!ene_path_.empty() && !energy_observer_output(step)) {throw NonFinite("Energy value becomes NaN");}
+    };
     uint64_t done = 0;
     while(done < total_step)
     {
-        if(done % save_step == 0 && !ene_path_.empty())
-        {
-            if(!energy_observer_output(done)) {throw NonFinite("Energy value becomes NaN");}
-        }
+        if(done % save_step == 0) {output(done);}
         const uint64_t next = std::min<uint64_t>(total_step, (done / save_step + 1) * save_step);
         step((size_t)(next - done));
         done = next;
     }
-    if(!ene_path_.empty() && !energy_observer_output(total_step)) {throw NonFinite("Energy value becomes NaN");}
+    output(total_step);
+    if(!dcd_prefix_.empty()) {dcd_end();}
 }
 
 double Engine::kinetic_energy()
@@ -1634,6 +1641,16 @@ mjg_status mjg_observer_energy_output(mjg_engine* e, uint64_t step)
     return guarded(e, [&](mjg::Engine& x) {
         if(!x.energy_observer_output(step)) {throw mjg::NonFinite("Energy value becomes NaN");}
     });
+}
+mjg_status mjg_observer_xyz_open(mjg_engine* e, const char* prefix, const char* const* names)
+{
+    if(!prefix) {return MJG_INVALID_ARG;}
+    return guarded(e, [&](mjg::Engine& x) {x.xyz_observer_open(prefix, names);});
+}
+mjg_status mjg_observer_dcd_open(mjg_engine* e, const char* prefix)
+{
+    if(!prefix) {return MJG_INVALID_ARG;}
+    return guarded(e, [&](mjg::Engine& x) {x.dcd_observer_open(prefix);});
 }
 mjg_status mjg_simulate(mjg_engine* e, uint64_t total_step, uint64_t save_step)
 {

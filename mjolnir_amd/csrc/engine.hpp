@@ -118,6 +118,8 @@ class Engine
     std::string interaction_name(size_t id) const;
     void energy_observer_open(const std::string& prefix);
     bool energy_observer_output(uint64_t step);
+    void xyz_observer_open(const std::string& prefix, const char* const* names);
+    void dcd_observer_open(const std::string& prefix);
     void simulate(uint64_t total_step, uint64_t save_step);
     double profile_interaction(int id, int reps, uint64_t* half_pairs);
 
@@ -155,6 +157,13 @@ class Engine
     bool has_T_ = false, has_I_ = false, initialized_ = false, integ_ready_ = false, state_set_ = false;
     double T_ = 0, I_ = 0;
     std::string ene_path_;          // open energy observer ("" if none)
+    std::string xyz_prefix_, dcd_prefix_;   // open trajectory observers ("" if none)
+    std::vector<std::string> names_;        // particle names of the XYZ frames
+    uint64_t dcd_frames_ = 0;
+    uint32_t n_molecules_ = 0;              // Topology::number_of_molecules
+    void trajectory_output(uint64_t step);
+    void dcd_begin(uint64_t total_step, uint64_t save_step);
+    void dcd_end();
 
     double4 *d_pos_ = nullptr, *d_vel_ = nullptr, *d_force_ = nullptr, *d_force_tmp_ = nullptr;
     double4 *d_bak_pos_ = nullptr, *d_bak_vel_ = nullptr, *d_bak_force_ = nullptr;

@@ -176,6 +176,7 @@ class SystemSpec:
     integrator: Optional[IntegratorSpec] = None
     constraint: Optional[ConstraintSpec] = None
     name: str = "system"
+    particle_names: Optional[List[str]] = None   # System::name(i); None: "X" for every particle
 
     def __post_init__(self):
         self.mass = np.ascontiguousarray(self.mass, dtype=np.float64).reshape(-1)

@@ -34,7 +34,7 @@ from __future__ import annotations
 import argparse
 import copy
 import os
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 from typing import Any, Dict, List, Optional
 
 import numpy as np
@@ -51,6 +51,7 @@ class SimulatorInput:
     total_step: int
     save_step: int
     output_prefix: str           # path + prefix of the output files
+    output_formats: List[str] = field(default_factory=list)   # trajectory observers: "xyz", "dcd"
 
 
 def _load_toml(path: str) -> Dict[str, Any]:
@@ -134,7 +135,9 @@ def _read_system(sys_t: Dict[str, Any], boundary: str, seed: int):
     if boundary == "periodic":
         b = sys_t["boundary_shape"]
         lower, upper = np.array(b["lower"], dtype=np.float64), np.array(b["upper"], dtype=np.float64)
-    return mass, pos, vel, attrs, lower, upper
+    # name defaults to "X" (read_system.hpp:277)
+    names = [str(p.get("name", "X")) for p in parts] if any("name" in p for p in parts) else None
+    return mass, pos, vel, attrs, lower, upper, names
 
 
 # ------------------------------------------------------------------------------------------
@@ -273,7 +276,8 @@ def load(path: str) -> SimulatorInput:
     systems = root["systems"]
     if len(systems) != 1:
         raise ValueError("exactly one [[systems]] table is supported")
-    mass, pos, vel, attrs, lower, upper = _read_system(_merge_include(systems[0], input_path, main_dir), boundary, seed)
+    mass, pos, vel, attrs, lower, upper, names = _read_system(_merge_include(systems[0], input_path, main_dir),
+                                                              boundary, seed)
     n = len(mass)
     ffs = root["forcefields"]
     if len(ffs) != 1:
@@ -302,10 +306,16 @@ def load(path: str) -> SimulatorInput:
     spec = SystemSpec(boundary=boundary, mass=mass, pos=pos, vel=vel, lower=lower, upper=upper, attributes=attrs,
                       local=local, global_=global_, integrator=it,
                       constraint=_read_constraint(constraints[0]) if constraints else None,
-                      name=os.path.splitext(os.path.basename(path))[0])
+                      name=os.path.splitext(os.path.basename(path))[0], particle_names=names)
     out = files.get("output", {})
     prefix = os.path.join(out.get("path", "./"), out.get("prefix", spec.name))
-    return SimulatorInput(spec, int(sim["total_step"]), int(sim["save_step"]), prefix)
+    # output.format: one name or a list (read_observer.hpp:61-101); the energy file is always written
+    fmt = out.get("format", [])
+    formats = [fmt] if isinstance(fmt, str) else list(fmt)
+    for f in formats:
+        if f not in ("xyz", "dcd"):
+            raise ValueError(f"output.format {f!r} is not supported (xyz, dcd)")
+    return SimulatorInput(spec, int(sim["total_step"]), int(sim["save_step"]), prefix, formats)
 
 
 # ------------------------------------------------------------------------------------------
@@ -352,10 +362,11 @@ def _nuc_rows(nucs):
     return rows
 
 
-def dump(spec: SystemSpec, total_step: int, save_step: int, path: str, prefix: str = "out"):
+def dump(spec: SystemSpec, total_step: int, save_step: int, path: str, prefix: str = "out", formats=("xyz",)):
     """Write ``spec`` as one input file in the reference's schema (no includes)."""
     inv_bead = {v: k for k, v in _BEAD.items()}
-    L = ["[files]", f'output.prefix = "{prefix}"', 'output.path = "./"', 'output.format = "xyz"', "",
+    fmt = ", ".join(f'"{f}"' for f in formats)
+    L = ["[files]", f'output.prefix = "{prefix}"', 'output.path = "./"', f"output.format = [{fmt}]", "",
          "[units]", 'length = "angstrom"', 'energy = "kcal/mol"', "", "[simulator]", 'type = "MolecularDynamics"',
          f'boundary_type = "{"PeriodicCuboid" if spec.boundary == "periodic" else "Unlimited"}"', 'precision = "double"',
          f"delta_t = {_f(spec.integrator.dt)}", f"total_step = {total_step}", f"save_step = {save_step}",
@@ -376,9 +387,10 @@ def dump(spec: SystemSpec, total_step: int, save_step: int, path: str, prefix: s
         L.append("boundary_shape.lower = [" + ", ".join(_f(x) for x in spec.lower) + "]")
         L.append("boundary_shape.upper = [" + ", ".join(_f(x) for x in spec.upper) + "]")
     L.append("particles = [")
-    for m, x, v in zip(spec.mass, spec.pos, spec.vel):
+    for i, (m, x, v) in enumerate(zip(spec.mass, spec.pos, spec.vel)):
+        nm = f', name = "{spec.particle_names[i]}"' if spec.particle_names is not None else ""
         L.append(f"{{mass = {_f(m)}, position = [{', '.join(_f(a) for a in x)}], "
-                 f"velocity = [{', '.join(_f(a) for a in v)}]}},")
+                 f"velocity = [{', '.join(_f(a) for a in v)}]{nm}}},")
     L += ["]", "", "[[forcefields]]"]
     nucs = None
     for l in spec.local:
@@ -434,12 +446,17 @@ def dump(spec: SystemSpec, total_step: int, save_step: int, path: str, prefix: s
 
 
 def run(path: str, total_step: Optional[int] = None) -> str:
-    """Run an input file on the FP64 engine: MolecularDynamicsSimulator with the energy observer
-    (prefix.ene); returns the path of the .ene file."""
+    """Run an input file on the FP64 engine: MolecularDynamicsSimulator with the observers of
+    output.format (prefix_{position,velocity}.xyz / .dcd) and the energy observer (prefix.ene);
+    returns the path of the .ene file."""
     from . import capi
     inp = load(path)
     e = capi.Engine(inp.spec)
     os.makedirs(os.path.dirname(os.path.abspath(inp.output_prefix)), exist_ok=True)
+    if "xyz" in inp.output_formats:
+        e.open_xyz_observer(inp.output_prefix, inp.spec.particle_names)
+    if "dcd" in inp.output_formats:
+        e.open_dcd_observer(inp.output_prefix)
     e.open_energy_observer(inp.output_prefix)
     e.simulate(inp.total_step if total_step is None else total_step, inp.save_step)
     e.close()

@@ -21,6 +21,8 @@
 //   ref_driver run    <spec> <nsteps> <out> [every]   integrate, dump x/v (every `every` steps)
 //   ref_driver bench  <spec> <nsteps>      time the step loop only, print one JSON line
 //   ref_driver save   <spec> <prefix>      MsgPack checkpoint of the initialized system
+//   ref_driver sim    <spec> <total> <save> <prefix>   simulator loop with the energy, XYZ and
+//                                          DCD observers (prefix.ene, prefix_position.xyz, ...)
 #include <mjolnir/core/SimulatorTraits.hpp>
 #include <mjolnir/core/BoundaryCondition.hpp>
 #include <mjolnir/core/System.hpp>
@@ -33,6 +35,8 @@
 #include <mjolnir/core/gBAOABLangevinIntegrator.hpp>
 #include <mjolnir/core/SystemMotionRemover.hpp>
 #include <mjolnir/core/EnergyObserver.hpp>
+#include <mjolnir/core/XYZObserver.hpp>
+#include <mjolnir/core/DCDObserver.hpp>
 #include <mjolnir/core/MsgPackSaver.hpp>
 #include <mjolnir/core/PeriodicGridCellList.hpp>
 #include <mjolnir/core/UnlimitedGridCellList.hpp>
@@ -726,7 +730,8 @@ static int run_impl(const SpecIn& s, std::size_t nsteps, const std::string& out,
 }
 
 // MolecularDynamicsSimulator::initialize/step/finalize (MolecularDynamicsSimulator.hpp:72-123)
-// with only the EnergyObserver attached: prefix.ene after `total` steps, output every `save`
+// with the EnergyObserver, XYZObserver and DCDObserver attached in that order: prefix.ene,
+// prefix_{position,velocity}.{xyz,dcd} after `total` steps, output every `save`
 template<typename traitsT, typename integT>
 static int sim_impl(const SpecIn& s, std::size_t total, std::size_t save, const std::string& prefix)
 {
@@ -739,15 +744,19 @@ static int sim_impl(const SpecIn& s, std::size_t total, std::size_t save, const 
     std::vector<real_type> coefs(s.coefs.begin(), s.coefs.end());
     integT integ(s.dt, std::move(coefs), SystemMotionRemover<traitsT>(s.rm_translation, s.rm_rotation, s.rm_rescale));
     integ.initialize(sys, b.ff, rng);
-    EnergyObserver<traitsT> obs(prefix);
-    obs.initialize(total, save, integ.delta_t(), sys, b.ff);
+    std::vector<std::unique_ptr<ObserverBase<traitsT>>> obs;
+    obs.emplace_back(new EnergyObserver<traitsT>(prefix));
+    obs.emplace_back(new XYZObserver<traitsT>(prefix));
+    obs.emplace_back(new DCDObserver<traitsT>(prefix));
+    for(auto& o : obs) {o->initialize(total, save, integ.delta_t(), sys, b.ff);}
     real_type t = 0;
     for(std::size_t step = 0; step < total; ++step)
     {
-        if(step % save == 0) {obs.output(step, integ.delta_t(), sys, b.ff);}
+        if(step % save == 0) {for(auto& o : obs) {o->output(step, integ.delta_t(), sys, b.ff);}}
         t = integ.step(t, sys, b.ff, rng);
     }
-    obs.output(total, integ.delta_t(), sys, b.ff);
+    for(auto& o : obs) {o->output(total, integ.delta_t(), sys, b.ff);}
+    for(auto& o : obs) {o->finalize(total, integ.delta_t(), sys, b.ff);}
     return 0;
 }
 

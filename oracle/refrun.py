@@ -122,12 +122,25 @@ def bench_many(specs, nsteps, procs: int, timeout: float = 600):
 
 def sim(spec, total: int, save: int) -> str:
     """Text of the .ene file the reference's EnergyObserver writes over a `total`-step run."""
+    return sim_files(spec, total, save)[".ene"]
+
+
+def sim_files(spec, total: int, save: int) -> dict:
+    """Every file the reference's observers write over a `total`-step run (output every `save`):
+    ".ene" (text), "_position.xyz", "_velocity.xyz" (text), "_position.dcd", "_velocity.dcd"
+    (bytes)."""
+    out = {}
     with tempfile.TemporaryDirectory() as d:
         p = _write(spec, d)
         prefix = os.path.join(d, "sim")
         subprocess.run([REF_DRIVER, "sim", p, str(total), str(save), prefix], check=True, stdout=subprocess.DEVNULL)
-        with open(prefix + ".ene") as f:
-            return f.read()
+        for suffix in (".ene", "_position.xyz", "_velocity.xyz"):
+            with open(prefix + suffix) as f:
+                out[suffix] = f.read()
+        for suffix in ("_position.dcd", "_velocity.dcd"):
+            with open(prefix + suffix, "rb") as f:
+                out[suffix] = f.read()
+    return out
 
 
 def save(spec) -> bytes:

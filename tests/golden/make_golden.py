@@ -216,16 +216,26 @@ def make(name: str, factory, full_pairs: bool, zt_steps: int):
     print(f"{name}: {len(names)} interactions, {os.path.getsize(path) / 1024:.1f} KiB")
 
 
+def sim_cases():
+    """fixture name -> (case of cases(), total steps, save step): zero-temperature simulator runs
+    with the reference's EnergyObserver, XYZObserver and DCDObserver attached."""
+    return {"sim_chain200": ("chain200", 300, 100), "sim_lj512_ul": ("lj512_ul", 300, 100)}
+
+
 def make_sim():
-    """The reference's EnergyObserver file for a 300-step zero-temperature run of chain200,
-    output every 100 steps (MolecularDynamicsSimulator cadence)."""
+    """The files the reference's observers write (MolecularDynamicsSimulator cadence: output at
+    every multiple of save and after the last step): .ene and XYZ as text, DCD as bytes."""
     import refrun
-    spec = workloads.zero_temperature(cases()["chain200"][0]())
-    text = refrun.sim(spec, 300, 100)
-    path = os.path.join(HERE, "sim_chain200.npz")
-    np.savez_compressed(path, spec_hash=np.array(spec_hash(spec)), ene=np.array(text), total=np.array(300),
-                        save=np.array(100))
-    print(f"sim_chain200: {len(text.splitlines())} lines")
+    for name, (case, total, save) in sim_cases().items():
+        spec = workloads.zero_temperature(cases()[case][0]())
+        f = refrun.sim_files(spec, total, save)
+        path = os.path.join(HERE, name + ".npz")
+        np.savez_compressed(path, spec_hash=np.array(spec_hash(spec)), ene=np.array(f[".ene"]), total=np.array(total),
+                            save=np.array(save), xyz_pos=np.array(f["_position.xyz"]),
+                            xyz_vel=np.array(f["_velocity.xyz"]),
+                            dcd_pos=np.frombuffer(f["_position.dcd"], dtype=np.uint8),
+                            dcd_vel=np.frombuffer(f["_velocity.dcd"], dtype=np.uint8))
+        print(f"{name}: {len(f['.ene'].splitlines())} energy lines, {len(f['_position.dcd'])} DCD bytes")
 
 
 def checkpoint_cases():
@@ -253,7 +263,7 @@ if __name__ == "__main__":
     if not refrun.available():
         sys.exit("oracle/_ref/ref_driver is missing: run `make -C oracle ref` where the reference sources exist")
     only = sys.argv[1:]
-    if not only or "sim_chain200" in only:
+    if not only or "sim" in only:
         make_sim()
     if not only or "checkpoints" in only:
         make_checkpoints()
