@@ -203,7 +203,7 @@ def bench_batch(args):
     print(f"[bench] timed {args.steps} steps in {sec:.3f} s", file=sys.stderr, flush=True)
 
     # dominant kernel: the segment kernel (all size classes of one launch), HIP events
-    ms, ps_launch = b.profile(SEGMENT, reps=5)
+    ms, ps_launch = b.profile(SEGMENT, reps=20)
     alg = batch_alg_bytes(b, ids, specs, SEGMENT)
     achieved = alg / (ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -191,10 +191,49 @@ __device__ __forceinline__ void add3(unsigned long long* F, uint32_t i, long lon
     atomicAdd(&F[3 * i + 1], (unsigned long long)y);
     atomicAdd(&F[3 * i + 2], (unsigned long long)z);
 }
+
+// The three components of a force in fixed point.  FP32: when every active lane's components
+// are below 2^31 units the 32-bit conversion serves them all, else every lane takes the FP64
+// route; both give the same integers, so the choice is made once per wave (one uniform branch)
+// instead of per component and lane.
+template<typename Real>
+__device__ __forceinline__ void fix3r(Real x, Real y, Real z, long long& a, long long& b, long long& c);
+template<>
+__device__ __forceinline__ void fix3r<double>(double x, double y, double z, long long& a, long long& b, long long& c)
+{
+    a = Fix<double>::conv(x); b = Fix<double>::conv(y); c = Fix<double>::conv(z);
+}
+template<>
+__device__ __forceinline__ void fix3r<float>(float x, float y, float z, long long& a, long long& b, long long& c)
+{
+    const float sx = x * 16777216.0f, sy = y * 16777216.0f, sz = z * 16777216.0f;
+    const bool small = fmaxf(fabsf(sx), fmaxf(fabsf(sy), fabsf(sz))) < 2147483520.0f;
+    if(__all(small))
+    {
+        a = (long long)__float2int_rn(sx); b = (long long)__float2int_rn(sy); c = (long long)__float2int_rn(sz);
+    }
+    else
+    {
+        a = Fix<float>::conv((double)x); b = Fix<float>::conv((double)y); c = Fix<float>::conv((double)z);
+    }
+}
 template<typename Real, typename V>
 __device__ __forceinline__ void acc3(unsigned long long* F, uint32_t i, V x, V y, V z)
 {
-    add3(F, i, Fix<Real>::conv(x), Fix<Real>::conv(y), Fix<Real>::conv(z));
+    long long a, b, c;
+    fix3r<Real>((Real)x, (Real)y, (Real)z, a, b, c);
+    add3(F, i, a, b, c);
+}
+
+// A wave-uniform value held in a scalar register for a whole loop: read back through
+// readfirstlane, so the compiler keeps (or spills) the register instead of re-loading the value
+// from memory inside the loop, where the scalar load's wait would also drain the LDS queue.
+__device__ __forceinline__ float uniform(float v) {return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));}
+__device__ __forceinline__ double uniform(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffff)), hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 template<typename Real>
 __device__ __forceinline__ double from_fix(unsigned long long v)
@@ -327,7 +366,8 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         const Real v0r_10 = v0r_6 * v0r_2 * v0r_2;
         const Real v0r_12 = v0r_10 * v0r_2;
         const Real coef = (Real)(-60) * k * r2 * (v0r_10 - v0r_12);
-        const long long fx = Fix<Real>::conv(coef * dx), fy = Fix<Real>::conv(coef * dy), fz = Fix<Real>::conv(coef * dz);
+        long long fx, fy, fz;
+        fix3r<Real>(coef * dx, coef * dy, coef * dz, fx, fy, fz);
         add3(F, i0, -fx, -fy, -fz);
         add3(F, i1, fx, fy, fz);
         if(ENERGY) {e = (double)(k * ((Real)5 * v0r_12 - (Real)6 * v0r_10));}
@@ -369,8 +409,9 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         const Real Fix_ = ci * (cos_t * eix - ekx), Fiy = ci * (cos_t * eiy - eky), Fiz = ci * (cos_t * eiz - ekz);
         const Real Fkx = ck * (cos_t * ekx - eix), Fky = ck * (cos_t * eky - eiy), Fkz = ck * (cos_t * ekz - eiz);
         // the middle body takes minus the sum of the ends, in integers (exact momentum balance)
-        const long long ix = Fix<Real>::conv(Fix_), iy = Fix<Real>::conv(Fiy), iz = Fix<Real>::conv(Fiz);
-        const long long kx = Fix<Real>::conv(Fkx), ky = Fix<Real>::conv(Fky), kz = Fix<Real>::conv(Fkz);
+        long long ix, iy, iz, kx, ky, kz;
+        fix3r<Real>(Fix_, Fiy, Fiz, ix, iy, iz);
+        fix3r<Real>(Fkx, Fky, Fkz, kx, ky, kz);
         add3(F, i0, ix, iy, iz);
         add3(F, i1, -(ix + kx), -(iy + ky), -(iz + kz));
         add3(F, i2, kx, ky, kz);
@@ -559,9 +600,129 @@ __device__ __forceinline__ void rep_pair(const RepGlobal& G, const Real* par, ui
         c = f_mag * rl;
         if(ENERGY) {e += (double)(qq * (ex * rl - (Real)G.coef_at_cutoff));}
     }
-    const long long fx = Fix<Real>::conv(dx * c), fy = Fix<Real>::conv(dy * c), fz = Fix<Real>::conv(dz * c);
+    long long fx, fy, fz;
+    fix3r<Real>(dx * c, dy * c, dz * c, fx, fy, fz);
     add3(F, i, fx, fy, fz);
     add3(F, j, -fx, -fy, -fz);
+}
+
+// Constants of one pair interaction in the arithmetic type, converted once per segment step and
+// held in scalar registers for the pair loop.
+template<typename Real> struct PairK
+{
+    Real cut2, cac, eps, qk, acut, idebye, debye;
+};
+template<typename Real>
+__device__ __forceinline__ PairK<Real> pair_consts(const RepGlobal& G)
+{
+    PairK<Real> k;
+    k.cut2 = uniform((Real)G.cutoff_ratio2);
+    k.cac = uniform((Real)G.coef_at_cutoff);
+    k.eps = uniform((Real)G.eps);
+    k.qk = uniform((Real)G.inv4pieps);
+    k.acut = uniform((Real)G.abs_cutoff);
+    k.idebye = uniform((Real)G.inv_debye);
+    k.debye = uniform((Real)G.debye);
+    return k;
+}
+
+// rep_pair for one interaction kind known at compile time (LJ, EV, DH): the same arithmetic,
+// with the kind's branches and constant loads out of the loop.
+template<typename Real, bool ENERGY, int KIND, bool PERIODIC>
+__device__ __forceinline__ void rep_pair_k(const PairK<Real>& K, const Real* par, uint32_t i, uint32_t j,
+                                           const Box& box, const PosL& P, unsigned long long* F, double& e)
+{
+    double ddx, ddy, ddz;
+    if constexpr(PERIODIC) {disp(box, P, i, j, ddx, ddy, ddz);}
+    else {ddx = P.x[j] - P.x[i]; ddy = P.y[j] - P.y[i]; ddz = P.z[j] - P.z[i];}
+    const Real dx = (Real)ddx, dy = (Real)ddy, dz = (Real)ddz;
+    const Real l_sq = dx * dx + dy * dy + dz * dz;
+    Real c;
+    if constexpr(KIND == kPairLJ)
+    {
+        // LorentzBerthelotRule (ParameterList.hpp:240-252), LennardJonesPotential.hpp:34-63
+        const Real s1 = par[2 * i], e1 = par[2 * i + 1], s2 = par[2 * j], e2 = par[2 * j + 1];
+        const Real sigma = (s1 + s2) / (Real)2;
+        const Real eps = (e1 == e2) ? e1 : RMath<Real>::sqrt(e1 * e2);
+        const Real sigma_sq = sigma * sigma;
+        if(sigma_sq * K.cut2 < l_sq) {return;}
+        const Real rcp = RMath<Real>::rcp(l_sq);
+        const Real s2l2 = sigma_sq * rcp;
+        const Real s6 = s2l2 * s2l2 * s2l2;
+        c = (Real)24 * eps * (s6 - (Real)2 * s6 * s6) * rcp;
+        if(ENERGY) {e += (double)((Real)4 * eps * (s6 * s6 - s6 - K.cac));}
+    }
+    else if constexpr(KIND == kPairEV)
+    {
+        // ExcludedVolumePotential.hpp:44-133
+        const Real radius = par[i] + par[j];
+        const Real sigma_sq = radius * radius;
+        if(sigma_sq * K.cut2 < l_sq) {return;}
+        const Real rcp = RMath<Real>::rcp(l_sq);
+        const Real s2l2 = sigma_sq * rcp;
+        const Real s6 = s2l2 * s2l2 * s2l2;
+        c = -(K.eps * (Real)12) * s6 * s6 * rcp;
+        if(ENERGY) {e += (double)(K.eps * (s6 * s6 - K.cac));}
+    }
+    else
+    {
+        // DebyeHuckelPotential.hpp:47-65
+        const Real qq = K.qk * par[i] * par[j];
+        const Real rl = RMath<Real>::rsqrt(l_sq);
+        const Real l = l_sq * rl;
+        if(K.acut <= l) {return;}
+        const Real ex = RMath<Real>::exp(-l * K.idebye);
+        const Real f_mag = -qq * (K.debye + l) * K.idebye * (rl * rl) * ex;
+        c = f_mag * rl;
+        if(ENERGY) {e += (double)(qq * (ex * rl - K.cac));}
+    }
+    long long fx, fy, fz;
+    fix3r<Real>(dx * c, dy * c, dz * c, fx, fy, fz);
+    add3(F, i, fx, fy, fz);
+    add3(F, j, -fx, -fy, -fz);
+}
+
+// The pair terms of one interaction: kPairBatch list entries per thread per load round, the
+// next round's entries loaded before the current round is computed (the round's LDS atomics
+// would otherwise hold the loads back).  KIND < 0: any kind, dispatched per pair (3SPN2).
+template<typename Real, bool ENERGY, bool TSP, int KIND, bool PERIODIC, int BS>
+__device__ __forceinline__ void rep_pair_loop(const RepGlobal& G, const Real* par, uint32_t np, const Box& box,
+                                              const PosL& P, unsigned long long* F, double& e_pair)
+{
+    constexpr int kPairBatch = RepBatching<BS>::pairs;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t* __restrict__ pairs = G.pairs;
+    PairK<Real> K{};
+    if constexpr(KIND >= 0) {K = pair_consts<Real>(G);}
+    uint32_t pk[kPairBatch];
+#pragma unroll
+    for(int m = 0; m < kPairBatch; ++m)
+    {
+        const uint32_t k = m * BS + tid;
+        pk[m] = (k < np) ? pairs[k] : 0xffffffffu;
+    }
+    for(uint32_t base = 0; base < np; base += BS * kPairBatch)
+    {
+        uint32_t nx[kPairBatch];
+#pragma unroll
+        for(int m = 0; m < kPairBatch; ++m)
+        {
+            const uint32_t k = base + BS * kPairBatch + m * BS + tid;
+            nx[m] = (k < np) ? pairs[k] : 0xffffffffu;
+        }
+#pragma unroll
+        for(int m = 0; m < kPairBatch; ++m)
+        {
+            if(pk[m] == 0xffffffffu) {continue;}
+            if constexpr(KIND >= 0)
+            {
+                rep_pair_k<Real, ENERGY, KIND, PERIODIC>(K, par, pk[m] >> 16, pk[m] & 0xffffu, box, P, F, e_pair);
+            }
+            else {rep_pair<Real, ENERGY, TSP>(G, par, pk[m] >> 16, pk[m] & 0xffffu, box, P, F, e_pair);}
+        }
+#pragma unroll
+        for(int m = 0; m < kPairBatch; ++m) {pk[m] = nx[m];}
+    }
 }
 
 // Exclusions of one list row (particle pa against higher indices), held in registers while
@@ -643,8 +804,8 @@ __device__ __forceinline__ void wave_append(bool want, uint32_t val, uint32_t* c
 // by the lower particle index) within the list radius that the rules allow.  Rows are dealt in
 // snake order so the triangular loop is balanced over threads; `part` is the LDS copy.
 template<int BS>
-__device__ void rep_build_rows(const RepGlobal& G, const RepDesc& R, const PosL& P, const uint16_t* part,
-                               uint32_t* count)
+__device__ void rep_build_rows(const RepGlobal& G, const RepDesc& R, const Box& box, const PosL& P,
+                               const uint16_t* part, uint32_t* count)
 {
     const uint32_t np = G.np;
     const uint32_t tid = threadIdx.x;
@@ -663,7 +824,7 @@ __device__ void rep_build_rows(const RepGlobal& G, const RepDesc& R, const PosL&
             if(want && G.use_distance)
             {
                 double dx, dy, dz;
-                adjust_direction(R.box, xa, ya, za, P.x[pb], P.y[pb], P.z[pb], dx, dy, dz);
+                adjust_direction(box, xa, ya, za, P.x[pb], P.y[pb], P.z[pb], dx, dy, dz);
                 want = length_sq(dx, dy, dz) < G.r_list2;
             }
             want = want && rep_allowed(G, rr, R.mol, a, b, pa, pb);
@@ -724,12 +885,13 @@ __device__ __forceinline__ int grid_wrap(const GridShape& g, int d, int c)
 }
 
 template<int BS>
-__device__ void rep_build_grid(const RepGlobal& G, const RepDesc& R, const PosL& P, const uint16_t* part,
-                               uint32_t* cell_end, uint16_t* key, uint16_t* sorted, uint32_t* count)
+__device__ void rep_build_grid(const RepGlobal& G, const RepDesc& R, const Box& box, const PosL& P,
+                               const uint16_t* part, uint32_t* cell_end, uint16_t* key, uint16_t* sorted,
+                               uint32_t* count)
 {
     const uint32_t np = G.np;
     const uint32_t tid = threadIdx.x;
-    const GridShape gs = grid_shape(R.box, sqrt(G.r_list2));
+    const GridShape gs = grid_shape(box, sqrt(G.r_list2));
     for(uint32_t c = tid; c < kGridCells; c += BS) {cell_end[c] = 0;}
     __syncthreads();
     for(uint32_t a = tid; a < np; a += BS)
@@ -788,7 +950,7 @@ __device__ void rep_build_grid(const RepGlobal& G, const RepDesc& R, const PosL&
                 if(want)
                 {
                     double ddx, ddy, ddz;
-                    adjust_direction(R.box, xa, ya, za, P.x[pb], P.y[pb], P.z[pb], ddx, ddy, ddz);
+                    adjust_direction(box, xa, ya, za, P.x[pb], P.y[pb], P.z[pb], ddx, ddy, ddz);
                     want = length_sq(ddx, ddy, ddz) < G.r_list2;
                 }
                 want = want && rep_allowed(G, rr, R.mol, a, b, pa, pb);
@@ -805,12 +967,27 @@ __device__ void rep_build_grid(const RepGlobal& G, const RepDesc& R, const PosL&
 // LDS: x[n], y[n], z[n] (double); F[3n] (int64 fixed point); parameters[par_cols][n] as
 // npar-column rows per interaction (Real); participant lists (uint16)
 // ------------------------------------------------------------------------------------
-template<typename Real, bool ENERGY, int BS, int PPT, bool TSP>
-__global__ void __launch_bounds__(BS)
-k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
+// Workgroup-wide scalars of a segment (static LDS of the kernel)
+template<int BS> struct SegShared
 {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const RepDesc& R = reps[blockIdx.x];
+    double red[BS / kWave];
+    double margin[kRepMaxGlobals];
+    uint32_t npairs[kRepMaxGlobals], maxpairs[kRepMaxGlobals], count[kRepMaxGlobals];
+    int rebuild[kRepMaxGlobals];
+    int any_rebuild, flags, force_rebuild;
+    unsigned long long rebuilds;
+    double energy[kRepMaxLocals + kRepMaxGlobals];
+};
+
+// One replica's segment.  PERIODIC is the replica's box kind as a compile-time constant: the
+// minimum-image branches of every displacement fold away (an unlimited box is a plain
+// difference) and no box field is re-read per pair or term.
+template<typename Real, bool ENERGY, int BS, int PPT, bool TSP, bool PERIODIC>
+__device__ __forceinline__ void rep_segment(const RepDesc& R, int nsteps, int eval_only, unsigned char* smem,
+                                            SegShared<BS>& sh)
+{
+    Box box = R.box;
+    box.periodic = PERIODIC ? 1 : 0;
     RepState& st = *R.st;
     const uint32_t n = R.n;
     const int ng = R.ng;
@@ -825,13 +1002,17 @@ k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
     uint16_t* grid_key = reinterpret_cast<uint16_t*>(grid_end + kGridCells);
     uint16_t* grid_sorted = grid_key + ((R.grid_np + 7u) & ~7u);
 
-    __shared__ double s_red[BS / kWave];
-    __shared__ double s_margin[kRepMaxGlobals];
-    __shared__ uint32_t s_npairs[kRepMaxGlobals], s_maxpairs[kRepMaxGlobals], s_count[kRepMaxGlobals];
-    __shared__ int s_rebuild[kRepMaxGlobals];
-    __shared__ int s_any_rebuild, s_flags, s_force_rebuild;
-    __shared__ unsigned long long s_rebuilds;
-    __shared__ double s_energy[kRepMaxLocals + kRepMaxGlobals];
+    double* s_red = sh.red;
+    double* s_margin = sh.margin;
+    uint32_t* s_npairs = sh.npairs;
+    uint32_t* s_maxpairs = sh.maxpairs;
+    uint32_t* s_count = sh.count;
+    int* s_rebuild = sh.rebuild;
+    int& s_any_rebuild = sh.any_rebuild;
+    int& s_flags = sh.flags;
+    int& s_force_rebuild = sh.force_rebuild;
+    unsigned long long& s_rebuilds = sh.rebuilds;
+    double* s_energy = sh.energy;
     const uint32_t tid = threadIdx.x;
     constexpr int kTermBatch = RepBatching<BS>::terms;
     constexpr int kPairBatch = RepBatching<BS>::pairs;
@@ -916,7 +1097,7 @@ k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
                     vz[k] = a * (vz[k] + dtr2m * fzk) + brm * bz;
                 }
                 double x = P.x[p] + dpx, y = P.y[p] + dpy, z = P.z[p] + dpz;
-                adjust_position(R.box, x, y, z);
+                adjust_position(box, x, y, z);
                 P.x[p] = x; P.y[p] = y; P.z[p] = z;
                 d2 = fmax(d2, length_sq(dpx, dpy, dpz));
                 if(!isfinite(x) || !isfinite(y) || !isfinite(z)) {bad = true;}
@@ -962,10 +1143,10 @@ k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
                 if(G.use_distance && G.np > kGridMinNp)
                 {
                     // the grid build synchronizes internally; the condition is uniform
-                    rep_build_grid<BS>(G, R, P, part_lds + G.part_off, grid_end, grid_key, grid_sorted, &s_count[g]);
+                    rep_build_grid<BS>(G, R, box, P, part_lds + G.part_off, grid_end, grid_key, grid_sorted, &s_count[g]);
                     __syncthreads();
                 }
-                else {rep_build_rows<BS>(G, R, P, part_lds + G.part_off, &s_count[g]);}
+                else {rep_build_rows<BS>(G, R, box, P, part_lds + G.part_off, &s_count[g]);}
             }
             __syncthreads();
             if(tid == 0)
@@ -1007,44 +1188,22 @@ k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
             {
                 if(kd[m] == 0) {continue;}
                 double e = 0.0;
-                rep_local<Real, ENERGY, TSP>((int)(kd[m] & 0xff), ix[m], p0[m], p1[m], p2[m], R.box, P, F, e);
+                rep_local<Real, ENERGY, TSP>((int)(kd[m] & 0xff), ix[m], p0[m], p1[m], p2[m], box, P, F, e);
                 if(ENERGY) {atomicAdd(&s_energy[kd[m] >> 8], e);}
             }
         }
-        // pair terms, kPairBatch list entries per thread per load round; the next round's
-        // entries are loaded before the current round is computed (the round's LDS atomics
-        // would otherwise hold the loads back)
+        // pair terms: one loop per interaction kind
         for(int g = 0; g < ng; ++g)
         {
             const RepGlobal& G = R.g[g];
             const Real* par = par_lds + (size_t)n * G.par_col;
             const uint32_t np = s_npairs[g];
             double e_pair = 0.0;
-            uint32_t pk[kPairBatch];
-#pragma unroll
-            for(int m = 0; m < kPairBatch; ++m)
-            {
-                const uint32_t k = m * BS + tid;
-                pk[m] = (k < np) ? G.pairs[k] : 0xffffffffu;
-            }
-            for(uint32_t base = 0; base < np; base += BS * kPairBatch)
-            {
-                uint32_t nx[kPairBatch];
-#pragma unroll
-                for(int m = 0; m < kPairBatch; ++m)
-                {
-                    const uint32_t k = base + BS * kPairBatch + m * BS + tid;
-                    nx[m] = (k < np) ? G.pairs[k] : 0xffffffffu;
-                }
-#pragma unroll
-                for(int m = 0; m < kPairBatch; ++m)
-                {
-                    if(pk[m] == 0xffffffffu) {continue;}
-                    rep_pair<Real, ENERGY, TSP>(G, par, pk[m] >> 16, pk[m] & 0xffffu, R.box, P, F, e_pair);
-                }
-#pragma unroll
-                for(int m = 0; m < kPairBatch; ++m) {pk[m] = nx[m];}
-            }
+            const int kind = G.kind;
+            if(kind == kPairLJ) {rep_pair_loop<Real, ENERGY, TSP, kPairLJ, PERIODIC, BS>(G, par, np, box, P, F, e_pair);}
+            else if(kind == kPairEV) {rep_pair_loop<Real, ENERGY, TSP, kPairEV, PERIODIC, BS>(G, par, np, box, P, F, e_pair);}
+            else if(kind == kPairDH) {rep_pair_loop<Real, ENERGY, TSP, kPairDH, PERIODIC, BS>(G, par, np, box, P, F, e_pair);}
+            else if(TSP) {rep_pair_loop<Real, ENERGY, TSP, -1, PERIODIC, BS>(G, par, np, box, P, F, e_pair);}
             if(ENERGY) {atomicAdd(&s_energy[R.n_local + g], e_pair);}
         }
         __syncthreads();
@@ -1100,6 +1259,17 @@ k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
         for(int g = 0; g < ng; ++g) {st.margin[g] = s_margin[g]; st.npairs[g] = s_npairs[g]; st.maxpairs[g] = s_maxpairs[g];}
         if(ENERGY) {for(int q = 0; q < R.n_local + ng; ++q) {st.energy[q] = s_energy[q];}}
     }
+}
+
+template<typename Real, bool ENERGY, int BS, int PPT, bool TSP>
+__global__ void __launch_bounds__(BS)
+k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ SegShared<BS> sh;
+    const RepDesc& R = reps[blockIdx.x];
+    if(R.box.periodic) {rep_segment<Real, ENERGY, BS, PPT, TSP, true>(R, nsteps, eval_only, smem, sh);}
+    else {rep_segment<Real, ENERGY, BS, PPT, TSP, false>(R, nsteps, eval_only, smem, sh);}
 }
 
 } // mjg
