@@ -73,6 +73,28 @@ static void hip_check(hipError_t e, const char* what)
 }
 static size_t align256(size_t x) {return (x + 255) & ~(size_t)255;}
 
+// Skin of a replica's neighbour list.  The replica kernel's forces are exact fixed-point sums
+// over the pairs within the cutoff, so the skin changes no result, only the balance between
+// the pair loop (list entries grow as (1 + m)^3) and the rebuilds (their rate grows as 1 / m):
+// the kernel uses its own margin, at most the input's -- 0.35 for the short-ranged LJ and
+// excluded-volume pairs, 0.1 for Debye-Hueckel, whose cutoff spans many particle spacings
+// (measured best on 2000-bead chains; MJG_LIST_MARGIN_SHORT / MJG_LIST_MARGIN_DH override them).
+static double list_margin(int pot, double input_margin)
+{
+    static const double m_short = [] {
+        const char* c = std::getenv("MJG_LIST_MARGIN_SHORT");
+        return c ? std::atof(c) : 0.35;
+    }();
+    static const double m_dh = [] {
+        const char* c = std::getenv("MJG_LIST_MARGIN_DH");
+        return c ? std::atof(c) : 0.1;
+    }();
+    double m = input_margin;
+    if(pot == kPairLJ || pot == kPairEV) {m = std::min(m, m_short);}
+    else if(pot == kPairDH) {m = std::min(m, m_dh);}
+    return std::max(m, 0.02);
+}
+
 class Batch
 {
   public:
@@ -752,11 +774,19 @@ class Batch
                 RG.inv4pieps = dh.inv4pieps; RG.coef_at_cutoff = dh.coef_at_cutoff;
                 max_cutoff = dh.debye * G.cutoff_ratio;
             }
-            const double r_list = max_cutoff * (1. + G.margin);
+            const double m = list_margin(G.pot, G.margin);
+            const double r_list = max_cutoff * (1. + m);
             RG.r_list2 = r_list * r_list;
-            RG.abs_margin = max_cutoff * G.margin;
+            RG.abs_margin = max_cutoff * m;
         }
-        r.lds = rep_lds_bytes(D.n, D.par_cols, D.part_total, real_size(), D.grid_np);
+        // the finer grid (16 cells per dimension) where it fits in LDS
+        D.grid_bits = 4;
+        r.lds = rep_lds_bytes(D.n, D.par_cols, D.part_total, real_size(), D.grid_np, D.grid_bits);
+        if(r.lds > kRepLdsLimit)
+        {
+            D.grid_bits = 3;
+            r.lds = rep_lds_bytes(D.n, D.par_cols, D.part_total, real_size(), D.grid_np, D.grid_bits);
+        }
         if(r.lds > kRepLdsLimit)
         {
             throw std::invalid_argument("replica needs " + std::to_string(r.lds) + " B of LDS (limit " +

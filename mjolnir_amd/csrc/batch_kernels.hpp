@@ -33,9 +33,10 @@ constexpr size_t kRepLdsLimit = 160 * 1024 - 1024;
 constexpr int kRepClasses = 3;
 
 // list rebuilds of interactions with more participants than kGridMinNp go through a cell grid
-// in LDS (at most kGridCells cells, 8 per dimension); smaller ones scan all participant pairs
+// in LDS: 2^grid_bits cells per dimension (8 or 16, the finer one where the replica's LDS
+// allows); smaller ones scan all participant pairs
 constexpr uint32_t kGridMinNp = 512;
-constexpr uint32_t kGridCells = 512;
+__host__ __device__ constexpr uint32_t grid_cells(uint32_t grid_bits) {return 1u << (3 * grid_bits);}
 constexpr uint32_t kRepClassMaxN[kRepClasses] = {256, 1024, 2048};
 
 // terms and list entries per thread per load round (fewer at 1024 threads, where a thread
@@ -105,6 +106,7 @@ struct RepDesc
     uint32_t par_cols;         // total LDS parameter columns
     uint32_t part_total;       // LDS participant entries (uint16, padded per interaction)
     uint32_t grid_np;          // largest participant count of a grid-built list (0: none)
+    uint32_t grid_bits;        // log2 of the grid cells per dimension (3 or 4)
     double dt, halfdt;
     uint64_t seed;
     Box box;
@@ -133,9 +135,10 @@ struct RepDesc
 
 // dynamic LDS bytes of one replica (layout in k_replica_segment)
 __host__ __device__ inline size_t rep_lds_bytes(uint32_t n, uint32_t par_cols, uint32_t part_total, size_t real_size,
-                                                uint32_t grid_np)
+                                                uint32_t grid_np, uint32_t grid_bits)
 {
-    const size_t grid = grid_np ? (size_t)kGridCells * 4 + (size_t)((grid_np + 7u) & ~7u) * 4 : 0;
+    // cell ends (uint32), cell keys (uint16) and the cell-sorted entries (uint32) of the participants
+    const size_t grid = grid_np ? (size_t)grid_cells(grid_bits) * 4 + (size_t)((grid_np + 7u) & ~7u) * 6 : 0;
     return (size_t)n * 48 + (size_t)n * par_cols * real_size + (size_t)part_total * 2 + grid;
 }
 
@@ -234,6 +237,15 @@ __device__ __forceinline__ double uniform(double v)
     const long long b = __double_as_longlong(v);
     const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffff)), hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ uint32_t uniform(uint32_t v) {return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);}
+__device__ __forceinline__ int uniform(int v) {return __builtin_amdgcn_readfirstlane(v);}
+template<typename T> __device__ __forceinline__ gptr<T> uniform_global(T* p)
+{
+    const unsigned long long b = (unsigned long long)p;
+    const unsigned long long lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(b & 0xffffffffu));
+    const unsigned long long hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return as_global(reinterpret_cast<T*>((hi << 32) | lo));
 }
 template<typename Real>
 __device__ __forceinline__ double from_fix(unsigned long long v)
@@ -691,7 +703,7 @@ __device__ __forceinline__ void rep_pair_loop(const RepGlobal& G, const Real* pa
 {
     constexpr int kPairBatch = RepBatching<BS>::pairs;
     const uint32_t tid = threadIdx.x;
-    const uint32_t* __restrict__ pairs = G.pairs;
+    const gptr<const uint32_t> pairs = uniform_global((const uint32_t*)G.pairs);
     PairK<Real> K{};
     if constexpr(KIND >= 0) {K = pair_consts<Real>(G);}
     uint32_t pk[kPairBatch];
@@ -736,15 +748,49 @@ struct RowRules
     uint32_t mol;
 };
 
-__device__ __forceinline__ RowRules row_rules(const RepGlobal& G, const uint32_t* __restrict__ mol, uint32_t pa)
+// List-build rules of one interaction: wave-uniform values held in registers and global views
+// of its tables, read once per rebuild instead of per candidate
+struct BuildK
+{
+    int pairwise, ignore_mol, base_rule, use_distance, sorted;
+    uint32_t last_slot, np, cap, npar;
+    double r_list2;
+    gptr<const unsigned long long> ex_near;
+    gptr<const uint32_t> far_start, far, mol;
+    gptr<const double> ppar;
+    gptr<uint32_t> pairs;
+};
+__device__ __forceinline__ BuildK build_consts(const RepGlobal& G, const RepDesc& R)
+{
+    BuildK k;
+    k.pairwise = uniform(G.pairwise);
+    k.ignore_mol = uniform(G.ignore_mol);
+    k.base_rule = uniform(G.base_rule);
+    k.use_distance = uniform(G.use_distance);
+    k.sorted = uniform(G.sorted);
+    k.last_slot = uniform(G.last_slot);
+    k.np = uniform(G.np);
+    k.cap = uniform(G.cap);
+    k.npar = uniform((uint32_t)G.npar);
+    k.r_list2 = uniform(G.r_list2);
+    k.ex_near = uniform_global(G.ex_near);
+    k.far_start = uniform_global(G.far_start);
+    k.far = uniform_global(G.far);
+    k.mol = uniform_global(R.mol);
+    k.ppar = uniform_global(G.ppar);
+    k.pairs = uniform_global(G.pairs);
+    return k;
+}
+
+__device__ __forceinline__ RowRules row_rules(const BuildK& K, uint32_t pa)
 {
     RowRules r;
-    r.near = G.ex_near[pa];
-    r.f0 = G.far_start[pa];
-    r.f1 = G.far_start[pa + 1];
+    r.near = K.ex_near[pa];
+    r.f0 = K.far_start[pa];
+    r.f1 = K.far_start[pa + 1];
 #pragma unroll
-    for(int k = 0; k < 4; ++k) {r.far[k] = (r.f0 + k < r.f1) ? G.far[r.f0 + k] : 0xffffffffu;}
-    r.mol = G.ignore_mol ? mol[pa] : 0u;
+    for(int k = 0; k < 4; ++k) {r.far[k] = (r.f0 + k < r.f1) ? K.far[r.f0 + k] : 0xffffffffu;}
+    r.mol = K.ignore_mol ? K.mol[pa] : 0u;
     return r;
 }
 
@@ -752,116 +798,168 @@ __device__ __forceinline__ RowRules row_rules(const RepGlobal& G, const uint32_t
 // a (of pa) and b (of pb): Verlet / Naive pair b after a in input order, cell list any pair
 // whose lower particle is not the last participant (LorentzBerthelotRule, ParameterList.hpp:
 // 240-252,285-289; VerletList.hpp:99-134)
-__device__ __forceinline__ bool rep_allowed(const RepGlobal& G, const RowRules& r, const uint32_t* __restrict__ mol,
-                                            uint32_t a, uint32_t b, uint32_t pa, uint32_t pb)
+__device__ __forceinline__ bool rep_allowed(const BuildK& K, const RowRules& r, uint32_t a, uint32_t b, uint32_t pa,
+                                            uint32_t pb)
 {
-    if(G.pairwise) {if(a > b) {return false;}}
-    else {if(a == G.last_slot) {return false;}}
-    if(G.ignore_mol == 1 && r.mol == mol[pb]) {return false;}
-    if(G.ignore_mol == 2 && r.mol != mol[pb]) {return false;}
-    if(G.base_rule == 1)
+    if(K.pairwise) {if(a > b) {return false;}}
+    else {if(a == K.last_slot) {return false;}}
+    if(K.ignore_mol == 1 && r.mol == K.mol[pb]) {return false;}
+    if(K.ignore_mol == 2 && r.mol != K.mol[pb]) {return false;}
+    if(K.base_rule == 1)
     {
         // ThreeSPN2ExcludedVolumePotential.hpp:326-364: bases that could pair never interact
-        const int ki = (int)G.ppar[pa], kj = (int)G.ppar[pb];
+        const int ki = (int)K.ppar[pa], kj = (int)K.ppar[pb];
         if(ki >= 2 && kj >= 2 && complementary_bases(ki - 2, kj - 2)) {return false;}
     }
-    else if(G.base_rule == 2)
+    else if(K.base_rule == 2)
     {
         // ThreeSPN2BasePairPotential.hpp:445-459 / ThreeSPN2CrossStackingPotential.hpp:437-451
-        if(!complementary_bases((int)G.ppar[(size_t)pa * G.npar], (int)G.ppar[(size_t)pb * G.npar])) {return false;}
+        if(!complementary_bases((int)K.ppar[(size_t)pa * K.npar], (int)K.ppar[(size_t)pb * K.npar])) {return false;}
     }
     const uint32_t d = pb - pa;
     if(d <= 64) {return ((r.near >> (d - 1)) & 1ull) == 0ull;}
     if(pb == r.far[0] || pb == r.far[1] || pb == r.far[2] || pb == r.far[3]) {return false;}
     for(uint32_t k = r.f0 + 4; k < r.f1; ++k)
     {
-        const uint32_t x = G.far[k];
+        const uint32_t x = K.far[k];
         if(x > pb) {break;}
         if(x == pb) {return false;}
     }
     return true;
 }
 
-// Append (pa << 16 | pb) for the lanes that want it: one LDS atomic per wave, lane offsets
-// from the ballot.  Every lane active at the call site must make the call.
-__device__ __forceinline__ void wave_append(bool want, uint32_t val, uint32_t* count, uint32_t cap, uint32_t* pairs)
+// Accepted list entries wait in a lane's registers (up to four) and are written for the whole
+// wave at once: one LDS atomic reserves the wave's slots; the lanes' first entries go first,
+// then their second ones, and so on, so neighbouring list slots come from different lanes
+// (different rows), which keeps the pair loop's force atomics on distinct addresses.  A lane's
+// slot for its j-th entry counts, with one ballot per j, the lanes before it that have a j-th
+// entry (valid in divergent code, where inactive lanes count zero).  The list is a set: its
+// order changes no force (the sums are exact).
+struct ListAppend
 {
-    const unsigned long long m = __ballot(want);
-    if(m == 0ull) {return;}
-    const int leader = __ffsll((long long)m) - 1;
-    const int lane = (int)__lane_id();
-    uint32_t base = 0;
-    if(lane == leader) {base = atomicAdd(count, (uint32_t)__popcll(m));}
-    base = (uint32_t)__shfl((int)base, leader, kWave);
-    if(want)
+    uint32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+    uint32_t k = 0;
+    __device__ __forceinline__ void flush(uint32_t* count, uint32_t cap, gptr<uint32_t> pairs)
     {
-        const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        if(k < cap) {pairs[k] = val;}
+        const unsigned long long m0 = __ballot(k > 0u), m1 = __ballot(k > 1u), m2 = __ballot(k > 2u),
+                                 m3 = __ballot(k > 3u);
+        const uint32_t c0 = (uint32_t)__popcll(m0), c1 = (uint32_t)__popcll(m1), c2 = (uint32_t)__popcll(m2);
+        const uint32_t total = c0 + c1 + c2 + (uint32_t)__popcll(m3);
+        if(total == 0) {return;}
+        const int lane = (int)__lane_id();
+        const unsigned long long lt = (1ull << lane) - 1ull;
+        const int leader = __ffsll((long long)__ballot(1)) - 1;
+        uint32_t base = 0;
+        if(lane == leader) {base = atomicAdd(count, total);}
+        base = (uint32_t)__shfl((int)base, leader, kWave);
+        const uint32_t s0 = base + (uint32_t)__popcll(m0 & lt);
+        const uint32_t s1 = base + c0 + (uint32_t)__popcll(m1 & lt);
+        const uint32_t s2 = base + c0 + c1 + (uint32_t)__popcll(m2 & lt);
+        const uint32_t s3 = base + c0 + c1 + c2 + (uint32_t)__popcll(m3 & lt);
+        if(k > 0u && s0 < cap) {pairs[s0] = v0;}
+        if(k > 1u && s1 < cap) {pairs[s1] = v1;}
+        if(k > 2u && s2 < cap) {pairs[s2] = v2;}
+        if(k > 3u && s3 < cap) {pairs[s3] = v3;}
+        k = 0;
     }
-}
+    __device__ __forceinline__ void add(bool want, uint32_t val, uint32_t* count, uint32_t cap, gptr<uint32_t> pairs)
+    {
+        if(want)
+        {
+            v0 = (k == 0u) ? val : v0; v1 = (k == 1u) ? val : v1;
+            v2 = (k == 2u) ? val : v2; v3 = (k == 3u) ? val : v3;
+            ++k;
+        }
+        if(__any(k == 4u)) {flush(count, cap, pairs);}
+    }
+};
 
 // List build of one interaction inside the workgroup: every participant pair (each once, owned
-// by the lower particle index) within the list radius that the rules allow.  Rows are dealt in
-// snake order so the triangular loop is balanced over threads; `part` is the LDS copy.
+// by the lower particle index) within the list radius that the rules allow.  With sorted
+// participants row a holds the np - 1 - a candidates after it; rows a and np - 2 - a are dealt
+// to the same threads (np candidates together), and each such folded row is split over
+// S = BS / (folded rows) threads (candidates s, s + S, ...) when there are fewer rows than
+// threads, so every thread scans about the same number of candidates.  Unsorted participants
+// scan all np candidates per row.
 template<int BS>
 __device__ void rep_build_rows(const RepGlobal& G, const RepDesc& R, const Box& box, const PosL& P,
                                const uint16_t* part, uint32_t* count)
 {
-    const uint32_t np = G.np;
+    const BuildK K = build_consts(G, R);
+    const uint32_t np = K.np;
+    if(np < 2u) {return;}
     const uint32_t tid = threadIdx.x;
-    const uint32_t nblk = (np + BS - 1) / BS;
-    for(uint32_t m = 0; m < nblk; ++m)
-    {
-        const uint32_t a = (m & 1) ? m * BS + (BS - 1 - tid) : m * BS + tid;
-        if(a >= np) {continue;}
+    const bool sorted = K.sorted != 0;
+    const uint32_t nrow = sorted ? np / 2u : np;
+    const uint32_t S = nrow >= (uint32_t)BS ? 1u : (uint32_t)BS / nrow;
+    ListAppend app;
+    auto scan_row = [&](uint32_t a, uint32_t first) {
         const uint32_t pa = part[a];
-        const RowRules rr = row_rules(G, R.mol, pa);
+        const RowRules rr = row_rules(K, pa);
         const double xa = P.x[pa], ya = P.y[pa], za = P.z[pa];
-        for(uint32_t b = G.sorted ? a + 1 : 0; b < np; ++b)
+        // the next candidate's particle is loaded while the current one is tested
+        uint32_t cur = (first < np) ? part[first] : 0u;
+        for(uint32_t b = first; b < np; b += S)
         {
-            const uint32_t pb = part[b];
+            const uint32_t pb = cur;
+            cur = (b + S < np) ? part[b + S] : 0u;
             bool want = pb > pa;
-            if(want && G.use_distance)
+            if(want && K.use_distance)
             {
                 double dx, dy, dz;
                 adjust_direction(box, xa, ya, za, P.x[pb], P.y[pb], P.z[pb], dx, dy, dz);
-                want = length_sq(dx, dy, dz) < G.r_list2;
+                want = length_sq(dx, dy, dz) < K.r_list2;
             }
-            want = want && rep_allowed(G, rr, R.mol, a, b, pa, pb);
-            wave_append(want, (pa << 16) | pb, count, G.cap, G.pairs);
+            want = want && rep_allowed(K, rr, a, b, pa, pb);
+            app.add(want, (pa << 16) | pb, count, K.cap, K.pairs);
         }
+    };
+    for(uint32_t u = tid; u < nrow * S; u += BS)
+    {
+        // neighbouring threads take different rows
+        const uint32_t sl = u / nrow, f = u - sl * nrow;
+        if(!sorted) {scan_row(f, sl); continue;}
+        scan_row(f, f + 1u + sl);
+        const uint32_t a2 = np - 2u - f;
+        if(a2 > f) {scan_row(a2, a2 + 1u + sl);}
     }
+    app.flush(count, K.cap, K.pairs);
 }
 
 // Cell-grid list build (same pair set as rep_build_rows).  Cells are at least r_list wide,
 // so every pair within r_list sits in the same or adjacent cells: a periodic box gets
-// min(8, max(3, floor(L / r_list))) cells per dimension with wrap-around; an unbounded system
-// hashes floor(x / r_list) modulo 8 per dimension (UnlimitedGridCellList.hpp:105-125 uses the
+// min(D, max(3, floor(L / r_list))) cells per dimension with wrap-around; an unbounded system
+// hashes floor(x / r_list) modulo D per dimension (UnlimitedGridCellList.hpp:105-125 uses the
 // same kind of hash), where far-apart particles can share a cell and the distance test
-// rejects them.  Participants are counting-sorted by cell in LDS.
+// rejects them.  D = 2^grid_bits (8 or 16).  Participants are counting-sorted by cell in LDS
+// and their rows are scanned in that order; three x-adjacent cells that do not wrap are one
+// index range.
 struct GridShape
 {
     int dims[3];
     double lo[3], inv_w[3];
     int periodic;
+    int bits;
 };
 
-__device__ __forceinline__ GridShape grid_shape(const Box& box, double r_list)
+__device__ __forceinline__ GridShape grid_shape(const Box& box, double r_list, int bits)
 {
     GridShape g;
     g.periodic = box.periodic;
+    g.bits = bits;
+    const double D = (double)(1 << bits);
     for(int d = 0; d < 3; ++d)
     {
         if(box.periodic)
         {
             const double f = floor(box.width[d] / r_list);
-            g.dims[d] = (int)fmin(8.0, fmax(3.0, f));
+            g.dims[d] = (int)fmin(D, fmax(3.0, f));
             g.lo[d] = box.lower[d];
             g.inv_w[d] = (double)g.dims[d] / box.width[d];
         }
         else
         {
-            g.dims[d] = 8;
+            g.dims[d] = 1 << bits;
             g.lo[d] = 0.0;
             g.inv_w[d] = 1.0 / r_list;
         }
@@ -875,39 +973,43 @@ __device__ __forceinline__ int grid_coord(const GridShape& g, int d, double x)
     c = fmin(fmax(c, -1.0e9), 1.0e9);
     const int ic = (int)c;
     if(g.periodic) {return ic < 0 ? 0 : (ic >= g.dims[d] ? g.dims[d] - 1 : ic);}
-    return ic & 7;
+    return ic & ((1 << g.bits) - 1);
 }
 
 __device__ __forceinline__ int grid_wrap(const GridShape& g, int d, int c)
 {
     if(g.periodic) {return c < 0 ? c + g.dims[d] : (c >= g.dims[d] ? c - g.dims[d] : c);}
-    return c & 7;
+    return c & ((1 << g.bits) - 1);
 }
 
 template<int BS>
 __device__ void rep_build_grid(const RepGlobal& G, const RepDesc& R, const Box& box, const PosL& P,
-                               const uint16_t* part, uint32_t* cell_end, uint16_t* key, uint16_t* sorted,
+                               const uint16_t* part, uint32_t* cell_end, uint16_t* key, uint32_t* sorted,
                                uint32_t* count)
 {
-    const uint32_t np = G.np;
+    const BuildK K = build_consts(G, R);
+    const uint32_t np = K.np;
     const uint32_t tid = threadIdx.x;
-    const GridShape gs = grid_shape(box, sqrt(G.r_list2));
-    for(uint32_t c = tid; c < kGridCells; c += BS) {cell_end[c] = 0;}
+    const int bits = (int)R.grid_bits;
+    const uint32_t ncell = grid_cells((uint32_t)bits);
+    const GridShape gs = grid_shape(box, sqrt(K.r_list2), bits);
+    for(uint32_t c = tid; c < ncell; c += BS) {cell_end[c] = 0;}
     __syncthreads();
     for(uint32_t a = tid; a < np; a += BS)
     {
         const uint32_t pa = part[a];
-        const int k = grid_coord(gs, 0, P.x[pa]) + 8 * grid_coord(gs, 1, P.y[pa]) + 64 * grid_coord(gs, 2, P.z[pa]);
+        const int k = grid_coord(gs, 0, P.x[pa]) + (grid_coord(gs, 1, P.y[pa]) << bits) +
+                      (grid_coord(gs, 2, P.z[pa]) << (2 * bits));
         key[a] = (uint16_t)k;
         atomicAdd(&cell_end[k], 1u);
     }
     __syncthreads();
-    // exclusive scan of the 512 counts in the first wave: 8 consecutive cells per lane
+    // exclusive scan of the counts in the first wave: ncell / 64 consecutive cells per lane
     if(tid < (uint32_t)kWave)
     {
-        uint32_t v[8], sum = 0;
-#pragma unroll
-        for(int q = 0; q < 8; ++q) {v[q] = cell_end[tid * 8 + q]; sum += v[q];}
+        const uint32_t per = ncell / kWave, c0 = tid * per;
+        uint32_t sum = 0;
+        for(uint32_t q = 0; q < per; ++q) {sum += cell_end[c0 + q];}
         uint32_t incl = sum;
 #pragma unroll
         for(int o = 1; o < kWave; o <<= 1)
@@ -916,48 +1018,68 @@ __device__ void rep_build_grid(const RepGlobal& G, const RepDesc& R, const Box& 
             if((int)tid >= o) {incl += t;}
         }
         uint32_t run = incl - sum;
-#pragma unroll
-        for(int q = 0; q < 8; ++q) {cell_end[tid * 8 + q] = run; run += v[q];}
+        for(uint32_t q = 0; q < per; ++q) {const uint32_t v = cell_end[c0 + q]; cell_end[c0 + q] = run; run += v;}
     }
     __syncthreads();
-    // scatter: afterwards cell_end[k] is the end of cell k (the start of k + 1)
+    // scatter: afterwards cell_end[k] is the end of cell k (the start of k + 1); an entry holds
+    // the participant slot and the particle (slot << 16 | particle), one load per candidate
     for(uint32_t a = tid; a < np; a += BS)
     {
         const uint32_t slot = atomicAdd(&cell_end[key[a]], 1u);
-        sorted[slot] = (uint16_t)a;
+        sorted[slot] = (a << 16) | (uint32_t)part[a];
     }
     __syncthreads();
-    const uint32_t nblk = (np + BS - 1) / BS;
-    for(uint32_t m = 0; m < nblk; ++m)
+    ListAppend app;
+    // rows in cell order: neighbouring lanes sit in the same or adjacent cells, so they scan
+    // nearly the same candidate ranges (the wave's trip count is close to each lane's) and read
+    // the same entries (LDS broadcast)
+    for(uint32_t i = tid; i < np; i += BS)
     {
-        const uint32_t a = (m & 1) ? m * BS + (BS - 1 - tid) : m * BS + tid;
-        if(a >= np) {continue;}
-        const uint32_t pa = part[a];
-        const RowRules rr = row_rules(G, R.mol, pa);
+        const uint32_t e = sorted[i];
+        const uint32_t a = e >> 16, pa = e & 0xffffu;
+        const RowRules rr = row_rules(K, pa);
         const double xa = P.x[pa], ya = P.y[pa], za = P.z[pa];
         const int cx = grid_coord(gs, 0, xa), cy = grid_coord(gs, 1, ya), cz = grid_coord(gs, 2, za);
-        for(int dz = -1; dz <= 1; ++dz)
-        for(int dy = -1; dy <= 1; ++dy)
-        for(int dx = -1; dx <= 1; ++dx)
-        {
-            const int c = grid_wrap(gs, 0, cx + dx) + 8 * grid_wrap(gs, 1, cy + dy) + 64 * grid_wrap(gs, 2, cz + dz);
-            const uint32_t beg = c ? cell_end[c - 1] : 0u, end = cell_end[c];
+        const bool run_x = cx >= 1 && cx + 1 < gs.dims[0];
+        // the next candidate's entry is loaded while the current one is tested
+        auto scan = [&](uint32_t beg, uint32_t end) {
+            uint32_t cur = (beg < end) ? sorted[beg] : 0u;
             for(uint32_t q = beg; q < end; ++q)
             {
-                const uint32_t b = sorted[q];
-                const uint32_t pb = part[b];
+                const uint32_t nxt = (q + 1u < end) ? sorted[q + 1u] : 0u;
+                const uint32_t b = cur >> 16, pb = cur & 0xffffu;
+                cur = nxt;
                 bool want = pb > pa;
                 if(want)
                 {
                     double ddx, ddy, ddz;
                     adjust_direction(box, xa, ya, za, P.x[pb], P.y[pb], P.z[pb], ddx, ddy, ddz);
-                    want = length_sq(ddx, ddy, ddz) < G.r_list2;
+                    want = length_sq(ddx, ddy, ddz) < K.r_list2;
                 }
-                want = want && rep_allowed(G, rr, R.mol, a, b, pa, pb);
-                wave_append(want, (pa << 16) | pb, count, G.cap, G.pairs);
+                want = want && rep_allowed(K, rr, a, b, pa, pb);
+                app.add(want, (pa << 16) | pb, count, K.cap, K.pairs);
+            }
+        };
+        for(int dz = -1; dz <= 1; ++dz)
+        for(int dy = -1; dy <= 1; ++dy)
+        {
+            const int yz = (grid_wrap(gs, 1, cy + dy) << bits) + (grid_wrap(gs, 2, cz + dz) << (2 * bits));
+            if(run_x)
+            {
+                const int c0 = yz + cx - 1, c2 = yz + cx + 1;
+                scan(c0 ? cell_end[c0 - 1] : 0u, cell_end[c2]);
+            }
+            else
+            {
+                for(int dx = -1; dx <= 1; ++dx)
+                {
+                    const int c = yz + grid_wrap(gs, 0, cx + dx);
+                    scan(c ? cell_end[c - 1] : 0u, cell_end[c]);
+                }
             }
         }
     }
+    app.flush(count, K.cap, K.pairs);
 }
 
 // ------------------------------------------------------------------------------------
@@ -999,8 +1121,8 @@ __device__ __forceinline__ void rep_segment(const RepDesc& R, int nsteps, int ev
     Real* par_lds = reinterpret_cast<Real*>(F + 3 * n);
     uint16_t* part_lds = reinterpret_cast<uint16_t*>(par_lds + (size_t)n * R.par_cols);
     uint32_t* grid_end = reinterpret_cast<uint32_t*>(part_lds + R.part_total);
-    uint16_t* grid_key = reinterpret_cast<uint16_t*>(grid_end + kGridCells);
-    uint16_t* grid_sorted = grid_key + ((R.grid_np + 7u) & ~7u);
+    uint16_t* grid_key = reinterpret_cast<uint16_t*>(grid_end + grid_cells(R.grid_bits));
+    uint32_t* grid_sorted = reinterpret_cast<uint32_t*>(grid_key + ((R.grid_np + 7u) & ~7u));
 
     double* s_red = sh.red;
     double* s_margin = sh.margin;
@@ -1069,7 +1191,7 @@ __device__ __forceinline__ void rep_segment(const RepDesc& R, int nsteps, int ev
             {
                 const uint32_t p = tid + k * BS;
                 if(p >= n) {continue;}
-                const double rmk = R.rmass[p], c1k = R.c1[p], c2k = R.c2[p];
+                const double rmk = as_global(R.rmass)[p], c1k = as_global(R.c1)[p], c2k = as_global(R.c2)[p];
                 const double fxk = from_fix<Real>(F[3 * p + 0]), fyk = from_fix<Real>(F[3 * p + 1]);
                 const double fzk = from_fix<Real>(F[3 * p + 2]);
                 double Rx, Ry, Rz;
@@ -1086,7 +1208,7 @@ __device__ __forceinline__ void rep_segment(const RepDesc& R, int nsteps, int ev
                 }
                 else
                 {
-                    const double r2m = 0.5 * rmk, b = c1k, bt = c2k, a = R.c3[p];
+                    const double r2m = 0.5 * rmk, b = c1k, bt = c2k, a = as_global(R.c3)[p];
                     const double bx = Rx * bt, by = Ry * bt, bz = Rz * bt, bdt = b * R.dt;
                     dpx = bdt * (vx[k] + r2m * (R.dt * fxk + bx));
                     dpy = bdt * (vy[k] + r2m * (R.dt * fyk + by));
@@ -1164,6 +1286,9 @@ __device__ __forceinline__ void rep_segment(const RepDesc& R, int nsteps, int ev
         }
         // local terms, kTermBatch per thread per load round
         const uint32_t nt = R.n_terms;
+        const auto t_idx = uniform_global(R.t_idx);
+        const auto t_kind = uniform_global(R.t_kind);
+        const auto t_par = uniform_global(R.t_par);
         for(uint32_t base = 0; base < nt; base += BS * kTermBatch)
         {
             unsigned long long ix[kTermBatch];
@@ -1176,11 +1301,11 @@ __device__ __forceinline__ void rep_segment(const RepDesc& R, int nsteps, int ev
                 kd[m] = 0;
                 if(t < nt)
                 {
-                    ix[m] = R.t_idx[t];
-                    kd[m] = R.t_kind[t];
-                    p0[m] = R.t_par[t];
-                    p1[m] = R.t_par[nt + t];
-                    p2[m] = R.t_par[2 * nt + t];
+                    ix[m] = t_idx[t];
+                    kd[m] = t_kind[t];
+                    p0[m] = t_par[t];
+                    p1[m] = t_par[nt + t];
+                    p2[m] = t_par[2 * nt + t];
                 }
             }
 #pragma unroll
@@ -1215,7 +1340,7 @@ __device__ __forceinline__ void rep_segment(const RepDesc& R, int nsteps, int ev
             {
                 const uint32_t p = tid + k * BS;
                 if(p >= n) {continue;}
-                const double hrm = R.halfdt * R.rmass[p];
+                const double hrm = R.halfdt * as_global(R.rmass)[p];
                 vx[k] += hrm * from_fix<Real>(F[3 * p + 0]);
                 vy[k] += hrm * from_fix<Real>(F[3 * p + 1]);
                 vz[k] += hrm * from_fix<Real>(F[3 * p + 2]);

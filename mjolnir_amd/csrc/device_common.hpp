@@ -14,6 +14,13 @@ namespace mjg
 
 constexpr int kWave = 64;
 
+// Global-memory view of a pointer.  Pointers read out of a descriptor are generic, and the
+// compiler then emits flat loads and stores, which count against the LDS wait counter as well:
+// every wait for an LDS result would also wait for outstanding memory loads.  Through this view
+// they become global (vector-memory) instructions.
+template<typename T> using gptr = __attribute__((address_space(1))) T*;
+template<typename T> __device__ __forceinline__ gptr<T> as_global(T* p) {return (gptr<T>)p;}
+
 // interaction kinds (the values of the C ABI's MJG_* enums)
 enum LocalKind : int
 {
