@@ -867,7 +867,7 @@ class Batch
     // more registers per thread, so its largest class runs 512 threads with 4 particles each)
     template<typename Real, int C> struct Shape
     {
-        static constexpr int bs = C == 0 ? 256 : (C == 1 || sizeof(Real) == 8) ? 512 : 1024;
+        static constexpr int bs = C == 0 ? 256 : sizeof(Real) == 8 ? 512 : 1024;
         static constexpr int ppt = (int)(kRepClassMaxN[C] / bs);
     };
     template<typename Real, int C>
