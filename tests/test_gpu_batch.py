@@ -73,18 +73,23 @@ def test_fp64_batch_forces_vs_oracle():
 
 
 def test_fp64_batch_zero_temperature_trajectory_vs_oracle():
+    """The 1200-bead chain's excluded-volume list is built on the cell grid and its
+    Debye-Hueckel list by the sliced row build; both are rebuilt during the run with the
+    replica kernel's own skin, so a pair missing from a rebuilt list would show here."""
     import oracle
     from mjolnir_amd import capi, workloads
-    specs = [workloads.zero_temperature(s) for s in _chains([90, 300])]
+    specs = [workloads.zero_temperature(s) for s in _chains([90, 300, 1200])]
     b, ids = _batch(specs, capi.FP64)
-    b.step(150)
+    b.step(200)
     for spec, r in zip(specs, ids):
         o = oracle.OracleEngine(spec)
-        o.step(150)
+        o.step(200)
         x, v, _ = b.state(r)
         xo, vo, _ = o.state()
         assert np.abs(x - xo).max() < 1e-8, np.abs(x - xo).max()
-        assert b.info(r)["step"] == 150
+        assert b.info(r)["step"] == 200
+        if spec.n == 1200:
+            assert b.info(r)["rebuilds"] > len(spec.global_), b.info(r)   # rebuilt after set-up
 
 
 def test_fp64_batch_langevin_and_gjf_vs_oracle_same_noise():
