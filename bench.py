@@ -232,7 +232,7 @@ def bench_batch(args):
 
 
 # ------------------------------------------------------------------------------------------
-# 3SPN2 duplex ensemble (config 3 in the FP32-bulk mode, batched)
+# 3SPN2 duplex ensemble: 200-bp duplexes in the FP32-bulk mode, batched
 # ------------------------------------------------------------------------------------------
 def bench_dna_ensemble(args, n_rep: int = 128):
     from mjolnir_amd import capi, workloads
