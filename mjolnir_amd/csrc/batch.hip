@@ -266,7 +266,10 @@ class Batch
     void set_integrator(int id, int type, double dt, const double* coefs, uint64_t seed)
     {
         Replica& r = rep(id);
-        if(type != MJG_INTEG_BAOAB && type != MJG_INTEG_GJF) {throw std::invalid_argument("unknown integrator");}
+        if(type != MJG_INTEG_BAOAB && type != MJG_INTEG_GJF && type != MJG_INTEG_UNDERDAMPED)
+        {
+            throw std::invalid_argument("unknown integrator");
+        }
         if(!(dt > 0) || !coefs) {throw std::invalid_argument("invalid integrator parameters");}
         r.integ = type; r.dt = dt; r.seed = seed;
         r.coefs.assign(coefs, coefs + r.n);
@@ -611,6 +614,7 @@ class Batch
         auto take = [&](size_t bytes) {const size_t o = off; off += align256(bytes); return o;};
         const size_t o_pos = take(n * sizeof(double4)), o_vel = take(n * sizeof(double4)), o_force = take(n * sizeof(double4));
         const size_t o_bpos = take(n * sizeof(double4)), o_bvel = take(n * sizeof(double4)), o_bforce = take(n * sizeof(double4));
+        const size_t o_acc = take(r.integ == MJG_INTEG_UNDERDAMPED ? n * sizeof(double4) : 0);
         const size_t o_rm = take(n * 8), o_c1 = take(n * 8), o_c2 = take(n * 8), o_c3 = take(n * 8);
         const size_t o_mass = take(n * 8);
         const size_t o_mol = take(n * 4), o_tidx = take(n_terms * 8), o_tkind = take(n_terms * 4);
@@ -644,7 +648,14 @@ class Batch
             p[i] = make_double4(x, y, zz, 0);
             v[i] = make_double4(r.vel[3 * i], r.vel[3 * i + 1], r.vel[3 * i + 2], 0);
             rm[i] = 1.0 / r.mass[i];
-            if(r.integ == MJG_INTEG_BAOAB)
+            if(r.integ == MJG_INTEG_UNDERDAMPED)
+            {
+                // UnderdampedLangevinIntegrator::update (:320-350): gamma, sqrt(2 kB T / dt) sqrt(gamma / m)
+                c1[i] = r.coefs[i];
+                c2[i] = std::sqrt(2 * kB_kcal() * r.T / r.dt) * std::sqrt(r.coefs[i] * rm[i]);
+                c3[i] = 0;
+            }
+            else if(r.integ == MJG_INTEG_BAOAB)
             {
                 const double gamma_dt = -1 * r.coefs[i] * r.dt;
                 c1[i] = std::exp(gamma_dt);
@@ -693,6 +704,7 @@ class Batch
         D.bak_pos = reinterpret_cast<double4*>(A + o_bpos);
         D.bak_vel = reinterpret_cast<double4*>(A + o_bvel);
         D.bak_force = reinterpret_cast<double4*>(A + o_bforce);
+        D.accel = reinterpret_cast<double4*>(A + o_acc);
         D.rmass = reinterpret_cast<const double*>(A + o_rm);
         D.mass = reinterpret_cast<const double*>(A + o_mass);
         D.c1 = reinterpret_cast<const double*>(A + o_c1);

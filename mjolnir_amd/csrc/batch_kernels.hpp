@@ -102,7 +102,7 @@ struct RepDesc
     uint32_t n_terms;
     int n_local;               // local interactions (energy slots 0..n_local-1)
     int ng;
-    int integ;                 // 1 BAOAB, 2 G-JF
+    int integ;                 // 1 BAOAB, 2 G-JF, 4 UnderdampedLangevin
     uint32_t par_cols;         // total LDS parameter columns
     uint32_t part_total;       // LDS participant entries (uint16, padded per interaction)
     uint32_t grid_np;          // largest participant count of a grid-built list (0: none)
@@ -129,6 +129,7 @@ struct RepDesc
     double4* bak_pos;
     double4* bak_vel;
     double4* bak_force;
+    double4* accel;            // UnderdampedLangevin: f/m + noise of the last force evaluation
     RepState* bak_st;
     RepGlobal g[kRepMaxGlobals];
 };
@@ -1194,10 +1195,30 @@ __device__ __forceinline__ void rep_segment(const RepDesc& R, int nsteps, int ev
                 const double rmk = as_global(R.rmass)[p], c1k = as_global(R.c1)[p], c2k = as_global(R.c2)[p];
                 const double fxk = from_fix<Real>(F[3 * p + 0]), fyk = from_fix<Real>(F[3 * p + 1]);
                 const double fzk = from_fix<Real>(F[3 * p + 2]);
-                double Rx, Ry, Rz;
-                rep_noise<Real>(R.seed, step, p, Rx, Ry, Rz);
+                double Rx = 0.0, Ry = 0.0, Rz = 0.0;
+                if(R.integ != 4) {rep_noise<Real>(R.seed, step, p, Rx, Ry, Rz);}
                 double dpx, dpy, dpz;
-                if(R.integ == 1)
+                if(R.integ == 4)
+                {
+                    // UnderdampedLangevinIntegrator::step (:464-488) with the stored acceleration; a
+                    // segment that starts from restored state (forced rebuild) recomputes it from the
+                    // forces and the noise of the current count, the value that was stored
+                    double4 ac;
+                    if(s == 0 && s_force_rebuild)
+                    {
+                        double Ax, Ay, Az;
+                        rep_noise<Real>(R.seed, step, p, Ax, Ay, Az);
+                        ac.x = fxk * rmk + c2k * Ax; ac.y = fyk * rmk + c2k * Ay; ac.z = fzk * rmk + c2k * Az;
+                    }
+                    else {ac = R.accel[p];}
+                    const double g2 = c1k * R.halfdt, omg = 1 - g2, cdt = R.dt * omg;
+                    const double h2 = R.dt * R.dt / 2;
+                    dpx = cdt * vx[k] + h2 * ac.x; dpy = cdt * vy[k] + h2 * ac.y; dpz = cdt * vz[k] + h2 * ac.z;
+                    const double sv = omg * (omg * omg + g2), hv = R.halfdt * omg;
+                    vx[k] *= sv; vy[k] *= sv; vz[k] *= sv;
+                    vx[k] += hv * ac.x; vy[k] += hv * ac.y; vz[k] += hv * ac.z;
+                }
+                else if(R.integ == 1)
                 {
                     const double hrm = R.halfdt * rmk, eg = c1k, nc = c2k;
                     vx[k] += hrm * fxk; vy[k] += hrm * fyk; vz[k] += hrm * fzk;
@@ -1332,8 +1353,35 @@ __device__ __forceinline__ void rep_segment(const RepDesc& R, int nsteps, int ev
             if(ENERGY) {atomicAdd(&s_energy[R.n_local + g], e_pair);}
         }
         __syncthreads();
+        if(R.integ == 4)
+        {
+            // UnderdampedLangevin (UnderdampedLangevinIntegrator.hpp:439-446, :514-523): a = f/m +
+            // noise, the noise of the count after this step (of the current count in an evaluation
+            // launch, which reproduces the stored value), then the closing velocity update
+#pragma unroll
+            for(int k = 0; k < PPT; ++k)
+            {
+                const uint32_t p = tid + k * BS;
+                if(p >= n) {continue;}
+                double Rx, Ry, Rz;
+                rep_noise<Real>(R.seed, eval_only ? step : step + 1, p, Rx, Ry, Rz);
+                const double rm = as_global(R.rmass)[p], nc = as_global(R.c2)[p];
+                double4 ac;
+                ac.x = from_fix<Real>(F[3 * p + 0]) * rm + nc * Rx;
+                ac.y = from_fix<Real>(F[3 * p + 1]) * rm + nc * Ry;
+                ac.z = from_fix<Real>(F[3 * p + 2]) * rm + nc * Rz;
+                ac.w = 0.0;
+                R.accel[p] = ac;
+                if(!eval_only)
+                {
+                    const double kv = R.halfdt * (1 - as_global(R.c1)[p] * R.halfdt);
+                    vx[k] += kv * ac.x; vy[k] += kv * ac.y; vz[k] += kv * ac.z;
+                }
+            }
+            if(!eval_only) {++step;}
+        }
         // closing half-kick (BAOABLangevinIntegrator.hpp:222-229 / GJFNVTLangevinIntegrator.hpp:222-229)
-        if(!eval_only)
+        else if(!eval_only)
         {
 #pragma unroll
             for(int k = 0; k < PPT; ++k)

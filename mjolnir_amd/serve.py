@@ -74,7 +74,8 @@ class Job:
 
 def batch_eligible(spec: SystemSpec, max_n: int = 2048) -> bool:
     return (spec.n <= max_n and len(spec.local) <= 8 and len(spec.global_) <= 4 and spec.constraint is None
-            and spec.integrator is not None and spec.integrator.type in ("BAOABLangevin", "GJFNVTLangevin")
+            and spec.integrator is not None
+            and spec.integrator.type in ("BAOABLangevin", "GJFNVTLangevin", "UnderdampedLangevin")
             and all((l.interaction, l.potential) in _BATCH_LOCALS for l in spec.local)
             and all(g.potential in _BATCH_GLOBALS for g in spec.global_))
 

@@ -250,3 +250,26 @@ def test_dna_batch_fp64_trajectory_and_mix():
     x1, v1, _ = b1.state(r1)
     x2, v2, _ = b2.state(ids[2])
     assert np.array_equal(x1, x2) and np.array_equal(v1, v2)
+
+
+def test_fp64_batch_underdamped_vs_oracle_same_noise():
+    """UnderdampedLangevin in the replica kernel (FP64 mode): the stored acceleration, its noise
+    at the count after each step, the drift and both velocity updates against the oracle with
+    the same noise stream; a periodic LJ box and a chain in one batch."""
+    import oracle
+    from mjolnir_amd import capi, workloads
+    specs = [make_golden.underdamped(make_golden.perturb(workloads.lj_fluid(n_side=6, seed=41), 0.1, 41), 1.0),
+             make_golden.underdamped(_chains([150], seed0=420)[0], 0.5)]
+    b, ids = _batch(specs, capi.FP64)
+    b.step(200)
+    for spec, r in zip(specs, ids):
+        o = oracle.OracleEngine(spec)
+        o.step(200)
+        x, v, _ = b.state(r)
+        xo, vo, _ = o.state()
+        d = x - xo
+        if spec.boundary == "periodic":
+            w = spec.upper - spec.lower
+            d = d - w * np.round(d / w)
+        assert np.abs(d).max() < 1e-7, (spec.name, np.abs(d).max())
+        assert np.abs(v - vo).max() < 1e-7, (spec.name, np.abs(v - vo).max())
