@@ -249,6 +249,10 @@ mjg_status mjg_batch_initialize(mjg_batch* b);
 mjg_status mjg_batch_step(mjg_batch* b, size_t n_steps);
 /* Steps one replica still has to take under mjg_batch_advance. */
 mjg_status mjg_batch_set_steps(mjg_batch* b, int replica, uint64_t steps);
+/* Position of one replica's noise stream, as mjg_set_step_count for an engine: a replica restored
+ * from a checkpoint continues the stream (MsgPackSaver.hpp:70-88 saves the generator's state). Its
+ * lists, and an UnderdampedLangevin replica's acceleration, are rebuilt at its next step. */
+mjg_status mjg_batch_set_step_count(mjg_batch* b, int replica, uint64_t step);
 /* One launch of up to `segment` steps for every resident replica with steps remaining. */
 mjg_status mjg_batch_advance(mjg_batch* b, int segment);
 /* Free a replica's device memory. */

@@ -61,7 +61,7 @@ EXPORTED = [
     "mjg_observer_energy_output", "mjg_observer_xyz_open", "mjg_observer_dcd_open", "mjg_simulate", "mjg_integrator_initialize",
     "mjg_step", "mjg_counters", "mjg_kinetic_energy", "mjg_profile_interaction",
     "mjg_batch_create", "mjg_batch_destroy", "mjg_batch_last_error", "mjg_batch_add_replica",
-    "mjg_batch_set_attribute", "mjg_batch_add_local", "mjg_batch_add_global", "mjg_batch_set_integrator",
+    "mjg_batch_set_attribute", "mjg_batch_add_local", "mjg_batch_add_global", "mjg_batch_set_integrator", "mjg_batch_set_step_count",
     "mjg_batch_initialize", "mjg_batch_step", "mjg_batch_set_steps", "mjg_batch_advance", "mjg_batch_retire",
     "mjg_batch_get_state", "mjg_batch_replica_info", "mjg_batch_list_sizes", "mjg_batch_evaluate", "mjg_batch_counters",
     "mjg_batch_observe", "mjg_batch_energies",
@@ -122,6 +122,7 @@ def lib():
     L.mjg_batch_add_global.argtypes = [vp, i32, i32, dbl, dbl, i32, dbl, i32, sz, C.POINTER(C.c_char_p), _up, sz,
                                        _up, _dp]
     L.mjg_batch_set_integrator.argtypes = [vp, i32, i32, dbl, _dp, u64]
+    L.mjg_batch_set_step_count.argtypes = [vp, i32, u64]
     L.mjg_batch_initialize.argtypes = [vp]
     L.mjg_batch_step.argtypes = [vp, sz]
     L.mjg_batch_set_steps.argtypes = [vp, i32, u64]
@@ -368,6 +369,9 @@ class Batch:
         x = np.zeros(3 * n); v = np.zeros(3 * n); f = np.zeros(3 * n)
         self._check(lib().mjg_batch_get_state(self.h, r, x.ctypes.data, v.ctypes.data, f.ctypes.data))
         return x.reshape(-1, 3), v.reshape(-1, 3), f.reshape(-1, 3)
+
+    def set_step_count(self, r: int, step: int):
+        self._check(lib().mjg_batch_set_step_count(self.h, r, int(step)))
 
     def info(self, r: int):
         s, rem, rb, fp = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_size_t()

@@ -169,3 +169,35 @@ def load_engine(spec, prefix: str, precision: int = 0, device: int = 0):
     e = capi.Engine(sp, precision, device=device)
     e.set_step_count(step)
     return e
+
+
+def save_batch_replica(batch, r: int, prefix: str):
+    """The same two files for one replica of a batch (its state, its step count)."""
+    spec = batch.specs[r]
+    x, v, f = batch.state(r)
+    periodic = spec.boundary == "periodic"
+    with open(prefix + "_system.msg", "wb") as fh:
+        fh.write(encode_system(spec.mass, x, v, f, spec.lower if periodic else None, spec.upper if periodic else None,
+                               spec.attributes, names=spec.particle_names))
+    with open(prefix + "_rng.msg", "wb") as fh:
+        fh.write(encode_rng(spec.integrator.seed, batch.info(r)["step"]))
+
+
+def load_batch_replica(batch, spec, prefix: str) -> int:
+    """Add a replica for ``spec`` (interactions, integrator) to ``batch`` from a checkpoint, make it
+    resident and continue its noise stream; returns the replica id."""
+    import copy
+    with open(prefix + "_system.msg", "rb") as fh:
+        s = decode_system(fh.read())
+    with open(prefix + "_rng.msg", "rb") as fh:
+        seed, step = decode_rng(fh.read())
+    sp = copy.deepcopy(spec)
+    sp.mass, sp.pos, sp.vel = s["mass"], s["pos"], s["vel"]
+    if s["lower"] is not None:
+        sp.lower, sp.upper = s["lower"], s["upper"]
+    sp.attributes.update(s["attributes"])
+    sp.integrator.seed = seed
+    r = batch.add(sp)
+    batch.initialize()
+    batch.set_step_count(r, step)
+    return r

@@ -328,6 +328,19 @@ class Batch
 
     void set_steps(int id, uint64_t steps) {set_remaining(rep(id), steps);}
 
+    // position of a replica's noise stream (a restored checkpoint continues it); its lists, and
+    // an UnderdampedLangevin replica's stored acceleration, are rebuilt at its next step
+    void set_step_count(int id, uint64_t step)
+    {
+        Replica& r = rep(id);
+        if(!r.ready) {throw std::logic_error("replica not initialized");}
+        RepState st = state_of(r);
+        st.step = step;
+        st.force_rebuild = 1;
+        hip_check(hipMemcpyAsync(r.desc.st, &st, sizeof(RepState), hipMemcpyHostToDevice, stream_), "upload");
+        hip_check(hipStreamSynchronize(stream_), "sync");
+    }
+
     // run `seg` steps of the replicas whose remaining count is positive (serving)
     void advance(int seg)
     {
@@ -1153,6 +1166,10 @@ mjg_status mjg_batch_step(mjg_batch* b, size_t n_steps) {return bguard(b, [&](mj
 mjg_status mjg_batch_set_steps(mjg_batch* b, int replica, uint64_t steps)
 {
     return bguard(b, [&](mjg::Batch& x) {x.set_steps(replica, steps);});
+}
+mjg_status mjg_batch_set_step_count(mjg_batch* b, int replica, uint64_t step)
+{
+    return bguard(b, [&](mjg::Batch& x) {x.set_step_count(replica, step);});
 }
 mjg_status mjg_batch_advance(mjg_batch* b, int segment) {return bguard(b, [&](mjg::Batch& x) {x.advance(segment);});}
 mjg_status mjg_batch_retire(mjg_batch* b, int replica) {return bguard(b, [&](mjg::Batch& x) {x.retire(replica);});}

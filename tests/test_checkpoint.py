@@ -3,7 +3,7 @@
 CPU: the reference's own saver output (tests/golden/*_system.msg, written by oracle/_ref from
 the same spec) decodes to the spec's state and the oracle's forces bit for bit, and our encoder
 reproduces those files byte for byte.  GPU: an engine restored from its own checkpoint
-continues the trajectory of the uninterrupted run."""
+continues the trajectory of the uninterrupted run; a batch replica continues it bit for bit."""
 import os
 
 import numpy as np
@@ -96,3 +96,31 @@ def test_underdamped_checkpoint_restart_continues_the_noise_stream(tmp_path):
     d = d - w * np.round(d / w)
     assert np.abs(d).max() < 1e-8, np.abs(d).max()
     assert np.abs(va - vb).max() < 1e-6, np.abs(va - vb).max()
+
+
+@pytest.mark.gpu
+def test_batch_replica_checkpoint_restart_is_bitwise(tmp_path):
+    """A replica of the batch engine (FP32 mode: exact fixed-point force sums, so the rebuilt lists
+    change nothing) saved at step 300 and restored into a new batch continues bit for bit, for
+    BAOAB and UnderdampedLangevin."""
+    from mjolnir_amd import capi, workloads
+    specs = [make_golden.perturb(workloads.cg_chain(n=300, seed=51), 0.1, 51),
+             make_golden.underdamped(make_golden.perturb(workloads.cg_chain(n=200, seed=52), 0.1, 52), 0.5)]
+    a = capi.Batch(capi.FP32_FIXED)
+    ids = [a.add(s) for s in specs]
+    a.initialize()
+    print("[gpu] batch checkpoint restart, chains N=300 (BAOAB), 200 (UnderdampedLangevin)", flush=True)
+    a.step(300)
+    b = capi.Batch(capi.FP32_FIXED)
+    restored = []
+    for k, (s, r) in enumerate(zip(specs, ids)):
+        prefix = str(tmp_path / f"rep{k}")
+        checkpoint.save_batch_replica(a, r, prefix)
+        restored.append(checkpoint.load_batch_replica(b, s, prefix))
+    a.step(100)
+    b.step(100)
+    for r, q in zip(ids, restored):
+        xa, va, _ = a.state(r)
+        xb, vb, _ = b.state(q)
+        assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+        assert a.info(r)["step"] == b.info(q)["step"] == 400
