@@ -146,3 +146,69 @@ def test_engine_observers_vs_reference(tmp_path, name):
     # energy file beside them
     got, ref = open(prefix + ".ene").read().splitlines(), str(g["ene"]).splitlines()
     assert len(got) == len(ref) and got[:2] == ref[:2]
+
+
+# ----------------------------------------------------------------------------------------
+# host-side writers (batch replicas): byte layout against the reference's files
+# ----------------------------------------------------------------------------------------
+@pytest.mark.parametrize("name", list(make_golden.sim_cases().keys()))
+def test_host_writers_reproduce_reference_files(tmp_path, name):
+    """Fed the reference's own frames (the DCD's float32 values; the initial state for the first
+    XYZ frame), the host writers give the reference's DCD bytes but the title record, and its
+    XYZ lines; the molecule count comes from the spec's bonds as construct_molecules counts."""
+    from mjolnir_amd import trajectory
+    spec, g, total, save = _case(name)
+    _, fp = read_dcd(g["dcd_pos"].tobytes())
+    _, fv = read_dcd(g["dcd_vel"].tobytes())
+    width = (spec.upper - spec.lower) if spec.boundary == "periodic" else None
+    prefix = str(tmp_path / "w")
+    w = trajectory.DCDWriter(prefix, spec.n, total, save, spec.integrator.dt, trajectory.n_molecules(spec), width)
+    for (_, p), (_, v) in zip(fp, fv):
+        w.write(p.T, v.T)
+    w.close()
+    for key, suffix in (("dcd_pos", "_position.dcd"), ("dcd_vel", "_velocity.dcd")):
+        ref = g[key].tobytes()
+        got = open(prefix + suffix, "rb").read()
+        assert len(got) == len(ref)
+        assert got[:100] == ref[:100] and got[180:] == ref[180:]
+    x = trajectory.XYZWriter(prefix)
+    x.write(0, spec.pos, spec.vel)
+    for key, suffix in (("xyz_pos", "_position.xyz"), ("xyz_vel", "_velocity.xyz")):
+        n_lines = spec.n + 2
+        assert open(prefix + suffix).read().splitlines() == str(g[key]).splitlines()[:n_lines]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", list(make_golden.sim_cases().keys()))
+def test_batch_replica_frames_vs_reference(tmp_path, name):
+    """The same run as one FP64 batch replica, its frames written by the host writers between
+    batch steps: the reference's DCD header bytes but the title, its XYZ steps and names, and
+    coordinates within the zero-temperature trajectory bar."""
+    from mjolnir_amd import capi, trajectory
+    spec, g, total, save = _case(name)
+    b = capi.Batch(capi.FP64)
+    r = b.add(spec)
+    b.initialize()
+    prefix = str(tmp_path / "rep")
+    print(f"[gpu] batch replica frames {name}", flush=True)
+    trajectory.batch_replica_frames(b, r, total, save, prefix)
+    assert b.info(r)["step"] == total
+    for key, suffix in (("dcd_pos", "_position.dcd"), ("dcd_vel", "_velocity.dcd")):
+        ref = g[key].tobytes()
+        got = open(prefix + suffix, "rb").read()
+        assert len(got) == len(ref)
+        hdr_end = 92 + 92 + 12
+        assert got[:100] == ref[:100] and got[180:hdr_end] == ref[180:hdr_end]
+        for (cr, ar), (cg, ag) in zip(read_dcd(ref)[1], read_dcd(got)[1]):
+            assert cr == cg
+            assert np.allclose(ag, ar, rtol=1e-6, atol=1e-6)
+    # the replica kernel rounds each force contribution to 2^-32 kcal/mol/A (exact int64 sums);
+    # over 300 steps that moves positions by ~1e-8 and velocities, which follow the force
+    # directly, by up to a few 1e-7: the velocity file is held to the DCD frames' 1e-6
+    for key, suffix, bar in (("xyz_pos", "_position.xyz", 1e-7), ("xyz_vel", "_velocity.xyz", 1e-6)):
+        ref = read_xyz(str(g[key]))
+        got = read_xyz(open(prefix + suffix).read())
+        assert [(s, n) for s, n, _ in got] == [(s, n) for s, n, _ in ref]
+        for (_, _, a), (_, _, c) in zip(got, ref):
+            assert np.abs(a - c).max() <= bar
+    b.close()
