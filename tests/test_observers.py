@@ -212,3 +212,40 @@ def test_batch_replica_frames_vs_reference(tmp_path, name):
         for (_, _, a), (_, _, c) in zip(got, ref):
             assert np.abs(a - c).max() <= bar
     b.close()
+
+
+def test_molecule_count_of_mixed_topology():
+    """construct_molecules on two bonded chains (10 and 6 beads), an angle written with its
+    indices out of order whose "bond" edges join two more beads, a "contact" edge (not a bond)
+    and free particles: the count equals the connected components of the bond graph, found
+    here by breadth-first search, and contact edges join nothing."""
+    from collections import deque
+    from mjolnir_amd import trajectory
+    from mjolnir_amd.spec import LocalSpec, SystemSpec
+    n = 24
+    bonds = [(i, i + 1) for i in range(9)] + [(i, i + 1) for i in range(10, 15)]
+    spec = SystemSpec("unlimited", np.ones(n), np.zeros((n, 3)), np.zeros((n, 3)))
+    spec.local.append(LocalSpec("BondLength", "Harmonic", "bond", np.array(bonds, dtype=np.uint32),
+                                np.ones((len(bonds), 2))))
+    spec.local.append(LocalSpec("BondAngle", "Gaussian", "bond", np.array([[18, 16, 17]], dtype=np.uint32),
+                                np.ones((1, 3))))
+    spec.local.append(LocalSpec("Contact", "GoContact", "contact", np.array([[20, 21]], dtype=np.uint32),
+                                np.ones((1, 2))))
+    adj = {i: set() for i in range(n)}
+    for a, b in bonds + [(18, 16), (18, 17), (16, 17)]:
+        adj[a].add(b)
+        adj[b].add(a)
+    seen, components = set(), 0
+    for s in range(n):
+        if s in seen:
+            continue
+        components += 1
+        todo = deque([s])
+        seen.add(s)
+        while todo:
+            for t in adj[todo.popleft()]:
+                if t not in seen:
+                    seen.add(t)
+                    todo.append(t)
+    assert components == 2 + 1 + 5      # two chains, the angle's three beads, 19..23 free
+    assert trajectory.n_molecules(spec) == components
