@@ -76,14 +76,15 @@ static size_t align256(size_t x) {return (x + 255) & ~(size_t)255;}
 // Skin of a replica's neighbour list.  The replica kernel's forces are exact fixed-point sums
 // over the pairs within the cutoff, so the skin changes no result, only the balance between
 // the pair loop (list entries grow as (1 + m)^3) and the rebuilds (their rate grows as 1 / m):
-// the kernel uses its own margin, at most the input's -- 0.35 for the short-ranged LJ and
+// the kernel uses its own margin, at most the input's -- 0.4 for the short-ranged LJ and
 // excluded-volume pairs, 0.1 for Debye-Hueckel, whose cutoff spans many particle spacings
-// (measured best on 2000-bead chains; MJG_LIST_MARGIN_SHORT / MJG_LIST_MARGIN_DH override them).
+// (measured best on the 512-chain mix of 50-2000 beads; MJG_LIST_MARGIN_SHORT /
+// MJG_LIST_MARGIN_DH override them).
 static double list_margin(int pot, double input_margin)
 {
     static const double m_short = [] {
         const char* c = std::getenv("MJG_LIST_MARGIN_SHORT");
-        return c ? std::atof(c) : 0.35;
+        return c ? std::atof(c) : 0.4;
     }();
     static const double m_dh = [] {
         const char* c = std::getenv("MJG_LIST_MARGIN_DH");
