@@ -245,7 +245,13 @@ mjg_status mjg_batch_set_integrator(mjg_batch* b, int replica, int type, double 
 /* ForceField::initialize + Integrator::initialize for every replica added since the last call
  * (ForceField.hpp:47-63, BAOABLangevinIntegrator.hpp:135-168); they become resident. */
 mjg_status mjg_batch_initialize(mjg_batch* b);
-/* Every resident replica takes n_steps Integrator::step calls (BAOABLangevinIntegrator.hpp:170-240). */
+/* Every resident replica takes n_steps Integrator::step calls (BAOABLangevinIntegrator.hpp:170-240).
+ * A replica whose positions or force contributions stop being finite is stopped at that step and
+ * takes no further steps; the other replicas complete the call, which then returns
+ * MJG_NAN_DETECTED (mjg_batch_last_error names the replicas; mjg_batch_replica_status tells each
+ * one's).  The reference has one system per process and only logs a NaN energy
+ * (GlobalForceField.hpp:181-186); here one failing replica never stops the others.  The same holds
+ * for mjg_batch_initialize, mjg_batch_advance and mjg_batch_profile. */
 mjg_status mjg_batch_step(mjg_batch* b, size_t n_steps);
 /* Steps one replica still has to take under mjg_batch_advance. */
 mjg_status mjg_batch_set_steps(mjg_batch* b, int replica, uint64_t steps);
@@ -258,6 +264,9 @@ mjg_status mjg_batch_advance(mjg_batch* b, int segment);
 /* Free a replica's device memory. */
 mjg_status mjg_batch_retire(mjg_batch* b, int replica);
 mjg_status mjg_batch_get_state(mjg_batch* b, int replica, double* pos, double* vel, double* force);
+/* MJG_OK for a replica that steps normally, MJG_NAN_DETECTED for one stopped with non-finite state
+ * (it keeps its last state until mjg_batch_retire). */
+mjg_status mjg_batch_replica_status(mjg_batch* b, int replica, int* status);
 /* Steps taken, steps remaining, list rebuilds and device bytes of one replica. */
 mjg_status mjg_batch_replica_info(mjg_batch* b, int replica, uint64_t* step, uint64_t* remaining, uint64_t* rebuilds,
                                   size_t* footprint_bytes);

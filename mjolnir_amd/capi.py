@@ -63,7 +63,7 @@ EXPORTED = [
     "mjg_batch_create", "mjg_batch_destroy", "mjg_batch_last_error", "mjg_batch_add_replica",
     "mjg_batch_set_attribute", "mjg_batch_add_local", "mjg_batch_add_global", "mjg_batch_set_integrator", "mjg_batch_set_step_count",
     "mjg_batch_initialize", "mjg_batch_step", "mjg_batch_set_steps", "mjg_batch_advance", "mjg_batch_retire",
-    "mjg_batch_get_state", "mjg_batch_replica_info", "mjg_batch_list_sizes", "mjg_batch_evaluate", "mjg_batch_counters",
+    "mjg_batch_get_state", "mjg_batch_replica_status", "mjg_batch_replica_info", "mjg_batch_list_sizes", "mjg_batch_evaluate", "mjg_batch_counters",
     "mjg_batch_observe", "mjg_batch_energies",
     "mjg_batch_profile",
 ]
@@ -130,6 +130,7 @@ def lib():
     L.mjg_batch_retire.argtypes = [vp, i32]
     L.mjg_batch_get_state.argtypes = [vp, i32, vp, vp, vp]
     L.mjg_batch_replica_info.argtypes = [vp, i32, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), C.POINTER(sz)]
+    L.mjg_batch_replica_status.argtypes = [vp, i32, C.POINTER(i32)]
     L.mjg_batch_evaluate.argtypes = [vp, i32, vp, C.POINTER(dbl), vp]
     L.mjg_batch_observe.argtypes = [vp]
     L.mjg_batch_energies.argtypes = [vp, i32, vp, C.POINTER(dbl)]
@@ -294,6 +295,10 @@ class Batch:
 
     ``add(spec)`` uploads one ``SystemSpec`` as a replica and returns its id; ``initialize()``
     makes every new replica resident (lists, first forces); ``step(n)`` advances all of them.
+
+    A replica whose state stops being finite is stopped by the library while the others go on:
+    ``initialize``, ``step``, ``advance`` and ``profile`` then record its id in ``failed`` (and
+    ``step`` / ``advance`` / ``initialize`` return the ids that failed during that call).
     """
 
     def __init__(self, precision: int = FP32_FIXED, device: int = 0):
@@ -304,11 +309,28 @@ class Batch:
             raise MjgError(st, "mjg_batch_create failed (is a GPU present?)")
         self.h = h
         self.specs = {}
+        self.failed = set()
 
     def _check(self, st: int):
         if st != OK:
             msg = lib().mjg_batch_last_error(self.h)
             raise MjgError(st, msg.decode() if msg else "")
+
+    def _check_run(self, st: int):
+        """Status of a call that advances replicas: MJG_NAN_DETECTED names replicas that were stopped
+        (the others completed the call); returns their ids."""
+        if st != NAN_DETECTED:
+            self._check(st)
+            return []
+        new = [r for r in self.specs if r not in self.failed and self.status(r) != OK]
+        self.failed.update(new)
+        return new
+
+    def status(self, r: int) -> int:
+        """OK, or NAN_DETECTED for a replica stopped with non-finite state."""
+        out = C.c_int()
+        self._check(lib().mjg_batch_replica_status(self.h, r, C.byref(out)))
+        return out.value
 
     def close(self):
         if getattr(self, "h", None):
@@ -349,20 +371,21 @@ class Batch:
         return r
 
     def initialize(self):
-        self._check(lib().mjg_batch_initialize(self.h))
+        return self._check_run(lib().mjg_batch_initialize(self.h))
 
     def step(self, n: int):
-        self._check(lib().mjg_batch_step(self.h, int(n)))
+        return self._check_run(lib().mjg_batch_step(self.h, int(n)))
 
     def set_steps(self, r: int, n: int):
         self._check(lib().mjg_batch_set_steps(self.h, r, int(n)))
 
     def advance(self, segment: int):
-        self._check(lib().mjg_batch_advance(self.h, int(segment)))
+        return self._check_run(lib().mjg_batch_advance(self.h, int(segment)))
 
     def retire(self, r: int):
         self._check(lib().mjg_batch_retire(self.h, r))
         self.specs.pop(r, None)
+        self.failed.discard(r)
 
     def state(self, r: int):
         n = self.specs[r].n
@@ -412,5 +435,5 @@ class Batch:
 
     def profile(self, segment_steps: int, reps: int = 3):
         ms, ps = C.c_double(), C.c_uint64()
-        self._check(lib().mjg_batch_profile(self.h, segment_steps, reps, C.byref(ms), C.byref(ps)))
+        self._check_run(lib().mjg_batch_profile(self.h, segment_steps, reps, C.byref(ms), C.byref(ps)))
         return ms.value, ps.value

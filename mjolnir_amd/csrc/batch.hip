@@ -20,6 +20,8 @@
 
 namespace mjg
 {
+// the segment kernels are compiled in batch_seg.hip
+MJG_REP_ALL_INSTANCES(extern)
 
 struct BatchDeviceError : std::runtime_error {using std::runtime_error::runtime_error;};
 struct BatchNonFinite : std::runtime_error {using std::runtime_error::runtime_error;};
@@ -64,6 +66,7 @@ struct Replica
     int cls = 0;                   // size class (block shape)
     bool ready = false, active = false;
     bool observed = false;          // an energy observation is held for this replica
+    bool failed = false;            // stopped with a non-finite position or force; never stepped again
     bool tsp = false;               // has 3SPN2 terms: runs in the kernel variant that holds them
 };
 
@@ -306,11 +309,19 @@ class Batch
             todo.swap(again);
         }
         if(!todo.empty()) {throw BatchDeviceError("replica list capacity did not converge");}
-        for(int id : fresh) {reps_[id]->active = true;}
+        download_states();
+        for(int id : fresh)
+        {
+            Replica& r = *reps_[id];
+            r.active = true;
+            // non-finite input (positions or first forces): the replica never steps
+            if(h_states_[r.slot].nonfinite) {r.failed = true; r.active = false; failed_now_.push_back(id);}
+        }
         descs_dirty_ = true;
     }
 
-    // Advance every active replica by up to n steps (fewer if it has fewer remaining).
+    // Advance every active replica by up to n steps (fewer if it has fewer remaining).  A replica
+    // that stops with non-finite state is deactivated; the others go on (failures_of_call()).
     void step(size_t n)
     {
         std::vector<int> act = active_ids();
@@ -322,10 +333,17 @@ class Batch
         while(done < n)
         {
             const int seg = (int)std::min<size_t>(segment_, n - done);
+            act = active_ids();
+            if(act.empty()) {break;}
             run_segment(act, seg);
             done += seg;
         }
     }
+
+    // replicas that failed (non-finite state) during the calls since the last clear
+    const std::vector<int>& failures_of_call() const {return failed_now_;}
+    void clear_failures() {failed_now_.clear();}
+    bool failed(int id) {return rep(id).failed;}
 
     void set_steps(int id, uint64_t steps) {set_remaining(rep(id), steps);}
 
@@ -460,7 +478,7 @@ class Batch
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
         download_states();
-        for(int id : act) {if(h_states_[reps_[id]->slot].nonfinite) {throw BatchNonFinite("non-finite position in a replica");}}
+        note_failures(act);
         if(particle_steps) {*particle_steps = ps * (uint64_t)nsteps;}
         return (double)ms / reps;
     }
@@ -889,13 +907,7 @@ class Batch
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRepLdsLimit),
                   "LDS attribute");
     }
-    // block shape of each size class: threads and particles per thread (the FP64 kernel needs
-    // more registers per thread, so its largest class runs 512 threads with 4 particles each)
-    template<typename Real, int C> struct Shape
-    {
-        static constexpr int bs = C == 0 ? 256 : sizeof(Real) == 8 ? 512 : 1024;
-        static constexpr int ppt = (int)(kRepClassMaxN[C] / bs);
-    };
+    template<typename Real, int C> using Shape = RepShape<Real, C>;
     template<typename Real, int C>
     void set_lds_limits_class()
     {
@@ -1024,17 +1036,32 @@ class Batch
         return s;
     }
 
+    // A replica whose state stopped being finite is taken out of the launches (its workgroup
+    // already ended its segment there); it stays queryable until it is retired.
+    void note_failures(const std::vector<int>& ids)
+    {
+        for(int id : ids)
+        {
+            Replica& r = *reps_[id];
+            if(r.failed || !h_states_[r.slot].nonfinite) {continue;}
+            r.failed = true;
+            r.active = false;
+            failed_now_.push_back(id);
+            descs_dirty_ = true;
+        }
+    }
+
     void run_segment(const std::vector<int>& act, int seg)
     {
         launch(act, seg, 0);
         download_states();
+        note_failures(act);
         // a replica whose list overflowed reruns the segment from its segment-start copy
         std::vector<int> again;
         for(int id : act)
         {
             const RepState& s = h_states_[reps_[id]->slot];
-            if(s.nonfinite) {throw BatchNonFinite("non-finite position in a replica");}
-            if(s.overflow) {again.push_back(id);}
+            if(s.overflow && !reps_[id]->failed) {again.push_back(id);}
         }
         for(int pass = 0; pass < 6 && !again.empty(); ++pass)
         {
@@ -1042,8 +1069,9 @@ class Batch
             for(int id : again) {grow_if_overflow(*reps_[id]);}
             launch(again, seg, 0);
             download_states();
+            note_failures(again);
             std::vector<int> still;
-            for(int id : again) {if(h_states_[reps_[id]->slot].overflow) {still.push_back(id);}}
+            for(int id : again) {if(h_states_[reps_[id]->slot].overflow && !reps_[id]->failed) {still.push_back(id);}}
             again.swap(still);
         }
         if(!again.empty()) {throw BatchDeviceError("replica list capacity did not converge");}
@@ -1066,6 +1094,7 @@ class Batch
     size_t states_cap_ = 0, n_slots_ = 0;
     std::vector<RepState> h_states_;
     std::vector<int> free_slots_;
+    std::vector<int> failed_now_;
 };
 
 } // mjg
@@ -1087,8 +1116,18 @@ mjg_status bguard(mjg_batch* b, F&& f)
     if(b == nullptr || b->impl == nullptr) {return MJG_INVALID_ARG;}
     try
     {
+        b->impl->clear_failures();
         f(*b->impl);
         b->err.clear();
+        // replicas that stopped with non-finite state during this call: the others completed it
+        const std::vector<int>& bad = b->impl->failures_of_call();
+        if(!bad.empty())
+        {
+            b->err = "non-finite state in replica(s)";
+            for(int id : bad) {b->err += " " + std::to_string(id);}
+            b->err += "; they were stopped, the other replicas completed the call";
+            return MJG_NAN_DETECTED;
+        }
         return MJG_OK;
     }
     catch(const mjg::BatchNonFinite& x) {b->err = x.what(); return MJG_NAN_DETECTED;}
@@ -1177,6 +1216,11 @@ mjg_status mjg_batch_retire(mjg_batch* b, int replica) {return bguard(b, [&](mjg
 mjg_status mjg_batch_get_state(mjg_batch* b, int replica, double* pos, double* vel, double* force)
 {
     return bguard(b, [&](mjg::Batch& x) {x.get_state(replica, pos, vel, force);});
+}
+mjg_status mjg_batch_replica_status(mjg_batch* b, int replica, int* status)
+{
+    if(status == nullptr) {return MJG_INVALID_ARG;}
+    return bguard(b, [&](mjg::Batch& x) {*status = x.failed(replica) ? MJG_NAN_DETECTED : MJG_OK;});
 }
 mjg_status mjg_batch_replica_info(mjg_batch* b, int replica, uint64_t* step, uint64_t* remaining, uint64_t* rebuilds,
                                   size_t* footprint_bytes)

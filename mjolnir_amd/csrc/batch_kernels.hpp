@@ -140,34 +140,37 @@ __host__ __device__ inline size_t rep_lds_bytes(uint32_t n, uint32_t par_cols, u
 {
     // cell ends (uint32), cell keys (uint16) and the cell-sorted entries (uint32) of the participants
     const size_t grid = grid_np ? (size_t)grid_cells(grid_bits) * 4 + (size_t)((grid_np + 7u) & ~7u) * 6 : 0;
-    return (size_t)n * 48 + (size_t)n * par_cols * real_size + (size_t)part_total * 2 + grid;
+    return (size_t)n * 48 + 8 + (size_t)n * par_cols * real_size + (size_t)part_total * 2 + grid;
 }
 
 // ------------------------------------------------------------------------------------
 // fixed-point accumulation: a force contribution v becomes the integer nearest v * 2^S
 // (round half to even, so conv(-v) == -conv(v)); S = 32 in the FP64 kernel, 24 in the FP32
 // kernel, where a contribution carries about 24 significant bits anyway.
+//
+// A contribution that is not finite, or whose image would not fit in 62 bits, has no integer to
+// become: it adds nothing and sets the flag word F[-1] just ahead of the replica's force sums
+// instead (a NaN would otherwise turn into an arbitrary finite integer and go unnoticed); the
+// segment then ends the replica as non-finite.
 // ------------------------------------------------------------------------------------
+constexpr double kFixLimit = 4611686018427387904.0;    // 2^62
+__device__ __forceinline__ long long fix_checked(double s, unsigned long long* F)
+{
+    if(!(fabs(s) < kFixLimit)) {atomicOr(&F[-1], 1ull); return 0;}   // NaN fails the test too
+    return __double2ll_rn(s);
+}
 template<typename Real> struct Fix;
 template<> struct Fix<double>
 {
     static constexpr double scale = 4294967296.0;          // 2^32
     static constexpr double inv = 2.3283064365386963e-10;  // 2^-32
-    __device__ static __forceinline__ long long conv(double v) {return __double2ll_rn(v * scale);}
+    __device__ static __forceinline__ long long conv(double v, unsigned long long* F) {return fix_checked(v * scale, F);}
 };
 template<> struct Fix<float>
 {
     static constexpr double scale = 16777216.0;            // 2^24
     static constexpr double inv = 5.9604644775390625e-08;  // 2^-24
-    __device__ static __forceinline__ long long conv(double v) {return __double2ll_rn(v * scale);}
-    // v 2^24 is exact in FP32, and rounding it to an integer in FP32 gives the integer the FP64
-    // path gives; below 2^31 that is a 32-bit conversion (two instructions instead of a dozen)
-    __device__ static __forceinline__ long long conv(float v)
-    {
-        const float s = v * 16777216.0f;
-        if(fabsf(s) < 2147483520.0f) {return (long long)__float2int_rn(s);}
-        return __double2ll_rn((double)v * scale);
-    }
+    __device__ static __forceinline__ long long conv(double v, unsigned long long* F) {return fix_checked(v * scale, F);}
 };
 
 // Reciprocal, reciprocal square root, square root and exp of the pair and local arithmetic.
@@ -200,15 +203,21 @@ __device__ __forceinline__ void add3(unsigned long long* F, uint32_t i, long lon
 // are below 2^31 units the 32-bit conversion serves them all, else every lane takes the FP64
 // route; both give the same integers, so the choice is made once per wave (one uniform branch)
 // instead of per component and lane.
+// (v 2^24 is exact in FP32, and rounding it to an integer in FP32 gives the integer the FP64 path
+// gives; below 2^31 that is a 32-bit conversion, two instructions instead of a dozen.  A NaN
+// fails the size test and takes the checked FP64 route.)
 template<typename Real>
-__device__ __forceinline__ void fix3r(Real x, Real y, Real z, long long& a, long long& b, long long& c);
+__device__ __forceinline__ void fix3r(Real x, Real y, Real z, long long& a, long long& b, long long& c,
+                                      unsigned long long* F);
 template<>
-__device__ __forceinline__ void fix3r<double>(double x, double y, double z, long long& a, long long& b, long long& c)
+__device__ __forceinline__ void fix3r<double>(double x, double y, double z, long long& a, long long& b, long long& c,
+                                              unsigned long long* F)
 {
-    a = Fix<double>::conv(x); b = Fix<double>::conv(y); c = Fix<double>::conv(z);
+    a = Fix<double>::conv(x, F); b = Fix<double>::conv(y, F); c = Fix<double>::conv(z, F);
 }
 template<>
-__device__ __forceinline__ void fix3r<float>(float x, float y, float z, long long& a, long long& b, long long& c)
+__device__ __forceinline__ void fix3r<float>(float x, float y, float z, long long& a, long long& b, long long& c,
+                                             unsigned long long* F)
 {
     const float sx = x * 16777216.0f, sy = y * 16777216.0f, sz = z * 16777216.0f;
     const bool small = fmaxf(fabsf(sx), fmaxf(fabsf(sy), fabsf(sz))) < 2147483520.0f;
@@ -218,14 +227,14 @@ __device__ __forceinline__ void fix3r<float>(float x, float y, float z, long lon
     }
     else
     {
-        a = Fix<float>::conv((double)x); b = Fix<float>::conv((double)y); c = Fix<float>::conv((double)z);
+        a = Fix<float>::conv((double)x, F); b = Fix<float>::conv((double)y, F); c = Fix<float>::conv((double)z, F);
     }
 }
 template<typename Real, typename V>
 __device__ __forceinline__ void acc3(unsigned long long* F, uint32_t i, V x, V y, V z)
 {
     long long a, b, c;
-    fix3r<Real>((Real)x, (Real)y, (Real)z, a, b, c);
+    fix3r<Real>((Real)x, (Real)y, (Real)z, a, b, c, F);
     add3(F, i, a, b, c);
 }
 
@@ -358,7 +367,7 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         const double rlen = 1.0 / sqrt(len2);
         const double len = len2 * rlen;
         const double c = (-1 * (2 * p0 * (len - p1))) * rlen;
-        const long long fx = Fix<Real>::conv(dx * c), fy = Fix<Real>::conv(dy * c), fz = Fix<Real>::conv(dz * c);
+        const long long fx = Fix<Real>::conv(dx * c, F), fy = Fix<Real>::conv(dy * c, F), fz = Fix<Real>::conv(dz * c, F);
         add3(F, i0, -fx, -fy, -fz);
         add3(F, i1, fx, fy, fz);
         if(ENERGY) {const double dv = len - p1; e = p0 * dv * dv;}
@@ -380,7 +389,7 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         const Real v0r_12 = v0r_10 * v0r_2;
         const Real coef = (Real)(-60) * k * r2 * (v0r_10 - v0r_12);
         long long fx, fy, fz;
-        fix3r<Real>(coef * dx, coef * dy, coef * dz, fx, fy, fz);
+        fix3r<Real>(coef * dx, coef * dy, coef * dz, fx, fy, fz, F);
         add3(F, i0, -fx, -fy, -fz);
         add3(F, i1, fx, fy, fz);
         if(ENERGY) {e = (double)(k * ((Real)5 * v0r_12 - (Real)6 * v0r_10));}
@@ -423,8 +432,8 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         const Real Fkx = ck * (cos_t * ekx - eix), Fky = ck * (cos_t * eky - eiy), Fkz = ck * (cos_t * ekz - eiz);
         // the middle body takes minus the sum of the ends, in integers (exact momentum balance)
         long long ix, iy, iz, kx, ky, kz;
-        fix3r<Real>(Fix_, Fiy, Fiz, ix, iy, iz);
-        fix3r<Real>(Fkx, Fky, Fkz, kx, ky, kz);
+        fix3r<Real>(Fix_, Fiy, Fiz, ix, iy, iz, F);
+        fix3r<Real>(Fkx, Fky, Fkz, kx, ky, kz, F);
         add3(F, i0, ix, iy, iz);
         add3(F, i1, -(ix + kx), -(iy + ky), -(iz + kz));
         add3(F, i2, kx, ky, kz);
@@ -486,7 +495,7 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         const double len = len2 * rlen;
         const double dv = len - p1;
         const double c = (-1 * (2 * p0 * (dv + 200 * dv * dv * dv))) * rlen;
-        const long long fx = Fix<Real>::conv(dx * c), fy = Fix<Real>::conv(dy * c), fz = Fix<Real>::conv(dz * c);
+        const long long fx = Fix<Real>::conv(dx * c, F), fy = Fix<Real>::conv(dy * c, F), fz = Fix<Real>::conv(dz * c, F);
         add3(F, i0, -fx, -fy, -fz);
         add3(F, i1, fx, fy, fz);
         if(ENERGY) {const double dv2 = dv * dv; e = p0 * (dv2 + 100 * dv2 * dv2);}
@@ -614,7 +623,7 @@ __device__ __forceinline__ void rep_pair(const RepGlobal& G, const Real* par, ui
         if(ENERGY) {e += (double)(qq * (ex * rl - (Real)G.coef_at_cutoff));}
     }
     long long fx, fy, fz;
-    fix3r<Real>(dx * c, dy * c, dz * c, fx, fy, fz);
+    fix3r<Real>(dx * c, dy * c, dz * c, fx, fy, fz, F);
     add3(F, i, fx, fy, fz);
     add3(F, j, -fx, -fy, -fz);
 }
@@ -690,7 +699,7 @@ __device__ __forceinline__ void rep_pair_k(const PairK<Real>& K, const Real* par
         if(ENERGY) {e += (double)(qq * (ex * rl - K.cac));}
     }
     long long fx, fy, fz;
-    fix3r<Real>(dx * c, dy * c, dz * c, fx, fy, fz);
+    fix3r<Real>(dx * c, dy * c, dz * c, fx, fy, fz, F);
     add3(F, i, fx, fy, fz);
     add3(F, j, -fx, -fy, -fz);
 }
@@ -1087,8 +1096,8 @@ __device__ void rep_build_grid(const RepGlobal& G, const RepDesc& R, const Box& 
 // one segment: `nsteps` steps (fewer if the replica has fewer remaining), or a pure force /
 // energy evaluation when eval_only (the lists are rebuilt first if the state asks for it)
 //
-// LDS: x[n], y[n], z[n] (double); F[3n] (int64 fixed point); parameters[par_cols][n] as
-// npar-column rows per interaction (Real); participant lists (uint16)
+// LDS: x[n], y[n], z[n] (double); the flag word F[-1] and F[3n] (int64 fixed point);
+// parameters[par_cols][n] as npar-column rows per interaction (Real); participant lists (uint16)
 // ------------------------------------------------------------------------------------
 // Workgroup-wide scalars of a segment (static LDS of the kernel)
 template<int BS> struct SegShared
@@ -1118,7 +1127,7 @@ __device__ __forceinline__ void rep_segment(const RepDesc& R, int nsteps, int ev
     P.x = reinterpret_cast<double*>(smem);
     P.y = P.x + n;
     P.z = P.y + n;
-    unsigned long long* F = reinterpret_cast<unsigned long long*>(P.z + n);
+    unsigned long long* F = reinterpret_cast<unsigned long long*>(P.z + n) + 1;
     Real* par_lds = reinterpret_cast<Real*>(F + 3 * n);
     uint16_t* part_lds = reinterpret_cast<uint16_t*>(par_lds + (size_t)n * R.par_cols);
     uint32_t* grid_end = reinterpret_cast<uint32_t*>(part_lds + R.part_total);
@@ -1154,9 +1163,9 @@ __device__ __forceinline__ void rep_segment(const RepDesc& R, int nsteps, int ev
             const double4 a = R.pos[p], v = R.vel[p], f = R.force[p];
             vx[k] = v.x; vy[k] = v.y; vz[k] = v.z;
             P.x[p] = a.x; P.y[p] = a.y; P.z[p] = a.z;
-            F[3 * p + 0] = (unsigned long long)Fix<Real>::conv(f.x);
-            F[3 * p + 1] = (unsigned long long)Fix<Real>::conv(f.y);
-            F[3 * p + 2] = (unsigned long long)Fix<Real>::conv(f.z);
+            F[3 * p + 0] = (unsigned long long)Fix<Real>::conv(f.x, F);
+            F[3 * p + 1] = (unsigned long long)Fix<Real>::conv(f.y, F);
+            F[3 * p + 2] = (unsigned long long)Fix<Real>::conv(f.z, F);
             if(!eval_only) {R.bak_pos[p] = a; R.bak_vel[p] = v; R.bak_force[p] = f;}
         }
     }
@@ -1173,12 +1182,15 @@ __device__ __forceinline__ void rep_segment(const RepDesc& R, int nsteps, int ev
         s_force_rebuild = st.force_rebuild;
         s_rebuilds = st.rebuilds;
         s_flags = 0;
+        // the stored forces came out of fixed point, so their conversion above never sets the flag
+        F[-1] = 0ull;
         for(int g = 0; g < ng; ++g) {s_margin[g] = st.margin[g]; s_npairs[g] = st.npairs[g]; s_maxpairs[g] = st.maxpairs[g];}
     }
     __syncthreads();
     unsigned long long steps_to_do = eval_only ? 1ull : (unsigned long long)nsteps;
     if(!eval_only && st.remaining < steps_to_do) {steps_to_do = st.remaining;}
     uint64_t step = st.step;
+    unsigned long long done = 0;
 
     for(unsigned long long s = 0; s < steps_to_do; ++s)
     {
@@ -1248,6 +1260,10 @@ __device__ __forceinline__ void rep_segment(const RepDesc& R, int nsteps, int ev
             if(bad) {atomicOr(&s_flags, 2);}
         }
         __syncthreads();
+        // A replica whose positions or force sums stopped being finite ends its segment here.
+        // Neither word is written between the barrier above and the next one (positions flag
+        // in the first half, force flag in the force phase), so every thread takes the same branch.
+        if(!eval_only && ((s_flags & 2) || F[-1] != 0ull)) {break;}
         for(uint32_t q = tid; q < 3 * n; q += BS) {F[q] = 0ull;}
         // skin check (ForceField::reduce_margin): 2 sqrt(max |dp|^2) off every margin
         d2 = wave_max(d2);
@@ -1395,6 +1411,7 @@ __device__ __forceinline__ void rep_segment(const RepDesc& R, int nsteps, int ev
             }
             ++step;
         }
+        ++done;
     }
 #pragma unroll
     for(int k = 0; k < PPT; ++k)
@@ -1424,11 +1441,13 @@ __device__ __forceinline__ void rep_segment(const RepDesc& R, int nsteps, int ev
     if(tid == 0)
     {
         if(ENERGY) {st.kinetic = ke;}
-        if(!eval_only) {st.step = step; st.remaining -= steps_to_do;}
+        if(!eval_only) {st.step = step; st.remaining -= done;}
         st.rebuilds = s_rebuilds;
-        st.force_rebuild = 0;
+        // the forced rebuild (restored or fresh state) is spent only by a launch that ran a step
+        // or an evaluation for this replica
+        if(done > 0) {st.force_rebuild = 0;}
         if(s_flags & 1) {st.overflow = 1;}
-        if(s_flags & 2) {st.nonfinite = 1;}
+        if((s_flags & 2) || F[-1] != 0ull) {st.nonfinite = 1;}
         for(int g = 0; g < ng; ++g) {st.margin[g] = s_margin[g]; st.npairs[g] = s_npairs[g]; st.maxpairs[g] = s_maxpairs[g];}
         if(ENERGY) {for(int q = 0; q < R.n_local + ng; ++q) {st.energy[q] = s_energy[q];}}
     }
@@ -1444,5 +1463,28 @@ k_replica_segment(const RepDesc* __restrict__ reps, int nsteps, int eval_only)
     if(R.box.periodic) {rep_segment<Real, ENERGY, BS, PPT, TSP, true>(R, nsteps, eval_only, smem, sh);}
     else {rep_segment<Real, ENERGY, BS, PPT, TSP, false>(R, nsteps, eval_only, smem, sh);}
 }
+
+// Block shape of each size class: threads and particles per thread (the FP64 kernel needs more
+// registers per thread, so its two larger classes run 512 threads with 2 and 4 particles each).
+template<typename Real, int C> struct RepShape
+{
+    static constexpr int bs = C == 0 ? 256 : sizeof(Real) == 8 ? 512 : 1024;
+    static constexpr int ppt = (int)(kRepClassMaxN[C] / bs);
+};
+
+// The kernel's instantiations: one per (Real, ENERGY, size class, TSP).  They are compiled in
+// separate translation units (batch_seg.hip, once per Real / ENERGY / TSP), and every other
+// translation unit only declares them.
+#define MJG_REP_KERNEL(Real, ENERGY, C, TSP) \
+    k_replica_segment<Real, ENERGY, RepShape<Real, C>::bs, RepShape<Real, C>::ppt, TSP>
+#define MJG_REP_INSTANCES(PREFIX, Real, ENERGY, TSP) \
+    PREFIX template __global__ void MJG_REP_KERNEL(Real, ENERGY, 0, TSP)(const RepDesc* __restrict__, int, int); \
+    PREFIX template __global__ void MJG_REP_KERNEL(Real, ENERGY, 1, TSP)(const RepDesc* __restrict__, int, int); \
+    PREFIX template __global__ void MJG_REP_KERNEL(Real, ENERGY, 2, TSP)(const RepDesc* __restrict__, int, int);
+#define MJG_REP_ALL_INSTANCES(PREFIX) \
+    MJG_REP_INSTANCES(PREFIX, float, false, false) MJG_REP_INSTANCES(PREFIX, float, false, true) \
+    MJG_REP_INSTANCES(PREFIX, float, true, false) MJG_REP_INSTANCES(PREFIX, float, true, true) \
+    MJG_REP_INSTANCES(PREFIX, double, false, false) MJG_REP_INSTANCES(PREFIX, double, false, true) \
+    MJG_REP_INSTANCES(PREFIX, double, true, false) MJG_REP_INSTANCES(PREFIX, double, true, true)
 
 } // mjg

@@ -556,7 +556,7 @@ void Engine::setup_global(GlobalIA& g)
     g.d_epair = dalloc<double>(np);
     g.d_fg = dalloc<double4>(n_);
     check(hipMemsetAsync(g.d_fg, 0, n_ * sizeof(double4), stream_), "memset");
-    if(g.multibody()) {g.d_acc = dalloc<unsigned long long>(3 * n_);}
+    if(g.multibody()) {g.d_acc = dalloc<unsigned long long>(3 * n_ + 1);}   // sums and the flag word
     sync();
     g.tpp = 8;
     alloc_list(g, 64);
@@ -877,11 +877,12 @@ void Engine::forces(double4* f, bool energy, int only, bool kick, bool advance)
 void Engine::launch_multibody(const GlobalIA& g, bool energy)
 {
     const uint32_t np = (uint32_t)g.part.size();
-    hipLaunchKernelGGL(k_zero_u64, dim3(blocks_for(3 * n_, 256)), dim3(256), 0, stream_, g.d_acc, (uint32_t)(3 * n_));
+    hipLaunchKernelGGL(k_zero_u64, dim3(blocks_for(3 * n_ + 1, 256)), dim3(256), 0, stream_, g.d_acc,
+                       (uint32_t)(3 * n_ + 1));
     TspSeg s{};
     s.kind = g.pot; s.np = np; s.cap = g.cap; s.tpp = g.mb_tpp;
     s.part = g.d_part; s.order = g.d_order; s.nl = g.d_nl; s.nn = g.d_nn;
-    s.ppar = g.d_ppar; s.acc = g.d_acc; s.eout = g.d_epair;
+    s.ppar = g.d_ppar; s.acc = g.d_acc; s.n = (uint32_t)n_; s.eout = g.d_epair;
     const uint32_t blocks = blocks_for((size_t)np * s.tpp, 256);
     if(energy) {hipLaunchKernelGGL(k_tsp_pairs<true>, dim3(blocks), dim3(256), 0, stream_, d_pos_, box_, s);}
     else {hipLaunchKernelGGL(k_tsp_pairs<false>, dim3(blocks), dim3(256), 0, stream_, d_pos_, box_, s);}

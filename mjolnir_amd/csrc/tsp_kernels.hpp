@@ -28,15 +28,24 @@ struct GlobalPos
         return d;
     }
 };
+// A contribution that is not finite, or whose fixed-point value would not fit in 62 bits, has
+// no integer to become: the accumulator's flag word (after the 3 n sums) is set instead, and the
+// conversion into forces then yields NaN, which the integrator's finite check reports.
+constexpr double kFixLimit = 4611686018427387904.0;       // 2^62
+__device__ __forceinline__ bool fix_ok(double s) {return fabs(s) < kFixLimit;}   // false for NaN
+
 // ... plus int64 fixed-point accumulation in global memory
 struct GlobalFixAcc : GlobalPos
 {
     unsigned long long* acc;
+    unsigned long long* flag;   // acc[3 n]
     __device__ __forceinline__ void add(uint32_t i, V3<double> f) const
     {
-        atomicAdd(&acc[3 * (size_t)i + 0], (unsigned long long)__double2ll_rn(f.x * kFixScale));
-        atomicAdd(&acc[3 * (size_t)i + 1], (unsigned long long)__double2ll_rn(f.y * kFixScale));
-        atomicAdd(&acc[3 * (size_t)i + 2], (unsigned long long)__double2ll_rn(f.z * kFixScale));
+        const double sx = f.x * kFixScale, sy = f.y * kFixScale, sz = f.z * kFixScale;
+        if(!(fix_ok(sx) && fix_ok(sy) && fix_ok(sz))) {atomicOr(flag, 1ull); return;}
+        atomicAdd(&acc[3 * (size_t)i + 0], (unsigned long long)__double2ll_rn(sx));
+        atomicAdd(&acc[3 * (size_t)i + 1], (unsigned long long)__double2ll_rn(sy));
+        atomicAdd(&acc[3 * (size_t)i + 2], (unsigned long long)__double2ll_rn(sz));
     }
 };
 } // tsp
@@ -54,7 +63,8 @@ struct TspSeg
     const uint32_t* nl;
     const uint32_t* nn;
     const double* ppar;
-    unsigned long long* acc;   // 3 per particle, int64 fixed point
+    unsigned long long* acc;   // 3 per particle, int64 fixed point, then the flag word
+    uint32_t n;                // particles of the system (the flag word is acc[3 n])
     double* eout;              // per row
 };
 
@@ -73,6 +83,7 @@ k_tsp_pairs(const double4* __restrict__ pos, Box box, TspSeg s)
     acc.pos = pos;
     acc.box = box;
     acc.acc = s.acc;
+    acc.flag = s.acc + 3 * (size_t)s.n;
     for(uint32_t k = l; k < cnt; k += s.tpp)
     {
         const uint32_t j = row[k];
@@ -93,14 +104,16 @@ __global__ void k_zero_u64(unsigned long long* __restrict__ p, uint32_t n)
     if(i < n) {p[i] = 0ull;}
 }
 
+// the sums into forces; a set flag word (acc[3 n]) makes every force NaN
 __global__ void k_fix_to_force(const unsigned long long* __restrict__ acc, double4* __restrict__ f, uint32_t n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if(i < n)
     {
-        f[i] = make_double4((double)(long long)acc[3 * (size_t)i] * tsp::kFixInv,
-                            (double)(long long)acc[3 * (size_t)i + 1] * tsp::kFixInv,
-                            (double)(long long)acc[3 * (size_t)i + 2] * tsp::kFixInv, 0.0);
+        const double bad = acc[3 * (size_t)n] ? __builtin_nan("") : 0.0;
+        f[i] = make_double4((double)(long long)acc[3 * (size_t)i] * tsp::kFixInv + bad,
+                            (double)(long long)acc[3 * (size_t)i + 1] * tsp::kFixInv + bad,
+                            (double)(long long)acc[3 * (size_t)i + 2] * tsp::kFixInv + bad, 0.0);
     }
 }
 

@@ -62,6 +62,8 @@ class Job:
     trace: List[Tuple[int, np.ndarray, float]] = field(default_factory=list)
     final_pos: Optional[np.ndarray] = None
     final_vel: Optional[np.ndarray] = None
+    failed: str = ""          # why the job stopped early (non-finite state); "" for a finished job
+    steps_done: int = 0
 
     @property
     def latency(self) -> float:
@@ -168,9 +170,15 @@ class Server:
                         k = min(self.segment, job.steps - done)
                         e.step(k)
                         done += k
+                        job.steps_done = done
                         self._record_engine(job, e, done)
                     x, v, _ = e.state()
                     job.final_pos, job.final_vel = x, v
+                except capi.MjgError as exc:
+                    # a job whose state stopped being finite fails alone; the server goes on
+                    if exc.status != capi.NAN_DETECTED:
+                        raise
+                    job.failed = str(exc)
                 finally:
                     e.close()
                 job.finished = time.perf_counter() - t0
@@ -184,6 +192,17 @@ class Server:
     def _record_engine(job: Job, e, step: int):
         per, _ = e.energies()
         job.trace.append((step, np.asarray(per, dtype=np.float64), e.kinetic_energy()))
+
+    def _fail_replicas(self, batch, resident, bad, tnow):
+        """Replicas the library stopped with non-finite state: their jobs fail alone (retired,
+        budget released); every other resident replica keeps running."""
+        for r in bad:
+            j = resident.pop(r)[0]
+            j.steps_done = batch.info(r)["step"]
+            j.failed = f"non-finite state at step {j.steps_done}"
+            batch.retire(r)
+            j.finished = tnow
+            self._release(j)
 
     # -- main loop ---------------------------------------------------------------------------
     def run(self, jobs: List[Job]) -> dict:
@@ -230,23 +249,27 @@ class Server:
                 waiting_b = still
                 if fresh:
                     ids = [batch.add(j.spec) for j in fresh]
-                    batch.initialize()
+                    bad = batch.initialize()
                     tnow = time.perf_counter() - t0
                     for r, j in zip(ids, fresh):
-                        batch.set_steps(r, j.steps)
                         j.admitted = tnow
                         resident[r] = [j, 0]
+                        if r not in bad:
+                            batch.set_steps(r, j.steps)
+                    self._fail_replicas(batch, resident, bad, tnow)
                     batch.observe()
                     for r, j in zip(ids, fresh):
                         per, ke = batch.energies(r)
                         j.trace.append((0, per.copy(), ke))
                 if resident:
-                    batch.advance(self.segment)
+                    bad = batch.advance(self.segment)
+                    self._fail_replicas(batch, resident, bad, time.perf_counter() - t0)
                     batch.observe()
                     done_ids = []
                     for r, rec in resident.items():
                         j = rec[0]
                         rec[1] = min(j.steps, rec[1] + self.segment)
+                        j.steps_done = rec[1]
                         per, ke = batch.energies(r)
                         j.trace.append((rec[1], per.copy(), ke))
                         if rec[1] >= j.steps:
@@ -281,8 +304,11 @@ class Server:
 
 
 def summarize(jobs: List[Job]) -> dict:
-    lat = np.array([j.latency for j in jobs])
-    ps = float(sum(j.spec.n * j.steps for j in jobs))
+    """Aggregate throughput (particle-steps of the work done, failed jobs counting the steps they
+    took) and latency percentiles over the finished jobs."""
+    ok = [j for j in jobs if not j.failed]
+    lat = np.array([j.latency for j in ok] or [math.nan])
+    ps = float(sum(j.spec.n * (j.steps if not j.failed else j.steps_done) for j in jobs))
     makespan = max(j.finished for j in jobs)
     last_arrival = max(j.arrival for j in jobs)
     out = {
@@ -294,9 +320,10 @@ def summarize(jobs: List[Job]) -> dict:
         "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
         "latency_p99_s": round(float(np.percentile(lat, 99)), 4),
         "drain_after_last_arrival_s": round(makespan - last_arrival, 4),
+        "failed_jobs": len(jobs) - len(ok),
     }
     for lane in ("batch", "engine"):
-        sel = [j for j in jobs if j.lane == lane]
+        sel = [j for j in ok if j.lane == lane]
         if sel:
             l = np.array([j.latency for j in sel])
             out[lane] = {"jobs": len(sel), "particle_steps": float(sum(j.spec.n * j.steps for j in sel)),

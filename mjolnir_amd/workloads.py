@@ -234,13 +234,53 @@ def _ensemble_member(job) -> SystemSpec:
     return cg_chain(n, seed=seed, T=T, max_bend_deg=max_bend_deg)
 
 
+def trimer_gas(n_trimers: int = 300, integrator: str = "BAOABLangevin", T: float = 300.0, friction: float = 1.0,
+               dt: float = 0.01, bond_k: float = 10.0, bond_r0: float = 3.8, angle_k: float = -2.0,
+               angle_sigma: float = 0.3, angle_theta0: float = 1.9, spacing: float = 40.0,
+               seed: int = 6) -> SystemSpec:
+    """Independent trimers i-j-k in an unbounded box: harmonic bonds i-j and j-k
+    (E = k (r - r0)^2, HarmonicPotential.hpp:23-32) and a Gaussian well on the angle i-j-k
+    (GaussianPotential.hpp:20-38), no pair interaction.  The Boltzmann density factorizes, so each
+    bond length has the density r^2 exp(-k (r - r0)^2 / kB T) and each angle
+    sin(theta) exp(-U(theta) / kB T): closed forms a thermostat must sample.  Friction `friction`
+    per particle (alpha = friction * mass for G-JF); unit masses."""
+    rng = np.random.default_rng(seed)
+    n = 3 * n_trimers
+    side = int(np.ceil(n_trimers ** (1.0 / 3.0)))
+    pos = np.zeros((n, 3))
+    for t in range(n_trimers):
+        c = np.array([t % side, (t // side) % side, t // (side * side)], dtype=np.float64) * spacing
+        a = rng.uniform(1.6, 2.2)
+        pos[3 * t] = c + [bond_r0, 0.0, 0.0]
+        pos[3 * t + 1] = c
+        pos[3 * t + 2] = c + bond_r0 * np.array([math.cos(a), math.sin(a), 0.0])
+    mass = np.ones(n)
+    vel = maxwell_boltzmann(rng, mass, T)
+    bonds = np.concatenate([np.stack([3 * np.arange(n_trimers), 3 * np.arange(n_trimers) + 1], axis=1),
+                            np.stack([3 * np.arange(n_trimers) + 1, 3 * np.arange(n_trimers) + 2], axis=1)])
+    ang = np.stack([3 * np.arange(n_trimers), 3 * np.arange(n_trimers) + 1, 3 * np.arange(n_trimers) + 2], axis=1)
+    local = [LocalSpec("BondLength", "Harmonic", "bond", bonds, np.tile([bond_k, bond_r0], (len(bonds), 1))),
+             LocalSpec("BondAngle", "Gaussian", "none", ang,
+                       np.tile([angle_k, angle_sigma, angle_theta0], (len(ang), 1)))]
+    coefs = np.full(n, friction) * (mass if integrator == "GJFNVTLangevin" else 1.0)
+    return SystemSpec(boundary="unlimited", mass=mass, pos=pos, vel=vel, attributes={"temperature": T},
+                      local=local, integrator=IntegratorSpec(integrator, dt, coefs, seed=seed),
+                      name=f"trimer_gas_{n_trimers}")
+
+
 def zero_temperature(spec: SystemSpec) -> SystemSpec:
     """Copy of ``spec`` with gamma (or alpha) = 0, which zeroes the friction and the noise
     amplitude: BAOAB and G-JF then reduce to velocity Verlet
     (BAOABLangevinIntegrator.hpp:79-82, GJFNVTLangevinIntegrator.hpp:116-118).  The
-    temperature attribute is kept because Debye-Hueckel parameters depend on it."""
+    temperature attribute is kept because Debye-Hueckel parameters depend on it.
+
+    A spec whose temperature attribute is already 0 is returned unchanged: its noise amplitudes
+    are zero at any friction (sqrt(kB T ...) in every integrator's reset_parameters), so its
+    damped trajectory is deterministic and the friction coefficients stay in it."""
     import copy
     s = copy.deepcopy(spec)
+    if s.attributes.get("temperature") == 0.0:
+        return s
     if s.integrator is not None:
         s.integrator.coefs = np.zeros_like(s.integrator.coefs)
     return s

@@ -139,7 +139,44 @@ def cases():
         "chain100_gbaoab": (lambda: perturb(workloads.constrained_chain(100, seed=5), 0.1, 13), True, 1000),
         "lj512_ul": (lambda: underdamped(perturb(workloads.lj_fluid(n_side=8, seed=4), 0.2, 14), 1.0), True, 1000),
         "chain200_ul": (lambda: underdamped(perturb(workloads.cg_chain(n=200, seed=6), 0.3, 15), 0.5), True, 1000),
+        # friction at zero temperature (deterministic damped trajectories, non-unit masses)
+        "lj512_damped": (lambda: damped(perturb(workloads.lj_fluid(n_side=8, seed=1), 0.2, 7), 31), True, 1000),
+        "lj512_gjf_damped": (lambda: damped(perturb(workloads.lj_fluid(n_side=8, seed=2), 0.2, 16), 32,
+                                            integrator="GJFNVTLangevin"), True, 1000),
+        "lj512_ul_damped": (lambda: damped(perturb(workloads.lj_fluid(n_side=8, seed=3), 0.2, 17), 33,
+                                           integrator="UnderdampedLangevin"), True, 1000),
+        "chain200_damped": (lambda: damped(perturb(workloads.cg_chain(n=200, seed=2), 0.3, 8), 34), True, 1000),
+        "chain200_gjf_damped": (lambda: damped(perturb(workloads.cg_chain(n=200, seed=9), 0.3, 18), 35,
+                                               integrator="GJFNVTLangevin"), True, 1000),
+        "dna12_damped": (lambda: damped(perturb(workloads.dna_duplex(12, seed=5), 0.2, 11), 36, gamma=(0.02, 0.5),
+                                        masses=None), True, 1000),
+        "chain100_gbaoab_damped": (lambda: damped(perturb(workloads.constrained_chain(100, seed=5), 0.1, 13), 37),
+                                   True, 1000),
     }
+
+
+def damped(spec: SystemSpec, seed: int, gamma=(0.05, 2.0), masses=(0.5, 3.0), integrator=None) -> SystemSpec:
+    """Zero temperature with friction.  The temperature attribute is 0, so every noise amplitude
+    is 0 (BAOABLangevinIntegrator.hpp:73-82, GJFNVTLangevinIntegrator.hpp:106-119,
+    UnderdampedLangevinIntegrator.hpp:320-350 all scale it by sqrt(kB T)) while the friction terms
+    act: e^{-gamma dt} (BAOAB, g-BAOAB), b and a of G-JF, the gamma dt / 2 factors of
+    UnderdampedLangevin.  The trajectory is deterministic, so the reference's own run pins the
+    friction coefficients.  Debye-Hueckel is dropped (its screening length needs T > 0).  Masses
+    are drawn from `masses` (None keeps the spec's, e.g. the DNA bead masses) and the per-particle
+    friction from `gamma`; G-JF takes alpha = gamma m.  `integrator` switches the type."""
+    s = copy.deepcopy(spec)
+    rng = np.random.default_rng(seed)
+    s.global_ = [g for g in s.global_ if g.potential != "DebyeHuckel"]
+    s.attributes = {"temperature": 0.0}
+    if masses is not None:
+        s.mass = rng.uniform(masses[0], masses[1], size=s.n)
+    it = s.integrator
+    typ = integrator or it.type
+    g = rng.uniform(gamma[0], gamma[1], size=s.n)
+    coefs = g * s.mass if typ == "GJFNVTLangevin" else g
+    s.integrator = IntegratorSpec(typ, it.dt, coefs, seed=it.seed, remove=it.remove)
+    s.name = s.name + "_damped"
+    return s
 
 
 def underdamped(spec: SystemSpec, gamma: float) -> SystemSpec:

@@ -273,3 +273,103 @@ def test_fp64_batch_underdamped_vs_oracle_same_noise():
             d = d - w * np.round(d / w)
         assert np.abs(d).max() < 1e-7, (spec.name, np.abs(d).max())
         assert np.abs(v - vo).max() < 1e-7, (spec.name, np.abs(v - vo).max())
+
+
+def test_nonfinite_replica_is_isolated():
+    """A replica whose state stops being finite is stopped alone: one whose position overflows at its
+    first step, and one with two coincident LJ particles (no finite force: the fixed-point guard
+    flags it at the first evaluation).  Every other replica of the mix ends bitwise equal to its
+    run alone, and the failed ones report their status."""
+    from mjolnir_amd import capi, workloads
+    good = _chains([150, 420, 900], seed0=800)
+    blow = copy.deepcopy(_chains([300], seed0=810)[0])
+    blow.vel[5] = [1e300, 0.0, 0.0]
+    coinc = workloads.lj_fluid(n_side=4, seed=3)
+    coinc.pos[1] = coinc.pos[0]
+    b, ids = _batch([good[0], blow, good[1], coinc, good[2]], capi.FP32_FIXED)
+    rb, rc = ids[1], ids[3]
+    assert rc in b.failed and rb not in b.failed          # no finite force at set-up
+    assert b.step(250) == [rb]                            # the overflowing position at its first step
+    assert b.status(rb) == capi.NAN_DETECTED and b.status(rc) == capi.NAN_DETECTED
+    assert b.info(rb)["step"] == 0 and b.info(rc)["step"] == 0
+    for spec, r in zip(good, [ids[0], ids[2], ids[4]]):
+        assert b.status(r) == capi.OK and b.info(r)["step"] == 250
+        a, (ra,) = _batch([spec], capi.FP32_FIXED)
+        a.step(250)
+        xa, va, fa = a.state(ra)
+        xm, vm, fm = b.state(r)
+        assert np.array_equal(xa, xm) and np.array_equal(va, vm) and np.array_equal(fa, fm)
+    b.retire(rb)
+    b.retire(rc)
+    assert b.step(50) == []
+
+
+def test_engine_coincident_particles_report_nonfinite():
+    """The FP64 engine with two coincident LJ particles: the step reports MJG_NAN_DETECTED."""
+    from mjolnir_amd import capi, workloads
+    spec = workloads.lj_fluid(n_side=4, seed=3)
+    spec.pos[1] = spec.pos[0]
+    print("[gpu] engine with coincident particles", flush=True)
+    e = capi.Engine(spec)
+    with pytest.raises(capi.MjgError) as exc:
+        e.step(10)
+    assert exc.value.status == capi.NAN_DETECTED
+
+
+def test_restored_replica_keeps_its_forced_rebuild_until_it_steps():
+    """A replica restored at a step count (forced list rebuild and, for UnderdampedLangevin, the
+    acceleration redrawn at that count) that takes part in a launch with no steps left keeps the
+    forced restart for its next real step: load -> advance -> set_steps -> advance equals the
+    uninterrupted run bitwise."""
+    from mjolnir_amd import capi
+    spec = make_golden.underdamped(_chains([240], seed0=820)[0], 0.5)
+    other = _chains([100], seed0=821)[0]
+    ref, (r0,) = _batch([spec], capi.FP32_FIXED)
+    ref.step(300)
+    x300, v300, f300 = ref.state(r0)
+    ref.step(200)
+    want = ref.state(r0)
+    s2 = copy.deepcopy(spec)
+    s2.pos, s2.vel = x300, v300
+    b, (ro, rr) = _batch([other, s2], capi.FP32_FIXED)
+    b.set_step_count(rr, 300)
+    b.set_steps(ro, 100)
+    b.set_steps(rr, 0)
+    b.advance(100)               # the restored replica has no steps left in this launch
+    b.set_steps(rr, 200)
+    b.advance(100)
+    b.advance(100)
+    got = b.state(rr)
+    assert b.info(rr)["step"] == 500
+    for a, c in zip(want, got):
+        assert np.array_equal(a, c)
+
+
+DAMPED_BATCH = ["lj512_damped", "lj512_gjf_damped", "lj512_ul_damped", "chain200_damped", "chain200_gjf_damped",
+                "dna12_damped"]
+
+
+def test_fp64_batch_damped_trajectories_vs_reference():
+    """Friction at zero temperature (deterministic damped trajectories, per-particle friction,
+    non-unit masses): the reference's own BAOAB, G-JF and UnderdampedLangevin runs reproduced by FP64
+    replicas, all in one batch, within 1e-8 A every 100 steps for 1000 steps."""
+    from conftest import golden_case
+    from mjolnir_amd import capi
+    cases = [golden_case(n) for n in DAMPED_BATCH]
+    b, ids = _batch([c[0] for c in cases], capi.FP64)
+    steps = [int(s) for s in cases[0][1]["zt_frame_steps"]]
+    done = 0
+    for k, step in enumerate(steps):
+        b.step(step - done)
+        done = step
+        for (spec, g), r in zip(cases, ids):
+            assert [int(s) for s in g["zt_frame_steps"]] == steps
+            x, _, _ = b.state(r)
+            d = x - g["zt_frames"][k]
+            if spec.boundary == "periodic":
+                w = spec.upper - spec.lower
+                d = d - w * np.round(d / w)
+            assert np.abs(d).max() <= 1e-8, (spec.name, step, np.abs(d).max())
+    for (spec, g), r in zip(cases, ids):
+        _, v, _ = b.state(r)
+        assert np.abs(v - g["zt_vel"]).max() <= 1e-8, spec.name

@@ -270,3 +270,27 @@ def test_energy_observer_and_simulator_vs_reference(tmp_path):
         y = np.array([float(t) for t in b.split()])
         assert np.allclose(x, y, rtol=1e-9, atol=2e-6), (a, b)
         assert len(a) == len(b)          # fixed-width columns
+
+
+@pytest.mark.parametrize("name", [n for n in CASES if make_golden.cases()[n][2] > 0 and make_golden.cases()[n][1]])
+def test_forces_on_reference_trajectory_frames(name):
+    """Forces at the reference's own trajectory frames (its zero-temperature run, steps 500 and
+    1000): the engine against the oracle (bit-identical to the reference) at 1e-10 per particle.
+    This follows the reference's dynamics past the point where ulp differences in transcendentals
+    make two integrations drift apart chaotically."""
+    import copy
+    import oracle
+    spec, g = golden_case(name)
+    if "zt_frames" not in g:
+        pytest.skip("no frames stored for this case")
+    steps = [int(s) for s in g["zt_frame_steps"]]
+    for step in (500, 1000):
+        s2 = copy.deepcopy(spec)
+        s2.pos = g["zt_frames"][steps.index(step)].copy()
+        e = _engine(s2, with_integrator=False)
+        o = oracle.OracleEngine(s2, with_integrator=False)
+        for k in range(len(spec.interaction_names())):
+            ref = o.force_of(k)
+            err = np.linalg.norm(e.force_of(k) - ref, axis=1)
+            assert np.all(err <= force_tolerance(ref, 1e-10)), (name, step, k, err.max())
+        e.close()
