@@ -102,6 +102,10 @@ const char* mjg_last_error(const mjg_engine* e);
  * vectors (System.hpp:139-150).  Positions are wrapped into the box as System::initialize
  * does (System.hpp:63-67). */
 mjg_status mjg_system_set_state(mjg_engine* e, const double* mass, const double* pos, const double* vel);
+/* Positions only (3*N doubles), wrapped into the box as above: what a host-side integrator that
+ * moved them hands to the force field before ForceFieldBase::calc_force(System&)
+ * (BAOABLangevinIntegrator.hpp:195-220).  Masses, velocities, lists and their margins stay. */
+mjg_status mjg_system_set_positions(mjg_engine* e, const double* pos);
 /* Download positions, velocities and the current forces (any pointer may be NULL). */
 mjg_status mjg_system_get_state(mjg_engine* e, double* pos, double* vel, double* force);
 /* System attributes "temperature" [K] and "ionic_strength" [M] (System.hpp:164-167). */
@@ -146,6 +150,15 @@ mjg_status mjg_calc_force_and_energy(mjg_engine* e, double* energy);
 /* Per-interaction energies in interaction-id order (what format_energy prints,
  * ForceField.hpp:154-161) and their sum; does not change the state. */
 mjg_status mjg_calc_energies(mjg_engine* e, double* per_interaction, double* total);
+/* ForceFieldBase::calc_force_and_virial / calc_force_virial_energy (ForceFieldBase.hpp:34,37):
+ * not provided.  No kernel accumulates the virial tensor, since no integrator on this path reads it
+ * (only the reference's constant-pressure integrators do).  Returns MJG_UNSUPPORTED and fills the 9
+ * entries of `virial` (if not NULL) with NaN, so that nothing downstream mistakes it for zero. */
+mjg_status mjg_calc_force_and_virial(mjg_engine* e, double* virial);
+/* Name of interaction `id` as the reference's name() prints it in the .ene header
+ * (LocalInteractionBase::name, GlobalInteractionBase::name; e.g. "BondLength:Harmonic",
+ * "GlobalPairLennardJones"); written NUL-terminated into buf (at most cap bytes), its length to *len. */
+mjg_status mjg_interaction_name(const mjg_engine* e, int id, char* buf, size_t cap, size_t* len);
 /* Force of interaction `id` alone, downloaded into force_out (3*N doubles). */
 mjg_status mjg_calc_force_of(mjg_engine* e, int id, double* force_out);
 /* ForceFieldBase::reduce_margin (ForceField.hpp:75-92): rebuild lists whose skin is spent. */

@@ -54,9 +54,10 @@ _up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 
 EXPORTED = [
     "mjg_version", "mjg_engine_create", "mjg_engine_destroy", "mjg_last_error",
-    "mjg_system_set_state", "mjg_system_get_state", "mjg_system_set_attribute",
+    "mjg_system_set_state", "mjg_system_set_positions", "mjg_system_get_state", "mjg_system_set_attribute",
     "mjg_add_local", "mjg_add_global", "mjg_add_constraint", "mjg_set_step_count", "mjg_forcefield_initialize", "mjg_num_interactions",
-    "mjg_calc_force", "mjg_calc_force_and_energy", "mjg_calc_energies", "mjg_calc_force_of",
+    "mjg_calc_force", "mjg_calc_force_and_energy", "mjg_calc_energies", "mjg_calc_force_and_virial",
+    "mjg_interaction_name", "mjg_calc_force_of",
     "mjg_reduce_margin", "mjg_neighbor_pairs", "mjg_integrator_set", "mjg_integrator_set_remover", "mjg_observer_energy_open",
     "mjg_observer_energy_output", "mjg_observer_xyz_open", "mjg_observer_dcd_open", "mjg_simulate", "mjg_integrator_initialize",
     "mjg_step", "mjg_counters", "mjg_kinetic_energy", "mjg_profile_interaction",

@@ -144,6 +144,27 @@ void Engine::set_state(const double* mass, const double* pos, const double* vel)
     state_set_ = true;
 }
 
+// Positions only (a host integrator moved them): wrapped into the box as set_state does; masses,
+// velocities, forces, neighbour lists and their margins are left as they are.
+void Engine::set_positions(const double* pos)
+{
+    if(!pos) {throw std::invalid_argument("null position pointer");}
+    if(!state_set_) {throw std::logic_error("set the state first");}
+    std::vector<double4> p(n_);
+    for(size_t i = 0; i < n_; ++i)
+    {
+        double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+        if(box_.periodic)
+        {
+            if(x < box_.lower[0]) {x += box_.width[0];} else if(x >= box_.upper[0]) {x -= box_.width[0];}
+            if(y < box_.lower[1]) {y += box_.width[1];} else if(y >= box_.upper[1]) {y -= box_.width[1];}
+            if(z < box_.lower[2]) {z += box_.width[2];} else if(z >= box_.upper[2]) {z -= box_.width[2];}
+        }
+        p[i] = make_double4(x, y, z, 0.0);
+    }
+    upload(d_pos_, p.data(), n_);
+}
+
 void Engine::get_state(double* pos, double* vel, double* force)
 {
     std::vector<double4> h(n_);
@@ -1560,6 +1581,10 @@ mjg_status mjg_system_set_state(mjg_engine* e, const double* mass, const double*
 {
     return guarded(e, [&](mjg::Engine& x) {x.set_state(mass, pos, vel);});
 }
+mjg_status mjg_system_set_positions(mjg_engine* e, const double* pos)
+{
+    return guarded(e, [&](mjg::Engine& x) {x.set_positions(pos);});
+}
 mjg_status mjg_system_get_state(mjg_engine* e, double* pos, double* vel, double* force)
 {
     return guarded(e, [&](mjg::Engine& x) {x.get_state(pos, vel, force);});
@@ -1611,6 +1636,27 @@ mjg_status mjg_calc_force_and_energy(mjg_engine* e, double* energy)
 mjg_status mjg_calc_energies(mjg_engine* e, double* per_interaction, double* total)
 {
     return guarded(e, [&](mjg::Engine& x) {x.calc_energies(per_interaction, total);});
+}
+mjg_status mjg_calc_force_and_virial(mjg_engine* e, double* virial)
+{
+    // no kernel accumulates the virial tensor (no integrator on this path reads it)
+    if(e == nullptr || e->impl == nullptr) {return MJG_INVALID_ARG;}
+    if(virial) {for(int k = 0; k < 9; ++k) {virial[k] = __builtin_nan("");}}
+    e->err = "the virial is not computed by this library (constant-volume integrators only)";
+    return MJG_UNSUPPORTED;
+}
+mjg_status mjg_interaction_name(const mjg_engine* e, int id, char* buf, size_t cap, size_t* len)
+{
+    if(e == nullptr || e->impl == nullptr || id < 0 || (size_t)id >= e->impl->num_interactions()) {return MJG_INVALID_ARG;}
+    const std::string nm = e->impl->interaction_name((size_t)id);
+    if(len) {*len = nm.size();}
+    if(buf && cap > 0)
+    {
+        const size_t k = std::min(cap - 1, nm.size());
+        std::memcpy(buf, nm.data(), k);
+        buf[k] = '\0';
+    }
+    return MJG_OK;
 }
 mjg_status mjg_calc_force_of(mjg_engine* e, int id, double* force_out)
 {

@@ -88,6 +88,7 @@ class Engine
     ~Engine();
 
     void set_state(const double* mass, const double* pos, const double* vel);
+    void set_positions(const double* pos);
     void get_state(double* pos, double* vel, double* force);
     void set_attribute(const std::string& key, double value);
     int add_local(int kind, const char* topology, size_t n_terms, const uint32_t* idx, const double* par, double margin);

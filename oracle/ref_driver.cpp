@@ -821,6 +821,7 @@ static int dispatch(const std::string& cmd, const SpecIn& s, int argc, char** ar
 }
 } // drv
 
+#ifndef REF_DRIVER_NO_MAIN
 int main(int argc, char** argv)
 {
     if(argc < 4)
@@ -842,3 +843,4 @@ int main(int argc, char** argv)
         return 1;
     }
 }
+#endif // REF_DRIVER_NO_MAIN
