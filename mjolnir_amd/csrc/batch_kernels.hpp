@@ -140,7 +140,7 @@ __host__ __device__ inline size_t rep_lds_bytes(uint32_t n, uint32_t par_cols, u
 {
     // cell ends (uint32), cell keys (uint16) and the cell-sorted entries (uint32) of the participants
     const size_t grid = grid_np ? (size_t)grid_cells(grid_bits) * 4 + (size_t)((grid_np + 7u) & ~7u) * 6 : 0;
-    return (size_t)n * 48 + 8 + (size_t)n * par_cols * real_size + (size_t)part_total * 2 + grid;
+    return (size_t)n * 48 + (size_t)n * par_cols * real_size + (size_t)part_total * 2 + grid;
 }
 
 // ------------------------------------------------------------------------------------
@@ -149,28 +149,38 @@ __host__ __device__ inline size_t rep_lds_bytes(uint32_t n, uint32_t par_cols, u
 // kernel, where a contribution carries about 24 significant bits anyway.
 //
 // A contribution that is not finite, or whose image would not fit in 62 bits, has no integer to
-// become: it adds nothing and sets the flag word F[-1] just ahead of the replica's force sums
-// instead (a NaN would otherwise turn into an arbitrary finite integer and go unnoticed); the
-// segment then ends the replica as non-finite.
+// become (a NaN would otherwise turn into an arbitrary finite integer and go unnoticed).  It
+// becomes kFixPoison instead, round(2^64 / golden ratio) read as a signed integer: k poisons sum
+// (mod 2^64) to 2^64 frac(k / phi), which stays above 2^57 in magnitude for every k up to 88, so a
+// particle's sum that took up to 88 of them leaves the range |sum| <= 2^57 that real forces keep
+// (2^33 kcal/mol/A in the FP32 kernel, 2^25 in the FP64 one).  The integrator reads every sum each
+// step and ends a replica whose sum left that range as non-finite.  The conversion itself stays
+// free of side effects (a flag store in this rarely taken branch costs the kernel registers).
 // ------------------------------------------------------------------------------------
 constexpr double kFixLimit = 4611686018427387904.0;    // 2^62
-__device__ __forceinline__ long long fix_checked(double s, unsigned long long* F)
+constexpr long long kFixPoison = (long long)0x9E3779B97F4A7C15ull;
+constexpr long long kFixSane = 1ll << 57;
+__device__ __forceinline__ long long fix_checked(double s)
 {
-    if(!(fabs(s) < kFixLimit)) {atomicOr(&F[-1], 1ull); return 0;}   // NaN fails the test too
-    return __double2ll_rn(s);
+    return (fabs(s) < kFixLimit) ? __double2ll_rn(s) : kFixPoison;   // NaN fails the test too
+}
+__device__ __forceinline__ bool fix_insane(unsigned long long v)
+{
+    const long long x = (long long)v;
+    return x > kFixSane || x < -kFixSane;
 }
 template<typename Real> struct Fix;
 template<> struct Fix<double>
 {
     static constexpr double scale = 4294967296.0;          // 2^32
     static constexpr double inv = 2.3283064365386963e-10;  // 2^-32
-    __device__ static __forceinline__ long long conv(double v, unsigned long long* F) {return fix_checked(v * scale, F);}
+    __device__ static __forceinline__ long long conv(double v) {return fix_checked(v * scale);}
 };
 template<> struct Fix<float>
 {
     static constexpr double scale = 16777216.0;            // 2^24
     static constexpr double inv = 5.9604644775390625e-08;  // 2^-24
-    __device__ static __forceinline__ long long conv(double v, unsigned long long* F) {return fix_checked(v * scale, F);}
+    __device__ static __forceinline__ long long conv(double v) {return fix_checked(v * scale);}
 };
 
 // Reciprocal, reciprocal square root, square root and exp of the pair and local arithmetic.
@@ -207,17 +217,14 @@ __device__ __forceinline__ void add3(unsigned long long* F, uint32_t i, long lon
 // gives; below 2^31 that is a 32-bit conversion, two instructions instead of a dozen.  A NaN
 // fails the size test and takes the checked FP64 route.)
 template<typename Real>
-__device__ __forceinline__ void fix3r(Real x, Real y, Real z, long long& a, long long& b, long long& c,
-                                      unsigned long long* F);
+__device__ __forceinline__ void fix3r(Real x, Real y, Real z, long long& a, long long& b, long long& c);
 template<>
-__device__ __forceinline__ void fix3r<double>(double x, double y, double z, long long& a, long long& b, long long& c,
-                                              unsigned long long* F)
+__device__ __forceinline__ void fix3r<double>(double x, double y, double z, long long& a, long long& b, long long& c)
 {
-    a = Fix<double>::conv(x, F); b = Fix<double>::conv(y, F); c = Fix<double>::conv(z, F);
+    a = Fix<double>::conv(x); b = Fix<double>::conv(y); c = Fix<double>::conv(z);
 }
 template<>
-__device__ __forceinline__ void fix3r<float>(float x, float y, float z, long long& a, long long& b, long long& c,
-                                             unsigned long long* F)
+__device__ __forceinline__ void fix3r<float>(float x, float y, float z, long long& a, long long& b, long long& c)
 {
     const float sx = x * 16777216.0f, sy = y * 16777216.0f, sz = z * 16777216.0f;
     const bool small = fmaxf(fabsf(sx), fmaxf(fabsf(sy), fabsf(sz))) < 2147483520.0f;
@@ -227,14 +234,14 @@ __device__ __forceinline__ void fix3r<float>(float x, float y, float z, long lon
     }
     else
     {
-        a = Fix<float>::conv((double)x, F); b = Fix<float>::conv((double)y, F); c = Fix<float>::conv((double)z, F);
+        a = Fix<float>::conv((double)x); b = Fix<float>::conv((double)y); c = Fix<float>::conv((double)z);
     }
 }
 template<typename Real, typename V>
 __device__ __forceinline__ void acc3(unsigned long long* F, uint32_t i, V x, V y, V z)
 {
     long long a, b, c;
-    fix3r<Real>((Real)x, (Real)y, (Real)z, a, b, c, F);
+    fix3r<Real>((Real)x, (Real)y, (Real)z, a, b, c);
     add3(F, i, a, b, c);
 }
 
@@ -367,7 +374,7 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         const double rlen = 1.0 / sqrt(len2);
         const double len = len2 * rlen;
         const double c = (-1 * (2 * p0 * (len - p1))) * rlen;
-        const long long fx = Fix<Real>::conv(dx * c, F), fy = Fix<Real>::conv(dy * c, F), fz = Fix<Real>::conv(dz * c, F);
+        const long long fx = Fix<Real>::conv(dx * c), fy = Fix<Real>::conv(dy * c), fz = Fix<Real>::conv(dz * c);
         add3(F, i0, -fx, -fy, -fz);
         add3(F, i1, fx, fy, fz);
         if(ENERGY) {const double dv = len - p1; e = p0 * dv * dv;}
@@ -389,7 +396,7 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         const Real v0r_12 = v0r_10 * v0r_2;
         const Real coef = (Real)(-60) * k * r2 * (v0r_10 - v0r_12);
         long long fx, fy, fz;
-        fix3r<Real>(coef * dx, coef * dy, coef * dz, fx, fy, fz, F);
+        fix3r<Real>(coef * dx, coef * dy, coef * dz, fx, fy, fz);
         add3(F, i0, -fx, -fy, -fz);
         add3(F, i1, fx, fy, fz);
         if(ENERGY) {e = (double)(k * ((Real)5 * v0r_12 - (Real)6 * v0r_10));}
@@ -432,8 +439,8 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         const Real Fkx = ck * (cos_t * ekx - eix), Fky = ck * (cos_t * eky - eiy), Fkz = ck * (cos_t * ekz - eiz);
         // the middle body takes minus the sum of the ends, in integers (exact momentum balance)
         long long ix, iy, iz, kx, ky, kz;
-        fix3r<Real>(Fix_, Fiy, Fiz, ix, iy, iz, F);
-        fix3r<Real>(Fkx, Fky, Fkz, kx, ky, kz, F);
+        fix3r<Real>(Fix_, Fiy, Fiz, ix, iy, iz);
+        fix3r<Real>(Fkx, Fky, Fkz, kx, ky, kz);
         add3(F, i0, ix, iy, iz);
         add3(F, i1, -(ix + kx), -(iy + ky), -(iz + kz));
         add3(F, i2, kx, ky, kz);
@@ -495,7 +502,7 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         const double len = len2 * rlen;
         const double dv = len - p1;
         const double c = (-1 * (2 * p0 * (dv + 200 * dv * dv * dv))) * rlen;
-        const long long fx = Fix<Real>::conv(dx * c, F), fy = Fix<Real>::conv(dy * c, F), fz = Fix<Real>::conv(dz * c, F);
+        const long long fx = Fix<Real>::conv(dx * c), fy = Fix<Real>::conv(dy * c), fz = Fix<Real>::conv(dz * c);
         add3(F, i0, -fx, -fy, -fz);
         add3(F, i1, fx, fy, fz);
         if(ENERGY) {const double dv2 = dv * dv; e = p0 * (dv2 + 100 * dv2 * dv2);}
@@ -623,7 +630,7 @@ __device__ __forceinline__ void rep_pair(const RepGlobal& G, const Real* par, ui
         if(ENERGY) {e += (double)(qq * (ex * rl - (Real)G.coef_at_cutoff));}
     }
     long long fx, fy, fz;
-    fix3r<Real>(dx * c, dy * c, dz * c, fx, fy, fz, F);
+    fix3r<Real>(dx * c, dy * c, dz * c, fx, fy, fz);
     add3(F, i, fx, fy, fz);
     add3(F, j, -fx, -fy, -fz);
 }
@@ -699,7 +706,7 @@ __device__ __forceinline__ void rep_pair_k(const PairK<Real>& K, const Real* par
         if(ENERGY) {e += (double)(qq * (ex * rl - K.cac));}
     }
     long long fx, fy, fz;
-    fix3r<Real>(dx * c, dy * c, dz * c, fx, fy, fz, F);
+    fix3r<Real>(dx * c, dy * c, dz * c, fx, fy, fz);
     add3(F, i, fx, fy, fz);
     add3(F, j, -fx, -fy, -fz);
 }
@@ -1096,8 +1103,8 @@ __device__ void rep_build_grid(const RepGlobal& G, const RepDesc& R, const Box& 
 // one segment: `nsteps` steps (fewer if the replica has fewer remaining), or a pure force /
 // energy evaluation when eval_only (the lists are rebuilt first if the state asks for it)
 //
-// LDS: x[n], y[n], z[n] (double); the flag word F[-1] and F[3n] (int64 fixed point);
-// parameters[par_cols][n] as npar-column rows per interaction (Real); participant lists (uint16)
+// LDS: x[n], y[n], z[n] (double); F[3n] (int64 fixed point); parameters[par_cols][n] as
+// npar-column rows per interaction (Real); participant lists (uint16)
 // ------------------------------------------------------------------------------------
 // Workgroup-wide scalars of a segment (static LDS of the kernel)
 template<int BS> struct SegShared
@@ -1127,7 +1134,7 @@ __device__ __forceinline__ void rep_segment(const RepDesc& R, int nsteps, int ev
     P.x = reinterpret_cast<double*>(smem);
     P.y = P.x + n;
     P.z = P.y + n;
-    unsigned long long* F = reinterpret_cast<unsigned long long*>(P.z + n) + 1;
+    unsigned long long* F = reinterpret_cast<unsigned long long*>(P.z + n);
     Real* par_lds = reinterpret_cast<Real*>(F + 3 * n);
     uint16_t* part_lds = reinterpret_cast<uint16_t*>(par_lds + (size_t)n * R.par_cols);
     uint32_t* grid_end = reinterpret_cast<uint32_t*>(part_lds + R.part_total);
@@ -1163,9 +1170,9 @@ __device__ __forceinline__ void rep_segment(const RepDesc& R, int nsteps, int ev
             const double4 a = R.pos[p], v = R.vel[p], f = R.force[p];
             vx[k] = v.x; vy[k] = v.y; vz[k] = v.z;
             P.x[p] = a.x; P.y[p] = a.y; P.z[p] = a.z;
-            F[3 * p + 0] = (unsigned long long)Fix<Real>::conv(f.x, F);
-            F[3 * p + 1] = (unsigned long long)Fix<Real>::conv(f.y, F);
-            F[3 * p + 2] = (unsigned long long)Fix<Real>::conv(f.z, F);
+            F[3 * p + 0] = (unsigned long long)Fix<Real>::conv(f.x);
+            F[3 * p + 1] = (unsigned long long)Fix<Real>::conv(f.y);
+            F[3 * p + 2] = (unsigned long long)Fix<Real>::conv(f.z);
             if(!eval_only) {R.bak_pos[p] = a; R.bak_vel[p] = v; R.bak_force[p] = f;}
         }
     }
@@ -1182,8 +1189,6 @@ __device__ __forceinline__ void rep_segment(const RepDesc& R, int nsteps, int ev
         s_force_rebuild = st.force_rebuild;
         s_rebuilds = st.rebuilds;
         s_flags = 0;
-        // the stored forces came out of fixed point, so their conversion above never sets the flag
-        F[-1] = 0ull;
         for(int g = 0; g < ng; ++g) {s_margin[g] = st.margin[g]; s_npairs[g] = st.npairs[g]; s_maxpairs[g] = st.maxpairs[g];}
     }
     __syncthreads();
@@ -1205,8 +1210,10 @@ __device__ __forceinline__ void rep_segment(const RepDesc& R, int nsteps, int ev
                 const uint32_t p = tid + k * BS;
                 if(p >= n) {continue;}
                 const double rmk = as_global(R.rmass)[p], c1k = as_global(R.c1)[p], c2k = as_global(R.c2)[p];
-                const double fxk = from_fix<Real>(F[3 * p + 0]), fyk = from_fix<Real>(F[3 * p + 1]);
-                const double fzk = from_fix<Real>(F[3 * p + 2]);
+                const unsigned long long ux = F[3 * p + 0], uy = F[3 * p + 1], uz = F[3 * p + 2];
+                // a sum that took a poisoned contribution (fix_checked) ends the replica
+                if(fix_insane(ux) || fix_insane(uy) || fix_insane(uz)) {bad = true;}
+                const double fxk = from_fix<Real>(ux), fyk = from_fix<Real>(uy), fzk = from_fix<Real>(uz);
                 double Rx = 0.0, Ry = 0.0, Rz = 0.0;
                 if(R.integ != 4) {rep_noise<Real>(R.seed, step, p, Rx, Ry, Rz);}
                 double dpx, dpy, dpz;
@@ -1260,10 +1267,10 @@ __device__ __forceinline__ void rep_segment(const RepDesc& R, int nsteps, int ev
             if(bad) {atomicOr(&s_flags, 2);}
         }
         __syncthreads();
-        // A replica whose positions or force sums stopped being finite ends its segment here.
-        // Neither word is written between the barrier above and the next one (positions flag
-        // in the first half, force flag in the force phase), so every thread takes the same branch.
-        if(!eval_only && ((s_flags & 2) || F[-1] != 0ull)) {break;}
+        // A replica whose positions or force sums stopped being finite (both checked in the first
+        // half) ends its segment here.  The flag is not written between the barrier above and the
+        // next one, so every thread takes the same branch.
+        if(!eval_only && (s_flags & 2)) {break;}
         for(uint32_t q = tid; q < 3 * n; q += BS) {F[q] = 0ull;}
         // skin check (ForceField::reduce_margin): 2 sqrt(max |dp|^2) off every margin
         d2 = wave_max(d2);
@@ -1421,10 +1428,12 @@ __device__ __forceinline__ void rep_segment(const RepDesc& R, int nsteps, int ev
         {
             R.pos[p] = make_double4(P.x[p], P.y[p], P.z[p], 0.0);
             R.vel[p] = make_double4(vx[k], vy[k], vz[k], 0.0);
-            R.force[p] = make_double4(from_fix<Real>(F[3 * p + 0]), from_fix<Real>(F[3 * p + 1]),
-                                      from_fix<Real>(F[3 * p + 2]), 0.0);
+            const unsigned long long ux = F[3 * p + 0], uy = F[3 * p + 1], uz = F[3 * p + 2];
+            R.force[p] = make_double4(from_fix<Real>(ux), from_fix<Real>(uy), from_fix<Real>(uz), 0.0);
+            if(fix_insane(ux) || fix_insane(uy) || fix_insane(uz)) {atomicOr(&s_flags, 2);}
         }
     }
+    __syncthreads();   // the last forces' check above is seen by the thread that writes the state
     double ke = 0.0;
     if(ENERGY)
     {
@@ -1447,7 +1456,7 @@ __device__ __forceinline__ void rep_segment(const RepDesc& R, int nsteps, int ev
         // or an evaluation for this replica
         if(done > 0) {st.force_rebuild = 0;}
         if(s_flags & 1) {st.overflow = 1;}
-        if((s_flags & 2) || F[-1] != 0ull) {st.nonfinite = 1;}
+        if(s_flags & 2) {st.nonfinite = 1;}
         for(int g = 0; g < ng; ++g) {st.margin[g] = s_margin[g]; st.npairs[g] = s_npairs[g]; st.maxpairs[g] = s_maxpairs[g];}
         if(ENERGY) {for(int q = 0; q < R.n_local + ng; ++q) {st.energy[q] = s_energy[q];}}
     }

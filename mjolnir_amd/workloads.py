@@ -299,7 +299,7 @@ _SBB_ANGLE = {"A": 156.54, "T": 135.78, "G": 159.81, "C": 141.16}
 
 
 def dna_duplex(n_bp: int = 200, seed: int = 3, T: float = 300.0, ionic_strength: float = 0.15,
-               dt: float = 0.02, gamma: float = 0.1, rise: float = 3.6, twist_deg: float = 36.0,
+               dt: float = 0.02, gamma: float = 0.1, rise: float = 3.38, twist_deg: float = 36.0,
                r_base: float = 3.0, bp_dist: float = 6.0, r_phos: float = 8.9, sb_bond: float = 4.4,
                margin: float = 0.4) -> SystemSpec:
     """Ideal right-handed B-form duplex of the 3SPN2 model (3 beads per nucleotide, no 5' terminal
@@ -309,9 +309,10 @@ def dna_duplex(n_bp: int = 200, seed: int = 3, T: float = 300.0, ionic_strength:
 
     Bases sit at radius r_base; the two bases of a pair lie in one plane, bp_dist apart (the
     partner turned by 2 asin(bp_dist / 2 r_base) about the axis; 180 degrees by default).  The
-    rise (3.6 A rather than B-DNA's 3.38) puts consecutive bases of a strand near the
-    base-stacking distances of the 3SPN2 parameter set, which start the duplex with pairing,
-    stacking and cross stacking formed and no strained contacts.
+    rise is B-DNA's 3.38 A; at that rise consecutive bases of a strand sit inside the 3SPN2
+    base-stacking distances, so the duplex starts with strained stacking (about +4 kcal/mol per
+    step) that relaxes within a few hundred steps.  A rise of 3.6 A starts it near those distances
+    instead, with pairing, stacking and cross stacking formed and no strained contacts.
     Each sugar lies in its base's plane, sb_bond from the base, at the angle S-B-B' of its base in
     the reference's pairing parameters, on the side away from the axis; phosphates sit at radius
     r_phos midway (angle and height) between consecutive sugars of their strand.  Bond, angle

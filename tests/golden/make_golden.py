@@ -134,8 +134,10 @@ def cases():
         "lj_pair_scan": (lj_pair_scan, True, 0),
         "lj_boundary": (lj_boundary_cases, True, 0),
         "chain200_remover": (chain_with_remover, True, 1000),
-        "dna12": (lambda: perturb(workloads.dna_duplex(12, seed=5), 0.2, 11), True, 1000),
-        "dna200": (lambda: perturb(workloads.dna_duplex(200, seed=3), 0.1, 12), False, 1000),
+        "dna12": (lambda: perturb(workloads.dna_duplex(12, seed=5, rise=3.6), 0.2, 11), True, 1000),
+        "dna200": (lambda: perturb(workloads.dna_duplex(200, seed=3, rise=3.6), 0.1, 12), False, 1000),
+        # config 3 as stated: ideal B-form at rise 3.38 A (no perturbation; the stacking starts strained)
+        "dna200_b338": (lambda: workloads.dna_duplex(200, seed=3), False, 1000),
         "chain100_gbaoab": (lambda: perturb(workloads.constrained_chain(100, seed=5), 0.1, 13), True, 1000),
         "lj512_ul": (lambda: underdamped(perturb(workloads.lj_fluid(n_side=8, seed=4), 0.2, 14), 1.0), True, 1000),
         "chain200_ul": (lambda: underdamped(perturb(workloads.cg_chain(n=200, seed=6), 0.3, 15), 0.5), True, 1000),
@@ -148,7 +150,7 @@ def cases():
         "chain200_damped": (lambda: damped(perturb(workloads.cg_chain(n=200, seed=2), 0.3, 8), 34), True, 1000),
         "chain200_gjf_damped": (lambda: damped(perturb(workloads.cg_chain(n=200, seed=9), 0.3, 18), 35,
                                                integrator="GJFNVTLangevin"), True, 1000),
-        "dna12_damped": (lambda: damped(perturb(workloads.dna_duplex(12, seed=5), 0.2, 11), 36, gamma=(0.02, 0.5),
+        "dna12_damped": (lambda: damped(perturb(workloads.dna_duplex(12, seed=5, rise=3.6), 0.2, 11), 36, gamma=(0.02, 0.5),
                                         masses=None), True, 1000),
         "chain100_gbaoab_damped": (lambda: damped(perturb(workloads.constrained_chain(100, seed=5), 0.1, 13), 37),
                                    True, 1000),

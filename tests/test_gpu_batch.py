@@ -43,7 +43,7 @@ def _batch(specs, precision):
 def test_mixed_precision_force_error_vs_oracle():
     import oracle
     from mjolnir_amd import capi
-    specs = _chains([60, 250, 700, 1500], temps=[280.0, 320.0, 360.0, 400.0])
+    specs = _chains([60, 250, 700, 1000, 1500], temps=[280.0, 320.0, 360.0, 300.0, 400.0])
     b, ids = _batch(specs, capi.FP32_FIXED)
     for spec, r in zip(specs, ids):
         o = oracle.OracleEngine(spec, with_integrator=False)
@@ -198,8 +198,9 @@ def test_periodic_lj_batch_vs_oracle():
 
 def _dna_specs():
     from mjolnir_amd import workloads
-    return [make_golden.perturb(workloads.dna_duplex(12, seed=5), 0.2, 11),
-            make_golden.perturb(workloads.dna_duplex(200, seed=3), 0.1, 12)]
+    return [make_golden.perturb(workloads.dna_duplex(12, seed=5, rise=3.6), 0.2, 11),
+            make_golden.perturb(workloads.dna_duplex(200, seed=3, rise=3.6), 0.1, 12),
+            workloads.dna_duplex(200, seed=3)]          # config 3 as stated: B-form, rise 3.38 A
 
 
 def test_dna_batch_forces_vs_oracle():
@@ -276,9 +277,10 @@ def test_fp64_batch_underdamped_vs_oracle_same_noise():
 
 
 def test_nonfinite_replica_is_isolated():
-    """A replica whose state stops being finite is stopped alone: one whose position overflows at its
-    first step, and one with two coincident LJ particles (no finite force: the fixed-point guard
-    flags it at the first evaluation).  Every other replica of the mix ends bitwise equal to its
+    """A replica whose state stops being finite is stopped alone: one whose bond force overflows the
+    fixed point after its first step (a particle launched at 1e300 A per time unit; its poisoned
+    force sum is seen when the next step reads it), and one with two coincident LJ particles (no
+    finite force: flagged at the first evaluation).  Every other replica of the mix ends bitwise equal to its
     run alone, and the failed ones report their status."""
     from mjolnir_amd import capi, workloads
     good = _chains([150, 420, 900], seed0=800)
@@ -289,9 +291,9 @@ def test_nonfinite_replica_is_isolated():
     b, ids = _batch([good[0], blow, good[1], coinc, good[2]], capi.FP32_FIXED)
     rb, rc = ids[1], ids[3]
     assert rc in b.failed and rb not in b.failed          # no finite force at set-up
-    assert b.step(250) == [rb]                            # the overflowing position at its first step
+    assert b.step(250) == [rb]
     assert b.status(rb) == capi.NAN_DETECTED and b.status(rc) == capi.NAN_DETECTED
-    assert b.info(rb)["step"] == 0 and b.info(rc)["step"] == 0
+    assert b.info(rb)["step"] == 1 and b.info(rc)["step"] == 0
     for spec, r in zip(good, [ids[0], ids[2], ids[4]]):
         assert b.status(r) == capi.OK and b.info(r)["step"] == 250
         a, (ra,) = _batch([spec], capi.FP32_FIXED)
@@ -373,3 +375,18 @@ def test_fp64_batch_damped_trajectories_vs_reference():
     for (spec, g), r in zip(cases, ids):
         _, v, _ = b.state(r)
         assert np.abs(v - g["zt_vel"]).max() <= 1e-8, spec.name
+
+
+def test_fp64_batch_config3_duplex_vs_reference():
+    """Config 3 as stated (200 bp, ideal B-form at rise 3.38 A, strained stacking at the start):
+    the reference's own 1000-step zero-temperature G-JF run reproduced by an FP64 replica within
+    1e-8 A (the reference's sensitivity here is about 1e-10 A)."""
+    from conftest import golden_case
+    from mjolnir_amd import capi
+    spec, g = golden_case("dna200_b338")
+    b, (r,) = _batch([spec], capi.FP64)
+    b.step(int(g["zt_steps"]))
+    x, v, _ = b.state(r)
+    print(f"[gpu] dna200_b338 FP64 replica: max|dx| {np.abs(x - g['zt_pos']).max():.2e}", flush=True)
+    assert np.abs(x - g["zt_pos"]).max() <= 1e-8
+    assert np.abs(v - g["zt_vel"]).max() <= 1e-8

@@ -1,0 +1,94 @@
+"""The benchmark configurations at their stated sizes on the GPU.
+
+* Config 4: the 512-replica ensemble of the benchmark (CG chains, lengths log-uniform in
+  [50, 2000], temperatures uniform in [280, 400] K), mixed precision, 10^4 steps in 100-step
+  segments.  Afterwards: every replica finite (a replica stopped as non-finite would be reported
+  and isolated, and is counted here); on a length-stratified sample of 16 replicas the force
+  error against the FP64 oracle at the final state, RMS <= 1e-5 of the RMS force (max reported);
+  three sampled replicas bitwise identical alone and in the 512-mix.
+* Config 5: the seeded 40-job Poisson stream (50 jobs/s) of the benchmark through the server;
+  a sample that includes the largest LJ job, the largest chain and a DNA job gives the same
+  result alone as in the mix (bitwise final state, energy traces within 1e-12).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _stratified(specs, k):
+    order = sorted(range(len(specs)), key=lambda i: specs[i].n)
+    step = max(1, len(order) // k)
+    return [order[i] for i in range(step // 2, len(order), step)][:k]
+
+
+def test_config4_ensemble_full_size_10k_steps():
+    import oracle
+    from mjolnir_amd import capi, workloads
+    specs = workloads.replica_ensemble(512)
+    print(f"[gpu] config 4: {len(specs)} replicas, {sum(s.n for s in specs)} particles, 10^4 steps", flush=True)
+    b = capi.Batch(capi.FP32_FIXED)
+    ids = [b.add(s) for s in specs]
+    assert b.initialize() == []
+    failed = b.step(10000)
+    print(f"[gpu] config 4: failed replicas {failed}", flush=True)
+    assert failed == []
+    ps, rebuilds = b.counters()
+    assert ps == 10000 * sum(s.n for s in specs)
+    sample = _stratified(specs, 16)
+    worst_rms, worst_max = 0.0, 0.0
+    for k in sample:
+        spec, r = specs[k], ids[k]
+        x, v, _ = b.state(r)
+        assert np.isfinite(x).all() and np.isfinite(v).all()
+        s2 = workloads.zero_temperature(spec)
+        s2.pos, s2.vel = x, v
+        o = oracle.OracleEngine(s2, with_integrator=False)
+        f_ref, _ = o.force_and_energy()
+        _, _, f = b.evaluate(r)
+        rms_ref = np.sqrt(np.mean(np.sum(f_ref ** 2, axis=1)))
+        rms_err = np.sqrt(np.mean(np.sum((f - f_ref) ** 2, axis=1))) / rms_ref
+        max_err = np.max(np.linalg.norm(f - f_ref, axis=1)) / rms_ref
+        worst_rms, worst_max = max(worst_rms, rms_err), max(worst_max, max_err)
+        print(f"[gpu] config 4 replica N={spec.n}: RMS force error {rms_err:.2e}, max {max_err:.2e} "
+              f"(relative to the RMS force)", flush=True)
+        assert rms_err <= 1e-5
+    print(f"[gpu] config 4 sample: worst RMS {worst_rms:.2e}, worst max {worst_max:.2e}", flush=True)
+    for k in (sample[0], sample[len(sample) // 2], sample[-1]):
+        a = capi.Batch(capi.FP32_FIXED)
+        (ra,) = [a.add(specs[k])]
+        a.initialize()
+        a.step(10000)
+        xa, va, fa = a.state(ra)
+        xm, vm, fm = b.state(ids[k])
+        assert np.array_equal(xa, xm) and np.array_equal(va, vm) and np.array_equal(fa, fm), specs[k].n
+        a.close()
+
+
+def _fresh(job):
+    from mjolnir_amd.serve import Job
+    return Job(id=job.id, kind=job.kind, spec=job.spec, steps=job.steps, arrival=0.0)
+
+
+def test_config5_stream_alone_vs_mix():
+    from mjolnir_amd import serve
+    jobs = serve.make_jobs(serve.job_trace(40, rate=50.0, seed=7), procs=16)
+    print(f"[gpu] config 5: 40 jobs, {sum(j.spec.n * j.steps for j in jobs):.3e} particle-steps", flush=True)
+    summary = serve.Server(workers=4).run(jobs)
+    print(f"[gpu] config 5 summary {summary}", flush=True)
+    assert {j.lane for j in jobs} == {"batch", "engine"}
+    lj = max((j for j in jobs if j.kind == "lj"), key=lambda j: j.spec.n)
+    ch = max((j for j in jobs if j.kind == "chain"), key=lambda j: j.spec.n)
+    dna = max((j for j in jobs if j.kind == "dna"), key=lambda j: j.spec.n)
+    small = min((j for j in jobs if j.lane == "batch"), key=lambda j: j.spec.n)
+    for j in (lj, ch, dna, small):
+        alone = _fresh(j)
+        print(f"[gpu] job {j.id} ({j.kind}, N={j.spec.n}, {j.steps} steps, {j.lane}) alone", flush=True)
+        serve.Server(workers=1).run([alone])
+        assert alone.lane == j.lane and alone.failed == j.failed
+        assert np.array_equal(alone.final_pos, j.final_pos), j.id
+        assert np.array_equal(alone.final_vel, j.final_vel), j.id
+        assert [t[0] for t in alone.trace] == [t[0] for t in j.trace]
+        for (s, e1, k1), (_, e2, k2) in zip(alone.trace, j.trace):
+            assert np.all(np.abs(e1 - e2) <= 1e-12 * np.maximum(np.abs(e1), 1.0)), (j.id, s)
+            assert abs(k1 - k2) <= 1e-12 * max(abs(k1), 1.0), (j.id, s)

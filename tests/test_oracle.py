@@ -118,7 +118,7 @@ def test_finite_difference_forces_dna(k):
     (3SPN2 bond, angle, periodic Gaussian dihedral, base stacking, base pairing, cross stacking,
     3SPN2 excluded volume, Debye-Hueckel).  The energy is the calc_energy path and the force
     the calc_force path, so the two restated paths check each other."""
-    spec = make_golden.perturb(workloads.dna_duplex(8, seed=31), 0.25, 32)
+    spec = make_golden.perturb(workloads.dna_duplex(8, seed=31, rise=3.6), 0.25, 32)
     base = oracle.OracleEngine(spec, with_integrator=False)
     f = base.force_of(k)
     assert np.abs(f).max() > 0 or k in (0, 1), "the term is inactive in this geometry"

@@ -40,7 +40,7 @@ def _same(a, b):
 @pytest.mark.parametrize("make", [
     lambda: workloads.cg_chain(n=60, seed=3, max_bend_deg=150.0),
     lambda: workloads.lj_fluid(n=64, seed=2),
-    lambda: workloads.dna_duplex(12, seed=5),
+    lambda: workloads.dna_duplex(12, seed=5, rise=3.6),
     lambda: workloads.constrained_chain(30, seed=3),
     lambda: make_golden.chain_with_remover(),
     lambda: make_golden.cases()["lj512_ul"][0](),
