@@ -47,7 +47,10 @@ enum {
 enum {
     MJG_FP64       = 0, /* FP64 throughout */
     MJG_FP32_FIXED = 1  /* FP32 pair/local arithmetic, int64 fixed-point force sums,
-                           FP64 integrator state */
+                           FP64 integrator state.  On an engine (mjg_engine_create) the
+                           LJ / excluded-volume / Debye-Hueckel / 3SPN2 excluded-volume pair
+                           kernel runs this way; local terms, 3SPN2 base pairing and cross
+                           stacking stay FP64 there.  Batches run every term this way. */
 };
 /* boundary (read_system.hpp:100-299: boundary_type "Unlimited" / "PeriodicCuboid") */
 enum { MJG_UNLIMITED = 0, MJG_PERIODIC = 1 };
@@ -277,6 +280,22 @@ mjg_status mjg_batch_advance(mjg_batch* b, int segment);
 /* Free a replica's device memory. */
 mjg_status mjg_batch_retire(mjg_batch* b, int replica);
 mjg_status mjg_batch_get_state(mjg_batch* b, int replica, double* pos, double* vel, double* force);
+/* Launch parameters of a replica that change no result (forces are exact fixed-point sums over the
+ * pairs inside the cutoff, whatever list holds them), with no counterpart in the reference (which
+ * fixes its cell size and skin from the input, PeriodicGridCellList.hpp:133-169,
+ * read_spatial_partition.hpp:62-116): the replica kernel's own list skin (relative margin, at most
+ * the input's) for LJ / excluded-volume pairs and for Debye-Hueckel, the cell-grid resolution of its
+ * list builds (grid_bits 4: 16 cells per dimension, 3: 8) and the lanes sharing one row of such a
+ * build (1, 2 or 4).  By default mjg_batch_initialize picks them with a measured search: replicas of
+ * one size bucket (half an octave), size class, box kind and interaction make-up share the result,
+ * found once per batch by timing a short segment of a clone of the first such replica for each
+ * candidate.  mjg_batch_set_autotune(b, 0) (or MJG_AUTOTUNE=0) uses fixed defaults instead;
+ * mjg_batch_set_tuning fixes one replica's (before mjg_batch_initialize). */
+mjg_status mjg_batch_set_autotune(mjg_batch* b, int on);
+mjg_status mjg_batch_set_tuning(mjg_batch* b, int replica, double list_skin_short, double list_skin_dh, int grid_bits,
+                                int grid_lanes);
+mjg_status mjg_batch_replica_tuning(mjg_batch* b, int replica, double* list_skin_short, double* list_skin_dh,
+                                    int* grid_bits, int* grid_lanes);
 /* MJG_OK for a replica that steps normally, MJG_NAN_DETECTED for one stopped with non-finite state
  * (it keeps its last state until mjg_batch_retire). */
 mjg_status mjg_batch_replica_status(mjg_batch* b, int replica, int* status);

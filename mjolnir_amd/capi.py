@@ -64,7 +64,8 @@ EXPORTED = [
     "mjg_batch_create", "mjg_batch_destroy", "mjg_batch_last_error", "mjg_batch_add_replica",
     "mjg_batch_set_attribute", "mjg_batch_add_local", "mjg_batch_add_global", "mjg_batch_set_integrator", "mjg_batch_set_step_count",
     "mjg_batch_initialize", "mjg_batch_step", "mjg_batch_set_steps", "mjg_batch_advance", "mjg_batch_retire",
-    "mjg_batch_get_state", "mjg_batch_replica_status", "mjg_batch_replica_info", "mjg_batch_list_sizes", "mjg_batch_evaluate", "mjg_batch_counters",
+    "mjg_batch_get_state", "mjg_batch_set_autotune", "mjg_batch_set_tuning", "mjg_batch_replica_tuning",
+    "mjg_batch_replica_status", "mjg_batch_replica_info", "mjg_batch_list_sizes", "mjg_batch_evaluate", "mjg_batch_counters",
     "mjg_batch_observe", "mjg_batch_energies",
     "mjg_batch_profile",
 ]
@@ -132,6 +133,9 @@ def lib():
     L.mjg_batch_get_state.argtypes = [vp, i32, vp, vp, vp]
     L.mjg_batch_replica_info.argtypes = [vp, i32, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), C.POINTER(sz)]
     L.mjg_batch_replica_status.argtypes = [vp, i32, C.POINTER(i32)]
+    L.mjg_batch_set_autotune.argtypes = [vp, i32]
+    L.mjg_batch_set_tuning.argtypes = [vp, i32, dbl, dbl, i32, i32]
+    L.mjg_batch_replica_tuning.argtypes = [vp, i32, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(i32), C.POINTER(i32)]
     L.mjg_batch_evaluate.argtypes = [vp, i32, vp, C.POINTER(dbl), vp]
     L.mjg_batch_observe.argtypes = [vp]
     L.mjg_batch_energies.argtypes = [vp, i32, vp, C.POINTER(dbl)]
@@ -326,6 +330,19 @@ class Batch:
         new = [r for r in self.specs if r not in self.failed and self.status(r) != OK]
         self.failed.update(new)
         return new
+
+    def set_autotune(self, on: bool):
+        self._check(lib().mjg_batch_set_autotune(self.h, int(bool(on))))
+
+    def set_tuning(self, r: int, skin_short: float, skin_dh: float, grid_bits: int, grid_lanes: int):
+        """Fix replica r's launch parameters (before initialize()); they change no result."""
+        self._check(lib().mjg_batch_set_tuning(self.h, r, float(skin_short), float(skin_dh), int(grid_bits),
+                                               int(grid_lanes)))
+
+    def tuning(self, r: int) -> dict:
+        a, b, c, d = C.c_double(), C.c_double(), C.c_int(), C.c_int()
+        self._check(lib().mjg_batch_replica_tuning(self.h, r, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
+        return {"skin_short": a.value, "skin_dh": b.value, "grid_bits": c.value, "grid_lanes": d.value}
 
     def status(self, r: int) -> int:
         """OK, or NAN_DETECTED for a replica stopped with non-finite state."""

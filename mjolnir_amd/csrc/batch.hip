@@ -9,7 +9,10 @@
 #include "../../include/mjg.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <map>
+#include <tuple>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -42,6 +45,19 @@ struct RepGlobalIn
     uint32_t cap = 0;
 };
 
+// Launch parameters of one replica that change no result (forces are exact fixed-point sums over
+// the pairs inside the cutoff, whatever list holds them): the replica kernel's own list skin for
+// LJ / excluded-volume pairs and for Debye-Hueckel (never above the input's margin), the cell-grid
+// resolution of its list builds (16 or 8 cells per dimension: the cell size) and the lanes that
+// share one row of a grid build.  Chosen per replica by Batch::tuned (or set explicitly).
+struct RepTune
+{
+    double m_short = 0.4;
+    double m_dh = 0.1;
+    int grid_bits = 4;
+    int grid_lanes = 1;
+};
+
 struct Replica
 {
     // host description
@@ -68,6 +84,8 @@ struct Replica
     bool observed = false;          // an energy observation is held for this replica
     bool failed = false;            // stopped with a non-finite position or force; never stepped again
     bool tsp = false;               // has 3SPN2 terms: runs in the kernel variant that holds them
+    RepTune tune;                   // launch parameters (see RepTune)
+    bool tune_set = false;          // set explicitly or tuned already
 };
 
 static void hip_check(hipError_t e, const char* what)
@@ -79,23 +97,21 @@ static size_t align256(size_t x) {return (x + 255) & ~(size_t)255;}
 // Skin of a replica's neighbour list.  The replica kernel's forces are exact fixed-point sums
 // over the pairs within the cutoff, so the skin changes no result, only the balance between
 // the pair loop (list entries grow as (1 + m)^3) and the rebuilds (their rate grows as 1 / m):
-// the kernel uses its own margin, at most the input's -- 0.4 for the short-ranged LJ and
-// excluded-volume pairs, 0.1 for Debye-Hueckel, whose cutoff spans many particle spacings
-// (measured best on the 512-chain mix of 50-2000 beads; MJG_LIST_MARGIN_SHORT /
-// MJG_LIST_MARGIN_DH override them).
-static double list_margin(int pot, double input_margin)
+// the kernel uses its own margin, at most the input's (RepTune; untuned defaults 0.4 for the
+// short-ranged LJ and excluded-volume pairs, 0.1 for Debye-Hueckel, whose cutoff spans many
+// particle spacings; MJG_LIST_MARGIN_SHORT / MJG_LIST_MARGIN_DH override those defaults).
+static RepTune default_tune()
 {
-    static const double m_short = [] {
-        const char* c = std::getenv("MJG_LIST_MARGIN_SHORT");
-        return c ? std::atof(c) : 0.4;
-    }();
-    static const double m_dh = [] {
-        const char* c = std::getenv("MJG_LIST_MARGIN_DH");
-        return c ? std::atof(c) : 0.1;
-    }();
+    RepTune t;
+    if(const char* c = std::getenv("MJG_LIST_MARGIN_SHORT")) {t.m_short = std::atof(c);}
+    if(const char* c = std::getenv("MJG_LIST_MARGIN_DH")) {t.m_dh = std::atof(c);}
+    return t;
+}
+static double list_margin(int pot, double input_margin, const RepTune& t)
+{
     double m = input_margin;
-    if(pot == kPairLJ || pot == kPairEV) {m = std::min(m, m_short);}
-    else if(pot == kPairDH) {m = std::min(m, m_dh);}
+    if(pot == kPairLJ || pot == kPairEV) {m = std::min(m, t.m_short);}
+    else if(pot == kPairDH) {m = std::min(m, t.m_dh);}
     return std::max(m, 0.02);
 }
 
@@ -116,6 +132,7 @@ class Batch
         const char* seg = std::getenv("MJG_SEGMENT");
         if(seg) {segment_ = std::max(1, std::atoi(seg));}
         trace_ = std::getenv("MJG_BATCH_TRACE") != nullptr;
+        if(const char* a = std::getenv("MJG_AUTOTUNE")) {autotune_ = a[0] != '0';}
         set_lds_limits<float>();
         set_lds_limits<double>();
     }
@@ -286,9 +303,47 @@ class Batch
         std::vector<int> fresh;
         for(size_t id = 0; id < reps_.size(); ++id)
         {
-            if(reps_[id] && !reps_[id]->ready) {prepare((int)id); fresh.push_back((int)id);}
+            if(reps_[id] && !reps_[id]->ready)
+            {
+                Replica& r = *reps_[id];
+                if(!r.tune_set) {r.tune = autotune_ ? tuned(r) : default_tune(); r.tune_set = true;}
+                prepare((int)id);
+                fresh.push_back((int)id);
+            }
         }
         if(fresh.empty()) {return;}
+        settle(fresh);
+        download_states();
+        for(int id : fresh)
+        {
+            Replica& r = *reps_[id];
+            r.active = true;
+            // non-finite input (positions or first forces): the replica never steps
+            if(h_states_[r.slot].nonfinite) {r.failed = true; r.active = false; failed_now_.push_back(id);}
+        }
+        descs_dirty_ = true;
+    }
+
+    // launch parameters of a replica (set before initialize; overrides the tuner)
+    void set_tuning(int id, double m_short, double m_dh, int grid_bits, int grid_lanes)
+    {
+        Replica& r = rep(id);
+        if(r.ready) {throw std::invalid_argument("replica already initialized");}
+        if(!(m_short > 0) || !(m_dh > 0) || (grid_bits != 3 && grid_bits != 4) ||
+           (grid_lanes != 1 && grid_lanes != 2 && grid_lanes != 4))
+        {
+            throw std::invalid_argument("tuning: margins > 0, grid_bits 3 or 4, grid_lanes 1, 2 or 4");
+        }
+        r.tune.m_short = m_short; r.tune.m_dh = m_dh; r.tune.grid_bits = grid_bits; r.tune.grid_lanes = grid_lanes;
+        r.tune_set = true;
+    }
+    RepTune tuning(int id) {return rep(id).tune;}
+    void set_autotune(bool on) {autotune_ = on;}
+
+  private:
+    // First evaluation of fresh replicas: lists, first forces; list capacities grown until they fit.
+    void settle(const std::vector<int>& fresh)
+    {
         std::vector<int> todo = fresh;
         for(int pass = 0; pass < 8 && !todo.empty(); ++pass)
         {
@@ -309,16 +364,109 @@ class Batch
             todo.swap(again);
         }
         if(!todo.empty()) {throw BatchDeviceError("replica list capacity did not converge");}
-        download_states();
-        for(int id : fresh)
-        {
-            Replica& r = *reps_[id];
-            r.active = true;
-            // non-finite input (positions or first forces): the replica never steps
-            if(h_states_[r.slot].nonfinite) {r.failed = true; r.active = false; failed_now_.push_back(id);}
-        }
-        descs_dirty_ = true;
     }
+
+    // ---- the autotuner ------------------------------------------------------------------
+    // Replicas of one size bucket (half an octave), class, box kind, precision and interaction
+    // make-up share their tuning, found once by a measured coordinate search: a clone of the first
+    // such replica runs a short segment alone (one workgroup, the replica's own per-step latency)
+    // for each candidate, and the fastest one is kept.  The candidates change no result.
+    struct TuneKey
+    {
+        int cls, tsp, periodic, bucket;
+        uint32_t kinds;
+        bool operator<(const TuneKey& o) const
+        {
+            return std::tie(cls, tsp, periodic, bucket, kinds) < std::tie(o.cls, o.tsp, o.periodic, o.bucket, o.kinds);
+        }
+    };
+    TuneKey tune_key(const Replica& r) const
+    {
+        TuneKey k{};
+        k.cls = 0;
+        while(r.n > kRepClassMaxN[k.cls]) {++k.cls;}
+        k.periodic = r.box.periodic;
+        k.bucket = (int)std::floor(2.0 * std::log2((double)r.n));
+        for(const auto& G : r.globals) {k.kinds |= 1u << G.pot;}
+        for(const auto& L : r.locals) {k.tsp |= (L.kind == kBond3SPN2 || L.kind == kDihedralPGauss || L.kind == kBaseStacking);}
+        return k;
+    }
+    RepTune tuned(const Replica& r)
+    {
+        const TuneKey key = tune_key(r);
+        const auto it = tune_cache_.find(key);
+        if(it != tune_cache_.end()) {return it->second;}
+        RepTune best = default_tune();
+        bool has_short = false, has_dh = false, has_grid = false;
+        for(const auto& G : r.globals)
+        {
+            has_short = has_short || G.pot == kPairLJ || G.pot == kPairEV;
+            has_dh = has_dh || G.pot == kPairDH;
+            has_grid = has_grid || (G.partition != MJG_PART_NAIVE && G.part.size() > kGridMinNp);
+        }
+        double best_t = trial(r, best);
+        auto consider = [&](RepTune t) {
+            const double x = trial(r, t);
+            if(x < best_t) {best_t = x; best = t;}
+        };
+        if(has_short) {for(double m : {0.2, 0.3, 0.5, 0.6}) {RepTune t = best; t.m_short = m; consider(t);}}
+        if(has_dh) {for(double m : {0.05, 0.2, 0.3}) {RepTune t = best; t.m_dh = m; consider(t);}}
+        if(has_grid)
+        {
+            const RepTune base = best;
+            for(int b : {3, 4})
+                for(int l : {1, 2, 4})
+                {
+                    if(b == base.grid_bits && l == base.grid_lanes) {continue;}
+                    RepTune t = base; t.grid_bits = b; t.grid_lanes = l; consider(t);
+                }
+        }
+        failed_now_.clear();
+        descs_dirty_ = true;
+        tune_cache_[key] = best;
+        if(trace_)
+        {
+            std::fprintf(stderr, "[mjg batch] tuned n=%zu: skin %.2f / DH %.2f, grid %d bits, %d lanes (%.3f ms)\n", r.n,
+                         best.m_short, best.m_dh, best.grid_bits, best.grid_lanes, best_t);
+        }
+        return best;
+    }
+    // time of kTuneSteps steps of a clone of `src` alone, after kTuneWarm steps (ms; infinite
+    // if the clone fails)
+    static constexpr int kTuneWarm = 100, kTuneSteps = 300;
+    double trial(const Replica& src, const RepTune& t)
+    {
+        auto c = std::make_unique<Replica>();
+        c->box = src.box; c->n = src.n; c->mass = src.mass; c->pos = src.pos; c->vel = src.vel;
+        c->T = src.T; c->I = src.I; c->has_T = src.has_T; c->has_I = src.has_I;
+        c->locals = src.locals; c->globals = src.globals;
+        for(auto& G : c->globals) {G.cap = 0;}
+        c->integ = src.integ; c->dt = src.dt; c->seed = src.seed; c->coefs = src.coefs;
+        c->topo.reset(c->n);
+        c->tune = t;
+        c->tune_set = true;
+        reps_.push_back(std::move(c));
+        const int id = (int)reps_.size() - 1;
+        double ms = INFINITY;
+        try
+        {
+            prepare(id);
+            settle({id});
+            reps_[id]->active = true;
+            set_remaining(*reps_[id], kTuneWarm + kTuneSteps);
+            run_segment({id}, kTuneWarm);
+            const auto t0 = std::chrono::steady_clock::now();
+            run_segment({id}, kTuneSteps);
+            const auto t1 = std::chrono::steady_clock::now();
+            if(!reps_[id]->failed) {ms = std::chrono::duration<double, std::milli>(t1 - t0).count();}
+        }
+        catch(const BatchDeviceError&) {throw;}
+        catch(const std::exception&) {ms = INFINITY;}
+        retire(id);
+        reps_.pop_back();
+        return ms;
+    }
+  public:
 
     // Advance every active replica by up to n steps (fewer if it has fewer remaining).  A replica
     // that stops with non-finite state is deactivated; the others go on (failures_of_call()).
@@ -818,13 +966,14 @@ class Batch
                 RG.inv4pieps = dh.inv4pieps; RG.coef_at_cutoff = dh.coef_at_cutoff;
                 max_cutoff = dh.debye * G.cutoff_ratio;
             }
-            const double m = list_margin(G.pot, G.margin);
+            const double m = list_margin(G.pot, G.margin, r.tune);
             const double r_list = max_cutoff * (1. + m);
             RG.r_list2 = r_list * r_list;
             RG.abs_margin = max_cutoff * m;
         }
-        // the finer grid (16 cells per dimension) where it fits in LDS
-        D.grid_bits = 4;
+        // the tuned grid (16 or 8 cells per dimension) where it fits in LDS
+        D.grid_bits = (uint32_t)r.tune.grid_bits;
+        D.grid_lanes = (uint32_t)r.tune.grid_lanes;
         r.lds = rep_lds_bytes(D.n, D.par_cols, D.part_total, real_size(), D.grid_np, D.grid_bits);
         if(r.lds > kRepLdsLimit)
         {
@@ -1087,6 +1236,8 @@ class Batch
     RepDesc* d_descs_ = nullptr;
     size_t desc_cap_ = 0;
     int segment_ = 100;
+    bool autotune_ = true;         // MJG_AUTOTUNE=0 starts a batch with the tuner off
+    std::map<TuneKey, RepTune> tune_cache_;
     bool trace_ = false;           // MJG_BATCH_TRACE: report segment reruns on stderr
     bool descs_dirty_ = true;
     std::vector<int> last_ids_;
@@ -1216,6 +1367,23 @@ mjg_status mjg_batch_retire(mjg_batch* b, int replica) {return bguard(b, [&](mjg
 mjg_status mjg_batch_get_state(mjg_batch* b, int replica, double* pos, double* vel, double* force)
 {
     return bguard(b, [&](mjg::Batch& x) {x.get_state(replica, pos, vel, force);});
+}
+mjg_status mjg_batch_set_autotune(mjg_batch* b, int on) {return bguard(b, [&](mjg::Batch& x) {x.set_autotune(on != 0);});}
+mjg_status mjg_batch_set_tuning(mjg_batch* b, int replica, double list_skin_short, double list_skin_dh, int grid_bits,
+                                int grid_lanes)
+{
+    return bguard(b, [&](mjg::Batch& x) {x.set_tuning(replica, list_skin_short, list_skin_dh, grid_bits, grid_lanes);});
+}
+mjg_status mjg_batch_replica_tuning(mjg_batch* b, int replica, double* list_skin_short, double* list_skin_dh,
+                                    int* grid_bits, int* grid_lanes)
+{
+    return bguard(b, [&](mjg::Batch& x) {
+        const mjg::RepTune t = x.tuning(replica);
+        if(list_skin_short) {*list_skin_short = t.m_short;}
+        if(list_skin_dh) {*list_skin_dh = t.m_dh;}
+        if(grid_bits) {*grid_bits = t.grid_bits;}
+        if(grid_lanes) {*grid_lanes = t.grid_lanes;}
+    });
 }
 mjg_status mjg_batch_replica_status(mjg_batch* b, int replica, int* status)
 {

@@ -19,6 +19,7 @@
 //   round), since one workgroup per CU cannot hide a dependent load per item.
 #pragma once
 #include "device_common.hpp"
+#include "mixed_math.hpp"
 #include "tsp_model.hpp"
 
 namespace mjg
@@ -107,6 +108,7 @@ struct RepDesc
     uint32_t part_total;       // LDS participant entries (uint16, padded per interaction)
     uint32_t grid_np;          // largest participant count of a grid-built list (0: none)
     uint32_t grid_bits;        // log2 of the grid cells per dimension (3 or 4)
+    uint32_t grid_lanes;       // lanes that share one row of a cell-grid list build (1, 2 or 4)
     double dt, halfdt;
     uint64_t seed;
     Box box;
@@ -143,65 +145,6 @@ __host__ __device__ inline size_t rep_lds_bytes(uint32_t n, uint32_t par_cols, u
     return (size_t)n * 48 + (size_t)n * par_cols * real_size + (size_t)part_total * 2 + grid;
 }
 
-// ------------------------------------------------------------------------------------
-// fixed-point accumulation: a force contribution v becomes the integer nearest v * 2^S
-// (round half to even, so conv(-v) == -conv(v)); S = 32 in the FP64 kernel, 24 in the FP32
-// kernel, where a contribution carries about 24 significant bits anyway.
-//
-// A contribution that is not finite, or whose image would not fit in 62 bits, has no integer to
-// become (a NaN would otherwise turn into an arbitrary finite integer and go unnoticed).  It
-// becomes kFixPoison instead, round(2^64 / golden ratio) read as a signed integer: k poisons sum
-// (mod 2^64) to 2^64 frac(k / phi), which stays above 2^57 in magnitude for every k up to 88, so a
-// particle's sum that took up to 88 of them leaves the range |sum| <= 2^57 that real forces keep
-// (2^33 kcal/mol/A in the FP32 kernel, 2^25 in the FP64 one).  The integrator reads every sum each
-// step and ends a replica whose sum left that range as non-finite.  The conversion itself stays
-// free of side effects (a flag store in this rarely taken branch costs the kernel registers).
-// ------------------------------------------------------------------------------------
-constexpr double kFixLimit = 4611686018427387904.0;    // 2^62
-constexpr long long kFixPoison = (long long)0x9E3779B97F4A7C15ull;
-constexpr long long kFixSane = 1ll << 57;
-__device__ __forceinline__ long long fix_checked(double s)
-{
-    return (fabs(s) < kFixLimit) ? __double2ll_rn(s) : kFixPoison;   // NaN fails the test too
-}
-__device__ __forceinline__ bool fix_insane(unsigned long long v)
-{
-    const long long x = (long long)v;
-    return x > kFixSane || x < -kFixSane;
-}
-template<typename Real> struct Fix;
-template<> struct Fix<double>
-{
-    static constexpr double scale = 4294967296.0;          // 2^32
-    static constexpr double inv = 2.3283064365386963e-10;  // 2^-32
-    __device__ static __forceinline__ long long conv(double v) {return fix_checked(v * scale);}
-};
-template<> struct Fix<float>
-{
-    static constexpr double scale = 16777216.0;            // 2^24
-    static constexpr double inv = 5.9604644775390625e-08;  // 2^-24
-    __device__ static __forceinline__ long long conv(double v) {return fix_checked(v * scale);}
-};
-
-// Reciprocal, reciprocal square root, square root and exp of the pair and local arithmetic.
-// The FP32 (mixed) mode uses the hardware instructions (about 1 ulp, within the mode's force
-// error budget); the FP64 mode keeps correctly rounded operations and the library exp.
-template<typename Real> struct RMath;
-template<> struct RMath<float>
-{
-    __device__ static __forceinline__ float rcp(float x) {return __builtin_amdgcn_rcpf(x);}
-    __device__ static __forceinline__ float rsqrt(float x) {return __builtin_amdgcn_rsqf(x);}
-    __device__ static __forceinline__ float sqrt(float x) {return __builtin_amdgcn_sqrtf(x);}
-    __device__ static __forceinline__ float exp(float x) {return __expf(x);}
-};
-template<> struct RMath<double>
-{
-    __device__ static __forceinline__ double rcp(double x) {return 1.0 / x;}
-    __device__ static __forceinline__ double rsqrt(double x) {return 1.0 / ::sqrt(x);}
-    __device__ static __forceinline__ double sqrt(double x) {return ::sqrt(x);}
-    __device__ static __forceinline__ double exp(double x) {return ::exp(x);}
-};
-
 __device__ __forceinline__ void add3(unsigned long long* F, uint32_t i, long long x, long long y, long long z)
 {
     atomicAdd(&F[3 * i + 0], (unsigned long long)x);
@@ -209,34 +152,6 @@ __device__ __forceinline__ void add3(unsigned long long* F, uint32_t i, long lon
     atomicAdd(&F[3 * i + 2], (unsigned long long)z);
 }
 
-// The three components of a force in fixed point.  FP32: when every active lane's components
-// are below 2^31 units the 32-bit conversion serves them all, else every lane takes the FP64
-// route; both give the same integers, so the choice is made once per wave (one uniform branch)
-// instead of per component and lane.
-// (v 2^24 is exact in FP32, and rounding it to an integer in FP32 gives the integer the FP64 path
-// gives; below 2^31 that is a 32-bit conversion, two instructions instead of a dozen.  A NaN
-// fails the size test and takes the checked FP64 route.)
-template<typename Real>
-__device__ __forceinline__ void fix3r(Real x, Real y, Real z, long long& a, long long& b, long long& c);
-template<>
-__device__ __forceinline__ void fix3r<double>(double x, double y, double z, long long& a, long long& b, long long& c)
-{
-    a = Fix<double>::conv(x); b = Fix<double>::conv(y); c = Fix<double>::conv(z);
-}
-template<>
-__device__ __forceinline__ void fix3r<float>(float x, float y, float z, long long& a, long long& b, long long& c)
-{
-    const float sx = x * 16777216.0f, sy = y * 16777216.0f, sz = z * 16777216.0f;
-    const bool small = fmaxf(fabsf(sx), fmaxf(fabsf(sy), fabsf(sz))) < 2147483520.0f;
-    if(__all(small))
-    {
-        a = (long long)__float2int_rn(sx); b = (long long)__float2int_rn(sy); c = (long long)__float2int_rn(sz);
-    }
-    else
-    {
-        a = Fix<float>::conv((double)x); b = Fix<float>::conv((double)y); c = Fix<float>::conv((double)z);
-    }
-}
 template<typename Real, typename V>
 __device__ __forceinline__ void acc3(unsigned long long* F, uint32_t i, V x, V y, V z)
 {
@@ -264,12 +179,6 @@ template<typename T> __device__ __forceinline__ gptr<T> uniform_global(T* p)
     const unsigned long long hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(b >> 32));
     return as_global(reinterpret_cast<T*>((hi << 32) | lo));
 }
-template<typename Real>
-__device__ __forceinline__ double from_fix(unsigned long long v)
-{
-    return (double)(long long)v * Fix<Real>::inv;
-}
-
 // Gaussian noise of the integrator: the Philox stream of noise3 (device_common.hpp); the FP32
 // kernel forms its Box-Muller transforms in FP32 from the top 24 bits of each draw.
 template<typename Real>
@@ -1049,9 +958,13 @@ __device__ void rep_build_grid(const RepGlobal& G, const RepDesc& R, const Box& 
     ListAppend app;
     // rows in cell order: neighbouring lanes sit in the same or adjacent cells, so they scan
     // nearly the same candidate ranges (the wave's trip count is close to each lane's) and read
-    // the same entries (LDS broadcast)
-    for(uint32_t i = tid; i < np; i += BS)
+    // the same entries (LDS broadcast).  L = grid_lanes neighbouring lanes share a row, each
+    // taking every L-th candidate of its cell ranges (the list is a set: the split changes only
+    // the order of its entries).
+    const uint32_t L = uniform(R.grid_lanes);
+    for(uint32_t u = tid; u < np * L; u += BS)
     {
+        const uint32_t i = u / L, sub = u - i * L;
         const uint32_t e = sorted[i];
         const uint32_t a = e >> 16, pa = e & 0xffffu;
         const RowRules rr = row_rules(K, pa);
@@ -1060,10 +973,11 @@ __device__ void rep_build_grid(const RepGlobal& G, const RepDesc& R, const Box& 
         const bool run_x = cx >= 1 && cx + 1 < gs.dims[0];
         // the next candidate's entry is loaded while the current one is tested
         auto scan = [&](uint32_t beg, uint32_t end) {
+            beg += sub;
             uint32_t cur = (beg < end) ? sorted[beg] : 0u;
-            for(uint32_t q = beg; q < end; ++q)
+            for(uint32_t q = beg; q < end; q += L)
             {
-                const uint32_t nxt = (q + 1u < end) ? sorted[q + 1u] : 0u;
+                const uint32_t nxt = (q + L < end) ? sorted[q + L] : 0u;
                 const uint32_t b = cur >> 16, pb = cur & 0xffffu;
                 cur = nxt;
                 bool want = pb > pa;

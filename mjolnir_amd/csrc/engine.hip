@@ -76,7 +76,6 @@ Engine::Engine(int precision, int boundary, const double* lower, const double* u
     if(precision != MJG_FP64 && precision != MJG_FP32_FIXED) {throw std::invalid_argument("unknown precision");}
     if(boundary != MJG_UNLIMITED && boundary != MJG_PERIODIC) {throw std::invalid_argument("unknown boundary");}
     if(n == 0 || n >= 0xffffffffull) {throw std::invalid_argument("particle count out of range");}
-    if(precision == MJG_FP32_FIXED) {throw std::invalid_argument("MJG_FP32_FIXED is served by the batched engine");}
     check(hipSetDevice(device), "hipSetDevice");
     check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     box_.periodic = (boundary == MJG_PERIODIC);
@@ -101,8 +100,14 @@ Engine::Engine(int precision, int boundary, const double* lower, const double* u
     upload(d_sctl_, &z, 1);
     scratch_n_ = n;
     d_scratch_ = dalloc<double>(n);
+    // Steps are captured into a hipGraph per segment, except with MJG_NO_GRAPH=1 and under a
+    // preloaded profiler library (rocprofv3 preloads librocprofiler-sdk-tool): a kernel-trace run
+    // of the graph path once ended in a segmentation fault inside the runtime and the profiler,
+    // whose cause was not found, so profiled runs launch the same kernels on the stream instead.
     const char* ng = std::getenv("MJG_NO_GRAPH");
-    use_graph_ = !(ng && ng[0] == '1');
+    const char* pre = std::getenv("LD_PRELOAD");
+    const bool profiled = pre && std::strstr(pre, "rocprof") != nullptr;
+    use_graph_ = !(ng && ng[0] == '1') && !profiled;
 }
 
 Engine::~Engine()
@@ -852,8 +857,16 @@ void Engine::forces(double4* f, bool energy, int only, bool kick, bool advance)
     {
         uint32_t blocks = 0;
         for(int k = 0; k < pt.n; ++k) {blocks += blocks_for((size_t)pt.seg[k].np * pt.seg[k].tpp, 256);}
-        if(energy) {hipLaunchKernelGGL(k_pairs<true>, dim3(blocks), dim3(256), 0, stream_, d_pos_, box_, pt);}
-        else {hipLaunchKernelGGL(k_pairs<false>, dim3(blocks), dim3(256), 0, stream_, d_pos_, box_, pt);}
+        if(precision_ == MJG_FP32_FIXED)
+        {
+            if(energy) {hipLaunchKernelGGL(k_pairs_mixed<true>, dim3(blocks), dim3(256), 0, stream_, d_pos_, box_, pt);}
+            else {hipLaunchKernelGGL(k_pairs_mixed<false>, dim3(blocks), dim3(256), 0, stream_, d_pos_, box_, pt);}
+        }
+        else
+        {
+            if(energy) {hipLaunchKernelGGL(k_pairs<true>, dim3(blocks), dim3(256), 0, stream_, d_pos_, box_, pt);}
+            else {hipLaunchKernelGGL(k_pairs<false>, dim3(blocks), dim3(256), 0, stream_, d_pos_, box_, pt);}
+        }
     }
     for(size_t gi = 0; gi < globals_.size(); ++gi)
     {
@@ -1482,6 +1495,10 @@ double Engine::profile_interaction(int id, int reps, uint64_t* half_pairs)
                                d_lidx_, d_lpar_, d_tf_, d_te_, id);
         }
         else if(globals_[id - nl].multibody()) {launch_multibody(globals_[id - nl], false);}
+        else if(precision_ == MJG_FP32_FIXED)
+        {
+            hipLaunchKernelGGL(k_pairs_mixed<false>, dim3(pblocks), dim3(256), 0, stream_, d_pos_, box_, pt);
+        }
         else
         {
             hipLaunchKernelGGL(k_pairs<false>, dim3(pblocks), dim3(256), 0, stream_, d_pos_, box_, pt);

@@ -12,6 +12,7 @@
 // what prepare_params cached for the pair (i < j) in the reference.
 #pragma once
 #include "device_common.hpp"
+#include "mixed_math.hpp"
 #include "tsp_kernels.hpp"
 
 namespace mjg
@@ -389,6 +390,12 @@ struct PairSegTable
     int n;
 };
 
+// A lane's entries of a row, kPairBatch at a time: the list words and then the partners'
+// positions of a batch are all in flight before the first pair is computed (one dependent load
+// pair per entry would leave the lane waiting on memory); the lane still takes its entries
+// k = l, l + tpp, ... in increasing order, so its sum order is the unbatched one.
+constexpr int kPairBatch = 4;
+
 template<int KIND, bool ENERGY>
 __device__ __forceinline__ void pair_row(const PairSeg& s, uint32_t t, uint32_t l, const double4* __restrict__ pos,
                                          const Box& box, double& fx, double& fy, double& fz, double& e, uint32_t& i)
@@ -397,13 +404,26 @@ __device__ __forceinline__ void pair_row(const PairSeg& s, uint32_t t, uint32_t 
     const double4 ri = pos[i];
     const uint32_t cnt = s.nn[t];
     const uint32_t* row = s.nl + (size_t)t * s.cap;
-    for(uint32_t k = l; k < cnt; k += s.tpp)
+    for(uint32_t k0 = l; k0 < cnt; k0 += kPairBatch * s.tpp)
     {
-        const uint32_t j = row[k];
-        const double4 rj = pos[j];
-        double dx, dy, dz;
-        adjust_direction(box, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, dx, dy, dz);
-        pair_term<KIND, ENERGY>(s.pp, i, j, s.ppar, dx, dy, dz, fx, fy, fz, e);
+        uint32_t j[kPairBatch];
+        double4 rj[kPairBatch];
+#pragma unroll
+        for(int m = 0; m < kPairBatch; ++m)
+        {
+            const uint32_t k = k0 + m * s.tpp;
+            j[m] = (k < cnt) ? row[k] : i;
+        }
+#pragma unroll
+        for(int m = 0; m < kPairBatch; ++m) {rj[m] = pos[j[m]];}
+#pragma unroll
+        for(int m = 0; m < kPairBatch; ++m)
+        {
+            if(k0 + m * s.tpp >= cnt) {break;}
+            double dx, dy, dz;
+            adjust_direction(box, ri.x, ri.y, ri.z, rj[m].x, rj[m].y, rj[m].z, dx, dy, dz);
+            pair_term<KIND, ENERGY>(s.pp, i, j[m], s.ppar, dx, dy, dz, fx, fy, fz, e);
+        }
     }
 }
 
@@ -433,6 +453,164 @@ k_pairs(const double4* __restrict__ pos, Box box, PairSegTable tab)
     if(l == 0)
     {
         s.fout[i] = make_double4(fx, fy, fz, 0.0);
+        if(ENERGY) {s.eout[t] = e;}
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// Mixed precision (MJG_FP32_FIXED): the same rows and lists, the displacement formed in FP64
+// as the reference forms it (minimum image) and rounded to FP32, the potential arithmetic in
+// FP32 with the hardware reciprocal / square root / exp, and each pair's force added to the
+// lane's int64 fixed-point sum (2^-24 kcal/mol/A per unit, mixed_math.hpp).  The lanes of a row
+// combine their sums with an integer butterfly, so the row's force is exact and independent of
+// the lane split.  A poisoned sum (a non-finite or overflowing contribution) gives a NaN force,
+// which the integrator's finite check reports.
+// ------------------------------------------------------------------------------------
+struct PairConstsF
+{
+    float cut2, cac, eps, qk, acut, idebye, debye;
+};
+__device__ __forceinline__ PairConstsF pair_consts_f(const PairParams& pp)
+{
+    PairConstsF k;
+    k.cut2 = (float)pp.cutoff_ratio2; k.cac = (float)pp.coef_at_cutoff; k.eps = (float)pp.eps;
+    k.qk = (float)pp.inv4pieps; k.acut = (float)pp.abs_cutoff; k.idebye = (float)pp.inv_debye; k.debye = (float)pp.debye;
+    return k;
+}
+
+// force on the owner i from j as FP32 components in (cx, cy, cz) times the displacement; returns
+// false outside the cutoff.  Formulas as pair_term (FP64) and the replica kernel's rep_pair_k.
+template<int KIND, bool ENERGY>
+__device__ __forceinline__ bool pair_term_f32(const PairConstsF& K, const double* __restrict__ ppar, uint32_t i, uint32_t j,
+                                              float dx, float dy, float dz, float& c, double& e)
+{
+    const float l_sq = dx * dx + dy * dy + dz * dz;
+    if(KIND == kPairLJ)
+    {
+        const float s1 = (float)ppar[2 * i], e1 = (float)ppar[2 * i + 1];
+        const float s2 = (float)ppar[2 * j], e2 = (float)ppar[2 * j + 1];
+        const float sigma = (s1 + s2) * 0.5f;
+        const float eps = (e1 == e2) ? e1 : RMath<float>::sqrt(e1 * e2);
+        const float sigma_sq = sigma * sigma;
+        if(sigma_sq * K.cut2 < l_sq) {return false;}
+        const float rcp = RMath<float>::rcp(l_sq);
+        const float s2l2 = sigma_sq * rcp;
+        const float s6 = s2l2 * s2l2 * s2l2;
+        c = 24.0f * eps * (s6 - 2.0f * s6 * s6) * rcp;
+        if(ENERGY && i < j) {e += (double)(4.0f * eps * (s6 * s6 - s6 - K.cac));}
+    }
+    else if(KIND == kPairEV)
+    {
+        const float radius = (float)ppar[i] + (float)ppar[j];
+        const float sigma_sq = radius * radius;
+        if(sigma_sq * K.cut2 < l_sq) {return false;}
+        const float rcp = RMath<float>::rcp(l_sq);
+        const float s2l2 = sigma_sq * rcp;
+        const float s6 = s2l2 * s2l2 * s2l2;
+        c = -(K.eps * 12.0f) * s6 * s6 * rcp;
+        if(ENERGY && i < j) {e += (double)(K.eps * (s6 * s6 - K.cac));}
+    }
+    else if(KIND == kPair3SPN2EV)
+    {
+        const float sigma = (float)((tsp::ev_sigma((int)ppar[i]) + tsp::ev_sigma((int)ppar[j])) * 0.5);
+        const float rl = RMath<float>::rsqrt(l_sq);
+        const float l = l_sq * rl;
+        if(sigma <= l) {return false;}
+        const float r1s1 = sigma * rl;
+        const float r3s3 = r1s1 * r1s1 * r1s1;
+        const float r6s6 = r3s3 * r3s3;
+        c = 12.0f * K.eps * rl * r6s6 * (1.0f - r6s6) * rl;
+        if(ENERGY && i < j) {e += (double)(K.eps * (r6s6 * (r6s6 - 2.0f) + 1.0f));}
+    }
+    else
+    {
+        const float qq = K.qk * (float)ppar[i] * (float)ppar[j];
+        const float rl = RMath<float>::rsqrt(l_sq);
+        const float l = l_sq * rl;
+        if(K.acut <= l) {return false;}
+        const float ex = RMath<float>::exp(-l * K.idebye);
+        const float f_mag = -qq * (K.debye + l) * K.idebye * (rl * rl) * ex;
+        c = f_mag * rl;
+        if(ENERGY && i < j) {e += (double)(qq * (ex * rl - K.cac));}
+    }
+    return true;
+}
+
+template<int KIND, bool ENERGY>
+__device__ __forceinline__ void pair_row_mixed(const PairSeg& s, uint32_t t, uint32_t l, const double4* __restrict__ pos,
+                                               const Box& box, long long& ax, long long& ay, long long& az, double& e,
+                                               uint32_t& i)
+{
+    const PairConstsF K = pair_consts_f(s.pp);
+    i = s.part[s.order[t]];
+    const double4 ri = pos[i];
+    const uint32_t cnt = s.nn[t];
+    const uint32_t* row = s.nl + (size_t)t * s.cap;
+    for(uint32_t k0 = l; k0 < cnt; k0 += kPairBatch * s.tpp)
+    {
+        uint32_t j[kPairBatch];
+        double4 rj[kPairBatch];
+#pragma unroll
+        for(int m = 0; m < kPairBatch; ++m)
+        {
+            const uint32_t k = k0 + m * s.tpp;
+            j[m] = (k < cnt) ? row[k] : i;
+        }
+#pragma unroll
+        for(int m = 0; m < kPairBatch; ++m) {rj[m] = pos[j[m]];}
+#pragma unroll
+        for(int m = 0; m < kPairBatch; ++m)
+        {
+            if(k0 + m * s.tpp >= cnt) {break;}
+            double ddx, ddy, ddz;
+            adjust_direction(box, ri.x, ri.y, ri.z, rj[m].x, rj[m].y, rj[m].z, ddx, ddy, ddz);
+            const float dx = (float)ddx, dy = (float)ddy, dz = (float)ddz;
+            float c;
+            if(!pair_term_f32<KIND, ENERGY>(K, s.ppar, i, j[m], dx, dy, dz, c, e)) {continue;}
+            long long fx, fy, fz;
+            fix3r<float>(dx * c, dy * c, dz * c, fx, fy, fz);
+            ax += fx; ay += fy; az += fz;
+        }
+    }
+}
+
+__device__ __forceinline__ long long shfl_xor_ll(long long v, int o)
+{
+    const int lo = __shfl_xor((int)(v & 0xffffffff), o, kWave), hi = __shfl_xor((int)(v >> 32), o, kWave);
+    return ((long long)hi << 32) | (unsigned int)lo;
+}
+
+template<bool ENERGY>
+__global__ void __launch_bounds__(256)
+k_pairs_mixed(const double4* __restrict__ pos, Box box, PairSegTable tab)
+{
+    int si = 0;
+    while(si + 1 < tab.n && blockIdx.x >= tab.seg[si + 1].block_begin) {++si;}
+    const PairSeg& s = tab.seg[si];
+    const uint32_t gt = (blockIdx.x - s.block_begin) * blockDim.x + threadIdx.x;
+    const uint32_t t = gt / s.tpp, l = gt % s.tpp;
+    if(t >= s.np) {return;}   // whole groups leave together
+    long long ax = 0, ay = 0, az = 0;
+    double e = 0;
+    uint32_t i = 0;
+    if(s.kind == kPairLJ) {pair_row_mixed<kPairLJ, ENERGY>(s, t, l, pos, box, ax, ay, az, e, i);}
+    else if(s.kind == kPairEV) {pair_row_mixed<kPairEV, ENERGY>(s, t, l, pos, box, ax, ay, az, e, i);}
+    else if(s.kind == kPair3SPN2EV) {pair_row_mixed<kPair3SPN2EV, ENERGY>(s, t, l, pos, box, ax, ay, az, e, i);}
+    else {pair_row_mixed<kPairDH, ENERGY>(s, t, l, pos, box, ax, ay, az, e, i);}
+    for(uint32_t o = s.tpp / 2; o > 0; o >>= 1)
+    {
+        ax += shfl_xor_ll(ax, (int)o);
+        ay += shfl_xor_ll(ay, (int)o);
+        az += shfl_xor_ll(az, (int)o);
+        if(ENERGY) {e += __shfl_xor(e, o, kWave);}
+    }
+    if(l == 0)
+    {
+        const bool bad = fix_insane((unsigned long long)ax) || fix_insane((unsigned long long)ay) ||
+                         fix_insane((unsigned long long)az);
+        const double nan = bad ? __builtin_nan("") : 0.0;
+        s.fout[i] = make_double4(from_fix<float>((unsigned long long)ax) + nan, from_fix<float>((unsigned long long)ay) + nan,
+                                 from_fix<float>((unsigned long long)az) + nan, 0.0);
         if(ENERGY) {s.eout[t] = e;}
     }
 }

@@ -12,8 +12,9 @@ Two lanes share the GPU:
   FP64 integrator).  Each segment advances every resident replica in one launch per size class
   and then records every replica's energies in one evaluation launch.  Between segments the
   finished replicas are retired and the waiting jobs admitted.
-* engine lane: larger systems (and constrained ones) run on FP64 engines, one per job and
-  ``workers`` at a time.  Each engine has its own HIP stream and host thread (the library's C
+* engine lane: larger systems (and constrained ones) run on engines, one per job and ``workers``
+  at a time, in ``engine_precision`` (default MJG_FP32_FIXED: FP32 pair arithmetic with int64
+  fixed-point sums, FP64 local terms and integrator; MJG_FP64 for the plain FP64 path).  Each engine has its own HIP stream and host thread (the library's C
   calls release the GIL), so their segments overlap on the device with the batch lane's.
 
 Admission checks the slot counts and a device-memory budget (``hbm_budget`` bytes against an
@@ -130,7 +131,9 @@ def make_jobs(trace, procs: int = 8) -> List[Job]:
 # ------------------------------------------------------------------------------------------
 class Server:
     def __init__(self, hbm_budget: float = 200e9, batch_slots: int = 1024, workers: int = 4, segment: int = 100,
-                 batch_max_n: int = 2048, device: int = 0):
+                 batch_max_n: int = 2048, device: int = 0, engine_precision: Optional[int] = None):
+        from . import capi
+        self.engine_precision = capi.FP32_FIXED if engine_precision is None else engine_precision
         self.hbm_budget = hbm_budget
         self.batch_slots = batch_slots
         self.workers = workers
@@ -162,7 +165,7 @@ class Server:
                 return
             try:
                 job.admitted = time.perf_counter() - t0
-                e = capi.Engine(job.spec, capi.FP64, device=self.device)
+                e = capi.Engine(job.spec, self.engine_precision, device=self.device)
                 try:
                     self._record_engine(job, e, 0)
                     done = 0
@@ -301,6 +304,40 @@ class Server:
         if self._errors:
             raise self._errors[0]
         return summarize(jobs)
+
+
+def one_at_a_time(jobs: List[Job], precision: int, segment: int = 100, device: int = 0) -> dict:
+    """The same jobs run one after another on single engines of the given precision, each with its
+    energy trace every `segment` steps, as a client without the server would run them: every job
+    waits for the ones before it.  Arrival times are kept, so latencies include that wait."""
+    from . import capi
+    t0 = time.perf_counter()
+    done = []
+    for job in sorted(jobs, key=lambda j: j.arrival):
+        now = time.perf_counter() - t0
+        if now < job.arrival:
+            time.sleep(job.arrival - now)
+        j = Job(id=job.id, kind=job.kind, spec=job.spec, steps=job.steps, arrival=job.arrival, lane="engine")
+        j.admitted = time.perf_counter() - t0
+        e = capi.Engine(j.spec, precision, device=device)
+        try:
+            Server._record_engine(j, e, 0)
+            k = 0
+            while k < j.steps:
+                n = min(segment, j.steps - k)
+                e.step(n)
+                k += n
+                j.steps_done = k
+                Server._record_engine(j, e, k)
+        except capi.MjgError as exc:
+            if exc.status != capi.NAN_DETECTED:
+                raise
+            j.failed = str(exc)
+        finally:
+            e.close()
+        j.finished = time.perf_counter() - t0
+        done.append(j)
+    return summarize(done)
 
 
 def summarize(jobs: List[Job]) -> dict:

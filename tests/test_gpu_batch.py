@@ -390,3 +390,27 @@ def test_fp64_batch_config3_duplex_vs_reference():
     print(f"[gpu] dna200_b338 FP64 replica: max|dx| {np.abs(x - g['zt_pos']).max():.2e}", flush=True)
     assert np.abs(x - g["zt_pos"]).max() <= 1e-8
     assert np.abs(v - g["zt_vel"]).max() <= 1e-8
+
+
+@pytest.mark.parametrize("kind", ["chain", "lj"])
+def test_tuning_changes_no_result(kind):
+    """The autotuner's knobs (list skins, cell-grid resolution, lanes per grid row) change no
+    result, because forces are exact fixed-point sums over the pairs inside the cutoff: one replica
+    under four fixed tunings and under the autotuner ends bitwise equal after 400 steps."""
+    from mjolnir_amd import capi, workloads
+    spec = _chains([1500], seed0=830)[0] if kind == "chain" else workloads.lj_fluid(n=1000, seed=31)
+    settings = [(0.2, 0.05, 3, 1), (0.6, 0.3, 4, 4), (0.4, 0.1, 4, 2), (0.3, 0.2, 3, 4), None]
+    states = []
+    for st in settings:
+        b = capi.Batch(capi.FP32_FIXED)
+        r = b.add(spec)
+        if st is not None:
+            b.set_tuning(r, *st)
+        b.initialize()
+        b.step(400)
+        states.append(b.state(r))
+        print(f"[gpu] {kind} tuning {b.tuning(r)}: rebuilds {b.info(r)['rebuilds']}", flush=True)
+        b.close()
+    for s in states[1:]:
+        for a, c in zip(states[0], s):
+            assert np.array_equal(a, c)

@@ -294,3 +294,56 @@ def test_forces_on_reference_trajectory_frames(name):
             err = np.linalg.norm(e.force_of(k) - ref, axis=1)
             assert np.all(err <= force_tolerance(ref, 1e-10)), (name, step, k, err.max())
         e.close()
+
+
+MIXED_CASES = ["lj4096", "chain1000", "dna200_b338", "lj_boundary", "two_chains_self"]
+
+
+def _rms_rel(f, ref):
+    rms = np.sqrt(np.mean(np.sum(ref ** 2, axis=1)))
+    return (np.sqrt(np.mean(np.sum((f - ref) ** 2, axis=1))) / rms,
+            float(np.max(np.linalg.norm(f - ref, axis=1)) / rms))
+
+
+@pytest.mark.parametrize("name", MIXED_CASES)
+def test_engine_mixed_precision_forces_vs_reference(name):
+    """The engine in MJG_FP32_FIXED (FP32 pair arithmetic, int64 fixed-point sums): total force
+    against the reference's, RMS error <= 1e-5 of the RMS force (the max is printed); neighbour
+    sets are the FP64 build's, so they stay bit-exact."""
+    from mjolnir_amd import capi
+    spec, g = golden_case(name)
+    e = _engine(spec, precision=capi.FP32_FIXED, with_integrator=False)
+    e.calc_force()
+    _, _, f = e.state()
+    rms, mx = _rms_rel(f, g["total_force"])
+    print(f"[gpu] {name} FP32_FIXED engine: RMS force error {rms:.2e}, max {mx:.2e} (relative to the RMS force)",
+          flush=True)
+    assert rms <= 1e-5
+    names = [str(x) for x in g["names"]]
+    for k in range(len(names)):
+        if f"npairs_{k}" in g:
+            p = e.pairs(k)
+            assert make_golden.pair_key_hash(p, spec.n) == str(g[f"pairhash_{k}"]), names[k]
+
+
+def test_engine_mixed_precision_lj65536_vs_oracle():
+    """A 65,536-particle LJ fluid (the size class of the served LJ jobs) in MJG_FP32_FIXED against
+    the FP64 oracle: RMS force error <= 1e-5 of the RMS force; the same box run 200 steps in both
+    precisions stays finite and the FP32 run's kinetic energy is within 1 % of the FP64 run's."""
+    import oracle
+    from mjolnir_amd import capi, workloads
+    spec = make_golden.perturb(workloads.lj_fluid(n_side=32, seed=21), 0.1, 22)
+    o = oracle.OracleEngine(spec, with_integrator=False)
+    f_ref, _ = o.force_and_energy()
+    e = _engine(spec, precision=capi.FP32_FIXED)
+    _, _, f = e.state()
+    rms, mx = _rms_rel(f, f_ref)
+    print(f"[gpu] LJ 65536 FP32_FIXED engine: RMS force error {rms:.2e}, max {mx:.2e}", flush=True)
+    assert rms <= 1e-5
+    e64 = _engine(spec)
+    e.step(200)
+    e64.step(200)
+    x, v, _ = e.state()
+    assert np.isfinite(x).all()
+    k32, k64 = e.kinetic_energy(), e64.kinetic_energy()
+    assert abs(k32 - k64) <= 0.01 * k64, (k32, k64)
