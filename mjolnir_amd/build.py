@@ -40,9 +40,18 @@ def units():
     return out
 
 
-def _compile(unit, verbose):
+def _newest_source(src: str) -> float:
+    """Latest modification time of a unit's source, every header in csrc/ and the C header."""
+    paths = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hpp")]
+    paths += [os.path.join(CSRC, src), os.path.join(HERE, "..", "include", "mjg.h")]
+    return max(os.path.getmtime(p) for p in paths if os.path.isfile(p))
+
+
+def _compile(unit, verbose, incremental=False):
     name, src, extra = unit
     obj = os.path.join(OBJ, name + ".o")
+    if incremental and os.path.exists(obj) and os.path.getmtime(obj) >= _newest_source(src):
+        return name, None
     cmd = [HIPCC] + FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", obj + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
@@ -53,12 +62,14 @@ def _compile(unit, verbose):
     return name, None
 
 
-def build(verbose: bool = False, jobs: int = 0) -> str:
+def build(verbose: bool = False, jobs: int = 0, incremental: bool = False) -> str:
+    """Compile and link libmjg.so.  incremental: skip translation units whose object is newer
+    than every source (a header change rebuilds all of them)."""
     os.makedirs(OBJ, exist_ok=True)
     us = units()
     jobs = jobs or max(1, min(len(us), os.cpu_count() or 1, 16))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        results = list(ex.map(lambda u: _compile(u, verbose), us))
+        results = list(ex.map(lambda u: _compile(u, verbose, incremental), us))
     failed = [(n, log) for n, log in results if log is not None]
     if failed:
         for n, log in failed:
@@ -77,4 +88,4 @@ def build(verbose: bool = False, jobs: int = 0) -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, incremental="--incremental" in sys.argv))
