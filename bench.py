@@ -337,8 +337,10 @@ def bench_batch(args):
 
     # the same ensemble in the FP64 replica kernel (isolates what precision buys), and the plain
     # FP64 engine one replica at a time
-    b64, _, m64 = run_batch(specs, capi.FP64, args.job_steps, args.warmup, args.steps)
-    b64.close()
+    m64 = None
+    if not args.no_fp64_batch:
+        b64, _, m64 = run_batch(specs, capi.FP64, args.job_steps, args.warmup, args.steps)
+        b64.close()
     fp64 = None if args.no_fp64_sample else fp64_engine_rate(specs)
     cpu = None if args.no_cpu_baseline else cpu_baseline_batch(specs)
     return {
@@ -358,10 +360,11 @@ def bench_batch(args):
                         "note": "the config's whole 10^4-step run from the generated state, 100-step segments"},
         "force_error": ferr,
         "force_error_per_config": per_config,
-        "fp64_batch": {"value": round(m64["value"], 1), "config4_job_value": round(m64["job_value"], 1),
-                       "unit": "particle-steps/s", "ratio_fp32_over_fp64": round(m["value"] / m64["value"], 3),
-                       "failed_replicas": len(m64["failed"]),
-                       "note": "the same ensemble and windows in the replica kernel's FP64 mode"},
+        "fp64_batch": None if m64 is None else {
+            "value": round(m64["value"], 1), "config4_job_value": round(m64["job_value"], 1),
+            "unit": "particle-steps/s", "ratio_fp32_over_fp64": round(m["value"] / m64["value"], 3),
+            "failed_replicas": len(m64["failed"]),
+            "note": "the same ensemble and windows in the replica kernel's FP64 mode"},
         "fp64_engine": None if fp64 is None else {
             "value": round(fp64, 1), "unit": "particle-steps/s",
             "sample": "12 replicas (stratified by length), plain FP64 engine one at a time",
@@ -496,6 +499,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-sample", action="store_true")
     ap.add_argument("--no-config-errors", action="store_true")
+    ap.add_argument("--no-fp64-batch", action="store_true",
+                    help="replicas512: skip the FP64 replica-kernel run of the same ensemble")
     ap.add_argument("--serve-jobs", type=int, default=40,
                     help="jobs of the serving stream (workload serve, and the default run's serving entry)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,

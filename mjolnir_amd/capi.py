@@ -155,6 +155,7 @@ class Engine:
 
     def __init__(self, spec: SystemSpec, precision: int = FP64, device: int = 0, with_integrator: bool = True):
         L = lib()
+        self._pid = os.getpid()
         self.spec = spec
         self.n = spec.n
         h = C.c_void_p()
@@ -200,8 +201,11 @@ class Engine:
             raise MjgError(st, msg.decode() if msg else "")
 
     def close(self):
+        # a handle belongs to the process that created it: a forked child that collects the
+        # object must not call into the HIP runtime with it
         if getattr(self, "h", None):
-            lib().mjg_engine_destroy(self.h)
+            if getattr(self, "_pid", os.getpid()) == os.getpid():
+                lib().mjg_engine_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -308,6 +312,7 @@ class Batch:
 
     def __init__(self, precision: int = FP32_FIXED, device: int = 0):
         L = lib()
+        self._pid = os.getpid()
         h = C.c_void_p()
         st = L.mjg_batch_create(precision, device, C.byref(h))
         if st != OK:
@@ -352,7 +357,8 @@ class Batch:
 
     def close(self):
         if getattr(self, "h", None):
-            lib().mjg_batch_destroy(self.h)
+            if getattr(self, "_pid", os.getpid()) == os.getpid():
+                lib().mjg_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -553,7 +553,7 @@ class Batch
     {
         Replica& r = rep(id);
         if(!r.ready) {throw std::logic_error("replica not initialized");}
-        launch({id}, 0, 2);
+        launch_eval({id});
         RepState s = state(id);
         double tot = 0;
         const int ne = (int)r.locals.size() + (int)r.globals.size();
@@ -568,9 +568,28 @@ class Batch
     {
         std::vector<int> act = active_ids();
         if(act.empty()) {return;}
-        launch(act, 0, 2);
-        download_states();
+        launch_eval(act);
         for(int id : act) {reps_[id]->observed = true;}
+    }
+
+    // One evaluation launch (forces and energies, no step); a replica whose list had to be rebuilt
+    // and did not fit (a forced rebuild after a restore) is evaluated again with a larger list.
+    void launch_eval(const std::vector<int>& ids)
+    {
+        launch(ids, 0, 2);
+        download_states();
+        std::vector<int> again;
+        for(int id : ids) {if(h_states_[reps_[id]->slot].overflow) {again.push_back(id);}}
+        for(int pass = 0; pass < 6 && !again.empty(); ++pass)
+        {
+            for(int id : again) {grow_if_overflow(*reps_[id], false);}
+            launch(again, 0, 2);
+            download_states();
+            std::vector<int> still;
+            for(int id : again) {if(h_states_[reps_[id]->slot].overflow) {still.push_back(id);}}
+            again.swap(still);
+        }
+        if(!again.empty()) {throw BatchDeviceError("replica list capacity did not converge");}
     }
 
     // the last observation of one replica: per-interaction energies and the kinetic energy
@@ -1128,9 +1147,10 @@ class Batch
         hip_check(hipStreamSynchronize(stream_), "replica sync");
     }
 
-    // after a step launch: grow the list capacities that overflowed and restart the replica from
-    // its segment-start copy (true if the replica must rerun the segment)
-    bool grow_if_overflow(Replica& r)
+    // after a launch: grow the list capacities that overflowed and reset the replica to run again,
+    // from its segment-start copy after a step launch (from_backup) or from its current state after
+    // an evaluation launch, which moved nothing (true if the replica must run again)
+    bool grow_if_overflow(Replica& r, bool from_backup = true)
     {
         const RepState s = state_of(r);
         bool over = false;
@@ -1144,11 +1164,12 @@ class Batch
         }
         if(!over) {return false;}
         std::vector<double> pos(3 * r.n), vel(3 * r.n), force(3 * r.n);
-        download_xyz(r, r.desc.bak_pos, pos);
-        download_xyz(r, r.desc.bak_vel, vel);
-        download_xyz(r, r.desc.bak_force, force);
+        download_xyz(r, from_backup ? r.desc.bak_pos : r.desc.pos, pos);
+        download_xyz(r, from_backup ? r.desc.bak_vel : r.desc.vel, vel);
+        download_xyz(r, from_backup ? r.desc.bak_force : r.desc.force, force);
         RepState bak{};
-        hip_check(hipMemcpyAsync(&bak, r.desc.bak_st, sizeof(RepState), hipMemcpyDeviceToHost, stream_), "download");
+        hip_check(hipMemcpyAsync(&bak, from_backup ? r.desc.bak_st : r.desc.st, sizeof(RepState),
+                                 hipMemcpyDeviceToHost, stream_), "download");
         hip_check(hipStreamSynchronize(stream_), "sync");
         r.pos = pos;
         r.vel = vel;
@@ -1192,7 +1213,8 @@ class Batch
         for(int id : ids)
         {
             Replica& r = *reps_[id];
-            if(r.failed || !h_states_[r.slot].nonfinite) {continue;}
+            // a replica whose list overflowed runs its segment again (its state is restored)
+            if(r.failed || h_states_[r.slot].overflow || !h_states_[r.slot].nonfinite) {continue;}
             r.failed = true;
             r.active = false;
             failed_now_.push_back(id);

@@ -20,6 +20,7 @@
 #pragma once
 #include "device_common.hpp"
 #include "mixed_math.hpp"
+#include <type_traits>
 #include "tsp_model.hpp"
 
 namespace mjg
@@ -150,6 +151,15 @@ __device__ __forceinline__ void add3(unsigned long long* F, uint32_t i, long lon
     atomicAdd(&F[3 * i + 0], (unsigned long long)x);
     atomicAdd(&F[3 * i + 1], (unsigned long long)y);
     atomicAdd(&F[3 * i + 2], (unsigned long long)z);
+}
+
+// F[i] -= (x, y, z): the LDS subtract (the same integer as adding the two's complement, without
+// negating the three 64-bit words first)
+__device__ __forceinline__ void sub3(unsigned long long* F, uint32_t i, long long x, long long y, long long z)
+{
+    __hip_atomic_fetch_sub(&F[3 * i + 0], (unsigned long long)x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_sub(&F[3 * i + 1], (unsigned long long)y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_sub(&F[3 * i + 2], (unsigned long long)z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 template<typename Real, typename V>
@@ -284,7 +294,7 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         const double len = len2 * rlen;
         const double c = (-1 * (2 * p0 * (len - p1))) * rlen;
         const long long fx = Fix<Real>::conv(dx * c), fy = Fix<Real>::conv(dy * c), fz = Fix<Real>::conv(dz * c);
-        add3(F, i0, -fx, -fy, -fz);
+        sub3(F, i0, fx, fy, fz);
         add3(F, i1, fx, fy, fz);
         if(ENERGY) {const double dv = len - p1; e = p0 * dv * dv;}
     }
@@ -306,7 +316,7 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         const Real coef = (Real)(-60) * k * r2 * (v0r_10 - v0r_12);
         long long fx, fy, fz;
         fix3r<Real>(coef * dx, coef * dy, coef * dz, fx, fy, fz);
-        add3(F, i0, -fx, -fy, -fz);
+        sub3(F, i0, fx, fy, fz);
         add3(F, i1, fx, fy, fz);
         if(ENERGY) {e = (double)(k * ((Real)5 * v0r_12 - (Real)6 * v0r_10));}
     }
@@ -351,7 +361,7 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         fix3r<Real>(Fix_, Fiy, Fiz, ix, iy, iz);
         fix3r<Real>(Fkx, Fky, Fkz, kx, ky, kz);
         add3(F, i0, ix, iy, iz);
-        add3(F, i1, -(ix + kx), -(iy + ky), -(iz + kz));
+        sub3(F, i1, ix + kx, iy + ky, iz + kz);
         add3(F, i2, kx, ky, kz);
         if(ENERGY) {e = (double)pot;}
     }
@@ -412,7 +422,7 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         const double dv = len - p1;
         const double c = (-1 * (2 * p0 * (dv + 200 * dv * dv * dv))) * rlen;
         const long long fx = Fix<Real>::conv(dx * c), fy = Fix<Real>::conv(dy * c), fz = Fix<Real>::conv(dz * c);
-        add3(F, i0, -fx, -fy, -fz);
+        sub3(F, i0, fx, fy, fz);
         add3(F, i1, fx, fy, fz);
         if(ENERGY) {const double dv2 = dv * dv; e = p0 * (dv2 + 100 * dv2 * dv2);}
     }
@@ -485,16 +495,20 @@ __device__ __forceinline__ void rep_pair(const RepGlobal& G, const Real* par, ui
     double ddx, ddy, ddz;
     disp(box, P, i, j, ddx, ddy, ddz);
     const Real dx = (Real)ddx, dy = (Real)ddy, dz = (Real)ddz;
-    const Real l_sq = dx * dx + dy * dy + dz * dz;
+    // cutoff decisions in FP64 in both modes (mixed_cut_sq): the pair set inside the cutoff is the
+    // FP64 path's, so no pair near the cutoff flips in or out in FP32
+    const double l_sq_d = length_sq(ddx, ddy, ddz);
+    const Real l_sq = (Real)l_sq_d;
     Real c;
     if(G.kind == kPairLJ)
     {
         // LorentzBerthelotRule (ParameterList.hpp:240-252), LennardJonesPotential.hpp:34-63
         const Real s1 = par[2 * i], e1 = par[2 * i + 1], s2 = par[2 * j], e2 = par[2 * j + 1];
-        const Real sigma = (s1 + s2) / (Real)2;
+        const double sigma_d = ((double)s1 + (double)s2) / 2;
         const Real eps = (e1 == e2) ? e1 : RMath<Real>::sqrt(e1 * e2);
-        const Real sigma_sq = sigma * sigma;
-        if(sigma_sq * (Real)G.cutoff_ratio2 < l_sq) {return;}
+        const double sigma_sq_d = sigma_d * sigma_d;
+        if(sigma_sq_d * G.cutoff_ratio2 < l_sq_d) {return;}
+        const Real sigma_sq = (Real)sigma_sq_d;
         const Real rcp = RMath<Real>::rcp(l_sq);
         const Real s2l2 = sigma_sq * rcp;
         const Real s6 = s2l2 * s2l2 * s2l2;
@@ -504,9 +518,10 @@ __device__ __forceinline__ void rep_pair(const RepGlobal& G, const Real* par, ui
     else if(G.kind == kPairEV)
     {
         // ExcludedVolumePotential.hpp:44-133
-        const Real radius = par[i] + par[j];
-        const Real sigma_sq = radius * radius;
-        if(sigma_sq * (Real)G.cutoff_ratio2 < l_sq) {return;}
+        const double radius_d = (double)par[i] + (double)par[j];
+        const double sigma_sq_d = radius_d * radius_d;
+        if(sigma_sq_d * G.cutoff_ratio2 < l_sq_d) {return;}
+        const Real sigma_sq = (Real)sigma_sq_d;
         const Real rcp = RMath<Real>::rcp(l_sq);
         const Real s2l2 = sigma_sq * rcp;
         const Real s6 = s2l2 * s2l2 * s2l2;
@@ -529,10 +544,17 @@ __device__ __forceinline__ void rep_pair(const RepGlobal& G, const Real* par, ui
     else
     {
         // DebyeHuckelPotential.hpp:47-65
+        if constexpr(std::is_same<Real, float>::value)
+        {
+            if(G.abs_cutoff * G.abs_cutoff <= l_sq_d) {return;}
+        }
         const Real qq = (Real)G.inv4pieps * par[i] * par[j];
         const Real rl = RMath<Real>::rsqrt(l_sq);
         const Real l = l_sq * rl;
-        if((Real)G.abs_cutoff <= l) {return;}
+        if constexpr(!std::is_same<Real, float>::value)
+        {
+            if((Real)G.abs_cutoff <= l) {return;}
+        }
         const Real ex = RMath<Real>::exp(-l * (Real)G.inv_debye);
         const Real f_mag = -qq * ((Real)G.debye + l) * (Real)G.inv_debye * (rl * rl) * ex;
         c = f_mag * rl;
@@ -541,7 +563,7 @@ __device__ __forceinline__ void rep_pair(const RepGlobal& G, const Real* par, ui
     long long fx, fy, fz;
     fix3r<Real>(dx * c, dy * c, dz * c, fx, fy, fz);
     add3(F, i, fx, fy, fz);
-    add3(F, j, -fx, -fy, -fz);
+    sub3(F, j, fx, fy, fz);
 }
 
 // Constants of one pair interaction in the arithmetic type, converted once per segment step and
@@ -549,11 +571,14 @@ __device__ __forceinline__ void rep_pair(const RepGlobal& G, const Real* par, ui
 template<typename Real> struct PairK
 {
     Real cut2, cac, eps, qk, acut, idebye, debye;
+    double cut2d, acut2d;   // the cutoff tests' FP64 operands
 };
 template<typename Real>
 __device__ __forceinline__ PairK<Real> pair_consts(const RepGlobal& G)
 {
     PairK<Real> k;
+    k.cut2d = uniform(G.cutoff_ratio2);
+    k.acut2d = uniform(G.abs_cutoff * G.abs_cutoff);
     k.cut2 = uniform((Real)G.cutoff_ratio2);
     k.cac = uniform((Real)G.coef_at_cutoff);
     k.eps = uniform((Real)G.eps);
@@ -574,16 +599,19 @@ __device__ __forceinline__ void rep_pair_k(const PairK<Real>& K, const Real* par
     if constexpr(PERIODIC) {disp(box, P, i, j, ddx, ddy, ddz);}
     else {ddx = P.x[j] - P.x[i]; ddy = P.y[j] - P.y[i]; ddz = P.z[j] - P.z[i];}
     const Real dx = (Real)ddx, dy = (Real)ddy, dz = (Real)ddz;
-    const Real l_sq = dx * dx + dy * dy + dz * dz;
+    // cutoff decisions in FP64 (as rep_pair)
+    const double l_sq_d = length_sq(ddx, ddy, ddz);
+    const Real l_sq = (Real)l_sq_d;
     Real c;
     if constexpr(KIND == kPairLJ)
     {
         // LorentzBerthelotRule (ParameterList.hpp:240-252), LennardJonesPotential.hpp:34-63
         const Real s1 = par[2 * i], e1 = par[2 * i + 1], s2 = par[2 * j], e2 = par[2 * j + 1];
-        const Real sigma = (s1 + s2) / (Real)2;
+        const double sigma_d = ((double)s1 + (double)s2) / 2;
         const Real eps = (e1 == e2) ? e1 : RMath<Real>::sqrt(e1 * e2);
-        const Real sigma_sq = sigma * sigma;
-        if(sigma_sq * K.cut2 < l_sq) {return;}
+        const double sigma_sq_d = sigma_d * sigma_d;
+        if(sigma_sq_d * K.cut2d < l_sq_d) {return;}
+        const Real sigma_sq = (Real)sigma_sq_d;
         const Real rcp = RMath<Real>::rcp(l_sq);
         const Real s2l2 = sigma_sq * rcp;
         const Real s6 = s2l2 * s2l2 * s2l2;
@@ -593,9 +621,10 @@ __device__ __forceinline__ void rep_pair_k(const PairK<Real>& K, const Real* par
     else if constexpr(KIND == kPairEV)
     {
         // ExcludedVolumePotential.hpp:44-133
-        const Real radius = par[i] + par[j];
-        const Real sigma_sq = radius * radius;
-        if(sigma_sq * K.cut2 < l_sq) {return;}
+        const double radius_d = (double)par[i] + (double)par[j];
+        const double sigma_sq_d = radius_d * radius_d;
+        if(sigma_sq_d * K.cut2d < l_sq_d) {return;}
+        const Real sigma_sq = (Real)sigma_sq_d;
         const Real rcp = RMath<Real>::rcp(l_sq);
         const Real s2l2 = sigma_sq * rcp;
         const Real s6 = s2l2 * s2l2 * s2l2;
@@ -605,10 +634,17 @@ __device__ __forceinline__ void rep_pair_k(const PairK<Real>& K, const Real* par
     else
     {
         // DebyeHuckelPotential.hpp:47-65
+        if constexpr(std::is_same<Real, float>::value)
+        {
+            if(K.acut2d <= l_sq_d) {return;}
+        }
         const Real qq = K.qk * par[i] * par[j];
         const Real rl = RMath<Real>::rsqrt(l_sq);
         const Real l = l_sq * rl;
-        if(K.acut <= l) {return;}
+        if constexpr(!std::is_same<Real, float>::value)
+        {
+            if(K.acut <= l) {return;}
+        }
         const Real ex = RMath<Real>::exp(-l * K.idebye);
         const Real f_mag = -qq * (K.debye + l) * K.idebye * (rl * rl) * ex;
         c = f_mag * rl;
@@ -617,7 +653,7 @@ __device__ __forceinline__ void rep_pair_k(const PairK<Real>& K, const Real* par
     long long fx, fy, fz;
     fix3r<Real>(dx * c, dy * c, dz * c, fx, fy, fz);
     add3(F, i, fx, fy, fz);
-    add3(F, j, -fx, -fy, -fz);
+    sub3(F, j, fx, fy, fz);
 }
 
 // The pair terms of one interaction: kPairBatch list entries per thread per load round, the
@@ -1241,6 +1277,11 @@ __device__ __forceinline__ void rep_segment(const RepDesc& R, int nsteps, int ev
                 }
             }
             __syncthreads();
+            // A list that did not fit holds only part of its pairs: the segment ends here, before
+            // any force is taken from it, and the host runs it again from the segment-start copy
+            // (an evaluation from the current state) with a larger list.  Uniform: the flag was
+            // written before the barrier above.
+            if(s_flags & 1) {break;}
         }
         // local terms, kTermBatch per thread per load round
         const uint32_t nt = R.n_terms;

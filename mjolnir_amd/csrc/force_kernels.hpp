@@ -478,21 +478,28 @@ __device__ __forceinline__ PairConstsF pair_consts_f(const PairParams& pp)
     return k;
 }
 
-// force on the owner i from j as FP32 components in (cx, cy, cz) times the displacement; returns
-// false outside the cutoff.  Formulas as pair_term (FP64) and the replica kernel's rep_pair_k.
+// force on the owner i from j as a FP32 factor c of the displacement; returns false outside the
+// cutoff.  Formulas as pair_term (FP64) and the replica kernel's rep_pair_k.  The cutoff decisions
+// are taken in FP64 from the FP64 displacement (ddx, ddy, ddz) and parameters, as pair_term takes
+// them (the Debye-Hueckel one on r^2 against the squared cutoff), so the set of pairs inside the
+// cutoff is the FP64 path's: a pair at the cutoff, where the LJ force is still 24 eps s^6 / r_c,
+// never flips in or out by FP32 rounding.
 template<int KIND, bool ENERGY>
-__device__ __forceinline__ bool pair_term_f32(const PairConstsF& K, const double* __restrict__ ppar, uint32_t i, uint32_t j,
-                                              float dx, float dy, float dz, float& c, double& e)
+__device__ __forceinline__ bool pair_term_f32(const PairConstsF& K, const PairParams& pp, const double* __restrict__ ppar,
+                                              uint32_t i, uint32_t j, double ddx, double ddy, double ddz, float& c,
+                                              double& e)
 {
-    const float l_sq = dx * dx + dy * dy + dz * dz;
+    const double l_sq_d = length_sq(ddx, ddy, ddz);
+    const float l_sq = (float)l_sq_d;
     if(KIND == kPairLJ)
     {
-        const float s1 = (float)ppar[2 * i], e1 = (float)ppar[2 * i + 1];
-        const float s2 = (float)ppar[2 * j], e2 = (float)ppar[2 * j + 1];
-        const float sigma = (s1 + s2) * 0.5f;
+        const double s1 = ppar[2 * i], s2 = ppar[2 * j];
+        const float e1 = (float)ppar[2 * i + 1], e2 = (float)ppar[2 * j + 1];
+        const double sigma_d = (s1 + s2) / 2;
+        const double sigma_sq_d = sigma_d * sigma_d;
+        if(sigma_sq_d * pp.cutoff_ratio2 < l_sq_d) {return false;}
         const float eps = (e1 == e2) ? e1 : RMath<float>::sqrt(e1 * e2);
-        const float sigma_sq = sigma * sigma;
-        if(sigma_sq * K.cut2 < l_sq) {return false;}
+        const float sigma_sq = (float)sigma_sq_d;
         const float rcp = RMath<float>::rcp(l_sq);
         const float s2l2 = sigma_sq * rcp;
         const float s6 = s2l2 * s2l2 * s2l2;
@@ -501,9 +508,10 @@ __device__ __forceinline__ bool pair_term_f32(const PairConstsF& K, const double
     }
     else if(KIND == kPairEV)
     {
-        const float radius = (float)ppar[i] + (float)ppar[j];
-        const float sigma_sq = radius * radius;
-        if(sigma_sq * K.cut2 < l_sq) {return false;}
+        const double radius_d = ppar[i] + ppar[j];
+        const double sigma_sq_d = radius_d * radius_d;
+        if(sigma_sq_d * pp.cutoff_ratio2 < l_sq_d) {return false;}
+        const float sigma_sq = (float)sigma_sq_d;
         const float rcp = RMath<float>::rcp(l_sq);
         const float s2l2 = sigma_sq * rcp;
         const float s6 = s2l2 * s2l2 * s2l2;
@@ -524,10 +532,10 @@ __device__ __forceinline__ bool pair_term_f32(const PairConstsF& K, const double
     }
     else
     {
+        if(pp.abs_cutoff * pp.abs_cutoff <= l_sq_d) {return false;}
         const float qq = K.qk * (float)ppar[i] * (float)ppar[j];
         const float rl = RMath<float>::rsqrt(l_sq);
         const float l = l_sq * rl;
-        if(K.acut <= l) {return false;}
         const float ex = RMath<float>::exp(-l * K.idebye);
         const float f_mag = -qq * (K.debye + l) * K.idebye * (rl * rl) * ex;
         c = f_mag * rl;
@@ -564,9 +572,9 @@ __device__ __forceinline__ void pair_row_mixed(const PairSeg& s, uint32_t t, uin
             if(k0 + m * s.tpp >= cnt) {break;}
             double ddx, ddy, ddz;
             adjust_direction(box, ri.x, ri.y, ri.z, rj[m].x, rj[m].y, rj[m].z, ddx, ddy, ddz);
-            const float dx = (float)ddx, dy = (float)ddy, dz = (float)ddz;
             float c;
-            if(!pair_term_f32<KIND, ENERGY>(K, s.ppar, i, j[m], dx, dy, dz, c, e)) {continue;}
+            if(!pair_term_f32<KIND, ENERGY>(K, s.pp, s.ppar, i, j[m], ddx, ddy, ddz, c, e)) {continue;}
+            const float dx = (float)ddx, dy = (float)ddy, dz = (float)ddz;
             long long fx, fy, fz;
             fix3r<float>(dx * c, dy * c, dz * c, fx, fy, fz);
             ax += fx; ay += fy; az += fz;

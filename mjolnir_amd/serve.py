@@ -118,7 +118,10 @@ def _make_spec(item) -> SystemSpec:
 def make_jobs(trace, procs: int = 8) -> List[Job]:
     """Build the jobs' systems (the requests' payloads) before the clock starts."""
     if procs > 1 and len(trace) > 1:
+        import gc
         import multiprocessing as mp
+        # forked children never call into the HIP runtime (see workloads.replica_ensemble)
+        gc.collect()
         with mp.get_context("fork").Pool(min(procs, len(trace))) as pool:
             specs = pool.map(_make_spec, trace, chunksize=1)
     else:

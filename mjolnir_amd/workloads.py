@@ -220,10 +220,16 @@ def replica_ensemble(n_replicas: int = 512, n_min: int = 50, n_max: int = 2000,
     temps = rng.uniform(T_min, T_max, size=n_replicas)
     seeds = rng.integers(1, 2 ** 31 - 1, size=n_replicas)
     jobs = [(int(n), int(s), float(t), max_bend_deg) for n, t, s in zip(lens, temps, seeds)]
-    workers = min(16, os.cpu_count() or 1)
+    # MJG_WORKLOAD_PROCS=1 builds them in this process (e.g. under a profiler that preloads itself)
+    workers = int(os.environ.get("MJG_WORKLOAD_PROCS", min(16, os.cpu_count() or 1)))
     if workers > 1 and n_replicas >= 32:
-        # every replica has its own seed, so the result does not depend on the pool
+        # every replica has its own seed, so the result does not depend on the pool.  The children
+        # are forked; a forked child of a process that holds engines or batches must never call
+        # into the HIP runtime: the handles skip their destroy call in any process but their
+        # creator's, and the parent's pending garbage is collected here first
+        import gc
         import multiprocessing as mp
+        gc.collect()
         with mp.get_context("fork").Pool(workers) as pool:
             return pool.map(_ensemble_member, jobs, chunksize=4)
     return [_ensemble_member(j) for j in jobs]

@@ -13,7 +13,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import force_tolerance, golden_case
+from conftest import base_pair_energy_paths, force_tolerance, golden_case, rms_force
 import make_golden
 
 pytestmark = pytest.mark.gpu
@@ -33,13 +33,16 @@ def test_forces_energies_pairs_vs_reference(name):
     e = _engine(spec, with_integrator=False)
     names = [str(x) for x in g["names"]]
     per, tot = e.energies()
+    f_scale = rms_force(g["total_force"])
+    ref_per = [float(g[f"energy_{k}"]) for k in range(len(names))]
+    bp = base_pair_energy_paths(names, ref_per, float(g["total_energy"]))
     for k in range(len(names)):
         ref_f = g[f"force_{k}"]
         f = e.force_of(k)
         err = np.linalg.norm(f - ref_f, axis=1)
-        assert np.all(err <= force_tolerance(ref_f, 1e-10)), (names[k], err.max())
-        ref_e = float(g[f"energy_{k}"])
-        assert abs(per[k] - ref_e) <= 1e-10 * max(abs(ref_e), 1.0), (names[k], per[k], ref_e)
+        assert np.all(err <= force_tolerance(ref_f, 1e-10, f_scale)), (names[k], err.max())
+        refs = bp.get(k, (ref_per[k],))
+        assert min(abs(per[k] - r) for r in refs) <= 1e-10 * max(abs(ref_per[k]), 1.0), (names[k], per[k], refs)
         if f"npairs_{k}" in g:
             p = e.pairs(k)
             assert len(p) == int(g[f"npairs_{k}"]), names[k]
@@ -315,9 +318,16 @@ def test_engine_mixed_precision_forces_vs_reference(name):
     e = _engine(spec, precision=capi.FP32_FIXED, with_integrator=False)
     e.calc_force()
     _, _, f = e.state()
-    rms, mx = _rms_rel(f, g["total_force"])
-    print(f"[gpu] {name} FP32_FIXED engine: RMS force error {rms:.2e}, max {mx:.2e} (relative to the RMS force)",
-          flush=True)
+    ref = g["total_force"]
+    # int64 fixed point at 2^-24 kcal/mol/A holds a force sum up to 2^57 units (8.6e9 kcal/mol/A);
+    # a particle whose force is beyond it (lj_boundary puts two particles 0.14 A apart: 2e19) comes
+    # back non-finite, the overflow report the integrator's finite check turns into NAN_DETECTED.
+    # The others are compared on their own RMS force.
+    huge = np.abs(ref).max(axis=1) > 2.0 ** 33
+    assert not np.isfinite(f[huge]).any()
+    rms, mx = _rms_rel(f[~huge], ref[~huge])
+    print(f"[gpu] {name} FP32_FIXED engine: RMS force error {rms:.2e}, max {mx:.2e} (relative to the RMS force); "
+          f"{int(huge.sum())} particles beyond the fixed-point range", flush=True)
     assert rms <= 1e-5
     names = [str(x) for x in g["names"]]
     for k in range(len(names)):

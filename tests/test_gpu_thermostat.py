@@ -24,9 +24,11 @@ def _run(spec, equilibrate=20000):
     print(f"[gpu] thermostat {spec.name} {spec.integrator.type} N={spec.n}, {STEPS} steps", flush=True)
     e = capi.Engine(spec)
     e.step(equilibrate)
-    for _ in range(STEPS // EVERY):
+    for k in range(STEPS // EVERY):
         e.step(EVERY)
         x, v, _ = e.state()
+        if (k + 1) % 100 == 0:
+            print(f"[gpu]   {(k + 1) * EVERY} steps", flush=True)
         yield x, v
 
 
