@@ -164,8 +164,13 @@ def bench_single(args, spec):
     # position read and 24 B force write per participant
     alg_bytes = 4 * hp + (8 + 24 + 24) * np_
     achieved = alg_bytes / (ms * 1e-3) / 1e9
-    kern = ("k_tsp_pairs" if g.potential in ("3SPN2BasePair", "3SPN2CrossStacking") else
-            ("k_pairs_mixed" if prec == capi.FP32_FIXED else "k_pairs"))
+    tiled = (spec.boundary == "periodic" and g.partition == "CellList" and np_ >= 16384 and
+             g.potential not in ("3SPN2BasePair", "3SPN2CrossStacking") and os.environ.get("MJG_NO_TILE") != "1")
+    if tiled:
+        kern = "k_pairs_tiled<mixed>" if prec == capi.FP32_FIXED else "k_pairs_tiled<fp64>"
+    else:
+        kern = ("k_tsp_pairs" if g.potential in ("3SPN2BasePair", "3SPN2CrossStacking") else
+                ("k_pairs_mixed" if prec == capi.FP32_FIXED else "k_pairs"))
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
                 "kernel": f"{kern} ({g.potential})", "avg_us": round(ms * 1e3, 3),

@@ -438,6 +438,20 @@ void Engine::alloc_list(GlobalIA& g, uint32_t cap)
     const uint32_t np = (uint32_t)g.part.size();
     g.cap = cap;
     g.d_nl = dalloc<uint32_t>((size_t)np * cap);
+    if(g.tiled) {g.d_nl16 = dalloc<uint16_t>((size_t)np * cap);}
+}
+
+// The cell-tile pair kernel (tile_kernels.hpp) serves a pair interaction on a periodic cell list
+// with at least kTileMinNp participants (MJG_TILE_MIN_NP overrides it; MJG_NO_TILE=1 turns the
+// kernel off): it needs a launch of one workgroup per cell, which fills the GPU only for large
+// grids.  Its staging capacity starts at twice the mean 27-cell neighbourhood and grows with the
+// largest one a build reports (ListCtl::maxnbr), up to what the LDS of one workgroup holds.
+constexpr uint32_t kTileMinNp = 16384;
+constexpr uint32_t kTileMaxCap = (uint32_t)((160 * 1024 - 1024) / kTileBytesPerParticle);
+static uint32_t tile_min_np()
+{
+    if(const char* v = std::getenv("MJG_TILE_MIN_NP")) {return (uint32_t)std::max(0L, std::atol(v));}
+    return kTileMinNp;
 }
 
 // lanes per participant: about six partners per lane, a power of two in [1, 64]
@@ -577,6 +591,22 @@ void Engine::setup_global(GlobalIA& g)
     {
         g.sb.keys = dalloc<uint32_t>(np); g.sb.vals = dalloc<uint32_t>(np);
     }
+    const char* no_tile = std::getenv("MJG_NO_TILE");
+    g.tiled = box_.periodic && g.partition == MJG_PART_CELLLIST && !g.multibody() && np >= tile_min_np() &&
+              !(no_tile && no_tile[0] == '1');
+    if(g.tiled)
+    {
+        const double mean_nbr = 27.0 * (double)np / (double)g.grid.ncell;
+        g.tile_cap = (uint32_t)std::min<double>(kTileMaxCap, std::max(256.0, 2.0 * mean_nbr + 64.0));
+        g.tile_tpp = 8;
+        // MJG_TILE_CAP fixes the staging capacity (no growth): a capacity below the largest
+        // neighbourhood serves every build from global memory
+        if(const char* v = std::getenv("MJG_TILE_CAP"))
+        {
+            g.tile_cap = (uint32_t)std::min<long>(kTileMaxCap, std::max(1L, std::atol(v)));
+            g.tile_fixed = true;
+        }
+    }
     g.d_nn = dalloc<uint32_t>(np);
     g.d_order = dalloc<uint32_t>(np);
     g.d_epair = dalloc<double>(np);
@@ -612,6 +642,7 @@ ListSegTable Engine::list_table()
         s.ex_start = g.d_ex_start;
         s.ex = g.d_ex;
         s.nl = g.d_nl;
+        s.nl16 = g.tiled ? g.d_nl16 : nullptr;
         s.nn = g.d_nn;
         s.order = g.d_order;
         s.ctl = d_lctl_ + gi;
@@ -630,7 +661,7 @@ PairSegTable Engine::pair_table(int only)
     {
         if(only >= 0 && only != nl + (int)gi) {continue;}
         GlobalIA& g = globals_[gi];
-        if(g.multibody()) {continue;}
+        if(g.multibody() || g.tiled) {continue;}
         PairSeg& s = t.seg[t.n++];
         s.kind = g.pot;
         s.tpp = g.tpp;
@@ -745,12 +776,23 @@ void Engine::build_lists_now(bool)
     download(ctl.data(), d_lctl_, globals_.size());
     for(auto& c : ctl) {c.rebuild = 0;}
     upload(d_lctl_, ctl.data(), globals_.size());
+    grow_tiles();
 }
 
 void Engine::initialize()
 {
     if(!state_set_) {throw std::invalid_argument("system state must be set before initialize");}
     if(initialized_) {throw std::invalid_argument("force field already initialized");}
+    // the cell-tile kernel stages up to kTileMaxCap particles in the LDS of one workgroup
+    const int tile_lds = (int)((size_t)kTileMaxCap * kTileBytesPerParticle);
+    check(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs_tiled<true, true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, tile_lds), "tile attribute");
+    check(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs_tiled<true, false>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, tile_lds), "tile attribute");
+    check(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs_tiled<false, true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, tile_lds), "tile attribute");
+    check(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs_tiled<false, false>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, tile_lds), "tile attribute");
     write_topology();
 
     // local tables: one segment per interaction, body slots in interaction/term/body order
@@ -870,7 +912,9 @@ void Engine::forces(double4* f, bool energy, int only, bool kick, bool advance)
     }
     for(size_t gi = 0; gi < globals_.size(); ++gi)
     {
-        if(globals_[gi].multibody() && (only < 0 || only == nl + (int)gi)) {launch_multibody(globals_[gi], energy);}
+        if(only >= 0 && only != nl + (int)gi) {continue;}
+        if(globals_[gi].multibody()) {launch_multibody(globals_[gi], energy);}
+        else if(globals_[gi].tiled) {launch_tiled(globals_[gi], (int)gi, energy);}
     }
     if(energy)
     {
@@ -921,6 +965,58 @@ void Engine::launch_multibody(const GlobalIA& g, bool energy)
     if(energy) {hipLaunchKernelGGL(k_tsp_pairs<true>, dim3(blocks), dim3(256), 0, stream_, d_pos_, box_, s);}
     else {hipLaunchKernelGGL(k_tsp_pairs<false>, dim3(blocks), dim3(256), 0, stream_, d_pos_, box_, s);}
     hipLaunchKernelGGL(k_fix_to_force, dim3(blocks_for(n_, 256)), dim3(256), 0, stream_, g.d_acc, g.d_fg, (uint32_t)n_);
+}
+
+// Cell-tile pair kernel of one interaction: one workgroup per cell, its LDS the staging of the
+// 27-cell neighbourhood.  FP64 runs the fused kernel's lanes per row (the same sums bit for bit);
+// the mixed kernel's fixed-point sums are exact for any lane split and use tile_tpp.
+void Engine::launch_tiled(GlobalIA& g, int gi, bool energy)
+{
+    TileSeg s{};
+    PairSeg& ps = s.ps;
+    ps.kind = g.pot; ps.tpp = g.tpp; ps.np = (uint32_t)g.part.size(); ps.block_begin = 0; ps.cap = g.cap;
+    ps.part = g.d_part; ps.order = g.d_order; ps.nl = g.d_nl; ps.nn = g.d_nn; ps.ppar = g.d_ppar;
+    ps.fout = g.d_fg; ps.eout = g.d_epair; ps.pp = g.pp;
+    s.nl16 = g.d_nl16; s.svals = g.sb.vals; s.cell_start = g.d_cell_start; s.cell_end = g.d_cell_end;
+    s.grid = g.d_grid; s.ctl = d_lctl_ + gi; s.cap_nbr = g.tile_cap;
+    const bool mixed = (precision_ == MJG_FP32_FIXED);
+    s.tpp = mixed ? g.tile_tpp : g.tpp;
+    const size_t lds = (size_t)g.tile_cap * kTileBytesPerParticle;
+    const dim3 grid(g.grid.ncell), block(kTileThreads);
+    if(mixed)
+    {
+        if(energy) {hipLaunchKernelGGL((k_pairs_tiled<true, true>), grid, block, lds, stream_, d_pos_, box_, s);}
+        else {hipLaunchKernelGGL((k_pairs_tiled<true, false>), grid, block, lds, stream_, d_pos_, box_, s);}
+    }
+    else
+    {
+        if(energy) {hipLaunchKernelGGL((k_pairs_tiled<false, true>), grid, block, lds, stream_, d_pos_, box_, s);}
+        else {hipLaunchKernelGGL((k_pairs_tiled<false, false>), grid, block, lds, stream_, d_pos_, box_, s);}
+    }
+}
+
+// After lists were built: a tiled interaction whose largest neighbourhood outgrew its staging
+// capacity gets a larger one (its launches served that build from global memory meanwhile).
+// True if a capacity changed, which invalidates a captured step graph.
+bool Engine::grow_tiles()
+{
+    bool any = false;
+    std::vector<ListCtl> ctl(globals_.size());
+    bool have = false;
+    for(size_t gi = 0; gi < globals_.size(); ++gi)
+    {
+        GlobalIA& g = globals_[gi];
+        if(!g.tiled) {continue;}
+        if(!have) {download(ctl.data(), d_lctl_, globals_.size()); have = true;}
+        const uint32_t need = ctl[gi].maxnbr;
+        if(need > g.tile_cap && g.tile_cap < kTileMaxCap && !g.tile_fixed)
+        {
+            g.tile_cap = std::min<uint32_t>(kTileMaxCap, need + need / 4 + 32);
+            any = true;
+        }
+    }
+    if(any && graph_exec_) {(void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr;}
+    return any;
 }
 
 void Engine::calc_force()
@@ -1287,6 +1383,7 @@ void Engine::run_segment(size_t n)
         if(!overflow)
         {
             if(sc.nonfinite) {throw NonFinite("non-finite particle position");}
+            grow_tiles();
             return;
         }
         // a partner list did not fit: restore the segment's start, force a rebuild with the
@@ -1495,6 +1592,7 @@ double Engine::profile_interaction(int id, int reps, uint64_t* half_pairs)
                                d_lidx_, d_lpar_, d_tf_, d_te_, id);
         }
         else if(globals_[id - nl].multibody()) {launch_multibody(globals_[id - nl], false);}
+        else if(globals_[id - nl].tiled) {launch_tiled(globals_[id - nl], id - nl, false);}
         else if(precision_ == MJG_FP32_FIXED)
         {
             hipLaunchKernelGGL(k_pairs_mixed<false>, dim3(pblocks), dim3(256), 0, stream_, d_pos_, box_, pt);

@@ -8,6 +8,7 @@
 #include "constraint_kernels.hpp"
 #include "nlist_kernels.hpp"
 #include "sort_kernels.hpp"
+#include "tile_kernels.hpp"
 
 #include <map>
 #include <stdexcept>
@@ -67,6 +68,12 @@ struct GlobalIA
     double* d_epair = nullptr;
     double4* d_fg = nullptr;      // this interaction's force on every particle
     unsigned long long* d_acc = nullptr;   // 3SPN2 base pairing / cross stacking: int64 fixed-point sums
+    // cell-tile pair kernel (periodic cell lists): staging slot of every list entry, the staging
+    // capacity in particles and the kernel's lanes per row
+    bool tiled = false;
+    uint16_t* d_nl16 = nullptr;
+    uint32_t tile_cap = 0, tile_tpp = 8;
+    bool tile_fixed = false;
     bool multibody() const {return pot == kBasePair || pot == kCrossStack;}
 };
 
@@ -137,6 +144,8 @@ class Engine
     ListSegTable list_table();
     PairSegTable pair_table(int only);
     void launch_multibody(const GlobalIA& g, bool energy);
+    void launch_tiled(GlobalIA& g, int gi, bool energy);
+    bool grow_tiles();
     void forces(double4* f, bool energy, int only, bool kick, bool advance = false);
     void launch_ul_accel(bool kick);
     void launch_step();

@@ -284,19 +284,31 @@ struct PairParams
     int npar;                // per-particle parameter columns
 };
 
-// Pair force on the owner i from j (full list).  d = adjust_direction(r_i, r_j).
-template<int KIND, bool ENERGY>
-__device__ __forceinline__ void pair_term(const PairParams& pp, uint32_t i, uint32_t j, const double* __restrict__ ppar,
-                                          double dx, double dy, double dz,
-                                          double& fx, double& fy, double& fz, double& e)
+// Per-particle pair parameters of a pair term, read through an accessor: column c of the
+// parameter row `key`.  GlobalPar reads the engine's per-particle table (key = particle index);
+// the cell-tile kernel reads the copies it staged in LDS (key = staging slot).
+template<int NPAR> struct GlobalPar
+{
+    const double* __restrict__ p;
+    __device__ __forceinline__ double get(uint32_t key, int c) const {return p[(size_t)NPAR * key + c];}
+};
+template<int KIND> constexpr int pair_npar() {return KIND == kPairLJ ? 2 : 1;}
+
+// Pair force on the owner i from j (full list).  d = adjust_direction(r_i, r_j).  i and j are
+// particle indices (the energy of a pair is counted from its lower particle, and the
+// Debye-Hueckel charge product is formed lower first); ki and kj are their parameter keys.
+template<int KIND, bool ENERGY, class Par>
+__device__ __forceinline__ void pair_term_p(const PairParams& pp, uint32_t i, uint32_t j, const Par& par, uint32_t ki,
+                                            uint32_t kj, double dx, double dy, double dz,
+                                            double& fx, double& fy, double& fz, double& e)
 {
     const double l_sq = length_sq(dx, dy, dz);
     if(KIND == kPairLJ)
     {
         // GlobalPairLennardJonesInteraction.hpp:157-188 with LorentzBerthelotRule
         // (ParameterList.hpp:240-252)
-        const double s1 = ppar[2 * i], e1 = ppar[2 * i + 1];
-        const double s2 = ppar[2 * j], e2 = ppar[2 * j + 1];
+        const double s1 = par.get(ki, 0), e1 = par.get(ki, 1);
+        const double s2 = par.get(kj, 0), e2 = par.get(kj, 1);
         const double sigma = (s1 + s2) / 2;
         const double eps = (e1 == e2) ? e1 : sqrt(e1 * e2);
         const double sigma_sq = sigma * sigma;
@@ -312,7 +324,7 @@ __device__ __forceinline__ void pair_term(const PairParams& pp, uint32_t i, uint
     {
         // GlobalPairExcludedVolumeInteraction.hpp:159-193; radius sum
         // (ExcludedVolumePotential.hpp:192-195)
-        const double radius = ppar[i] + ppar[j];
+        const double radius = par.get(ki, 0) + par.get(kj, 0);
         const double sigma_sq = radius * radius;
         if(sigma_sq * pp.cutoff_ratio2 < l_sq) {return;}
         const double rcp = 1.0 / l_sq;
@@ -326,7 +338,7 @@ __device__ __forceinline__ void pair_term(const PairParams& pp, uint32_t i, uint
     {
         // GlobalPairInteraction.hpp:148-189 with ThreeSPN2ExcludedVolumePotential.hpp:47-64;
         // sigma of the pair (ThreeSPN2ExcludedVolumeParameterList::prepare_params, :198-202)
-        const double sigma = (tsp::ev_sigma((int)ppar[i]) + tsp::ev_sigma((int)ppar[j])) * 0.5;
+        const double sigma = (tsp::ev_sigma((int)par.get(ki, 0)) + tsp::ev_sigma((int)par.get(kj, 0))) * 0.5;
         const double rl = 1.0 / sqrt(l_sq);
         const double l = l_sq * rl;
         if(sigma <= l) {return;}
@@ -350,8 +362,8 @@ __device__ __forceinline__ void pair_term(const PairParams& pp, uint32_t i, uint
     {
         // GlobalPairInteraction.hpp:164-185 with DebyeHuckelPotential.hpp:47-65; charge
         // product cached as (1/(4 pi eps0 epsk) q_lo) q_hi (DebyeHuckelPotential.hpp:249-254)
-        const uint32_t lo = (i < j) ? i : j, hi = (i < j) ? j : i;
-        const double qq = pp.inv4pieps * ppar[lo] * ppar[hi];
+        const bool io = i < j;
+        const double qq = pp.inv4pieps * par.get(io ? ki : kj, 0) * par.get(io ? kj : ki, 0);
         const double rl = 1.0 / sqrt(l_sq);
         const double l = l_sq * rl;
         if(pp.abs_cutoff <= l) {return;}
@@ -361,6 +373,14 @@ __device__ __forceinline__ void pair_term(const PairParams& pp, uint32_t i, uint
         fx += dx * c; fy += dy * c; fz += dz * c;
         if(ENERGY && i < j) {e += qq * (ex / l - pp.coef_at_cutoff);}
     }
+}
+
+template<int KIND, bool ENERGY>
+__device__ __forceinline__ void pair_term(const PairParams& pp, uint32_t i, uint32_t j, const double* __restrict__ ppar,
+                                          double dx, double dy, double dz,
+                                          double& fx, double& fy, double& fz, double& e)
+{
+    pair_term_p<KIND, ENERGY>(pp, i, j, GlobalPar<pair_npar<KIND>()>{ppar}, i, j, dx, dy, dz, fx, fy, fz, e);
 }
 
 // ------------------------------------------------------------------------------------
@@ -484,17 +504,17 @@ __device__ __forceinline__ PairConstsF pair_consts_f(const PairParams& pp)
 // them (the Debye-Hueckel one on r^2 against the squared cutoff), so the set of pairs inside the
 // cutoff is the FP64 path's: a pair at the cutoff, where the LJ force is still 24 eps s^6 / r_c,
 // never flips in or out by FP32 rounding.
-template<int KIND, bool ENERGY>
-__device__ __forceinline__ bool pair_term_f32(const PairConstsF& K, const PairParams& pp, const double* __restrict__ ppar,
-                                              uint32_t i, uint32_t j, double ddx, double ddy, double ddz, float& c,
-                                              double& e)
+template<int KIND, bool ENERGY, class Par>
+__device__ __forceinline__ bool pair_term_f32(const PairConstsF& K, const PairParams& pp, const Par& par, uint32_t i,
+                                              uint32_t j, uint32_t ki, uint32_t kj, double ddx, double ddy, double ddz,
+                                              float& c, double& e)
 {
     const double l_sq_d = length_sq(ddx, ddy, ddz);
     const float l_sq = (float)l_sq_d;
     if(KIND == kPairLJ)
     {
-        const double s1 = ppar[2 * i], s2 = ppar[2 * j];
-        const float e1 = (float)ppar[2 * i + 1], e2 = (float)ppar[2 * j + 1];
+        const double s1 = par.get(ki, 0), s2 = par.get(kj, 0);
+        const float e1 = (float)par.get(ki, 1), e2 = (float)par.get(kj, 1);
         const double sigma_d = (s1 + s2) / 2;
         const double sigma_sq_d = sigma_d * sigma_d;
         if(sigma_sq_d * pp.cutoff_ratio2 < l_sq_d) {return false;}
@@ -508,7 +528,7 @@ __device__ __forceinline__ bool pair_term_f32(const PairConstsF& K, const PairPa
     }
     else if(KIND == kPairEV)
     {
-        const double radius_d = ppar[i] + ppar[j];
+        const double radius_d = par.get(ki, 0) + par.get(kj, 0);
         const double sigma_sq_d = radius_d * radius_d;
         if(sigma_sq_d * pp.cutoff_ratio2 < l_sq_d) {return false;}
         const float sigma_sq = (float)sigma_sq_d;
@@ -520,7 +540,7 @@ __device__ __forceinline__ bool pair_term_f32(const PairConstsF& K, const PairPa
     }
     else if(KIND == kPair3SPN2EV)
     {
-        const float sigma = (float)((tsp::ev_sigma((int)ppar[i]) + tsp::ev_sigma((int)ppar[j])) * 0.5);
+        const float sigma = (float)((tsp::ev_sigma((int)par.get(ki, 0)) + tsp::ev_sigma((int)par.get(kj, 0))) * 0.5);
         const float rl = RMath<float>::rsqrt(l_sq);
         const float l = l_sq * rl;
         if(sigma <= l) {return false;}
@@ -533,7 +553,7 @@ __device__ __forceinline__ bool pair_term_f32(const PairConstsF& K, const PairPa
     else
     {
         if(pp.abs_cutoff * pp.abs_cutoff <= l_sq_d) {return false;}
-        const float qq = K.qk * (float)ppar[i] * (float)ppar[j];
+        const float qq = K.qk * (float)par.get(ki, 0) * (float)par.get(kj, 0);
         const float rl = RMath<float>::rsqrt(l_sq);
         const float l = l_sq * rl;
         const float ex = RMath<float>::exp(-l * K.idebye);
@@ -573,7 +593,11 @@ __device__ __forceinline__ void pair_row_mixed(const PairSeg& s, uint32_t t, uin
             double ddx, ddy, ddz;
             adjust_direction(box, ri.x, ri.y, ri.z, rj[m].x, rj[m].y, rj[m].z, ddx, ddy, ddz);
             float c;
-            if(!pair_term_f32<KIND, ENERGY>(K, s.pp, s.ppar, i, j[m], ddx, ddy, ddz, c, e)) {continue;}
+            if(!pair_term_f32<KIND, ENERGY>(K, s.pp, GlobalPar<pair_npar<KIND>()>{s.ppar}, i, j[m], i, j[m], ddx, ddy,
+                                            ddz, c, e))
+            {
+                continue;
+            }
             const float dx = (float)ddx, dy = (float)ddy, dz = (float)ddz;
             long long fx, fy, fz;
             fix3r<float>(dx * c, dy * c, dz * c, fx, fy, fz);

@@ -49,6 +49,7 @@ struct ListCtl
 {
     int rebuild;                   // 1: rebuild this step
     unsigned int maxcount;         // largest partner count seen at the last build
+    unsigned int maxnbr;           // largest 27-cell neighbourhood of a row at the last build
     unsigned long long bbox_min[3];
     unsigned long long bbox_max[3];
     double margin;                 // current_margin of the partition
@@ -235,6 +236,7 @@ struct ListSeg
     const uint32_t* ex_start;
     const uint32_t* ex;
     uint32_t* nl;
+    uint16_t* nl16;                    // non-null: also the staging slot of each entry (k_pairs_tiled)
     uint32_t* nn;
     uint32_t* order;
     ListCtl* ctl;
@@ -276,8 +278,10 @@ k_verlet(const double4* __restrict__ pos, Box box, ListSegTable tab)
     const double4 ri = pos[i];
     uint32_t count = 0;
     uint32_t* row = s.nl + (size_t)t * s.cap;
+    uint16_t* row16 = s.nl16 ? s.nl16 + (size_t)t * s.cap : nullptr;
+    uint32_t nbr = 0;   // staging slot of the next candidate cell's first particle (27-cell order)
 
-    auto consider = [&](uint32_t j, bool valid) {
+    auto consider = [&](uint32_t j, bool valid, uint32_t slot16) {
         bool accept = false;
         if(valid && j != i)
         {
@@ -301,7 +305,11 @@ k_verlet(const double4* __restrict__ pos, Box box, ListSegTable tab)
         if(accept)
         {
             const uint32_t k = count + (uint32_t)__popcll(b & below);
-            if(k < s.cap) {row[k] = j;}
+            if(k < s.cap)
+            {
+                row[k] = j;
+                if(row16) {row16[k] = (uint16_t)slot16;}
+            }
         }
         count += (uint32_t)__popcll(b);
     };
@@ -311,7 +319,7 @@ k_verlet(const double4* __restrict__ pos, Box box, ListSegTable tab)
         for(uint32_t base = 0; base < s.np; base += s.tpp)
         {
             const uint32_t q = base + l;
-            consider((q < s.np) ? s.part[q] : 0u, q < s.np);
+            consider((q < s.np) ? s.part[q] : 0u, q < s.np, 0u);
         }
     }
     else
@@ -341,8 +349,9 @@ k_verlet(const double4* __restrict__ pos, Box box, ListSegTable tab)
             for(uint32_t base = sb; base < se; base += s.tpp)
             {
                 const uint32_t q = base + l;
-                consider((q < se) ? s.part[s.svals[q]] : 0u, q < se);
+                consider((q < se) ? s.part[s.svals[q]] : 0u, q < se, nbr + (q - sb));
             }
+            nbr += (se > sb) ? se - sb : 0u;
         }
     }
     (void)ri;
@@ -350,6 +359,7 @@ k_verlet(const double4* __restrict__ pos, Box box, ListSegTable tab)
     {
         s.nn[t] = (count < s.cap) ? count : s.cap;
         atomicMax(&s.ctl->maxcount, count);
+        if(row16) {atomicMax(&s.ctl->maxnbr, nbr);}
     }
 }
 
@@ -391,13 +401,13 @@ __device__ __forceinline__ int margin_decision(ListCtl& c, double dmargin, doubl
     if(never && !force) {c.rebuild = 0; return 0;}
     if(force)
     {
-        c.rebuild = 1; c.margin = never ? INFINITY : abs_margin; c.maxcount = 0;
+        c.rebuild = 1; c.margin = never ? INFINITY : abs_margin; c.maxcount = 0; c.maxnbr = 0;
         return 1;
     }
     const double m = c.margin - dmargin;
     if(m < 0)
     {
-        c.rebuild = 1; c.margin = abs_margin; c.maxcount = 0;
+        c.rebuild = 1; c.margin = abs_margin; c.maxcount = 0; c.maxnbr = 0;
         for(int d = 0; d < 3; ++d) {c.bbox_min[d] = ~0ull; c.bbox_max[d] = 0ull;}
         return 1;
     }

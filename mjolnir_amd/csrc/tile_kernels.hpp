@@ -1,0 +1,253 @@
+// Cell-tile pair kernel of the engine (periodic cell lists).
+//
+// One workgroup per cell of the interaction's grid.  It stages the positions, per-particle
+// parameters and particle indices of the 27 surrounding cells in LDS, in the order the list
+// build walks them (z outer, then y, then x; each cell's sorted slots in order), and walks the
+// rows of its own cell's participants.  A row entry's staging slot is written by k_verlet beside
+// its particle index (nl16), so a partner is read from LDS instead of gathered from global memory:
+// the rows of one cell share their 27-cell neighbourhood, which the full-list kernels fetch again
+// for every row entry (about 130 scattered 32-byte loads per particle at r_l = 15 A in the
+// Lennard-Jones fluid).
+//
+// The lanes of a row and their sum order are those of k_pairs / k_pairs_mixed when tpp is theirs
+// (entries k = l, l + tpp, ... per lane, then the same butterfly): the FP64 result is theirs bit
+// for bit; the mixed result is an exact fixed-point sum, the same for any lane split.  A build
+// whose largest neighbourhood exceeded the staging capacity (ListCtl::maxnbr > cap_nbr) is served
+// by those kernels' global gathers instead, inside this kernel, until the host grows the capacity.
+#pragma once
+#include "force_kernels.hpp"
+#include "nlist_kernels.hpp"
+
+namespace mjg
+{
+struct TileSeg
+{
+    PairSeg ps;                   // rows, lists, parameters and outputs as k_pairs reads them
+    const uint16_t* nl16;         // staging slot of each row entry (row-major like nl)
+    const uint32_t* svals;        // sorted slot -> participant
+    const uint32_t* cell_start;
+    const uint32_t* cell_end;
+    const Grid* grid;
+    const ListCtl* ctl;
+    uint32_t cap_nbr;             // staging capacity in particles
+    uint32_t tpp;                 // lanes per row of this kernel
+};
+constexpr uint32_t kTileThreads = 256;
+
+// staged per-particle parameters (column 0 and 1)
+struct StagePar
+{
+    const double* s0;
+    const double* s1;
+    __device__ __forceinline__ double get(uint32_t k, int c) const {return c ? s1[k] : s0[k];}
+};
+
+// LDS bytes per staged particle: x, y, z, two parameter columns, particle index
+constexpr size_t kTileBytesPerParticle = 5 * sizeof(double) + sizeof(uint32_t);
+
+template<bool MIXED, bool ENERGY, int KIND>
+__device__ __forceinline__ void tile_rows(const TileSeg& s, const Box& box, uint32_t cs, uint32_t nhome, uint32_t home0,
+                                          const double* sx, const double* sy, const double* sz, const StagePar& par,
+                                          const uint32_t* sid)
+{
+    const PairSeg& ps = s.ps;
+    const uint32_t tpp = s.tpp;
+    const uint32_t tid = threadIdx.x;
+    const PairConstsF K = pair_consts_f(ps.pp);
+    for(uint32_t u = tid; u < nhome * tpp; u += kTileThreads)
+    {
+        const uint32_t h = u / tpp, l = u % tpp;
+        const uint32_t t = cs + h, ki = home0 + h;
+        const uint32_t i = sid[ki];
+        const double xi = sx[ki], yi = sy[ki], zi = sz[ki];
+        const uint32_t cnt = ps.nn[t];
+        const uint16_t* row = s.nl16 + (size_t)t * ps.cap;
+        double fx = 0, fy = 0, fz = 0, e = 0;
+        long long ax = 0, ay = 0, az = 0;
+        for(uint32_t k = l; k < cnt; k += tpp)
+        {
+            const uint32_t kj = row[k];
+            const uint32_t j = sid[kj];
+            double dx, dy, dz;
+            adjust_direction(box, xi, yi, zi, sx[kj], sy[kj], sz[kj], dx, dy, dz);
+            if constexpr(MIXED)
+            {
+                float c;
+                if(!pair_term_f32<KIND, ENERGY>(K, ps.pp, par, i, j, ki, kj, dx, dy, dz, c, e)) {continue;}
+                long long gx, gy, gz;
+                fix3r<float>((float)dx * c, (float)dy * c, (float)dz * c, gx, gy, gz);
+                ax += gx; ay += gy; az += gz;
+            }
+            else
+            {
+                pair_term_p<KIND, ENERGY>(ps.pp, i, j, par, ki, kj, dx, dy, dz, fx, fy, fz, e);
+            }
+        }
+        for(uint32_t o = tpp / 2; o > 0; o >>= 1)
+        {
+            if constexpr(MIXED)
+            {
+                ax += shfl_xor_ll(ax, (int)o);
+                ay += shfl_xor_ll(ay, (int)o);
+                az += shfl_xor_ll(az, (int)o);
+            }
+            else
+            {
+                fx += __shfl_xor(fx, o, kWave);
+                fy += __shfl_xor(fy, o, kWave);
+                fz += __shfl_xor(fz, o, kWave);
+            }
+            if(ENERGY) {e += __shfl_xor(e, o, kWave);}
+        }
+        if(l == 0)
+        {
+            if constexpr(MIXED)
+            {
+                const bool bad = fix_insane((unsigned long long)ax) || fix_insane((unsigned long long)ay) ||
+                                 fix_insane((unsigned long long)az);
+                const double nan = bad ? __builtin_nan("") : 0.0;
+                ps.fout[i] = make_double4(from_fix<float>((unsigned long long)ax) + nan,
+                                          from_fix<float>((unsigned long long)ay) + nan,
+                                          from_fix<float>((unsigned long long)az) + nan, 0.0);
+            }
+            else
+            {
+                ps.fout[i] = make_double4(fx, fy, fz, 0.0);
+            }
+            if(ENERGY) {ps.eout[t] = e;}
+        }
+    }
+}
+
+// The rows of one cell with the full-list kernels' global gathers (a neighbourhood that does not
+// fit the staging capacity).
+template<bool MIXED, bool ENERGY, int KIND>
+__device__ __forceinline__ void tile_rows_global(const TileSeg& s, const double4* __restrict__ pos, const Box& box,
+                                                 uint32_t cs, uint32_t nhome)
+{
+    const PairSeg& ps = s.ps;
+    const uint32_t tpp = ps.tpp;
+    for(uint32_t u = threadIdx.x; u < nhome * tpp; u += kTileThreads)
+    {
+        const uint32_t t = cs + u / tpp, l = u % tpp;
+        uint32_t i = 0;
+        double e = 0;
+        if constexpr(MIXED)
+        {
+            long long ax = 0, ay = 0, az = 0;
+            pair_row_mixed<KIND, ENERGY>(ps, t, l, pos, box, ax, ay, az, e, i);
+            for(uint32_t o = tpp / 2; o > 0; o >>= 1)
+            {
+                ax += shfl_xor_ll(ax, (int)o);
+                ay += shfl_xor_ll(ay, (int)o);
+                az += shfl_xor_ll(az, (int)o);
+                if(ENERGY) {e += __shfl_xor(e, o, kWave);}
+            }
+            if(l == 0)
+            {
+                const bool bad = fix_insane((unsigned long long)ax) || fix_insane((unsigned long long)ay) ||
+                                 fix_insane((unsigned long long)az);
+                const double nan = bad ? __builtin_nan("") : 0.0;
+                ps.fout[i] = make_double4(from_fix<float>((unsigned long long)ax) + nan,
+                                          from_fix<float>((unsigned long long)ay) + nan,
+                                          from_fix<float>((unsigned long long)az) + nan, 0.0);
+                if(ENERGY) {ps.eout[t] = e;}
+            }
+        }
+        else
+        {
+            double fx = 0, fy = 0, fz = 0;
+            pair_row<KIND, ENERGY>(ps, t, l, pos, box, fx, fy, fz, e, i);
+            for(uint32_t o = tpp / 2; o > 0; o >>= 1)
+            {
+                fx += __shfl_xor(fx, o, kWave);
+                fy += __shfl_xor(fy, o, kWave);
+                fz += __shfl_xor(fz, o, kWave);
+                if(ENERGY) {e += __shfl_xor(e, o, kWave);}
+            }
+            if(l == 0)
+            {
+                ps.fout[i] = make_double4(fx, fy, fz, 0.0);
+                if(ENERGY) {ps.eout[t] = e;}
+            }
+        }
+    }
+}
+
+template<bool MIXED, bool ENERGY, int KIND>
+__device__ __forceinline__ void tile_cell(const TileSeg& s, const double4* __restrict__ pos, const Box& box,
+                                          unsigned char* smem)
+{
+    __shared__ uint32_t c_beg[27], c_off[28];
+    const uint32_t cell = blockIdx.x;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t cs = s.cell_start[cell], ce = s.cell_end[cell];
+    if(ce <= cs) {return;}                                   // empty cell: no rows (uniform)
+    const uint32_t nhome = ce - cs;
+    if(s.ctl->maxnbr > s.cap_nbr)                            // uniform: one word for the launch
+    {
+        tile_rows_global<MIXED, ENERGY, KIND>(s, pos, box, cs, nhome);
+        return;
+    }
+    const Grid g = *s.grid;
+    if(tid < 27)
+    {
+        // neighbour cell m = 9 (dz + 1) + 3 (dy + 1) + (dx + 1), wrapped as k_verlet wraps it
+        const int dz = (int)(tid / 9) - 1, dy = (int)((tid / 3) % 3) - 1, dx = (int)(tid % 3) - 1;
+        const int cx = (int)(cell % (uint32_t)g.dim[0]);
+        const int cy = (int)((cell / (uint32_t)g.dim[0]) % (uint32_t)g.dim[1]);
+        const int cz = (int)(cell / ((uint32_t)g.dim[0] * (uint32_t)g.dim[1]));
+        int x = cx + dx, y = cy + dy, z = cz + dz;
+        x = (x < 0) ? g.dim[0] - 1 : (x >= g.dim[0] ? 0 : x);
+        y = (y < 0) ? g.dim[1] - 1 : (y >= g.dim[1] ? 0 : y);
+        z = (z < 0) ? g.dim[2] - 1 : (z >= g.dim[2] ? 0 : z);
+        const uint32_t c2 = (uint32_t)x + (uint32_t)g.dim[0] * ((uint32_t)y + (uint32_t)g.dim[1] * (uint32_t)z);
+        const uint32_t b = s.cell_start[c2], en = s.cell_end[c2];
+        c_beg[tid] = b;
+        c_off[tid + 1] = (en > b) ? en - b : 0u;
+    }
+    __syncthreads();
+    if(tid == 0)
+    {
+        c_off[0] = 0;
+        for(int m = 0; m < 27; ++m) {c_off[m + 1] += c_off[m];}
+    }
+    __syncthreads();
+    const uint32_t cap = s.cap_nbr;
+    double* sx = reinterpret_cast<double*>(smem);
+    double* sy = sx + cap;
+    double* sz = sy + cap;
+    double* s0 = sz + cap;
+    double* s1 = s0 + cap;
+    uint32_t* sid = reinterpret_cast<uint32_t*>(s1 + cap);
+    const int npar = s.ps.pp.npar;
+    for(int m = 0; m < 27; ++m)
+    {
+        const uint32_t b = c_beg[m], o = c_off[m], c = c_off[m + 1] - o;
+        for(uint32_t q = tid; q < c; q += kTileThreads)
+        {
+            const uint32_t i = s.ps.part[s.svals[b + q]];
+            const double4 r = pos[i];
+            sx[o + q] = r.x; sy[o + q] = r.y; sz[o + q] = r.z;
+            s0[o + q] = s.ps.ppar[(size_t)i * npar];
+            s1[o + q] = (npar > 1) ? s.ps.ppar[(size_t)i * npar + 1] : 0.0;
+            sid[o + q] = i;
+        }
+    }
+    __syncthreads();
+    tile_rows<MIXED, ENERGY, KIND>(s, box, cs, nhome, c_off[13], sx, sy, sz, StagePar{s0, s1}, sid);
+}
+
+template<bool MIXED, bool ENERGY>
+__global__ void __launch_bounds__(kTileThreads)
+k_pairs_tiled(const double4* __restrict__ pos, Box box, TileSeg s)
+{
+    extern __shared__ __align__(16) unsigned char tile_smem[];
+    const int kind = s.ps.kind;
+    if(kind == kPairLJ) {tile_cell<MIXED, ENERGY, kPairLJ>(s, pos, box, tile_smem);}
+    else if(kind == kPairEV) {tile_cell<MIXED, ENERGY, kPairEV>(s, pos, box, tile_smem);}
+    else if(kind == kPair3SPN2EV) {tile_cell<MIXED, ENERGY, kPair3SPN2EV>(s, pos, box, tile_smem);}
+    else {tile_cell<MIXED, ENERGY, kPairDH>(s, pos, box, tile_smem);}
+}
+
+} // mjg

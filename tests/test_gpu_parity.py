@@ -357,3 +357,71 @@ def test_engine_mixed_precision_lj65536_vs_oracle():
     assert np.isfinite(x).all()
     k32, k64 = e.kinetic_energy(), e64.kinetic_energy()
     assert abs(k32 - k64) <= 0.01 * k64, (k32, k64)
+
+
+def _tile_engine(spec, precision, monkeypatch, **env):
+    for k in ("MJG_NO_TILE", "MJG_TILE_MIN_NP", "MJG_TILE_CAP"):
+        monkeypatch.delenv(k, raising=False)
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    e = _engine(spec, precision=precision)
+    for k in env:
+        monkeypatch.delenv(k)
+    return e
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_cell_tile_kernel_equals_the_gather_kernel(precision, monkeypatch):
+    """The cell-tile pair kernel (periodic cell lists of >= 16384 participants: one workgroup per
+    cell, the 27-cell neighbourhood staged in LDS) against the full-list gather kernel on a 32,768-
+    particle LJ fluid: forces bit for bit (FP64: the same lanes per row and sum order; mixed: exact
+    fixed-point sums), energies to 1e-12 (mixed: the lanes split the FP64 energy sum differently),
+    and 300 Langevin steps through list rebuilds bit for bit; the same with a staging capacity too
+    small for any neighbourhood, which serves every build from global memory inside the kernel."""
+    from mjolnir_amd import capi, workloads
+    prec = capi.FP64 if precision == "fp64" else capi.FP32_FIXED
+    spec = make_golden.perturb(workloads.lj_fluid(n_side=32, seed=41), 0.1, 42)
+    ref = _tile_engine(spec, prec, monkeypatch, MJG_NO_TILE="1")
+    for env in ({}, {"MJG_TILE_CAP": "16"}):
+        e = _tile_engine(spec, prec, monkeypatch, **env)
+        print(f"[gpu] cell-tile kernel {precision} {env}", flush=True)
+        e.calc_force()
+        ref.calc_force()
+        assert np.array_equal(e.state()[2], ref.state()[2])
+        pe, te = e.energies()
+        pr, tr = ref.energies()
+        assert abs(te - tr) <= 1e-12 * abs(tr)
+        e.close()
+    a = _tile_engine(spec, prec, monkeypatch)
+    a.step(300)
+    ref.step(300)
+    xa, va, fa = a.state()
+    xr, vr, fr = ref.state()
+    assert a.counters()[1] > 1
+    assert np.array_equal(xa, xr) and np.array_equal(va, vr) and np.array_equal(fa, fr)
+
+
+@pytest.mark.parametrize("name", [n for n in CASES if n.startswith("lj")])
+def test_cell_tile_kernel_vs_reference(name, monkeypatch):
+    """The cell-tile kernel switched on for the small periodic golden cases (MJG_TILE_MIN_NP=0):
+    per-particle forces at 1e-10 and energies at 1e-10 against the reference, in FP64; the mixed
+    path at the mixed bar (RMS <= 1e-5 of the RMS force, over particles within the fixed-point
+    range)."""
+    from mjolnir_amd import capi
+    spec, g = golden_case(name)
+    if spec.boundary != "periodic":
+        pytest.skip("the cell-tile kernel serves periodic cell lists")
+    e = _tile_engine(spec, capi.FP64, monkeypatch, MJG_TILE_MIN_NP="0")
+    e.calc_force()
+    ref = g["total_force"]
+    err = np.linalg.norm(e.state()[2] - ref, axis=1)
+    assert np.all(err <= force_tolerance(ref, 1e-10)), err.max()
+    _, tot = e.energies()
+    assert abs(tot - float(g["total_energy"])) <= 1e-10 * max(abs(float(g["total_energy"])), 1.0)
+    m = _tile_engine(spec, capi.FP32_FIXED, monkeypatch, MJG_TILE_MIN_NP="0")
+    m.calc_force()
+    f = m.state()[2]
+    ok = np.abs(ref).max(axis=1) <= 2.0 ** 33
+    rms, mx = _rms_rel(f[ok], ref[ok])
+    print(f"[gpu] {name} cell-tile FP32_FIXED: RMS {rms:.2e}, max {mx:.2e}", flush=True)
+    assert rms <= 1e-5
