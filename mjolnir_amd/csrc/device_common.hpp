@@ -68,9 +68,12 @@ __device__ __forceinline__ void adjust_position(const Box& b, double& x, double&
     }
 }
 
-// math::length_sq (math/Vector.hpp:85-88): x*x + y*y + z*z, left to right.
+// math::length_sq (math/Vector.hpp:85-88): x*x + y*y + z*z, left to right, never fused into
+// multiply-adds (the units built with contraction on keep the reference's rounding here: the
+// cutoff and list decisions taken on it are the FP64 path's).
 __device__ __forceinline__ double length_sq(double x, double y, double z)
 {
+#pragma clang fp contract(off)
     return x * x + y * y + z * z;
 }
 

@@ -597,8 +597,7 @@ void Engine::setup_global(GlobalIA& g)
     if(g.tiled)
     {
         const double mean_nbr = 27.0 * (double)np / (double)g.grid.ncell;
-        g.tile_cap = (uint32_t)std::min<double>(kTileMaxCap, std::max(256.0, 2.0 * mean_nbr + 64.0));
-        g.tile_tpp = 8;
+        g.tile_cap = (uint32_t)std::min<double>(kTileMaxCap, std::max(256.0, 1.25 * mean_nbr + 64.0));
         // MJG_TILE_CAP fixes the staging capacity (no growth): a capacity below the largest
         // neighbourhood serves every build from global memory
         if(const char* v = std::getenv("MJG_TILE_CAP"))
@@ -609,6 +608,8 @@ void Engine::setup_global(GlobalIA& g)
     }
     g.d_nn = dalloc<uint32_t>(np);
     g.d_order = dalloc<uint32_t>(np);
+    g.d_spart = dalloc<uint32_t>(np);
+    if(g.partition == MJG_PART_CELLLIST) {g.d_spos = dalloc<double4>(np);}
     g.d_epair = dalloc<double>(np);
     g.d_fg = dalloc<double4>(n_);
     check(hipMemsetAsync(g.d_fg, 0, n_ * sizeof(double4), stream_), "memset");
@@ -645,9 +646,12 @@ ListSegTable Engine::list_table()
         s.nl16 = g.tiled ? g.d_nl16 : nullptr;
         s.nn = g.d_nn;
         s.order = g.d_order;
+        s.spart = g.d_spart;
+        s.spos = g.d_spos;
         s.ctl = d_lctl_ + gi;
         s.rule = g.rule;
-        block += blocks_for((size_t)s.np * s.tpp, 256);
+        s.nblocks = std::min<uint32_t>(blocks_for((size_t)s.np * s.tpp, 256), kVerletMaxBlocks);
+        block += s.nblocks;
     }
     return t;
 }
@@ -736,9 +740,17 @@ void Engine::launch_rebuilds(int force)
                                g.d_cell_start, g.d_cell_end, ctl);
         }
     }
+    for(size_t gi = 0; gi < globals_.size(); ++gi)
+    {
+        GlobalIA& g = globals_[gi];
+        if(g.partition != MJG_PART_CELLLIST) {continue;}
+        const uint32_t np = (uint32_t)g.part.size();
+        hipLaunchKernelGGL(k_sorted_slots, dim3(std::min<uint32_t>(blocks_for(np, 256), 1024)), dim3(256), 0, stream_,
+                           d_pos_, g.d_part, g.sb.vals, np, g.d_spart, g.d_spos, d_lctl_ + gi);
+    }
     const ListSegTable lt = list_table();
     uint32_t blocks = 0;
-    for(int k = 0; k < lt.n; ++k) {blocks += blocks_for((size_t)lt.seg[k].np * lt.seg[k].tpp, 256);}
+    for(int k = 0; k < lt.n; ++k) {blocks += lt.seg[k].nblocks;}
     hipLaunchKernelGGL(k_verlet, dim3(blocks), dim3(256), 0, stream_, d_pos_, box_, lt);
 }
 
@@ -968,8 +980,7 @@ void Engine::launch_multibody(const GlobalIA& g, bool energy)
 }
 
 // Cell-tile pair kernel of one interaction: one workgroup per cell, its LDS the staging of the
-// 27-cell neighbourhood.  FP64 runs the fused kernel's lanes per row (the same sums bit for bit);
-// the mixed kernel's fixed-point sums are exact for any lane split and use tile_tpp.
+// 27-cell neighbourhood, the fused kernel's lanes per row (the same sums bit for bit).
 void Engine::launch_tiled(GlobalIA& g, int gi, bool energy)
 {
     TileSeg s{};
@@ -977,10 +988,10 @@ void Engine::launch_tiled(GlobalIA& g, int gi, bool energy)
     ps.kind = g.pot; ps.tpp = g.tpp; ps.np = (uint32_t)g.part.size(); ps.block_begin = 0; ps.cap = g.cap;
     ps.part = g.d_part; ps.order = g.d_order; ps.nl = g.d_nl; ps.nn = g.d_nn; ps.ppar = g.d_ppar;
     ps.fout = g.d_fg; ps.eout = g.d_epair; ps.pp = g.pp;
-    s.nl16 = g.d_nl16; s.svals = g.sb.vals; s.cell_start = g.d_cell_start; s.cell_end = g.d_cell_end;
+    s.nl16 = g.d_nl16; s.spart = g.d_spart; s.cell_start = g.d_cell_start; s.cell_end = g.d_cell_end;
     s.grid = g.d_grid; s.ctl = d_lctl_ + gi; s.cap_nbr = g.tile_cap;
     const bool mixed = (precision_ == MJG_FP32_FIXED);
-    s.tpp = mixed ? g.tile_tpp : g.tpp;
+    s.tpp = g.tpp;
     const size_t lds = (size_t)g.tile_cap * kTileBytesPerParticle;
     const dim3 grid(g.grid.ncell), block(kTileThreads);
     if(mixed)

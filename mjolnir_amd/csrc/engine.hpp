@@ -60,7 +60,8 @@ struct GlobalIA
     Grid* d_grid = nullptr;
     SortBuffers sb;
     uint32_t *d_cell_start = nullptr, *d_cell_end = nullptr;
-    uint32_t *d_nl = nullptr, *d_nn = nullptr, *d_order = nullptr;
+    uint32_t *d_nl = nullptr, *d_nn = nullptr, *d_order = nullptr, *d_spart = nullptr;
+    double4* d_spos = nullptr;    // position of each sorted slot at the last cell-list build
     uint32_t cap = 0;             // row length of the list: nl[t * cap + k]
     uint32_t tpp = 8;             // lanes per participant in k_verlet / k_pairs
     uint32_t mb_tpp = 16;         // lanes per participant in k_tsp_pairs (one entry per lane)
@@ -68,11 +69,11 @@ struct GlobalIA
     double* d_epair = nullptr;
     double4* d_fg = nullptr;      // this interaction's force on every particle
     unsigned long long* d_acc = nullptr;   // 3SPN2 base pairing / cross stacking: int64 fixed-point sums
-    // cell-tile pair kernel (periodic cell lists): staging slot of every list entry, the staging
-    // capacity in particles and the kernel's lanes per row
+    // cell-tile pair kernel (periodic cell lists): staging slot of every list entry and the
+    // staging capacity in particles
     bool tiled = false;
     uint16_t* d_nl16 = nullptr;
-    uint32_t tile_cap = 0, tile_tpp = 8;
+    uint32_t tile_cap = 0;
     bool tile_fixed = false;
     bool multibody() const {return pot == kBasePair || pot == kCrossStack;}
 };

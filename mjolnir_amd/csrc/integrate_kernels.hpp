@@ -24,12 +24,22 @@ struct StepCtl
 };
 
 // max |dp|^2 of this step into slot step & 1; the other slot (read by the previous step's
-// skin check) is cleared for the next step
+// skin check) is cleared for the next step.  One atomic per block (the waves' maxima are combined
+// in LDS first): every block of a launch updates the same word, and per-wave atomics on it
+// serialise in the L2.  Called by every thread of the block.
 __device__ __forceinline__ void block_max_to(double v, StepCtl* ctl, uint64_t step)
 {
+    __shared__ double wmax[16];
     v = wave_max(v);
-    if(threadIdx.x % kWave == 0) {atomicMax(&ctl->maxdisp2[step & 1], (unsigned long long)__double_as_longlong(v));}
-    if(blockIdx.x == 0 && threadIdx.x == 0) {ctl->maxdisp2[(step + 1) & 1] = 0ull;}
+    if(threadIdx.x % kWave == 0) {wmax[threadIdx.x / kWave] = v;}
+    __syncthreads();
+    if(threadIdx.x == 0)
+    {
+        double m = wmax[0];
+        for(uint32_t w = 1; w < blockDim.x / kWave; ++w) {m = fmax(m, wmax[w]);}
+        atomicMax(&ctl->maxdisp2[step & 1], (unsigned long long)__double_as_longlong(m));
+        if(blockIdx.x == 0) {ctl->maxdisp2[(step + 1) & 1] = 0ull;}
+    }
 }
 
 __global__ void __launch_bounds__(256)

@@ -223,7 +223,7 @@ __device__ __forceinline__ bool pair_allowed(uint32_t lo, uint32_t hi, const Lis
 // ------------------------------------------------------------------------------------
 struct ListSeg
 {
-    uint32_t np, tpp, block_begin, cap;
+    uint32_t np, tpp, block_begin, nblocks, cap;
     int allpairs;                      // Verlet / Naive partitions: scan every participant
     const uint32_t* part;
     const uint32_t* skeys;
@@ -239,6 +239,8 @@ struct ListSeg
     uint16_t* nl16;                    // non-null: also the staging slot of each entry (k_pairs_tiled)
     uint32_t* nn;
     uint32_t* order;
+    uint32_t* spart;                   // particle of each sorted slot
+    const double4* spos;               // its position at this build (cell lists)
     ListCtl* ctl;
     ListRule rule;
 };
@@ -255,33 +257,24 @@ __device__ __forceinline__ uint64_t group_mask_of(uint32_t tpp, uint32_t lane)
     return m << (lane & ~(tpp - 1u));
 }
 
-__global__ void __launch_bounds__(256)
-k_verlet(const double4* __restrict__ pos, Box box, ListSegTable tab)
+// One row (sorted slot t) of the list, built by the tpp lanes of one group (lane l of the group).
+__device__ __forceinline__ void verlet_row(const ListSeg& s, uint32_t t, uint32_t l, const double4* __restrict__ pos,
+                                           const Box& box)
 {
-    int si = 0;
-    while(si + 1 < tab.n && blockIdx.x >= tab.seg[si + 1].block_begin) {++si;}
-    const ListSeg& s = tab.seg[si];
-    if(s.ctl->rebuild == 0) {return;}
-    const uint32_t gt = (blockIdx.x - s.block_begin) * blockDim.x + threadIdx.x;
-    const uint32_t t = gt / s.tpp;
     const uint32_t lane = threadIdx.x % kWave;
-    const uint32_t l = gt % s.tpp;
-    const bool active = t < s.np;
-    // all lanes of a group share t, so whole groups are active or inactive together
-    if(!active) {return;}
     const uint64_t gmask = group_mask_of(s.tpp, lane);
     const uint64_t below = gmask & ((1ull << lane) - 1ull);
     uint32_t p, i;
     if(s.allpairs) {p = t;} else {p = s.svals[t];}
     i = s.part[p];
-    if(l == 0) {s.order[t] = p;}
+    if(l == 0) {s.order[t] = p; s.spart[t] = i;}
     const double4 ri = pos[i];
     uint32_t count = 0;
     uint32_t* row = s.nl + (size_t)t * s.cap;
     uint16_t* row16 = s.nl16 ? s.nl16 + (size_t)t * s.cap : nullptr;
     uint32_t nbr = 0;   // staging slot of the next candidate cell's first particle (27-cell order)
 
-    auto consider = [&](uint32_t j, bool valid, uint32_t slot16) {
+    auto consider = [&](uint32_t j, bool valid, uint32_t slot16, const double4& rj) {
         bool accept = false;
         if(valid && j != i)
         {
@@ -290,7 +283,9 @@ k_verlet(const double4* __restrict__ pos, Box box, ListSegTable tab)
             {
                 if(s.rule.use_distance)
                 {
-                    const double4 a = pos[lo], b = pos[hi];
+                    // the reference measures the pair from its lower index (same operands, same
+                    // order); the row's own position is in registers
+                    const double4 a = (i < j) ? ri : rj, b = (i < j) ? rj : ri;
                     double ddx, ddy, ddz;
                     adjust_direction(box, a.x, a.y, a.z, b.x, b.y, b.z, ddx, ddy, ddz);
                     accept = length_sq(ddx, ddy, ddz) < s.rule.r_list2;
@@ -319,7 +314,8 @@ k_verlet(const double4* __restrict__ pos, Box box, ListSegTable tab)
         for(uint32_t base = 0; base < s.np; base += s.tpp)
         {
             const uint32_t q = base + l;
-            consider((q < s.np) ? s.part[q] : 0u, q < s.np, 0u);
+            const uint32_t j = (q < s.np) ? s.part[q] : 0u;
+            consider(j, q < s.np, 0u, pos[j]);
         }
     }
     else
@@ -348,8 +344,11 @@ k_verlet(const double4* __restrict__ pos, Box box, ListSegTable tab)
             const uint32_t sb = s.cell_start[c2], se = s.cell_end[c2];
             for(uint32_t base = sb; base < se; base += s.tpp)
             {
+                // the cell's candidates from the slot-ordered copies (k_sorted_slots): one
+                // contiguous run instead of two index lookups and a gather per candidate
                 const uint32_t q = base + l;
-                consider((q < se) ? s.part[s.svals[q]] : 0u, q < se, nbr + (q - sb));
+                const bool v = q < se;
+                consider(v ? s.spart[q] : 0u, v, nbr + (q - sb), v ? s.spos[q] : ri);
             }
             nbr += (se > sb) ? se - sb : 0u;
         }
@@ -360,6 +359,39 @@ k_verlet(const double4* __restrict__ pos, Box box, ListSegTable tab)
         s.nn[t] = (count < s.cap) ? count : s.cap;
         atomicMax(&s.ctl->maxcount, count);
         if(row16) {atomicMax(&s.ctl->maxnbr, nbr);}
+    }
+}
+
+// At a cell-list rebuild, after the sort: the particle and the position of every sorted slot (a
+// cell's particles are then one contiguous run of both).
+__global__ void __launch_bounds__(256)
+k_sorted_slots(const double4* __restrict__ pos, const uint32_t* __restrict__ part, const uint32_t* __restrict__ svals,
+               uint32_t np, uint32_t* __restrict__ spart, double4* __restrict__ spos, const ListCtl* __restrict__ ctl)
+{
+    if(ctl->rebuild == 0) {return;}
+    for(uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < np; t += gridDim.x * blockDim.x)
+    {
+        const uint32_t i = part[svals[t]];
+        spart[t] = i;
+        spos[t] = pos[i];
+    }
+}
+
+// A segment's blocks stride over its rows (at most kVerletMaxBlocks of them, so that the launch a
+// step makes when no list is rebuilt, every block leaving at once, stays small).  All lanes of a
+// group share their row, so whole groups enter and leave the loop together.
+constexpr uint32_t kVerletMaxBlocks = 2048;
+__global__ void __launch_bounds__(256)
+k_verlet(const double4* __restrict__ pos, Box box, ListSegTable tab)
+{
+    int si = 0;
+    while(si + 1 < tab.n && blockIdx.x >= tab.seg[si + 1].block_begin) {++si;}
+    const ListSeg& s = tab.seg[si];
+    if(s.ctl->rebuild == 0) {return;}
+    const uint32_t stride = s.nblocks * blockDim.x;
+    for(uint32_t gt = (blockIdx.x - s.block_begin) * blockDim.x + threadIdx.x; gt / s.tpp < s.np; gt += stride)
+    {
+        verlet_row(s, gt / s.tpp, gt % s.tpp, pos, box);
     }
 }
 

@@ -24,7 +24,7 @@ struct TileSeg
 {
     PairSeg ps;                   // rows, lists, parameters and outputs as k_pairs reads them
     const uint16_t* nl16;         // staging slot of each row entry (row-major like nl)
-    const uint32_t* svals;        // sorted slot -> participant
+    const uint32_t* spart;        // sorted slot -> particle (written by k_verlet)
     const uint32_t* cell_start;
     const uint32_t* cell_end;
     const Grid* grid;
@@ -33,6 +33,7 @@ struct TileSeg
     uint32_t tpp;                 // lanes per row of this kernel
 };
 constexpr uint32_t kTileThreads = 256;
+constexpr int kTileBatch = 4;
 
 // staged per-particle parameters (column 0 and 1)
 struct StagePar
@@ -64,23 +65,42 @@ __device__ __forceinline__ void tile_rows(const TileSeg& s, const Box& box, uint
         const uint16_t* row = s.nl16 + (size_t)t * ps.cap;
         double fx = 0, fy = 0, fz = 0, e = 0;
         long long ax = 0, ay = 0, az = 0;
-        for(uint32_t k = l; k < cnt; k += tpp)
+        // the entries k = l, l + tpp, ... in order, kTileBatch at a time: a batch's row words come
+        // from global memory, so the next batch's are requested before the current one is computed
+        constexpr int B = kTileBatch;
+        uint32_t nxt[B];
+#pragma unroll
+        for(int m = 0; m < B; ++m) {const uint32_t k = l + m * tpp; nxt[m] = (k < cnt) ? row[k] : ki;}
+        for(uint32_t k0 = l; k0 < cnt; k0 += B * tpp)
         {
-            const uint32_t kj = row[k];
-            const uint32_t j = sid[kj];
-            double dx, dy, dz;
-            adjust_direction(box, xi, yi, zi, sx[kj], sy[kj], sz[kj], dx, dy, dz);
-            if constexpr(MIXED)
+            uint32_t kj[B];
+#pragma unroll
+            for(int m = 0; m < B; ++m) {kj[m] = nxt[m];}
+#pragma unroll
+            for(int m = 0; m < B; ++m)
             {
-                float c;
-                if(!pair_term_f32<KIND, ENERGY>(K, ps.pp, par, i, j, ki, kj, dx, dy, dz, c, e)) {continue;}
-                long long gx, gy, gz;
-                fix3r<float>((float)dx * c, (float)dy * c, (float)dz * c, gx, gy, gz);
-                ax += gx; ay += gy; az += gz;
+                const uint32_t k = k0 + (B + m) * tpp;
+                nxt[m] = (k < cnt) ? row[k] : ki;
             }
-            else
+#pragma unroll
+            for(int m = 0; m < B; ++m)
             {
-                pair_term_p<KIND, ENERGY>(ps.pp, i, j, par, ki, kj, dx, dy, dz, fx, fy, fz, e);
+                if(k0 + m * tpp >= cnt) {break;}
+                const uint32_t j = sid[kj[m]];
+                double dx, dy, dz;
+                adjust_direction(box, xi, yi, zi, sx[kj[m]], sy[kj[m]], sz[kj[m]], dx, dy, dz);
+                if constexpr(MIXED)
+                {
+                    float c;
+                    if(!pair_term_f32<KIND, ENERGY>(K, ps.pp, par, i, j, ki, kj[m], dx, dy, dz, c, e)) {continue;}
+                    long long gx, gy, gz;
+                    fix3r<float>((float)dx * c, (float)dy * c, (float)dz * c, gx, gy, gz);
+                    ax += gx; ay += gy; az += gz;
+                }
+                else
+                {
+                    pair_term_p<KIND, ENERGY>(ps.pp, i, j, par, ki, kj[m], dx, dy, dz, fx, fy, fz, e);
+                }
             }
         }
         for(uint32_t o = tpp / 2; o > 0; o >>= 1)
@@ -221,18 +241,20 @@ __device__ __forceinline__ void tile_cell(const TileSeg& s, const double4* __res
     double* s1 = s0 + cap;
     uint32_t* sid = reinterpret_cast<uint32_t*>(s1 + cap);
     const int npar = s.ps.pp.npar;
-    for(int m = 0; m < 27; ++m)
+    // every thread stages the particles k = tid, tid + 256, ... of the neighbourhood: its cell m is
+    // found among the 27 offsets (binary search), so all the loads of a round are in flight at once
+    const uint32_t total = c_off[27];
+    for(uint32_t k = tid; k < total; k += kTileThreads)
     {
-        const uint32_t b = c_beg[m], o = c_off[m], c = c_off[m + 1] - o;
-        for(uint32_t q = tid; q < c; q += kTileThreads)
-        {
-            const uint32_t i = s.ps.part[s.svals[b + q]];
-            const double4 r = pos[i];
-            sx[o + q] = r.x; sy[o + q] = r.y; sz[o + q] = r.z;
-            s0[o + q] = s.ps.ppar[(size_t)i * npar];
-            s1[o + q] = (npar > 1) ? s.ps.ppar[(size_t)i * npar + 1] : 0.0;
-            sid[o + q] = i;
-        }
+        uint32_t m = 0;
+#pragma unroll
+        for(uint32_t w = 16; w > 0; w >>= 1) {if(m + w < 27 && c_off[m + w] <= k) {m += w;}}
+        const uint32_t i = s.spart[c_beg[m] + (k - c_off[m])];
+        const double4 r = pos[i];
+        sx[k] = r.x; sy[k] = r.y; sz[k] = r.z;
+        s0[k] = s.ps.ppar[(size_t)i * npar];
+        s1[k] = (npar > 1) ? s.ps.ppar[(size_t)i * npar + 1] : 0.0;
+        sid[k] = i;
     }
     __syncthreads();
     tile_rows<MIXED, ENERGY, KIND>(s, box, cs, nhome, c_off[13], sx, sy, sz, StagePar{s0, s1}, sid);
