@@ -1163,6 +1163,14 @@ class Batch
             }
         }
         if(!over) {return false;}
+        relayout(r, from_backup);
+        return true;
+    }
+
+    // lay the replica's arena out again with its (new) list capacities and restore its state: the
+    // segment-start copy (from_backup) or the current one; the lists are rebuilt at its next step
+    void relayout(Replica& r, bool from_backup)
+    {
         std::vector<double> pos(3 * r.n), vel(3 * r.n), force(3 * r.n);
         download_xyz(r, from_backup ? r.desc.bak_pos : r.desc.pos, pos);
         download_xyz(r, from_backup ? r.desc.bak_vel : r.desc.vel, vel);
@@ -1178,7 +1186,7 @@ class Batch
         int id = -1;
         for(size_t k = 0; k < reps_.size(); ++k) {if(reps_[k].get() == &r) {id = (int)k;}}
         prepare(id);
-        // positions are already wrapped; forces and counters are the segment start's
+        // positions are already wrapped; forces and counters are the restored ones
         std::vector<double4> f(r.n);
         for(size_t i = 0; i < r.n; ++i) {f[i] = make_double4(force[3 * i], force[3 * i + 1], force[3 * i + 2], 0.0);}
         hip_check(hipMemcpyAsync(r.desc.force, f.data(), r.n * sizeof(double4), hipMemcpyHostToDevice, stream_), "upload");
@@ -1187,7 +1195,6 @@ class Batch
         for(size_t g = 0; g < r.globals.size(); ++g) {bak.maxpairs[g] = 0;}
         hip_check(hipMemcpyAsync(r.desc.st, &bak, sizeof(RepState), hipMemcpyHostToDevice, stream_), "upload");
         hip_check(hipStreamSynchronize(stream_), "sync");
-        return true;
     }
 
     void download_xyz(Replica& r, const double4* d, std::vector<double>& out)
@@ -1246,6 +1253,36 @@ class Batch
             again.swap(still);
         }
         if(!again.empty()) {throw BatchDeviceError("replica list capacity did not converge");}
+        // A list that came within an eighth of its capacity (chains compact over a run, and their
+        // Debye-Hueckel lists grow) is enlarged now, from the replica's current state, so that it
+        // does not overflow in a later segment, which would run that segment a second time.
+        for(int id : act)
+        {
+            Replica& r = *reps_[id];
+            if(r.failed) {continue;}
+            const RepState& st = h_states_[r.slot];
+            bool tight = false;
+            for(size_t g = 0; g < r.globals.size(); ++g)
+            {
+                const uint32_t cap = r.globals[g].cap;
+                tight = tight || (st.maxpairs[g] > cap - cap / 8);
+            }
+            if(tight) {grow_ahead(r);}
+        }
+    }
+
+    // enlarge the replica's nearly full lists as an overflow at their capacity would, keeping its
+    // current state (its next step rebuilds the lists)
+    void grow_ahead(Replica& r)
+    {
+        const RepState st = state_of(r);
+        for(size_t g = 0; g < r.globals.size(); ++g)
+        {
+            const uint32_t cap = r.globals[g].cap;
+            if(st.maxpairs[g] > cap - cap / 8) {r.globals[g].cap = grown_cap(std::max(st.maxpairs[g], cap));}
+        }
+        relayout(r, false);
+        h_states_[r.slot] = state_of(r);
     }
 
     int precision_;

@@ -445,9 +445,10 @@ void Engine::alloc_list(GlobalIA& g, uint32_t cap)
 // with at least kTileMinNp participants (MJG_TILE_MIN_NP overrides it; MJG_NO_TILE=1 turns the
 // kernel off): it needs a launch of one workgroup per cell, which fills the GPU only for large
 // grids.  Its staging capacity starts at twice the mean 27-cell neighbourhood and grows with the
-// largest one a build reports (ListCtl::maxnbr), up to what the LDS of one workgroup holds.
+// largest one a build reports (ListCtl::maxnbr), up to what the LDS of one workgroup holds after
+// the kernel's static LDS (896 bytes in the current build; 4 KiB are set aside for it).
 constexpr uint32_t kTileMinNp = 16384;
-constexpr uint32_t kTileMaxCap = (uint32_t)((160 * 1024 - 1024) / kTileBytesPerParticle);
+constexpr uint32_t kTileMaxCap = (uint32_t)((160 * 1024 - 4096) / kTileBytesPerParticle);
 static uint32_t tile_min_np()
 {
     if(const char* v = std::getenv("MJG_TILE_MIN_NP")) {return (uint32_t)std::max(0L, std::atol(v));}
@@ -992,7 +993,7 @@ void Engine::launch_tiled(GlobalIA& g, int gi, bool energy)
     s.grid = g.d_grid; s.ctl = d_lctl_ + gi; s.cap_nbr = g.tile_cap;
     const bool mixed = (precision_ == MJG_FP32_FIXED);
     s.tpp = g.tpp;
-    const size_t lds = (size_t)g.tile_cap * kTileBytesPerParticle;
+    const size_t lds = (size_t)g.tile_cap * (mixed ? kTileBytesPerParticleF32 : kTileBytesPerParticle);
     const dim3 grid(g.grid.ncell), block(kTileThreads);
     if(mixed)
     {

@@ -194,25 +194,14 @@ __device__ __forceinline__ void tile_rows_global(const TileSeg& s, const double4
     }
 }
 
-template<bool MIXED, bool ENERGY, int KIND>
-__device__ __forceinline__ void tile_cell(const TileSeg& s, const double4* __restrict__ pos, const Box& box,
-                                          unsigned char* smem)
+// The 27 neighbour cells of `cell` (m = 9 (dz + 1) + 3 (dy + 1) + (dx + 1), wrapped as k_verlet wraps
+// them): first sorted slot and staging offsets (c_off[27] = neighbourhood size).
+__device__ __forceinline__ void tile_neighbourhood(const TileSeg& s, const Grid& g, uint32_t cell, uint32_t* c_beg,
+                                                   uint32_t* c_off)
 {
-    __shared__ uint32_t c_beg[27], c_off[28];
-    const uint32_t cell = blockIdx.x;
     const uint32_t tid = threadIdx.x;
-    const uint32_t cs = s.cell_start[cell], ce = s.cell_end[cell];
-    if(ce <= cs) {return;}                                   // empty cell: no rows (uniform)
-    const uint32_t nhome = ce - cs;
-    if(s.ctl->maxnbr > s.cap_nbr)                            // uniform: one word for the launch
-    {
-        tile_rows_global<MIXED, ENERGY, KIND>(s, pos, box, cs, nhome);
-        return;
-    }
-    const Grid g = *s.grid;
     if(tid < 27)
     {
-        // neighbour cell m = 9 (dz + 1) + 3 (dy + 1) + (dx + 1), wrapped as k_verlet wraps it
         const int dz = (int)(tid / 9) - 1, dy = (int)((tid / 3) % 3) - 1, dx = (int)(tid % 3) - 1;
         const int cx = (int)(cell % (uint32_t)g.dim[0]);
         const int cy = (int)((cell / (uint32_t)g.dim[0]) % (uint32_t)g.dim[1]);
@@ -233,6 +222,34 @@ __device__ __forceinline__ void tile_cell(const TileSeg& s, const double4* __res
         for(int m = 0; m < 27; ++m) {c_off[m + 1] += c_off[m];}
     }
     __syncthreads();
+}
+
+// neighbour cell of staging slot k (binary search over the 27 offsets)
+__device__ __forceinline__ uint32_t tile_cell_of(const uint32_t* c_off, uint32_t k)
+{
+    uint32_t m = 0;
+#pragma unroll
+    for(uint32_t w = 16; w > 0; w >>= 1) {if(m + w < 27 && c_off[m + w] <= k) {m += w;}}
+    return m;
+}
+
+template<bool MIXED, bool ENERGY, int KIND>
+__device__ __forceinline__ void tile_cell(const TileSeg& s, const double4* __restrict__ pos, const Box& box,
+                                          unsigned char* smem)
+{
+    __shared__ uint32_t c_beg[27], c_off[28];
+    const uint32_t cell = blockIdx.x;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t cs = s.cell_start[cell], ce = s.cell_end[cell];
+    if(ce <= cs) {return;}                                   // empty cell: no rows (uniform)
+    const uint32_t nhome = ce - cs;
+    if(s.ctl->maxnbr > s.cap_nbr)                            // uniform: one word for the launch
+    {
+        tile_rows_global<MIXED, ENERGY, KIND>(s, pos, box, cs, nhome);
+        return;
+    }
+    const Grid g = *s.grid;
+    tile_neighbourhood(s, g, cell, c_beg, c_off);
     const uint32_t cap = s.cap_nbr;
     double* sx = reinterpret_cast<double*>(smem);
     double* sy = sx + cap;
@@ -246,9 +263,7 @@ __device__ __forceinline__ void tile_cell(const TileSeg& s, const double4* __res
     const uint32_t total = c_off[27];
     for(uint32_t k = tid; k < total; k += kTileThreads)
     {
-        uint32_t m = 0;
-#pragma unroll
-        for(uint32_t w = 16; w > 0; w >>= 1) {if(m + w < 27 && c_off[m + w] <= k) {m += w;}}
+        const uint32_t m = tile_cell_of(c_off, k);
         const uint32_t i = s.spart[c_beg[m] + (k - c_off[m])];
         const double4 r = pos[i];
         sx[k] = r.x; sy[k] = r.y; sz[k] = r.z;
@@ -260,16 +275,247 @@ __device__ __forceinline__ void tile_cell(const TileSeg& s, const double4* __res
     tile_rows<MIXED, ENERGY, KIND>(s, box, cs, nhome, c_off[13], sx, sy, sz, StagePar{s0, s1}, sid);
 }
 
+// ------------------------------------------------------------------------------------
+// Mixed precision (MJG_FP32_FIXED) with FP32 staging.  Each staged particle holds its position
+// relative to the home cell's centre, seen through the neighbour cell's periodic image, split into
+// two FP32 parts (hi = the FP32 rounding of the FP64 offset, lo = the FP32 rounding of what
+// remains), so that (hi_j - hi_i) + (lo_j - lo_i) is the displacement to about FP32 precision of
+// the displacement itself; no minimum-image branches remain in the pair loop.  The cutoff test is
+// taken on that r^2 in FP32 where it is clear by more than kTileBand (relative); a pair inside the
+// band is decided by the FP64 path's own test (adjust_direction on the FP64 positions and the FP64
+// parameters), so the pair set inside the cutoff stays the FP64 path's.  The staged parameters are
+// FP32 (column 0 with the hi part, column 1 with the lo part).
+// ------------------------------------------------------------------------------------
+constexpr float kTileBand = 1.0e-5f;          // the FP32 r^2 and threshold errors are below 2e-6 relative
+constexpr size_t kTileBytesPerParticleF32 = 2 * sizeof(float4) + sizeof(uint32_t);
+
+// the FP64 path's inside-cutoff decision of the pair (i, j) (pair_term_f32's)
+template<int KIND>
+__device__ __noinline__ bool tile_inside_fp64(const PairParams& pp, const double* __restrict__ ppar,
+                                              const double4* __restrict__ pos, const Box& box, uint32_t i, uint32_t j)
+{
+    const double4 ri = pos[i], rj = pos[j];
+    double dx, dy, dz;
+    adjust_direction(box, ri.x, ri.y, ri.z, rj.x, rj.y, rj.z, dx, dy, dz);
+    const double l_sq = length_sq(dx, dy, dz);
+    if(KIND == kPairLJ)
+    {
+        const double sg = (ppar[2 * (size_t)i] + ppar[2 * (size_t)j]) / 2;
+        return !(sg * sg * pp.cutoff_ratio2 < l_sq);
+    }
+    if(KIND == kPairEV)
+    {
+        const double rad = ppar[i] + ppar[j];
+        return !(rad * rad * pp.cutoff_ratio2 < l_sq);
+    }
+    if(KIND == kPair3SPN2EV)
+    {
+        const double sg = (tsp::ev_sigma((int)ppar[i]) + tsp::ev_sigma((int)ppar[j])) * 0.5;
+        return sg * sg > l_sq;
+    }
+    return pp.abs_cutoff * pp.abs_cutoff > l_sq;
+}
+
+template<bool ENERGY, int KIND>
+__device__ __forceinline__ void tile_rows_f32(const TileSeg& s, const double4* __restrict__ pos, const Box& box,
+                                              uint32_t cs, uint32_t nhome, uint32_t home0, const float4* hi,
+                                              const float4* lo, const uint32_t* sid)
+{
+#pragma clang fp contract(fast)
+    const PairSeg& ps = s.ps;
+    const uint32_t tpp = s.tpp;
+    const uint32_t tid = threadIdx.x;
+    const PairConstsF K = pair_consts_f(ps.pp);
+    const float acut2 = (float)(ps.pp.abs_cutoff * ps.pp.abs_cutoff);
+    for(uint32_t u = tid; u < nhome * tpp; u += kTileThreads)
+    {
+        const uint32_t h = u / tpp, l = u % tpp;
+        const uint32_t t = cs + h, ki = home0 + h;
+        const uint32_t i = sid[ki];
+        const float4 ah = hi[ki], al = lo[ki];
+        const uint32_t cnt = ps.nn[t];
+        const uint16_t* row = s.nl16 + (size_t)t * ps.cap;
+        double e = 0;
+        long long ax = 0, ay = 0, az = 0;
+        constexpr int B = kTileBatch;
+        uint32_t nxt[B];
+#pragma unroll
+        for(int m = 0; m < B; ++m) {const uint32_t k = l + m * tpp; nxt[m] = (k < cnt) ? row[k] : ki;}
+        for(uint32_t k0 = l; k0 < cnt; k0 += B * tpp)
+        {
+            uint32_t kj[B];
+#pragma unroll
+            for(int m = 0; m < B; ++m) {kj[m] = nxt[m];}
+#pragma unroll
+            for(int m = 0; m < B; ++m)
+            {
+                const uint32_t k = k0 + (B + m) * tpp;
+                nxt[m] = (k < cnt) ? row[k] : ki;
+            }
+#pragma unroll
+            for(int m = 0; m < B; ++m)
+            {
+                if(k0 + m * tpp >= cnt) {break;}
+                const float4 bh = hi[kj[m]], bl = lo[kj[m]];
+                const float dx = (bh.x - ah.x) + (bl.x - al.x);
+                const float dy = (bh.y - ah.y) + (bl.y - al.y);
+                const float dz = (bh.z - ah.z) + (bl.z - al.z);
+                const float l_sq = dx * dx + dy * dy + dz * dz;
+                float thr, sg = 0.0f;
+                if(KIND == kPairLJ) {sg = (ah.w + bh.w) * 0.5f; thr = sg * sg * K.cut2;}
+                else if(KIND == kPairEV) {sg = ah.w + bh.w; thr = sg * sg * K.cut2;}
+                else if(KIND == kPair3SPN2EV)
+                {
+                    sg = (float)((tsp::ev_sigma((int)ah.w) + tsp::ev_sigma((int)bh.w)) * 0.5);
+                    thr = sg * sg;
+                }
+                else {thr = acut2;}
+                if(l_sq > thr * (1.0f + kTileBand)) {continue;}
+                const uint32_t j = (ENERGY || l_sq >= thr * (1.0f - kTileBand)) ? sid[kj[m]] : 0u;
+                if(l_sq >= thr * (1.0f - kTileBand) && !tile_inside_fp64<KIND>(ps.pp, ps.ppar, pos, box, i, j)) {continue;}
+                float c;
+                if(KIND == kPairLJ)
+                {
+                    const float e1 = al.w, e2 = bl.w;
+                    const float eps = (e1 == e2) ? e1 : RMath<float>::sqrt(e1 * e2);
+                    const float rcp = RMath<float>::rcp(l_sq);
+                    const float s2l2 = sg * sg * rcp;
+                    const float s6 = s2l2 * s2l2 * s2l2;
+                    c = 24.0f * eps * (s6 - 2.0f * s6 * s6) * rcp;
+                    if(ENERGY && i < j) {e += (double)(4.0f * eps * (s6 * s6 - s6 - K.cac));}
+                }
+                else if(KIND == kPairEV)
+                {
+                    const float rcp = RMath<float>::rcp(l_sq);
+                    const float s2l2 = sg * sg * rcp;
+                    const float s6 = s2l2 * s2l2 * s2l2;
+                    c = -(K.eps * 12.0f) * s6 * s6 * rcp;
+                    if(ENERGY && i < j) {e += (double)(K.eps * (s6 * s6 - K.cac));}
+                }
+                else if(KIND == kPair3SPN2EV)
+                {
+                    const float rl = RMath<float>::rsqrt(l_sq);
+                    const float r1s1 = sg * rl;
+                    const float r3s3 = r1s1 * r1s1 * r1s1;
+                    const float r6s6 = r3s3 * r3s3;
+                    c = 12.0f * K.eps * rl * r6s6 * (1.0f - r6s6) * rl;
+                    if(ENERGY && i < j) {e += (double)(K.eps * (r6s6 * (r6s6 - 2.0f) + 1.0f));}
+                }
+                else
+                {
+                    const float qq = K.qk * ah.w * bh.w;
+                    const float rl = RMath<float>::rsqrt(l_sq);
+                    const float ll = l_sq * rl;
+                    const float ex = RMath<float>::exp(-ll * K.idebye);
+                    const float f_mag = -qq * (K.debye + ll) * K.idebye * (rl * rl) * ex;
+                    c = f_mag * rl;
+                    if(ENERGY && i < j) {e += (double)(qq * (ex * rl - K.cac));}
+                }
+                long long gx, gy, gz;
+                fix3r<float>(dx * c, dy * c, dz * c, gx, gy, gz);
+                ax += gx; ay += gy; az += gz;
+            }
+        }
+        for(uint32_t o = tpp / 2; o > 0; o >>= 1)
+        {
+            ax += shfl_xor_ll(ax, (int)o);
+            ay += shfl_xor_ll(ay, (int)o);
+            az += shfl_xor_ll(az, (int)o);
+            if(ENERGY) {e += __shfl_xor(e, o, kWave);}
+        }
+        if(l == 0)
+        {
+            const bool bad = fix_insane((unsigned long long)ax) || fix_insane((unsigned long long)ay) ||
+                             fix_insane((unsigned long long)az);
+            const double nan = bad ? __builtin_nan("") : 0.0;
+            ps.fout[i] = make_double4(from_fix<float>((unsigned long long)ax) + nan,
+                                      from_fix<float>((unsigned long long)ay) + nan,
+                                      from_fix<float>((unsigned long long)az) + nan, 0.0);
+            if(ENERGY) {ps.eout[t] = e;}
+        }
+    }
+}
+
+template<bool ENERGY, int KIND>
+__device__ __forceinline__ void tile_cell_f32(const TileSeg& s, const double4* __restrict__ pos, const Box& box,
+                                              unsigned char* smem)
+{
+    __shared__ uint32_t c_beg[27], c_off[28];
+    const uint32_t cell = blockIdx.x;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t cs = s.cell_start[cell], ce = s.cell_end[cell];
+    if(ce <= cs) {return;}
+    const uint32_t nhome = ce - cs;
+    if(s.ctl->maxnbr > s.cap_nbr)
+    {
+        tile_rows_global<true, ENERGY, KIND>(s, pos, box, cs, nhome);
+        return;
+    }
+    const Grid g = *s.grid;
+    tile_neighbourhood(s, g, cell, c_beg, c_off);
+    const uint32_t cap = s.cap_nbr;
+    float4* hi = reinterpret_cast<float4*>(smem);
+    float4* lo = hi + cap;
+    uint32_t* sid = reinterpret_cast<uint32_t*>(lo + cap);
+    // the home cell's centre and cell widths
+    const int cx = (int)(cell % (uint32_t)g.dim[0]);
+    const int cy = (int)((cell / (uint32_t)g.dim[0]) % (uint32_t)g.dim[1]);
+    const int cz = (int)(cell / ((uint32_t)g.dim[0] * (uint32_t)g.dim[1]));
+    const double wx = box.width[0] / g.dim[0], wy = box.width[1] / g.dim[1], wz = box.width[2] / g.dim[2];
+    const double ox = box.lower[0] + ((double)cx + 0.5) * wx;
+    const double oy = box.lower[1] + ((double)cy + 0.5) * wy;
+    const double oz = box.lower[2] + ((double)cz + 0.5) * wz;
+    const double ilx = 1.0 / box.width[0], ily = 1.0 / box.width[1], ilz = 1.0 / box.width[2];
+    const int npar = s.ps.pp.npar;
+    const uint32_t total = c_off[27];
+    for(uint32_t k = tid; k < total; k += kTileThreads)
+    {
+        const uint32_t m = tile_cell_of(c_off, k);
+        const uint32_t i = s.spart[c_beg[m] + (k - c_off[m])];
+        const double4 r = pos[i];
+        // The particle sat in neighbour cell m at the last build and has moved less than half the
+        // skin since; positions are wrapped into the box every step, so it may have crossed the
+        // boundary after the build.  Its image nearest to cell m's centre, seen from the home cell
+        // (offset (dx, dy, dz) cell widths), is the one within half a cell plus the drift of it:
+        // fewer than half a box width, since a periodic grid has at least three cells a dimension.
+        const double ex = ((double)((int)(m % 3) - 1)) * wx;
+        const double ey = ((double)((int)((m / 3) % 3) - 1)) * wy;
+        const double ez = ((double)((int)(m / 9) - 1)) * wz;
+        double X = r.x - ox, Y = r.y - oy, Z = r.z - oz;
+        X -= box.width[0] * rint((X - ex) * ilx);
+        Y -= box.width[1] * rint((Y - ey) * ily);
+        Z -= box.width[2] * rint((Z - ez) * ilz);
+        const float hx = (float)X, hy = (float)Y, hz = (float)Z;
+        const float p0 = (float)s.ps.ppar[(size_t)i * npar];
+        const float p1 = (npar > 1) ? (float)s.ps.ppar[(size_t)i * npar + 1] : 0.0f;
+        hi[k] = make_float4(hx, hy, hz, p0);
+        lo[k] = make_float4((float)(X - (double)hx), (float)(Y - (double)hy), (float)(Z - (double)hz), p1);
+        sid[k] = i;
+    }
+    __syncthreads();
+    tile_rows_f32<ENERGY, KIND>(s, pos, box, cs, nhome, c_off[13], hi, lo, sid);
+}
+
 template<bool MIXED, bool ENERGY>
 __global__ void __launch_bounds__(kTileThreads)
 k_pairs_tiled(const double4* __restrict__ pos, Box box, TileSeg s)
 {
     extern __shared__ __align__(16) unsigned char tile_smem[];
     const int kind = s.ps.kind;
-    if(kind == kPairLJ) {tile_cell<MIXED, ENERGY, kPairLJ>(s, pos, box, tile_smem);}
-    else if(kind == kPairEV) {tile_cell<MIXED, ENERGY, kPairEV>(s, pos, box, tile_smem);}
-    else if(kind == kPair3SPN2EV) {tile_cell<MIXED, ENERGY, kPair3SPN2EV>(s, pos, box, tile_smem);}
-    else {tile_cell<MIXED, ENERGY, kPairDH>(s, pos, box, tile_smem);}
+    if constexpr(MIXED)
+    {
+        if(kind == kPairLJ) {tile_cell_f32<ENERGY, kPairLJ>(s, pos, box, tile_smem);}
+        else if(kind == kPairEV) {tile_cell_f32<ENERGY, kPairEV>(s, pos, box, tile_smem);}
+        else if(kind == kPair3SPN2EV) {tile_cell_f32<ENERGY, kPair3SPN2EV>(s, pos, box, tile_smem);}
+        else {tile_cell_f32<ENERGY, kPairDH>(s, pos, box, tile_smem);}
+    }
+    else
+    {
+        if(kind == kPairLJ) {tile_cell<false, ENERGY, kPairLJ>(s, pos, box, tile_smem);}
+        else if(kind == kPairEV) {tile_cell<false, ENERGY, kPairEV>(s, pos, box, tile_smem);}
+        else if(kind == kPair3SPN2EV) {tile_cell<false, ENERGY, kPair3SPN2EV>(s, pos, box, tile_smem);}
+        else {tile_cell<false, ENERGY, kPairDH>(s, pos, box, tile_smem);}
+    }
 }
 
 } // mjg

@@ -370,35 +370,99 @@ def _tile_engine(spec, precision, monkeypatch, **env):
     return e
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_cell_tile_kernel_equals_the_gather_kernel(precision, monkeypatch):
-    """The cell-tile pair kernel (periodic cell lists of >= 16384 participants: one workgroup per
-    cell, the 27-cell neighbourhood staged in LDS) against the full-list gather kernel on a 32,768-
-    particle LJ fluid: forces bit for bit (FP64: the same lanes per row and sum order; mixed: exact
-    fixed-point sums), energies to 1e-12 (mixed: the lanes split the FP64 energy sum differently),
-    and 300 Langevin steps through list rebuilds bit for bit; the same with a staging capacity too
-    small for any neighbourhood, which serves every build from global memory inside the kernel."""
+def test_cell_tile_kernel_fp64_equals_the_gather_kernel(monkeypatch):
+    """The FP64 cell-tile pair kernel (periodic cell lists of >= 16384 participants: one workgroup
+    per cell, the 27-cell neighbourhood staged in LDS) against the full-list gather kernel on a
+    32,768-particle LJ fluid: forces and energies bit for bit (the same lanes per row and sum
+    order), and 300 Langevin steps through list rebuilds bit for bit; the same with a staging
+    capacity too small for any neighbourhood, which serves every build from global memory."""
     from mjolnir_amd import capi, workloads
-    prec = capi.FP64 if precision == "fp64" else capi.FP32_FIXED
     spec = make_golden.perturb(workloads.lj_fluid(n_side=32, seed=41), 0.1, 42)
-    ref = _tile_engine(spec, prec, monkeypatch, MJG_NO_TILE="1")
+    ref = _tile_engine(spec, capi.FP64, monkeypatch, MJG_NO_TILE="1")
     for env in ({}, {"MJG_TILE_CAP": "16"}):
-        e = _tile_engine(spec, prec, monkeypatch, **env)
-        print(f"[gpu] cell-tile kernel {precision} {env}", flush=True)
+        e = _tile_engine(spec, capi.FP64, monkeypatch, **env)
+        print(f"[gpu] cell-tile kernel fp64 {env}", flush=True)
         e.calc_force()
         ref.calc_force()
         assert np.array_equal(e.state()[2], ref.state()[2])
-        pe, te = e.energies()
-        pr, tr = ref.energies()
-        assert abs(te - tr) <= 1e-12 * abs(tr)
+        assert e.energies()[1] == ref.energies()[1]
         e.close()
-    a = _tile_engine(spec, prec, monkeypatch)
+    a = _tile_engine(spec, capi.FP64, monkeypatch)
     a.step(300)
     ref.step(300)
     xa, va, fa = a.state()
     xr, vr, fr = ref.state()
     assert a.counters()[1] > 1
     assert np.array_equal(xa, xr) and np.array_equal(va, vr) and np.array_equal(fa, fr)
+
+
+def test_cell_tile_kernel_mixed(monkeypatch):
+    """The mixed-precision cell-tile kernel (FP32 offsets from the home cell's centre, hi + lo parts)
+    on the same 32,768-particle fluid: forces against the FP64 engine at the mixed bar (RMS <= 1e-5
+    of the RMS force), within 2e-6 of the gather kernel's, energies at 1e-6; a staging capacity too
+    small for any neighbourhood gives the gather kernel's forces bit for bit (the same global path);
+    two runs of 300 Langevin steps end bit for bit equal, and after 300 steps (particles cross the
+    periodic boundary between list builds, so the staging must pick each one's image anew) the
+    kinetic energy is within 1 % of the FP64 engine's, as the mixed gather kernel's is."""
+    from mjolnir_amd import capi, workloads
+    spec = make_golden.perturb(workloads.lj_fluid(n_side=32, seed=41), 0.1, 42)
+    f64 = _tile_engine(spec, capi.FP64, monkeypatch)
+    gat = _tile_engine(spec, capi.FP32_FIXED, monkeypatch, MJG_NO_TILE="1")
+    til = _tile_engine(spec, capi.FP32_FIXED, monkeypatch)
+    fb = _tile_engine(spec, capi.FP32_FIXED, monkeypatch, MJG_TILE_CAP="16")
+    for e in (f64, gat, til, fb):
+        e.calc_force()
+    ref = f64.state()[2]
+    rms_t, mx_t = _rms_rel(til.state()[2], ref)
+    rms_g, _ = _rms_rel(gat.state()[2], ref)
+    d_tg, _ = _rms_rel(til.state()[2], gat.state()[2])
+    print(f"[gpu] mixed cell-tile: RMS {rms_t:.2e} (max {mx_t:.2e}), gather {rms_g:.2e}, tile vs gather {d_tg:.2e}",
+          flush=True)
+    assert rms_t <= 1e-5 and d_tg <= 2e-6
+    assert np.array_equal(fb.state()[2], gat.state()[2])
+    assert abs(til.energies()[1] - f64.energies()[1]) <= 1e-6 * abs(f64.energies()[1])
+    runs = []
+    for _ in range(2):
+        a = _tile_engine(spec, capi.FP32_FIXED, monkeypatch)
+        a.step(300)
+        runs.append(a.state())
+        ke = a.kinetic_energy()
+        a.close()
+    assert all(np.array_equal(x, y) for x, y in zip(*runs))
+    f64.step(300)
+    gat.step(300)
+    ke64, keg = f64.kinetic_energy(), gat.kinetic_energy()
+    print(f"[gpu] kinetic energy after 300 steps: FP64 {ke64:.2f}, mixed gather {keg:.2f}, mixed tile {ke:.2f}",
+          flush=True)
+    # positions are wrapped into the box every step, so particles cross the boundary between builds
+    assert np.any(np.abs(runs[0][0] - spec.pos) > 0.5 * (spec.upper - spec.lower))
+    assert abs(ke - ke64) <= 0.01 * ke64 and abs(keg - ke64) <= 0.01 * ke64
+
+
+@pytest.mark.parametrize("delta", [-2e-7, 0.0, 2e-7])
+def test_cell_tile_mixed_cutoff_decision_is_the_fp64_one(delta, monkeypatch):
+    """Two LJ particles at r_c (1 + delta) in a periodic box, beyond the FP32 resolution of r^2: the
+    mixed cell-tile kernel takes the FP64 path's decision (inside at r = r_c, the reference's
+    sigma^2 ratio^2 < r^2 test) and its force equals the FP64 engine's to FP32 precision."""
+    from mjolnir_amd import capi, workloads
+    spec = workloads.lj_fluid(n_side=2, seed=3)          # 8 particles, sigma 4, cutoff ratio 2.5
+    w = 60.0
+    spec.lower, spec.upper = np.zeros(3), np.full(3, w)
+    rc = 4.0 * 2.5
+    # particles 0 and 1 alone within the cutoff of each other; the other six far from everything
+    pos = np.array([[10.0, 30.0, 30.0], [10.0 + rc * (1.0 + delta), 30.0, 30.0],
+                    [45.0, 5.0, 5.0], [45.0, 45.0, 5.0], [5.0, 45.0, 45.0], [45.0, 5.0, 45.0],
+                    [5.0, 5.0, 45.0], [45.0, 45.0, 45.0]])
+    spec.pos = pos
+    spec.vel = np.zeros_like(pos)
+    e64 = _tile_engine(spec, capi.FP64, monkeypatch, MJG_TILE_MIN_NP="0")
+    e32 = _tile_engine(spec, capi.FP32_FIXED, monkeypatch, MJG_TILE_MIN_NP="0")
+    f64 = e64.force_of(0)[0]
+    f32 = e32.force_of(0)[0]
+    print(f"[gpu] delta {delta}: FP64 {f64}, mixed {f32}", flush=True)
+    assert (np.abs(f64).max() > 0) == (delta <= 0.0)
+    # FP32 arithmetic, fixed point at 2^-24 kcal/mol/A
+    assert np.abs(f32 - f64).max() <= 1e-5 * np.abs(f64).max() + 1e-7
 
 
 @pytest.mark.parametrize("name", [n for n in CASES if n.startswith("lj")])
