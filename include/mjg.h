@@ -197,6 +197,14 @@ mjg_status mjg_step(mjg_engine* e, size_t n_steps);
 mjg_status mjg_counters(mjg_engine* e, uint64_t* steps, uint64_t* rebuilds);
 /* Kinetic energy sum_i m_i v_i^2 / 2 (EnergyObserver.hpp:89-150 prints it beside the terms). */
 mjg_status mjg_kinetic_energy(mjg_engine* e, double* ke);
+/* mjg_kinetic_energy without any device allocation, for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  The per-particle terms and the sum go to a static device array of the library (8 MiB
+ * per GPU, held until the process ends) instead of the engine's scratch buffer; the kernels and
+ * the sum order are mjg_kinetic_energy's, so the result is the same bit for bit.  At most 1048576
+ * particles (MJG_INVALID_ARG above that).  Calls on engines of one GPU share the array: calls that
+ * overlap in time on one GPU race and must be serialized by the caller. */
+mjg_status mjg_kinetic_energy_static(mjg_engine* e, double* ke);
 
 /* EnergyObserver (EnergyObserver.hpp:27-150): create prefix.ene (truncating it) and write the
  * unit line and the column header: "# timestep", one column per interaction named as the

@@ -60,7 +60,7 @@ EXPORTED = [
     "mjg_interaction_name", "mjg_calc_force_of",
     "mjg_reduce_margin", "mjg_neighbor_pairs", "mjg_integrator_set", "mjg_integrator_set_remover", "mjg_observer_energy_open",
     "mjg_observer_energy_output", "mjg_observer_xyz_open", "mjg_observer_dcd_open", "mjg_simulate", "mjg_integrator_initialize",
-    "mjg_step", "mjg_counters", "mjg_kinetic_energy", "mjg_profile_interaction",
+    "mjg_step", "mjg_counters", "mjg_kinetic_energy", "mjg_kinetic_energy_static", "mjg_profile_interaction",
     "mjg_batch_create", "mjg_batch_destroy", "mjg_batch_last_error", "mjg_batch_add_replica",
     "mjg_batch_set_attribute", "mjg_batch_add_local", "mjg_batch_add_global", "mjg_batch_set_integrator", "mjg_batch_set_step_count",
     "mjg_batch_initialize", "mjg_batch_step", "mjg_batch_set_steps", "mjg_batch_advance", "mjg_batch_retire",
@@ -113,6 +113,7 @@ def lib():
     L.mjg_step.argtypes = [vp, sz]
     L.mjg_counters.argtypes = [vp, C.POINTER(u64), C.POINTER(u64)]
     L.mjg_kinetic_energy.argtypes = [vp, C.POINTER(dbl)]
+    L.mjg_kinetic_energy_static.argtypes = [vp, C.POINTER(dbl)]
     L.mjg_profile_interaction.argtypes = [vp, i32, i32, C.POINTER(dbl), C.POINTER(u64)]
     L.mjg_batch_create.argtypes = [i32, i32, C.POINTER(vp)]
     L.mjg_batch_destroy.argtypes = [vp]
@@ -264,6 +265,13 @@ class Engine:
     def kinetic_energy(self) -> float:
         e = C.c_double()
         self._check(lib().mjg_kinetic_energy(self.h, C.byref(e)))
+        return e.value
+
+    def kinetic_energy_static(self) -> float:
+        """mjg_kinetic_energy_static: the same sum in the library's static device scratch (no
+        device allocation; at most 1048576 particles)."""
+        e = C.c_double()
+        self._check(lib().mjg_kinetic_energy_static(self.h, C.byref(e)))
         return e.value
 
     def profile_interaction(self, k: int, reps: int = 50):

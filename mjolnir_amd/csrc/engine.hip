@@ -9,6 +9,7 @@
 #include <cstring>
 #include <fstream>
 #include <iomanip>
+#include <mutex>
 #include <new>
 #include <sstream>
 
@@ -1583,6 +1584,61 @@ double Engine::kinetic_energy()
     return ke;
 }
 
+// ------------------------------------------------------------------------------------
+// Allocation-free kinetic energy: the per-particle terms and their sum live in a file-scope device
+// array, `scratch`, instead of the engine's scratch buffer.  Every GPU the library runs on holds
+// its own copy (8 MiB) until the process ends; its address on a device is looked up once, with
+// that device current, and kept.  Calls on one GPU share the array: overlapping calls race.
+// Nothing in it has to start at zero (k_kinetic writes every term it sums, k_sum the result).
+// ------------------------------------------------------------------------------------
+constexpr size_t kStaticMaxParticles = size_t(1) << 20;
+static __device__ __align__(256) double scratch[kStaticMaxParticles + 1];
+
+namespace
+{
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}
+
+static double* static_scratch(int device)
+{
+    if(device < 0 || device >= kMaxDevices) {throw std::out_of_range("device ordinal out of range");}
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if(g_scratch_ptr[device] == nullptr)
+    {
+        auto fail = [](hipError_t e, const char* what) {
+            if(e != hipSuccess) {throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));}
+        };
+        int prev = 0;
+        fail(hipGetDevice(&prev), "hipGetDevice");
+        fail(hipSetDevice(device), "hipSetDevice");
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        (void)hipSetDevice(prev);
+        fail(e, "hipGetSymbolAddress(scratch)");
+        if(p == nullptr) {throw DeviceError("hipGetSymbolAddress(scratch) returned no address");}
+        g_scratch_ptr[device] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[device];
+}
+
+double Engine::kinetic_energy_static()
+{
+    if(n_ > kStaticMaxParticles)
+    {
+        throw std::invalid_argument("mjg_kinetic_energy_static: " + std::to_string(n_) +
+                                    " particles exceed its static scratch of 1048576; use mjg_kinetic_energy");
+    }
+    double* s = static_scratch(device_);
+    hipLaunchKernelGGL(k_kinetic, dim3(blocks_for(n_, 256)), dim3(256), 0, stream_, (uint32_t)n_, d_vel_, d_mass_, s);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, stream_, s, (uint32_t)n_, s + kStaticMaxParticles);
+    check(hipGetLastError(), "kinetic launch");
+    double ke = 0;
+    download(&ke, s + kStaticMaxParticles, 1);
+    return ke;
+}
+
 double Engine::profile_interaction(int id, int reps, uint64_t* half_pairs)
 {
     if(!initialized_) {throw std::logic_error("not initialized");}
@@ -1849,6 +1905,10 @@ mjg_status mjg_counters(mjg_engine* e, uint64_t* steps, uint64_t* rebuilds)
 mjg_status mjg_kinetic_energy(mjg_engine* e, double* ke)
 {
     return guarded(e, [&](mjg::Engine& x) {const double v = x.kinetic_energy(); if(ke) {*ke = v;}});
+}
+mjg_status mjg_kinetic_energy_static(mjg_engine* e, double* ke)
+{
+    return guarded(e, [&](mjg::Engine& x) {const double v = x.kinetic_energy_static(); if(ke) {*ke = v;}});
 }
 
 mjg_status mjg_profile_interaction(mjg_engine* e, int id, int reps, double* avg_ms, uint64_t* half_pairs)

@@ -123,6 +123,7 @@ class Engine
     void counters(uint64_t* steps, uint64_t* rebuilds);
     void set_step_count(uint64_t step);
     double kinetic_energy();
+    double kinetic_energy_static();   // same sum, in the library's static device scratch
     // observers and the simulator loop
     std::string interaction_name(size_t id) const;
     void energy_observer_open(const std::string& prefix);
