@@ -621,6 +621,36 @@ void Engine::setup_global(GlobalIA& g)
     alloc_list(g, 64);
 }
 
+ListSeg Engine::list_seg(GlobalIA& g, size_t gi)
+{
+    ListSeg s{};
+    s.np = (uint32_t)g.part.size();
+    s.tpp = g.tpp;
+    s.cap = g.cap;
+    s.allpairs = (g.partition != MJG_PART_CELLLIST);
+    s.part = g.d_part;
+    s.skeys = g.sb.keys;
+    s.svals = g.sb.vals;
+    s.cell_start = g.d_cell_start;
+    s.cell_end = g.d_cell_end;
+    s.grid = g.d_grid;
+    s.pslot = g.d_pslot;
+    s.mol = g.d_mol;
+    s.ex_start = g.d_ex_start;
+    s.ex = g.d_ex;
+    s.nl = g.d_nl;
+    s.nl16 = g.tiled ? g.d_nl16 : nullptr;
+    s.nn = g.d_nn;
+    s.order = g.d_order;
+    s.spart = g.d_spart;
+    s.spos = g.d_spos;
+    s.ctl = d_lctl_ + gi;
+    s.rule = g.rule;
+    return s;
+}
+
+// k_verlet's segments: every cell-list interaction the cell-tile kernel does not serve (those get
+// their own build, k_verlet_tiled)
 ListSegTable Engine::list_table()
 {
     ListSegTable t{};
@@ -628,30 +658,10 @@ ListSegTable Engine::list_table()
     for(size_t gi = 0; gi < globals_.size(); ++gi)
     {
         GlobalIA& g = globals_[gi];
+        if(g.tiled) {continue;}
         ListSeg& s = t.seg[t.n++];
-        s.np = (uint32_t)g.part.size();
-        s.tpp = g.tpp;
+        s = list_seg(g, gi);
         s.block_begin = block;
-        s.cap = g.cap;
-        s.allpairs = (g.partition != MJG_PART_CELLLIST);
-        s.part = g.d_part;
-        s.skeys = g.sb.keys;
-        s.svals = g.sb.vals;
-        s.cell_start = g.d_cell_start;
-        s.cell_end = g.d_cell_end;
-        s.grid = g.d_grid;
-        s.pslot = g.d_pslot;
-        s.mol = g.d_mol;
-        s.ex_start = g.d_ex_start;
-        s.ex = g.d_ex;
-        s.nl = g.d_nl;
-        s.nl16 = g.tiled ? g.d_nl16 : nullptr;
-        s.nn = g.d_nn;
-        s.order = g.d_order;
-        s.spart = g.d_spart;
-        s.spos = g.d_spos;
-        s.ctl = d_lctl_ + gi;
-        s.rule = g.rule;
         s.nblocks = std::min<uint32_t>(blocks_for((size_t)s.np * s.tpp, 256), kVerletMaxBlocks);
         block += s.nblocks;
     }
@@ -753,7 +763,16 @@ void Engine::launch_rebuilds(int force)
     const ListSegTable lt = list_table();
     uint32_t blocks = 0;
     for(int k = 0; k < lt.n; ++k) {blocks += lt.seg[k].nblocks;}
-    hipLaunchKernelGGL(k_verlet, dim3(blocks), dim3(256), 0, stream_, d_pos_, box_, lt);
+    if(blocks > 0) {hipLaunchKernelGGL(k_verlet, dim3(blocks), dim3(256), 0, stream_, d_pos_, box_, lt);}
+    for(size_t gi = 0; gi < globals_.size(); ++gi)
+    {
+        GlobalIA& g = globals_[gi];
+        if(!g.tiled) {continue;}
+        const ListSeg s = list_seg(g, gi);
+        const size_t lds = (size_t)g.tile_cap * kBuildBytesPerParticle;
+        hipLaunchKernelGGL(k_verlet_tiled, dim3(std::min<uint32_t>(g.grid.ncell, kVerletMaxBlocks)), dim3(kTileThreads),
+                           lds, stream_, d_pos_, box_, s, g.tile_cap);
+    }
 }
 
 // Build the lists of every global interaction now (initialize / explicit reduce_margin),
@@ -807,6 +826,8 @@ void Engine::initialize()
                               hipFuncAttributeMaxDynamicSharedMemorySize, tile_lds), "tile attribute");
     check(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pairs_tiled<false, false>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, tile_lds), "tile attribute");
+    check(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_verlet_tiled), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)((size_t)kTileMaxCap * kBuildBytesPerParticle)), "tile attribute");
     write_topology();
 
     // local tables: one segment per interaction, body slots in interaction/term/body order
@@ -993,7 +1014,9 @@ void Engine::launch_tiled(GlobalIA& g, int gi, bool energy)
     s.nl16 = g.d_nl16; s.spart = g.d_spart; s.cell_start = g.d_cell_start; s.cell_end = g.d_cell_end;
     s.grid = g.d_grid; s.ctl = d_lctl_ + gi; s.cap_nbr = g.tile_cap;
     const bool mixed = (precision_ == MJG_FP32_FIXED);
-    s.tpp = g.tpp;
+    // FP64 keeps the gather kernel's lanes per row (its sums in the same order); the mixed sums are
+    // exact, so its lanes per row are chosen for the tile alone
+    s.tpp = mixed ? tile_lanes(g) : g.tpp;
     const size_t lds = (size_t)g.tile_cap * (mixed ? kTileBytesPerParticleF32 : kTileBytesPerParticle);
     const dim3 grid(g.grid.ncell), block(kTileThreads);
     if(mixed)
@@ -1006,6 +1029,19 @@ void Engine::launch_tiled(GlobalIA& g, int gi, bool energy)
         if(energy) {hipLaunchKernelGGL((k_pairs_tiled<false, true>), grid, block, lds, stream_, d_pos_, box_, s);}
         else {hipLaunchKernelGGL((k_pairs_tiled<false, false>), grid, block, lds, stream_, d_pos_, box_, s);}
     }
+}
+
+// Lanes per list row of the mixed cell-tile kernel: MJG_TILE_TPP (a power of two up to 64) or half
+// the gather kernel's (about twelve entries a lane for the longest row; measured on a 262,144-particle
+// LJ fluid, 8 and 16 lanes per row ran 10 % faster than 4 or 32).
+uint32_t Engine::tile_lanes(const GlobalIA& g) const
+{
+    if(const char* v = std::getenv("MJG_TILE_TPP"))
+    {
+        const long t = std::atol(v);
+        if(t >= 1 && t <= 64 && (t & (t - 1)) == 0) {return (uint32_t)t;}
+    }
+    return std::max<uint32_t>(1, g.tpp / 2);
 }
 
 // After lists were built: a tiled interaction whose largest neighbourhood outgrew its staging

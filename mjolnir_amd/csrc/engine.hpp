@@ -143,11 +143,13 @@ class Engine
     void alloc_list(GlobalIA& g, uint32_t cap);
     void launch_rebuilds(int force);
     void build_lists_now(bool all);
+    ListSeg list_seg(GlobalIA& g, size_t gi);
     ListSegTable list_table();
     PairSegTable pair_table(int only);
     void launch_multibody(const GlobalIA& g, bool energy);
     void launch_tiled(GlobalIA& g, int gi, bool energy);
     bool grow_tiles();
+    uint32_t tile_lanes(const GlobalIA& g) const;
     void forces(double4* f, bool energy, int only, bool kick, bool advance = false);
     void launch_ul_accel(bool kick);
     void launch_step();

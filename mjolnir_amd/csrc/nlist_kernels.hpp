@@ -257,56 +257,103 @@ __device__ __forceinline__ uint64_t group_mask_of(uint32_t tpp, uint32_t lane)
     return m << (lane & ~(tpp - 1u));
 }
 
+// One candidate j of row i (position ri) for one lane of the row's group: accepted partners are
+// appended with a ballot, a candidate's position being the number of accepted candidates before it
+// (in lane order), so the row's order is the candidates' order.  slot16 is the candidate's staging
+// slot (nl16).
+struct RowBuild
+{
+    uint32_t i;
+    double4 ri;
+    uint64_t gmask, below;
+    uint32_t count;
+    uint32_t* row;
+    uint16_t* row16;
+};
+
+__device__ __forceinline__ void verlet_consider(const ListSeg& s, const Box& box, RowBuild& rb, uint32_t j, bool valid,
+                                                uint32_t slot16, const double4& rj)
+{
+    const uint32_t i = rb.i;
+    bool accept = false;
+    if(valid && j != i)
+    {
+        // Both tests must pass and neither has side effects, so the distance (operands in
+        // registers) goes first and only the candidates within r_list read the exclusion and
+        // participant tables from global memory.
+        bool near = true;
+        if(s.rule.use_distance)
+        {
+            // the reference measures the pair from its lower index (same operands, same order)
+            const double4 a = (i < j) ? rb.ri : rj, b = (i < j) ? rj : rb.ri;
+            double ddx, ddy, ddz;
+            adjust_direction(box, a.x, a.y, a.z, b.x, b.y, b.z, ddx, ddy, ddz);
+            near = length_sq(ddx, ddy, ddz) < s.rule.r_list2;
+        }
+        if(near)
+        {
+            const uint32_t lo = (i < j) ? i : j, hi = (i < j) ? j : i;
+            accept = pair_allowed(lo, hi, s.rule, s.pslot, s.mol, s.ex_start, s.ex);
+        }
+    }
+    const uint64_t b = __ballot(accept) & rb.gmask;
+    if(accept)
+    {
+        const uint32_t k = rb.count + (uint32_t)__popcll(b & rb.below);
+        if(k < s.cap)
+        {
+            rb.row[k] = j;
+            if(rb.row16) {rb.row16[k] = (uint16_t)slot16;}
+        }
+    }
+    rb.count += (uint32_t)__popcll(b);
+}
+
+// Block-wide maxima of two per-thread values, then one atomicMax each into *pa and *pb (pb may be
+// null): a build's longest row and largest neighbourhood, which every row would otherwise send to
+// the same two words (hundreds of thousands of atomics on one address, serialised in L2).  Every
+// thread of the block calls it.
+__device__ __forceinline__ void block_max2_to(uint32_t a, uint32_t b, unsigned int* pa, unsigned int* pb)
+{
+    __shared__ uint32_t wa[16], wb[16];
+    for(int o = kWave / 2; o > 0; o >>= 1)
+    {
+        a = max(a, (uint32_t)__shfl_xor((int)a, o, kWave));
+        b = max(b, (uint32_t)__shfl_xor((int)b, o, kWave));
+    }
+    const uint32_t w = threadIdx.x / kWave, nw = (blockDim.x + kWave - 1) / kWave;
+    if(threadIdx.x % kWave == 0) {wa[w] = a; wb[w] = b;}
+    __syncthreads();
+    if(threadIdx.x == 0)
+    {
+        for(uint32_t k = 1; k < nw; ++k) {a = max(a, wa[k]); b = max(b, wb[k]);}
+        if(a > 0) {atomicMax(pa, a);}
+        if(pb && b > 0) {atomicMax(pb, b);}
+    }
+}
+
 // One row (sorted slot t) of the list, built by the tpp lanes of one group (lane l of the group).
+// The row's length and neighbourhood size are folded into the thread's maxima mc and mn.
 __device__ __forceinline__ void verlet_row(const ListSeg& s, uint32_t t, uint32_t l, const double4* __restrict__ pos,
-                                           const Box& box)
+                                           const Box& box, uint32_t& mc, uint32_t& mn)
 {
     const uint32_t lane = threadIdx.x % kWave;
-    const uint64_t gmask = group_mask_of(s.tpp, lane);
-    const uint64_t below = gmask & ((1ull << lane) - 1ull);
     uint32_t p, i;
     if(s.allpairs) {p = t;} else {p = s.svals[t];}
     i = s.part[p];
     if(l == 0) {s.order[t] = p; s.spart[t] = i;}
-    const double4 ri = pos[i];
-    uint32_t count = 0;
-    uint32_t* row = s.nl + (size_t)t * s.cap;
-    uint16_t* row16 = s.nl16 ? s.nl16 + (size_t)t * s.cap : nullptr;
+    RowBuild rb;
+    rb.i = i;
+    rb.ri = pos[i];
+    rb.gmask = group_mask_of(s.tpp, lane);
+    rb.below = rb.gmask & ((1ull << lane) - 1ull);
+    rb.count = 0;
+    rb.row = s.nl + (size_t)t * s.cap;
+    rb.row16 = s.nl16 ? s.nl16 + (size_t)t * s.cap : nullptr;
+    const double4 ri = rb.ri;
     uint32_t nbr = 0;   // staging slot of the next candidate cell's first particle (27-cell order)
-
     auto consider = [&](uint32_t j, bool valid, uint32_t slot16, const double4& rj) {
-        bool accept = false;
-        if(valid && j != i)
-        {
-            const uint32_t lo = (i < j) ? i : j, hi = (i < j) ? j : i;
-            if(pair_allowed(lo, hi, s.rule, s.pslot, s.mol, s.ex_start, s.ex))
-            {
-                if(s.rule.use_distance)
-                {
-                    // the reference measures the pair from its lower index (same operands, same
-                    // order); the row's own position is in registers
-                    const double4 a = (i < j) ? ri : rj, b = (i < j) ? rj : ri;
-                    double ddx, ddy, ddz;
-                    adjust_direction(box, a.x, a.y, a.z, b.x, b.y, b.z, ddx, ddy, ddz);
-                    accept = length_sq(ddx, ddy, ddz) < s.rule.r_list2;
-                }
-                else
-                {
-                    accept = true;
-                }
-            }
-        }
-        const uint64_t b = __ballot(accept) & gmask;
-        if(accept)
-        {
-            const uint32_t k = count + (uint32_t)__popcll(b & below);
-            if(k < s.cap)
-            {
-                row[k] = j;
-                if(row16) {row16[k] = (uint16_t)slot16;}
-            }
-        }
-        count += (uint32_t)__popcll(b);
+        verlet_consider(s, box, rb, j, valid, slot16, rj);
     };
 
     if(s.allpairs)
@@ -354,12 +401,9 @@ __device__ __forceinline__ void verlet_row(const ListSeg& s, uint32_t t, uint32_
         }
     }
     (void)ri;
-    if(l == 0)
-    {
-        s.nn[t] = (count < s.cap) ? count : s.cap;
-        atomicMax(&s.ctl->maxcount, count);
-        if(row16) {atomicMax(&s.ctl->maxnbr, nbr);}
-    }
+    if(l == 0) {s.nn[t] = (rb.count < s.cap) ? rb.count : s.cap;}
+    mc = max(mc, rb.count);
+    mn = max(mn, nbr);
 }
 
 // At a cell-list rebuild, after the sort: the particle and the position of every sorted slot (a
@@ -389,10 +433,12 @@ k_verlet(const double4* __restrict__ pos, Box box, ListSegTable tab)
     const ListSeg& s = tab.seg[si];
     if(s.ctl->rebuild == 0) {return;}
     const uint32_t stride = s.nblocks * blockDim.x;
+    uint32_t mc = 0, mn = 0;
     for(uint32_t gt = (blockIdx.x - s.block_begin) * blockDim.x + threadIdx.x; gt / s.tpp < s.np; gt += stride)
     {
-        verlet_row(s, gt / s.tpp, gt % s.tpp, pos, box);
+        verlet_row(s, gt / s.tpp, gt % s.tpp, pos, box, mc, mn);
     }
+    block_max2_to(mc, mn, &s.ctl->maxcount, s.nl16 ? &s.ctl->maxnbr : nullptr);
 }
 
 // ------------------------------------------------------------------------------------

@@ -45,6 +45,8 @@ struct StagePar
 
 // LDS bytes per staged particle: x, y, z, two parameter columns, particle index
 constexpr size_t kTileBytesPerParticle = 5 * sizeof(double) + sizeof(uint32_t);
+// staged particles per thread whose loads are issued together
+constexpr int kTileStage = 4;
 
 template<bool MIXED, bool ENERGY, int KIND>
 __device__ __forceinline__ void tile_rows(const TileSeg& s, const Box& box, uint32_t cs, uint32_t nhome, uint32_t home0,
@@ -196,8 +198,9 @@ __device__ __forceinline__ void tile_rows_global(const TileSeg& s, const double4
 
 // The 27 neighbour cells of `cell` (m = 9 (dz + 1) + 3 (dy + 1) + (dx + 1), wrapped as k_verlet wraps
 // them): first sorted slot and staging offsets (c_off[27] = neighbourhood size).
-__device__ __forceinline__ void tile_neighbourhood(const TileSeg& s, const Grid& g, uint32_t cell, uint32_t* c_beg,
-                                                   uint32_t* c_off)
+__device__ __forceinline__ void tile_neighbourhood(const uint32_t* __restrict__ cell_start,
+                                                   const uint32_t* __restrict__ cell_end, const Grid& g, uint32_t cell,
+                                                   uint32_t* c_beg, uint32_t* c_off)
 {
     const uint32_t tid = threadIdx.x;
     if(tid < 27)
@@ -211,7 +214,7 @@ __device__ __forceinline__ void tile_neighbourhood(const TileSeg& s, const Grid&
         y = (y < 0) ? g.dim[1] - 1 : (y >= g.dim[1] ? 0 : y);
         z = (z < 0) ? g.dim[2] - 1 : (z >= g.dim[2] ? 0 : z);
         const uint32_t c2 = (uint32_t)x + (uint32_t)g.dim[0] * ((uint32_t)y + (uint32_t)g.dim[1] * (uint32_t)z);
-        const uint32_t b = s.cell_start[c2], en = s.cell_end[c2];
+        const uint32_t b = cell_start[c2], en = cell_end[c2];
         c_beg[tid] = b;
         c_off[tid + 1] = (en > b) ? en - b : 0u;
     }
@@ -249,7 +252,7 @@ __device__ __forceinline__ void tile_cell(const TileSeg& s, const double4* __res
         return;
     }
     const Grid g = *s.grid;
-    tile_neighbourhood(s, g, cell, c_beg, c_off);
+    tile_neighbourhood(s.cell_start, s.cell_end, g, cell, c_beg, c_off);
     const uint32_t cap = s.cap_nbr;
     double* sx = reinterpret_cast<double*>(smem);
     double* sy = sx + cap;
@@ -258,18 +261,32 @@ __device__ __forceinline__ void tile_cell(const TileSeg& s, const double4* __res
     double* s1 = s0 + cap;
     uint32_t* sid = reinterpret_cast<uint32_t*>(s1 + cap);
     const int npar = s.ps.pp.npar;
-    // every thread stages the particles k = tid, tid + 256, ... of the neighbourhood: its cell m is
-    // found among the 27 offsets (binary search), so all the loads of a round are in flight at once
+    // every thread stages the particles k = tid, tid + 256, ... of the neighbourhood, kTileStage
+    // at a time: their cells m are found among the 27 offsets (binary search), then all their index
+    // loads, then all their position and parameter loads are in flight at once
     const uint32_t total = c_off[27];
-    for(uint32_t k = tid; k < total; k += kTileThreads)
+    for(uint32_t k0 = tid; k0 < total; k0 += kTileStage * kTileThreads)
     {
-        const uint32_t m = tile_cell_of(c_off, k);
-        const uint32_t i = s.spart[c_beg[m] + (k - c_off[m])];
-        const double4 r = pos[i];
-        sx[k] = r.x; sy[k] = r.y; sz[k] = r.z;
-        s0[k] = s.ps.ppar[(size_t)i * npar];
-        s1[k] = (npar > 1) ? s.ps.ppar[(size_t)i * npar + 1] : 0.0;
-        sid[k] = i;
+        uint32_t id[kTileStage];
+#pragma unroll
+        for(int q = 0; q < kTileStage; ++q)
+        {
+            const uint32_t k = k0 + q * kTileThreads;
+            id[q] = 0;
+            if(k < total) {const uint32_t m = tile_cell_of(c_off, k); id[q] = s.spart[c_beg[m] + (k - c_off[m])];}
+        }
+#pragma unroll
+        for(int q = 0; q < kTileStage; ++q)
+        {
+            const uint32_t k = k0 + q * kTileThreads;
+            if(k >= total) {break;}
+            const uint32_t i = id[q];
+            const double4 r = pos[i];
+            sx[k] = r.x; sy[k] = r.y; sz[k] = r.z;
+            s0[k] = s.ps.ppar[(size_t)i * npar];
+            s1[k] = (npar > 1) ? s.ps.ppar[(size_t)i * npar + 1] : 0.0;
+            sid[k] = i;
+        }
     }
     __syncthreads();
     tile_rows<MIXED, ENERGY, KIND>(s, box, cs, nhome, c_off[13], sx, sy, sz, StagePar{s0, s1}, sid);
@@ -284,10 +301,13 @@ __device__ __forceinline__ void tile_cell(const TileSeg& s, const double4* __res
 // taken on that r^2 in FP32 where it is clear by more than kTileBand (relative); a pair inside the
 // band is decided by the FP64 path's own test (adjust_direction on the FP64 positions and the FP64
 // parameters), so the pair set inside the cutoff stays the FP64 path's.  The staged parameters are
-// FP32 (column 0 with the hi part, column 1 with the lo part).
+// FP32 (column 0 with the hi part, column 1 with the lo part).  No particle index is staged: a
+// row's own index comes from the cell-sorted order, a neighbour's only for a pair inside the band
+// (through its cell, as staging found it), and each row adds half of every pair energy, so a
+// pair's energy is counted once over its two rows.
 // ------------------------------------------------------------------------------------
 constexpr float kTileBand = 1.0e-5f;          // the FP32 r^2 and threshold errors are below 2e-6 relative
-constexpr size_t kTileBytesPerParticleF32 = 2 * sizeof(float4) + sizeof(uint32_t);
+constexpr size_t kTileBytesPerParticleF32 = 2 * sizeof(float4);
 
 // the FP64 path's inside-cutoff decision of the pair (i, j) (pair_term_f32's)
 template<int KIND>
@@ -316,10 +336,18 @@ __device__ __noinline__ bool tile_inside_fp64(const PairParams& pp, const double
     return pp.abs_cutoff * pp.abs_cutoff > l_sq;
 }
 
+// the particle index of staging slot k (its cell found among the 27 offsets)
+__device__ __forceinline__ uint32_t tile_slot_particle(const TileSeg& s, const uint32_t* c_beg, const uint32_t* c_off,
+                                                       uint32_t k)
+{
+    const uint32_t m = tile_cell_of(c_off, k);
+    return s.spart[c_beg[m] + (k - c_off[m])];
+}
+
 template<bool ENERGY, int KIND>
 __device__ __forceinline__ void tile_rows_f32(const TileSeg& s, const double4* __restrict__ pos, const Box& box,
                                               uint32_t cs, uint32_t nhome, uint32_t home0, const float4* hi,
-                                              const float4* lo, const uint32_t* sid)
+                                              const float4* lo, const uint32_t* c_beg, const uint32_t* c_off)
 {
 #pragma clang fp contract(fast)
     const PairSeg& ps = s.ps;
@@ -331,7 +359,7 @@ __device__ __forceinline__ void tile_rows_f32(const TileSeg& s, const double4* _
     {
         const uint32_t h = u / tpp, l = u % tpp;
         const uint32_t t = cs + h, ki = home0 + h;
-        const uint32_t i = sid[ki];
+        const uint32_t i = s.spart[t];
         const float4 ah = hi[ki], al = lo[ki];
         const uint32_t cnt = ps.nn[t];
         const uint16_t* row = s.nl16 + (size_t)t * ps.cap;
@@ -371,8 +399,11 @@ __device__ __forceinline__ void tile_rows_f32(const TileSeg& s, const double4* _
                 }
                 else {thr = acut2;}
                 if(l_sq > thr * (1.0f + kTileBand)) {continue;}
-                const uint32_t j = (ENERGY || l_sq >= thr * (1.0f - kTileBand)) ? sid[kj[m]] : 0u;
-                if(l_sq >= thr * (1.0f - kTileBand) && !tile_inside_fp64<KIND>(ps.pp, ps.ppar, pos, box, i, j)) {continue;}
+                if(l_sq >= thr * (1.0f - kTileBand) &&
+                   !tile_inside_fp64<KIND>(ps.pp, ps.ppar, pos, box, i, tile_slot_particle(s, c_beg, c_off, kj[m])))
+                {
+                    continue;
+                }
                 float c;
                 if(KIND == kPairLJ)
                 {
@@ -382,7 +413,7 @@ __device__ __forceinline__ void tile_rows_f32(const TileSeg& s, const double4* _
                     const float s2l2 = sg * sg * rcp;
                     const float s6 = s2l2 * s2l2 * s2l2;
                     c = 24.0f * eps * (s6 - 2.0f * s6 * s6) * rcp;
-                    if(ENERGY && i < j) {e += (double)(4.0f * eps * (s6 * s6 - s6 - K.cac));}
+                    if(ENERGY) {e += 0.5 * (double)(4.0f * eps * (s6 * s6 - s6 - K.cac));}
                 }
                 else if(KIND == kPairEV)
                 {
@@ -390,7 +421,7 @@ __device__ __forceinline__ void tile_rows_f32(const TileSeg& s, const double4* _
                     const float s2l2 = sg * sg * rcp;
                     const float s6 = s2l2 * s2l2 * s2l2;
                     c = -(K.eps * 12.0f) * s6 * s6 * rcp;
-                    if(ENERGY && i < j) {e += (double)(K.eps * (s6 * s6 - K.cac));}
+                    if(ENERGY) {e += 0.5 * (double)(K.eps * (s6 * s6 - K.cac));}
                 }
                 else if(KIND == kPair3SPN2EV)
                 {
@@ -399,7 +430,7 @@ __device__ __forceinline__ void tile_rows_f32(const TileSeg& s, const double4* _
                     const float r3s3 = r1s1 * r1s1 * r1s1;
                     const float r6s6 = r3s3 * r3s3;
                     c = 12.0f * K.eps * rl * r6s6 * (1.0f - r6s6) * rl;
-                    if(ENERGY && i < j) {e += (double)(K.eps * (r6s6 * (r6s6 - 2.0f) + 1.0f));}
+                    if(ENERGY) {e += 0.5 * (double)(K.eps * (r6s6 * (r6s6 - 2.0f) + 1.0f));}
                 }
                 else
                 {
@@ -409,7 +440,7 @@ __device__ __forceinline__ void tile_rows_f32(const TileSeg& s, const double4* _
                     const float ex = RMath<float>::exp(-ll * K.idebye);
                     const float f_mag = -qq * (K.debye + ll) * K.idebye * (rl * rl) * ex;
                     c = f_mag * rl;
-                    if(ENERGY && i < j) {e += (double)(qq * (ex * rl - K.cac));}
+                    if(ENERGY) {e += 0.5 * (double)(qq * (ex * rl - K.cac));}
                 }
                 long long gx, gy, gz;
                 fix3r<float>(dx * c, dy * c, dz * c, gx, gy, gz);
@@ -452,11 +483,10 @@ __device__ __forceinline__ void tile_cell_f32(const TileSeg& s, const double4* _
         return;
     }
     const Grid g = *s.grid;
-    tile_neighbourhood(s, g, cell, c_beg, c_off);
+    tile_neighbourhood(s.cell_start, s.cell_end, g, cell, c_beg, c_off);
     const uint32_t cap = s.cap_nbr;
     float4* hi = reinterpret_cast<float4*>(smem);
     float4* lo = hi + cap;
-    uint32_t* sid = reinterpret_cast<uint32_t*>(lo + cap);
     // the home cell's centre and cell widths
     const int cx = (int)(cell % (uint32_t)g.dim[0]);
     const int cy = (int)((cell / (uint32_t)g.dim[0]) % (uint32_t)g.dim[1]);
@@ -468,32 +498,137 @@ __device__ __forceinline__ void tile_cell_f32(const TileSeg& s, const double4* _
     const double ilx = 1.0 / box.width[0], ily = 1.0 / box.width[1], ilz = 1.0 / box.width[2];
     const int npar = s.ps.pp.npar;
     const uint32_t total = c_off[27];
-    for(uint32_t k = tid; k < total; k += kTileThreads)
+    for(uint32_t k0 = tid; k0 < total; k0 += kTileStage * kTileThreads)
     {
-        const uint32_t m = tile_cell_of(c_off, k);
-        const uint32_t i = s.spart[c_beg[m] + (k - c_off[m])];
-        const double4 r = pos[i];
-        // The particle sat in neighbour cell m at the last build and has moved less than half the
-        // skin since; positions are wrapped into the box every step, so it may have crossed the
-        // boundary after the build.  Its image nearest to cell m's centre, seen from the home cell
-        // (offset (dx, dy, dz) cell widths), is the one within half a cell plus the drift of it:
-        // fewer than half a box width, since a periodic grid has at least three cells a dimension.
-        const double ex = ((double)((int)(m % 3) - 1)) * wx;
-        const double ey = ((double)((int)((m / 3) % 3) - 1)) * wy;
-        const double ez = ((double)((int)(m / 9) - 1)) * wz;
-        double X = r.x - ox, Y = r.y - oy, Z = r.z - oz;
-        X -= box.width[0] * rint((X - ex) * ilx);
-        Y -= box.width[1] * rint((Y - ey) * ily);
-        Z -= box.width[2] * rint((Z - ez) * ilz);
-        const float hx = (float)X, hy = (float)Y, hz = (float)Z;
-        const float p0 = (float)s.ps.ppar[(size_t)i * npar];
-        const float p1 = (npar > 1) ? (float)s.ps.ppar[(size_t)i * npar + 1] : 0.0f;
-        hi[k] = make_float4(hx, hy, hz, p0);
-        lo[k] = make_float4((float)(X - (double)hx), (float)(Y - (double)hy), (float)(Z - (double)hz), p1);
-        sid[k] = i;
+        uint32_t id[kTileStage], mm[kTileStage];
+#pragma unroll
+        for(int q = 0; q < kTileStage; ++q)
+        {
+            const uint32_t k = k0 + q * kTileThreads;
+            id[q] = 0; mm[q] = 13;
+            if(k < total) {const uint32_t m = tile_cell_of(c_off, k); mm[q] = m; id[q] = s.spart[c_beg[m] + (k - c_off[m])];}
+        }
+#pragma unroll
+        for(int q = 0; q < kTileStage; ++q)
+        {
+            const uint32_t k = k0 + q * kTileThreads;
+            if(k >= total) {break;}
+            const uint32_t i = id[q], m = mm[q];
+            const double4 r = pos[i];
+            // The particle sat in neighbour cell m at the last build and has moved less than half the
+            // skin since; positions are wrapped into the box every step, so it may have crossed the
+            // boundary after the build.  Its image nearest to cell m's centre, seen from the home cell
+            // (offset (dx, dy, dz) cell widths), is the one within half a cell plus the drift of it:
+            // fewer than half a box width, since a periodic grid has at least three cells a dimension.
+            const double ex = ((double)((int)(m % 3) - 1)) * wx;
+            const double ey = ((double)((int)((m / 3) % 3) - 1)) * wy;
+            const double ez = ((double)((int)(m / 9) - 1)) * wz;
+            double X = r.x - ox, Y = r.y - oy, Z = r.z - oz;
+            X -= box.width[0] * rint((X - ex) * ilx);
+            Y -= box.width[1] * rint((Y - ey) * ily);
+            Z -= box.width[2] * rint((Z - ez) * ilz);
+            const float hx = (float)X, hy = (float)Y, hz = (float)Z;
+            const float p0 = (float)s.ps.ppar[(size_t)i * npar];
+            const float p1 = (npar > 1) ? (float)s.ps.ppar[(size_t)i * npar + 1] : 0.0f;
+            hi[k] = make_float4(hx, hy, hz, p0);
+            lo[k] = make_float4((float)(X - (double)hx), (float)(Y - (double)hy), (float)(Z - (double)hz), p1);
+        }
     }
     __syncthreads();
-    tile_rows_f32<ENERGY, KIND>(s, pos, box, cs, nhome, c_off[13], hi, lo, sid);
+    tile_rows_f32<ENERGY, KIND>(s, pos, box, cs, nhome, c_off[13], hi, lo, c_beg, c_off);
+}
+
+// ------------------------------------------------------------------------------------
+// Cell-tiled list build for the interactions the cell-tile kernel serves.  A workgroup takes a
+// cell at a time (a grid-stride loop over the cells, at most kVerletMaxBlocks workgroups, so that
+// the launch a step makes when no list is rebuilt stays small), stages the slot-ordered positions
+// (k_sorted_slots) and particles of its 27-cell neighbourhood in LDS in k_verlet's order, and builds
+// the cell's rows from there: each candidate is loaded once per cell instead of once per row.  The
+// candidates are tested in k_verlet's order by its own test (verlet_consider), so every row is
+// k_verlet's, entry for entry, whatever the lanes per row.  A neighbourhood larger than the staging
+// capacity builds its rows as k_verlet does.
+// ------------------------------------------------------------------------------------
+constexpr size_t kBuildBytesPerParticle = 3 * sizeof(double) + sizeof(uint32_t);
+
+__global__ void __launch_bounds__(kTileThreads)
+k_verlet_tiled(const double4* __restrict__ pos, Box box, ListSeg s, uint32_t cap_nbr)
+{
+    if(s.ctl->rebuild == 0) {return;}
+    extern __shared__ __align__(16) unsigned char tile_smem[];
+    __shared__ uint32_t c_beg[27], c_off[28];
+    double* sx = reinterpret_cast<double*>(tile_smem);
+    double* sy = sx + cap_nbr;
+    double* sz = sy + cap_nbr;
+    uint32_t* sj = reinterpret_cast<uint32_t*>(sz + cap_nbr);
+    const Grid g = *s.grid;
+    const uint32_t tid = threadIdx.x, lane = tid % kWave;
+    const uint32_t tpp = s.tpp;
+    uint32_t mc = 0, mn = 0;                                 // longest row, largest neighbourhood
+    for(uint32_t cell = blockIdx.x; cell < g.ncell; cell += gridDim.x)
+    {
+        const uint32_t cs = s.cell_start[cell], ce = s.cell_end[cell];
+        if(ce <= cs) {continue;}                             // uniform over the workgroup
+        const uint32_t nhome = ce - cs;
+        tile_neighbourhood(s.cell_start, s.cell_end, g, cell, c_beg, c_off);
+        const uint32_t total = c_off[27];
+        if(total > cap_nbr)
+        {
+            for(uint32_t u = tid; u < nhome * tpp; u += kTileThreads)
+            {
+                verlet_row(s, cs + u / tpp, u % tpp, pos, box, mc, mn);
+            }
+        }
+        else
+        {
+            for(uint32_t k0 = tid; k0 < total; k0 += kTileStage * kTileThreads)
+            {
+                uint32_t q[kTileStage];
+#pragma unroll
+                for(int r = 0; r < kTileStage; ++r)
+                {
+                    const uint32_t k = k0 + r * kTileThreads;
+                    q[r] = 0;
+                    if(k < total) {const uint32_t m = tile_cell_of(c_off, k); q[r] = c_beg[m] + (k - c_off[m]);}
+                }
+#pragma unroll
+                for(int r = 0; r < kTileStage; ++r)
+                {
+                    const uint32_t k = k0 + r * kTileThreads;
+                    if(k >= total) {break;}
+                    const double4 rj = s.spos[q[r]];
+                    sx[k] = rj.x; sy[k] = rj.y; sz[k] = rj.z;
+                    sj[k] = s.spart[q[r]];
+                }
+            }
+            __syncthreads();
+            for(uint32_t u = tid; u < nhome * tpp; u += kTileThreads)
+            {
+                const uint32_t h = u / tpp, l = u % tpp;
+                const uint32_t t = cs + h, ki = c_off[13] + h;
+                if(l == 0) {s.order[t] = s.svals[t];}
+                RowBuild rb;
+                rb.i = sj[ki];
+                rb.ri = make_double4(sx[ki], sy[ki], sz[ki], 0.0);
+                rb.gmask = group_mask_of(tpp, lane);
+                rb.below = rb.gmask & ((1ull << lane) - 1ull);
+                rb.count = 0;
+                rb.row = s.nl + (size_t)t * s.cap;
+                rb.row16 = s.nl16 + (size_t)t * s.cap;
+                for(uint32_t base = 0; base < total; base += tpp)
+                {
+                    const uint32_t k = base + l;
+                    const bool v = k < total;
+                    verlet_consider(s, box, rb, v ? sj[k] : 0u, v, k,
+                                    v ? make_double4(sx[k], sy[k], sz[k], 0.0) : rb.ri);
+                }
+                if(l == 0) {s.nn[t] = (rb.count < s.cap) ? rb.count : s.cap;}
+                mc = max(mc, rb.count);
+                mn = max(mn, total);
+            }
+        }
+        __syncthreads();                                     // the next cell's staging overwrites this one's
+    }
+    block_max2_to(mc, mn, &s.ctl->maxcount, &s.ctl->maxnbr);
 }
 
 template<bool MIXED, bool ENERGY>

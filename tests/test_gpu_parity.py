@@ -374,8 +374,10 @@ def test_cell_tile_kernel_fp64_equals_the_gather_kernel(monkeypatch):
     """The FP64 cell-tile pair kernel (periodic cell lists of >= 16384 participants: one workgroup
     per cell, the 27-cell neighbourhood staged in LDS) against the full-list gather kernel on a
     32,768-particle LJ fluid: forces and energies bit for bit (the same lanes per row and sum
-    order), and 300 Langevin steps through list rebuilds bit for bit; the same with a staging
-    capacity too small for any neighbourhood, which serves every build from global memory."""
+    order), and 300 Langevin steps through list rebuilds bit for bit (the tiled engine's lists come
+    from the cell-tiled build, the gather engine's from k_verlet, so this also holds the two builds
+    to the same rows entry for entry); the same with a staging capacity too small for any
+    neighbourhood, which serves every pair evaluation and build from global memory."""
     from mjolnir_amd import capi, workloads
     spec = make_golden.perturb(workloads.lj_fluid(n_side=32, seed=41), 0.1, 42)
     ref = _tile_engine(spec, capi.FP64, monkeypatch, MJG_NO_TILE="1")
@@ -467,8 +469,9 @@ def test_cell_tile_mixed_cutoff_decision_is_the_fp64_one(delta, monkeypatch):
 
 @pytest.mark.parametrize("name", [n for n in CASES if n.startswith("lj")])
 def test_cell_tile_kernel_vs_reference(name, monkeypatch):
-    """The cell-tile kernel switched on for the small periodic golden cases (MJG_TILE_MIN_NP=0):
-    per-particle forces at 1e-10 and energies at 1e-10 against the reference, in FP64; the mixed
+    """The cell-tile kernel and the cell-tiled list build switched on for the small periodic golden
+    cases (MJG_TILE_MIN_NP=0): per-particle forces at 1e-10, energies at 1e-10 and the neighbour
+    sets against the reference, in FP64; the mixed
     path at the mixed bar (RMS <= 1e-5 of the RMS force, over particles within the fixed-point
     range)."""
     from mjolnir_amd import capi
@@ -482,6 +485,13 @@ def test_cell_tile_kernel_vs_reference(name, monkeypatch):
     assert np.all(err <= force_tolerance(ref, 1e-10)), err.max()
     _, tot = e.energies()
     assert abs(tot - float(g["total_energy"])) <= 1e-10 * max(abs(float(g["total_energy"])), 1.0)
+    # the lists come from the cell-tiled build: the reference's pair sets
+    names = [str(x) for x in g["names"]]
+    for k in range(len(names)):
+        if f"npairs_{k}" in g:
+            p = e.pairs(k)
+            assert len(p) == int(g[f"npairs_{k}"]), names[k]
+            assert make_golden.pair_key_hash(p, spec.n) == str(g[f"pairhash_{k}"]), names[k]
     m = _tile_engine(spec, capi.FP32_FIXED, monkeypatch, MJG_TILE_MIN_NP="0")
     m.calc_force()
     f = m.state()[2]
