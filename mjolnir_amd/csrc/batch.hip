@@ -800,10 +800,15 @@ class Batch
             }
             if(G.cap == 0)
             {
-                // first guess: a tenth of all participant pairs, at least 1024; grown on overflow.
-                // MJG_BATCH_INITIAL_CAP overrides the guess (tests use it to exercise the growth path)
+                // Every participant pair: at most 2,096,128 entries (8 MiB) for the largest replica,
+                // which HBM holds for thousands of replicas, so a list never overflows and no
+                // segment runs twice or stops for a host-side re-layout (a chain's Debye-Hueckel
+                // list grows several-fold as it compacts).  Above kRepAllPairsMax entries the first
+                // guess is a tenth of the pairs, grown on overflow.  MJG_BATCH_INITIAL_CAP
+                // overrides the guess (tests use it to exercise the growth path).
                 const uint64_t np = G.part.size();
-                G.cap = (uint32_t)std::min<uint64_t>(np * (np - 1) / 2 + 1, std::max<uint64_t>(1024, np * (np - 1) / 10));
+                const uint64_t all = np * (np - 1) / 2 + 1;
+                G.cap = (uint32_t)(all <= kRepAllPairsMax ? all : std::max<uint64_t>(1024, np * (np - 1) / 10));
                 if(const char* c = std::getenv("MJG_BATCH_INITIAL_CAP")) {G.cap = (uint32_t)std::max(1, std::atoi(c));}
             }
         }
@@ -1064,6 +1069,7 @@ class Batch
 
     // list capacity after an overflow at `seen` entries
     static uint32_t grown_cap(uint32_t seen) {return seen + seen / 2 + 1024;}
+    static constexpr uint64_t kRepAllPairsMax = uint64_t(1) << 21;
 
     template<typename Real, bool ENERGY, int BS, int PPT>
     void set_lds_limit()

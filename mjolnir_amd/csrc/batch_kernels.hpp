@@ -656,6 +656,98 @@ __device__ __forceinline__ void rep_pair_k(const PairK<Real>& K, const Real* par
     sub3(F, j, fx, fy, fz);
 }
 
+// Operands of one pair of a compile-time kind, as the pipelined pair loop reads them from LDS:
+// both positions and the parameters.  They are read one pair ahead, before the current pair's
+// force atomics are issued, so the in-order LDS queue returns them without first draining those
+// atomics (and a pair's parameters no longer wait for its cutoff test).
+template<typename Real> struct PairIn
+{
+    uint32_t i, j;
+    double xi, yi, zi, xj, yj, zj;
+    Real a0, a1, b0, b1;          // LJ: sigma_i, eps_i, sigma_j, eps_j; EV / DH: par[i], -, par[j], -
+};
+
+template<typename Real, int KIND>
+__device__ __forceinline__ PairIn<Real> rep_pair_fetch(const Real* par, const PosL& P, uint32_t word)
+{
+    PairIn<Real> q;
+    const bool v = word != 0xffffffffu;
+    q.i = v ? word >> 16 : 0u;
+    q.j = v ? word & 0xffffu : 0u;
+    q.xi = P.x[q.i]; q.yi = P.y[q.i]; q.zi = P.z[q.i];
+    q.xj = P.x[q.j]; q.yj = P.y[q.j]; q.zj = P.z[q.j];
+    if constexpr(KIND == kPairLJ)
+    {
+        q.a0 = par[2 * q.i]; q.a1 = par[2 * q.i + 1]; q.b0 = par[2 * q.j]; q.b1 = par[2 * q.j + 1];
+    }
+    else
+    {
+        q.a0 = par[q.i]; q.a1 = (Real)0; q.b0 = par[q.j]; q.b1 = (Real)0;
+    }
+    return q;
+}
+
+// rep_pair_k's arithmetic on fetched operands: false for a pair beyond the cutoff, else its
+// fixed-point force on j (the negative on i).
+template<typename Real, bool ENERGY, int KIND, bool PERIODIC>
+__device__ __forceinline__ bool rep_pair_eval(const PairK<Real>& K, const PairIn<Real>& q, const Box& box,
+                                              long long& fx, long long& fy, long long& fz, double& e)
+{
+    double ddx, ddy, ddz;
+    if constexpr(PERIODIC) {adjust_direction(box, q.xi, q.yi, q.zi, q.xj, q.yj, q.zj, ddx, ddy, ddz);}
+    else {ddx = q.xj - q.xi; ddy = q.yj - q.yi; ddz = q.zj - q.zi;}
+    const Real dx = (Real)ddx, dy = (Real)ddy, dz = (Real)ddz;
+    const double l_sq_d = length_sq(ddx, ddy, ddz);
+    const Real l_sq = (Real)l_sq_d;
+    Real c;
+    if constexpr(KIND == kPairLJ)
+    {
+        const Real s1 = q.a0, e1 = q.a1, s2 = q.b0, e2 = q.b1;
+        const double sigma_d = ((double)s1 + (double)s2) / 2;
+        const Real eps = (e1 == e2) ? e1 : RMath<Real>::sqrt(e1 * e2);
+        const double sigma_sq_d = sigma_d * sigma_d;
+        if(sigma_sq_d * K.cut2d < l_sq_d) {return false;}
+        const Real sigma_sq = (Real)sigma_sq_d;
+        const Real rcp = RMath<Real>::rcp(l_sq);
+        const Real s2l2 = sigma_sq * rcp;
+        const Real s6 = s2l2 * s2l2 * s2l2;
+        c = (Real)24 * eps * (s6 - (Real)2 * s6 * s6) * rcp;
+        if(ENERGY) {e += (double)((Real)4 * eps * (s6 * s6 - s6 - K.cac));}
+    }
+    else if constexpr(KIND == kPairEV)
+    {
+        const double radius_d = (double)q.a0 + (double)q.b0;
+        const double sigma_sq_d = radius_d * radius_d;
+        if(sigma_sq_d * K.cut2d < l_sq_d) {return false;}
+        const Real sigma_sq = (Real)sigma_sq_d;
+        const Real rcp = RMath<Real>::rcp(l_sq);
+        const Real s2l2 = sigma_sq * rcp;
+        const Real s6 = s2l2 * s2l2 * s2l2;
+        c = -(K.eps * (Real)12) * s6 * s6 * rcp;
+        if(ENERGY) {e += (double)(K.eps * (s6 * s6 - K.cac));}
+    }
+    else
+    {
+        if constexpr(std::is_same<Real, float>::value)
+        {
+            if(K.acut2d <= l_sq_d) {return false;}
+        }
+        const Real qq = K.qk * q.a0 * q.b0;
+        const Real rl = RMath<Real>::rsqrt(l_sq);
+        const Real l = l_sq * rl;
+        if constexpr(!std::is_same<Real, float>::value)
+        {
+            if(K.acut <= l) {return false;}
+        }
+        const Real ex = RMath<Real>::exp(-l * K.idebye);
+        const Real f_mag = -qq * (K.debye + l) * K.idebye * (rl * rl) * ex;
+        c = f_mag * rl;
+        if(ENERGY) {e += (double)(qq * (ex * rl - K.cac));}
+    }
+    fix3r<Real>(dx * c, dy * c, dz * c, fx, fy, fz);
+    return true;
+}
+
 // The pair terms of one interaction: kPairBatch list entries per thread per load round, the
 // next round's entries loaded before the current round is computed (the round's LDS atomics
 // would otherwise hold the loads back).  KIND < 0: any kind, dispatched per pair (3SPN2).
@@ -674,6 +766,47 @@ __device__ __forceinline__ void rep_pair_loop(const RepGlobal& G, const Real* pa
     {
         const uint32_t k = m * BS + tid;
         pk[m] = (k < np) ? pairs[k] : 0xffffffffu;
+    }
+    if constexpr(KIND >= 0)
+    {
+        // Software pipeline over the thread's pairs: the operands of the next pair are read from
+        // LDS one pair ahead and a pair's force atomics go out one pair late, after those reads,
+        // so the in-order LDS queue serves the reads without draining the atomics first.
+        PairIn<Real> cur = rep_pair_fetch<Real, KIND>(par, P, pk[0]);
+        long long ax = 0, ay = 0, az = 0;
+        uint32_t ai = 0, aj = 0;
+        bool pending = false;
+        for(uint32_t base = 0; base < np; base += BS * kPairBatch)
+        {
+            uint32_t nx[kPairBatch];
+#pragma unroll
+            for(int m = 0; m < kPairBatch; ++m)
+            {
+                const uint32_t k = base + BS * kPairBatch + m * BS + tid;
+                nx[m] = (k < np) ? pairs[k] : 0xffffffffu;
+            }
+#pragma unroll
+            for(int m = 0; m < kPairBatch; ++m)
+            {
+                const PairIn<Real> nxt = rep_pair_fetch<Real, KIND>(par, P, (m + 1 < kPairBatch) ? pk[m + 1] : nx[0]);
+                if(pending)
+                {
+                    add3(F, ai, ax, ay, az);
+                    sub3(F, aj, ax, ay, az);
+                }
+                ai = cur.i; aj = cur.j;
+                pending = pk[m] != 0xffffffffu && rep_pair_eval<Real, ENERGY, KIND, PERIODIC>(K, cur, box, ax, ay, az, e_pair);
+                cur = nxt;
+            }
+#pragma unroll
+            for(int m = 0; m < kPairBatch; ++m) {pk[m] = nx[m];}
+        }
+        if(pending)
+        {
+            add3(F, ai, ax, ay, az);
+            sub3(F, aj, ax, ay, az);
+        }
+        return;
     }
     for(uint32_t base = 0; base < np; base += BS * kPairBatch)
     {

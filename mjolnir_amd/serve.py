@@ -72,7 +72,11 @@ class Job:
 
     @property
     def footprint(self) -> int:
-        return self.spec.n * _BYTES_PER_PARTICLE + (_ENGINE_OVERHEAD if self.lane == "engine" else 0)
+        base = self.spec.n * _BYTES_PER_PARTICLE
+        if self.lane == "engine":
+            return base + _ENGINE_OVERHEAD
+        # a batch replica holds each pair list at its bound, every participant pair (4 B an entry)
+        return base + sum(4 * len(g.participants) * (len(g.participants) - 1) // 2 for g in self.spec.global_)
 
 
 def batch_eligible(spec: SystemSpec, max_n: int = 2048) -> bool:
