@@ -311,7 +311,7 @@ constexpr size_t kTileBytesPerParticleF32 = 2 * sizeof(float4);
 
 // the FP64 path's inside-cutoff decision of the pair (i, j) (pair_term_f32's)
 template<int KIND>
-__device__ __noinline__ bool tile_inside_fp64(const PairParams& pp, const double* __restrict__ ppar,
+__device__ __forceinline__ bool tile_inside_fp64(const PairParams& pp, const double* __restrict__ ppar,
                                               const double4* __restrict__ pos, const Box& box, uint32_t i, uint32_t j)
 {
     const double4 ri = pos[i], rj = pos[j];
