@@ -1268,13 +1268,17 @@ class Batch
             if(r.failed) {continue;}
             const RepState& st = h_states_[r.slot];
             bool tight = false;
-            for(size_t g = 0; g < r.globals.size(); ++g)
-            {
-                const uint32_t cap = r.globals[g].cap;
-                tight = tight || (st.maxpairs[g] > cap - cap / 8);
-            }
+            for(size_t g = 0; g < r.globals.size(); ++g) {tight = tight || list_tight(r, g, st);}
             if(tight) {grow_ahead(r);}
         }
+    }
+
+    // A list within an eighth of its capacity, unless it holds every participant pair (which no
+    // build can exceed)
+    static bool list_tight(const Replica& r, size_t g, const RepState& st)
+    {
+        const uint64_t np = r.globals[g].part.size(), cap = r.globals[g].cap;
+        return cap < np * (np - 1) / 2 + 1 && st.maxpairs[g] > cap - cap / 8;
     }
 
     // enlarge the replica's nearly full lists as an overflow at their capacity would, keeping its
@@ -1285,7 +1289,15 @@ class Batch
         for(size_t g = 0; g < r.globals.size(); ++g)
         {
             const uint32_t cap = r.globals[g].cap;
-            if(st.maxpairs[g] > cap - cap / 8) {r.globals[g].cap = grown_cap(std::max(st.maxpairs[g], cap));}
+            if(list_tight(r, g, st))
+            {
+                if(trace_)
+                {
+                    std::fprintf(stderr, "[mjg batch] replica n=%zu: list %zu at %u of %u entries; enlarged\n", r.n, g,
+                                 st.maxpairs[g], cap);
+                }
+                r.globals[g].cap = grown_cap(std::max(st.maxpairs[g], cap));
+            }
         }
         relayout(r, false);
         h_states_[r.slot] = state_of(r);
