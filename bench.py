@@ -113,8 +113,11 @@ def cpu_baseline_single(spec, target_s: float = 12.0):
     threads = host_cores()
     if not refrun.available(omp=True):
         return None
+    print(f"[bench] CPU baseline: probing the reference build on {spec.name} ({threads} threads)", file=sys.stderr,
+          flush=True)
     probe = refrun.bench(spec, 50, threads=threads)
     steps = max(50, int(target_s * probe["particle_steps_per_s"] / spec.n))
+    print(f"[bench] CPU baseline: {steps} steps", file=sys.stderr, flush=True)
     r = refrun.bench(spec, steps, threads=threads, timeout=300)
     return {"value": r["particle_steps_per_s"], "unit": "particle-steps/s", "cores": r["threads"],
             "kind": "reference", "host": host_info(), "build": CPU_NOTE,
@@ -503,7 +506,7 @@ def bench_serve(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=500)
     ap.add_argument("--job-steps", type=int, default=CONFIG4_STEPS,
                     help="replicas512: steps of the config-4 job timed before the warm-up and timed window")

@@ -278,10 +278,11 @@ __device__ __forceinline__ double rep_tsp_pair(int kind, uint32_t i, uint32_t j,
 // ------------------------------------------------------------------------------------
 // local terms (Real arithmetic except the harmonic bond)
 // ------------------------------------------------------------------------------------
-template<typename Real, bool ENERGY, bool TSP>
+template<typename RealK, bool ENERGY, bool TSP>
 __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, double p0, double p1, double p2,
                                           const Box& box, const PosL& P, unsigned long long* F, double& e)
 {
+    using Real = RealK;
     const uint32_t i0 = (uint32_t)(ix & 0xffff), i1 = (uint32_t)((ix >> 16) & 0xffff);
     const uint32_t i2 = (uint32_t)((ix >> 32) & 0xffff), i3 = (uint32_t)(ix >> 48);
     if(kind == kRBondHarmonic)
@@ -322,7 +323,9 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
     }
     else if(kind == kRAngleHarmonic || kind == kRAngleGaussian)
     {
-        // BondAngleInteraction.hpp:129-189; GaussianPotential.hpp:20-38
+        // BondAngleInteraction.hpp:129-189; GaussianPotential.hpp:20-38.  In the 3SPN2 variant in FP64
+        // (the duplex's near-straight angles lose FP32 precision in acos and 1 / sin)
+        using Real = typename std::conditional<TSP, double, RealK>::type;
         double a, b, c;
         disp(box, P, i1, i0, a, b, c);
         const Real ijx = (Real)a, ijy = (Real)b, ijz = (Real)c;
@@ -358,8 +361,8 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         const Real Fkx = ck * (cos_t * ekx - eix), Fky = ck * (cos_t * eky - eiy), Fkz = ck * (cos_t * ekz - eiz);
         // the middle body takes minus the sum of the ends, in integers (exact momentum balance)
         long long ix, iy, iz, kx, ky, kz;
-        fix3r<Real>(Fix_, Fiy, Fiz, ix, iy, iz);
-        fix3r<Real>(Fkx, Fky, Fkz, kx, ky, kz);
+        fix3r<RealK>(Fix_, Fiy, Fiz, ix, iy, iz);
+        fix3r<RealK>(Fkx, Fky, Fkz, kx, ky, kz);
         add3(F, i0, ix, iy, iz);
         sub3(F, i1, ix + kx, iy + ky, iz + kz);
         add3(F, i2, kx, ky, kz);
@@ -428,7 +431,9 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
     }
     else if(kind == kRDihedralPGauss)
     {
-        // DihedralAngleInteraction.hpp:157-227; PeriodicGaussianPotential.hpp:21-44
+        // DihedralAngleInteraction.hpp:157-227; PeriodicGaussianPotential.hpp:21-44 (FP64, as the
+        // 3SPN2 angles)
+        using Real = double;
         double a, b, c;
         disp(box, P, i1, i0, a, b, c);
         const Real ijx = (Real)a, ijy = (Real)b, ijz = (Real)c;
@@ -459,12 +464,12 @@ __device__ __forceinline__ void rep_local(int kind, unsigned long long ix, doubl
         const Real Flx = cl * nx, Fly = cl * ny, Flz = cl * nz;
         const Real cijk = (ijx * kjx + ijy * kjy + ijz * kjz) * r_kj_rlensq;
         const Real cjkl = (klx * kjx + kly * kjy + klz * kjz) * r_kj_rlensq;
-        acc3<Real>(F, i0, Fix, Fiy, Fiz);
-        acc3<Real>(F, i1, (cijk - (Real)1) * Fix - cjkl * Flx, (cijk - (Real)1) * Fiy - cjkl * Fly,
+        acc3<RealK>(F, i0, Fix, Fiy, Fiz);
+        acc3<RealK>(F, i1, (cijk - (Real)1) * Fix - cjkl * Flx, (cijk - (Real)1) * Fiy - cjkl * Fly,
                    (cijk - (Real)1) * Fiz - cjkl * Flz);
-        acc3<Real>(F, i2, (cjkl - (Real)1) * Flx - cijk * Fix, (cjkl - (Real)1) * Fly - cijk * Fiy,
+        acc3<RealK>(F, i2, (cjkl - (Real)1) * Flx - cijk * Fix, (cjkl - (Real)1) * Fly - cijk * Fiy,
                    (cjkl - (Real)1) * Flz - cijk * Fiz);
-        acc3<Real>(F, i3, Flx, Fly, Flz);
+        acc3<RealK>(F, i3, Flx, Fly, Flz);
     }
     else if(kind == kRBaseStacking)
     {
