@@ -142,8 +142,9 @@ def test_list_overflow_is_recovered(monkeypatch):
 def test_thermostat_mixed_precision_kinetic_temperature():
     """Kinetic temperature of 8 + 8 LJ-fluid replicas (216 particles, gamma = 0.5) at two
     temperatures, pooled per temperature.  The FP64 oracle gives 0.998 for this set-up (BAOAB's
-    kinetic-temperature bias at dt = 0.01 is below 0.5 % here; a CG chain with stiff bonds shows
-    several per cent at the same dt in either precision, so it cannot test the thermostat).
+    kinetic-temperature bias at dt = 0.01 is below 0.5 % here).  The config-4 chains, which sit a
+    few per cent above 1 in either precision, are compared mixed against FP64 in
+    test_gpu_thermostat.py.
     The kinetic energy of 216 particles fluctuates by sqrt(2 / (3 N)) = 5.5 % and decorrelates
     within about 100 steps; 160 samples 50 steps apart from 8 replicas give a pooled mean with a
     spread near 0.3 %; the bar is 2 %."""
