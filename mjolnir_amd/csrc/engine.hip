@@ -743,19 +743,17 @@ void Engine::launch_rebuilds(int force)
                 hipLaunchKernelGGL(k_grid_open, dim3(1), dim3(1), 0, stream_, ctl, g.d_grid, g.r_list);
             }
             hipLaunchKernelGGL(k_cellkey, dim3(blocks_for(np, 256)), dim3(256), 0, stream_, d_pos_, g.d_part, np,
-                               g.d_grid, g.sb.keys, g.sb.vals, ctl);
+                               g.d_grid, g.sb.keys, g.sb.vals, g.d_cell_start, g.d_cell_end, ctl);
             radix_sort(g.sb, g.key_bits, &ctl->rebuild, stream_);
-            const uint32_t ncell_launch = box_.periodic ? g.grid.ncell : kMaxCells;
-            hipLaunchKernelGGL(k_cell_clear, dim3(blocks_for(ncell_launch, 256)), dim3(256), 0, stream_,
-                               g.d_cell_start, g.d_cell_end, g.d_grid, ctl);
             hipLaunchKernelGGL(k_cell_range, dim3(blocks_for(np, 256)), dim3(256), 0, stream_, g.sb.keys, np,
-                               g.d_cell_start, g.d_cell_end, ctl);
+                               g.d_cell_start, g.d_cell_end, d_pos_, g.d_part, g.sb.vals, g.d_spart, g.d_spos, ctl);
         }
     }
+    // the slot copies of the interactions that k_prep_small prepares (k_cell_range made the others')
     for(size_t gi = 0; gi < globals_.size(); ++gi)
     {
         GlobalIA& g = globals_[gi];
-        if(g.partition != MJG_PART_CELLLIST) {continue;}
+        if(g.partition != MJG_PART_CELLLIST || !g.small) {continue;}
         const uint32_t np = (uint32_t)g.part.size();
         hipLaunchKernelGGL(k_sorted_slots, dim3(std::min<uint32_t>(blocks_for(np, 256), 1024)), dim3(256), 0, stream_,
                            d_pos_, g.d_part, g.sb.vals, np, g.d_spart, g.d_spos, d_lctl_ + gi);

@@ -131,14 +131,18 @@ __device__ __forceinline__ int cell_coord(double v, double lo, double inv_w, int
     return (int)cc;
 }
 
+// Cell key of every participant, and the cell ranges cleared for k_cell_range (the ranges are
+// read only after the sort, so clearing them here saves a dependent launch per step).
 __global__ void k_cellkey(const double4* __restrict__ pos, const uint32_t* __restrict__ part, uint32_t np,
                           const Grid* __restrict__ grid, uint32_t* __restrict__ keys,
-                          uint32_t* __restrict__ vals, const ListCtl* __restrict__ ctl)
+                          uint32_t* __restrict__ vals, uint32_t* __restrict__ cell_start,
+                          uint32_t* __restrict__ cell_end, const ListCtl* __restrict__ ctl)
 {
     if(ctl->rebuild == 0) {return;}
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if(p >= np) {return;}
     const Grid g = *grid;
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    for(uint32_t c = p; c < g.ncell; c += gridDim.x * blockDim.x) {cell_start[c] = 0; cell_end[c] = 0;}
+    if(p >= np) {return;}
     const double4 r = pos[part[p]];
     const int cx = cell_coord(r.x, g.lo[0], g.inv_w[0], g.dim[0]);
     const int cy = cell_coord(r.y, g.lo[1], g.inv_w[1], g.dim[1]);
@@ -147,16 +151,12 @@ __global__ void k_cellkey(const double4* __restrict__ pos, const uint32_t* __res
     vals[p] = p;
 }
 
-__global__ void k_cell_clear(uint32_t* __restrict__ cell_start, uint32_t* __restrict__ cell_end,
-                             const Grid* __restrict__ grid, const ListCtl* __restrict__ ctl)
-{
-    if(ctl->rebuild == 0) {return;}
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if(c < grid->ncell) {cell_start[c] = 0; cell_end[c] = 0;}
-}
-
+// After the sort: each cell's range of sorted slots, and the particle and position of every sorted
+// slot (k_sorted_slots' copies, made here for the interactions whose cells this kernel ranges).
 __global__ void k_cell_range(const uint32_t* __restrict__ skeys, uint32_t np, uint32_t* __restrict__ cell_start,
-                             uint32_t* __restrict__ cell_end, const ListCtl* __restrict__ ctl)
+                             uint32_t* __restrict__ cell_end, const double4* __restrict__ pos,
+                             const uint32_t* __restrict__ part, const uint32_t* __restrict__ svals,
+                             uint32_t* __restrict__ spart, double4* __restrict__ spos, const ListCtl* __restrict__ ctl)
 {
     if(ctl->rebuild == 0) {return;}
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -164,6 +164,9 @@ __global__ void k_cell_range(const uint32_t* __restrict__ skeys, uint32_t np, ui
     const uint32_t k = skeys[t];
     if(t == 0 || skeys[t - 1] != k) {cell_start[k] = t;}
     if(t == np - 1 || skeys[t + 1] != k) {cell_end[k] = t + 1;}
+    const uint32_t i = part[svals[t]];
+    spart[t] = i;
+    spos[t] = pos[i];
 }
 
 struct ListRule
