@@ -41,12 +41,14 @@ constexpr uint32_t kGridMinNp = 512;
 __host__ __device__ constexpr uint32_t grid_cells(uint32_t grid_bits) {return 1u << (3 * grid_bits);}
 constexpr uint32_t kRepClassMaxN[kRepClasses] = {256, 1024, 2048};
 
-// terms and list entries per thread per load round (fewer at 1024 threads, where a thread
-// has at most 128 VGPRs)
+// terms and list entries per thread per load round: one term and two list entries at 1024
+// threads (at most 128 VGPRs; larger batches spilled registers and measured 1-5 % slower) and at
+// 256 threads (no spill, three waves per SIMD; the same speed as four and eight); the FP64
+// kernel's 512-thread classes (256 VGPRs) take four and eight
 template<int BS> struct RepBatching
 {
-    static constexpr int terms = BS >= 1024 ? 2 : 4;
-    static constexpr int pairs = BS >= 1024 ? 4 : 8;
+    static constexpr int terms = (BS >= 1024 || BS == 256) ? 1 : 4;
+    static constexpr int pairs = (BS >= 1024 || BS == 256) ? 2 : 8;
 };
 
 enum RepTermKind : int
