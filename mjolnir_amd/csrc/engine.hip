@@ -571,18 +571,11 @@ void Engine::setup_global(GlobalIA& g)
             gr.ncell = (uint32_t)nc;
             g.grid = gr;
             upload(g.d_grid, &gr, 1);
-            g.key_bits = 1;
-            while((1u << g.key_bits) < gr.ncell) {++g.key_bits;}
         }
-        else
-        {
-            g.key_bits = 18;
-        }
-        g.sb.n = np;
-        g.sb.tiles = blocks_for(np, kSortTile);
         g.sb.keys = dalloc<uint32_t>(np); g.sb.vals = dalloc<uint32_t>(np);
-        g.sb.keys_alt = dalloc<uint32_t>(np); g.sb.vals_alt = dalloc<uint32_t>(np);
-        g.sb.hist = dalloc<uint32_t>(256 * (size_t)g.sb.tiles);
+        g.sb.ukeys = dalloc<uint32_t>(np);
+        g.sb.count = dalloc<uint32_t>(kMaxCells);
+        check(hipMemsetAsync(g.sb.count, 0, kMaxCells * sizeof(uint32_t), stream_), "memset");
         g.d_cell_start = dalloc<uint32_t>(kMaxCells);
         g.d_cell_end = dalloc<uint32_t>(kMaxCells);
     }
@@ -742,14 +735,20 @@ void Engine::launch_rebuilds(int force)
                                    d_pos_, g.d_part, np, ctl);
                 hipLaunchKernelGGL(k_grid_open, dim3(1), dim3(1), 0, stream_, ctl, g.d_grid, g.r_list);
             }
+            // a periodic grid's cell count is known here; an open grid's is set on the device
+            const uint32_t cells = box_.periodic ? g.grid.ncell : kMaxCells;
             hipLaunchKernelGGL(k_cellkey, dim3(blocks_for(np, 256)), dim3(256), 0, stream_, d_pos_, g.d_part, np,
-                               g.d_grid, g.sb.keys, g.sb.vals, g.d_cell_start, g.d_cell_end, ctl);
-            radix_sort(g.sb, g.key_bits, &ctl->rebuild, stream_);
-            hipLaunchKernelGGL(k_cell_range, dim3(blocks_for(np, 256)), dim3(256), 0, stream_, g.sb.keys, np,
-                               g.d_cell_start, g.d_cell_end, d_pos_, g.d_part, g.sb.vals, g.d_spart, g.d_spos, ctl);
+                               g.d_grid, g.sb.ukeys, g.sb.count, ctl);
+            hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, stream_, g.d_grid, g.sb.count, g.d_cell_start,
+                               g.d_cell_end, ctl);
+            hipLaunchKernelGGL(k_cell_place, dim3(blocks_for(np, 256)), dim3(256), 0, stream_, g.sb.ukeys, np,
+                               g.d_cell_end, g.sb.keys, g.sb.vals, ctl);
+            hipLaunchKernelGGL(k_cell_order, dim3(std::min<uint32_t>(blocks_for(cells, 256), 1024)), dim3(256), 0,
+                               stream_, g.d_grid, g.d_cell_start, g.d_cell_end, g.sb.count, g.sb.vals, d_pos_,
+                               g.d_part, g.d_spart, g.d_spos, ctl);
         }
     }
-    // the slot copies of the interactions that k_prep_small prepares (k_cell_range made the others')
+    // the slot copies of the interactions that k_prep_small prepares (k_cell_order made the others')
     for(size_t gi = 0; gi < globals_.size(); ++gi)
     {
         GlobalIA& g = globals_[gi];

@@ -7,7 +7,6 @@
 #include "integrate_kernels.hpp"
 #include "constraint_kernels.hpp"
 #include "nlist_kernels.hpp"
-#include "sort_kernels.hpp"
 #include "tile_kernels.hpp"
 
 #include <map>
@@ -52,13 +51,12 @@ struct GlobalIA
     double max_cutoff = 0, r_list = 0;
     ListRule rule{};
     Grid grid{};          // periodic grid (host copy)
-    int key_bits = 0;
 
     // device
     uint32_t *d_part = nullptr, *d_pslot = nullptr, *d_mol = nullptr, *d_ex_start = nullptr, *d_ex = nullptr;
     double* d_ppar = nullptr;
     Grid* d_grid = nullptr;
-    SortBuffers sb;
+    CellSort sb;
     uint32_t *d_cell_start = nullptr, *d_cell_end = nullptr;
     uint32_t *d_nl = nullptr, *d_nn = nullptr, *d_order = nullptr, *d_spart = nullptr;
     double4* d_spos = nullptr;    // position of each sorted slot at the last cell-list build

@@ -99,8 +99,10 @@ __global__ void k_grid_open(ListCtl* __restrict__ ctl, Grid* __restrict__ grid, 
     }
     double w = r_list * (1.0 + 1e-6);
     if(!(w > 0.0)) {w = 1.0;}
+    // widened until the grid fits kMaxCells; ext is finite, so w outgrows it (or becomes
+    // infinite, one cell) and the loop ends
     int dim[3];
-    for(int it = 0; it < 64; ++it)
+    for(;;)
     {
         unsigned long long prod = 1;
         for(int d = 0; d < 3; ++d)
@@ -131,42 +133,127 @@ __device__ __forceinline__ int cell_coord(double v, double lo, double inv_w, int
     return (int)cc;
 }
 
-// Cell key of every participant, and the cell ranges cleared for k_cell_range (the ranges are
-// read only after the sort, so clearing them here saves a dependent launch per step).
+// Device buffers of one interaction's cell sort: the sorted keys, the participant of every
+// sorted slot, the unsorted keys and the per-cell counts.
+struct CellSort
+{
+    uint32_t* keys = nullptr;
+    uint32_t* vals = nullptr;
+    uint32_t* ukeys = nullptr;
+    uint32_t* count = nullptr;
+};
+
+// Sorting the participants by cell (the reference sorts (participant, cell) pairs with std::sort,
+// PeriodicGridCellList.hpp:260-274) as a counting sort in four launches:
+//   k_cellkey     the cell of every participant, counted per cell;
+//   k_cell_scan   the counts turned into cell ranges;
+//   k_cell_place  every participant dropped into a free slot of its cell;
+//   k_cell_order  each cell's slots sorted by participant index, the slot copies made, the counts
+//                 cleared for the next rebuild.
+// The result (sorted keys, participant of every slot, cell ranges) is what a stable sort of the
+// keys gives, so the lists and every force sum built from them are deterministic.  On a step
+// without a rebuild each kernel returns at once: four dependent launches per step where a radix
+// sort of 16-bit keys takes eight.
+//
+// count[] holds zeros outside a rebuild (zeroed at allocation, cleared again by k_cell_order for
+// every cell a rebuild counted).
 __global__ void k_cellkey(const double4* __restrict__ pos, const uint32_t* __restrict__ part, uint32_t np,
-                          const Grid* __restrict__ grid, uint32_t* __restrict__ keys,
-                          uint32_t* __restrict__ vals, uint32_t* __restrict__ cell_start,
-                          uint32_t* __restrict__ cell_end, const ListCtl* __restrict__ ctl)
+                          const Grid* __restrict__ grid, uint32_t* __restrict__ ukeys,
+                          uint32_t* __restrict__ count, const ListCtl* __restrict__ ctl)
 {
     if(ctl->rebuild == 0) {return;}
-    const Grid g = *grid;
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    for(uint32_t c = p; c < g.ncell; c += gridDim.x * blockDim.x) {cell_start[c] = 0; cell_end[c] = 0;}
     if(p >= np) {return;}
+    const Grid g = *grid;
     const double4 r = pos[part[p]];
     const int cx = cell_coord(r.x, g.lo[0], g.inv_w[0], g.dim[0]);
     const int cy = cell_coord(r.y, g.lo[1], g.inv_w[1], g.dim[1]);
     const int cz = cell_coord(r.z, g.lo[2], g.inv_w[2], g.dim[2]);
-    keys[p] = (uint32_t)cx + (uint32_t)g.dim[0] * ((uint32_t)cy + (uint32_t)g.dim[1] * (uint32_t)cz);
-    vals[p] = p;
+    const uint32_t k = (uint32_t)cx + (uint32_t)g.dim[0] * ((uint32_t)cy + (uint32_t)g.dim[1] * (uint32_t)cz);
+    ukeys[p] = k;
+    atomicAdd(&count[k], 1u);
 }
 
-// After the sort: each cell's range of sorted slots, and the particle and position of every sorted
-// slot (k_sorted_slots' copies, made here for the interactions whose cells this kernel ranges).
-__global__ void k_cell_range(const uint32_t* __restrict__ skeys, uint32_t np, uint32_t* __restrict__ cell_start,
-                             uint32_t* __restrict__ cell_end, const double4* __restrict__ pos,
-                             const uint32_t* __restrict__ part, const uint32_t* __restrict__ svals,
-                             uint32_t* __restrict__ spart, double4* __restrict__ spos, const ListCtl* __restrict__ ctl)
+// Exclusive prefix of the counts into cell_start; cell_end starts equal (k_cell_place's cursor).
+// One workgroup of 1024 threads, each owning a contiguous run of at most kMaxCells / 1024 cells.
+__global__ void __launch_bounds__(1024)
+k_cell_scan(const Grid* __restrict__ grid, const uint32_t* __restrict__ count, uint32_t* __restrict__ cell_start,
+            uint32_t* __restrict__ cell_end, const ListCtl* __restrict__ ctl)
 {
     if(ctl->rebuild == 0) {return;}
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if(t >= np) {return;}
-    const uint32_t k = skeys[t];
-    if(t == 0 || skeys[t - 1] != k) {cell_start[k] = t;}
-    if(t == np - 1 || skeys[t + 1] != k) {cell_end[k] = t + 1;}
-    const uint32_t i = part[svals[t]];
-    spart[t] = i;
-    spos[t] = pos[i];
+    __shared__ uint32_t s[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t n = grid->ncell;
+    const uint32_t chunk = (n + 1023) / 1024;
+    const uint32_t b = t * chunk;
+    const uint32_t e = (b + chunk < n) ? b + chunk : n;
+    uint32_t sum = 0;
+    for(uint32_t c = b; c < e; ++c) {sum += count[c];}
+    s[t] = sum;
+    __syncthreads();
+    for(uint32_t off = 1; off < 1024; off <<= 1)
+    {
+        const uint32_t v = (t >= off) ? s[t - off] : 0u;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = s[t] - sum;
+    for(uint32_t c = b; c < e; ++c)
+    {
+        cell_start[c] = run;
+        cell_end[c] = run;
+        run += count[c];
+    }
+}
+
+// A free slot of its cell for every participant; cell_end ends as the end of each range.
+__global__ void k_cell_place(const uint32_t* __restrict__ ukeys, uint32_t np, uint32_t* __restrict__ cell_end,
+                             uint32_t* __restrict__ skeys, uint32_t* __restrict__ svals,
+                             const ListCtl* __restrict__ ctl)
+{
+    if(ctl->rebuild == 0) {return;}
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if(p >= np) {return;}
+    const uint32_t k = ukeys[p];
+    const uint32_t slot = atomicAdd(&cell_end[k], 1u);
+    skeys[slot] = k;
+    svals[slot] = p;
+}
+
+// One thread per cell: the cell's slots put in participant order (an insertion sort, the cells
+// hold tens of participants), then the particle and position of every slot (the slot copies the
+// list builds read), and the cell's count cleared.
+__global__ void k_cell_order(const Grid* __restrict__ grid, const uint32_t* __restrict__ cell_start,
+                             const uint32_t* __restrict__ cell_end, uint32_t* __restrict__ count,
+                             uint32_t* __restrict__ svals, const double4* __restrict__ pos,
+                             const uint32_t* __restrict__ part, uint32_t* __restrict__ spart,
+                             double4* __restrict__ spos, const ListCtl* __restrict__ ctl)
+{
+    if(ctl->rebuild == 0) {return;}
+    const uint32_t n = grid->ncell;
+    for(uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+    {
+        count[c] = 0;
+        const uint32_t b = cell_start[c], e = cell_end[c];
+        for(uint32_t x = b + 1; x < e; ++x)
+        {
+            const uint32_t v = svals[x];
+            uint32_t y = x;
+            while(y > b && svals[y - 1] > v)
+            {
+                svals[y] = svals[y - 1];
+                --y;
+            }
+            svals[y] = v;
+        }
+        for(uint32_t x = b; x < e; ++x)
+        {
+            const uint32_t i = part[svals[x]];
+            spart[x] = i;
+            spos[x] = pos[i];
+        }
+    }
 }
 
 struct ListRule
@@ -624,7 +711,7 @@ k_prep_small(const double4* __restrict__ pos, PrepSegTable tab, StepCtl* __restr
     }
 }
 
-// Skin check for interactions on the multi-kernel (radix-sort) path.
+// Skin check for interactions on the multi-kernel (counting-sort) path.
 __global__ void k_margin_large(PrepSegTable tab, StepCtl* __restrict__ sctl, int force)
 {
     const int g = threadIdx.x;
