@@ -266,6 +266,8 @@ struct ListRule
     int base_rule;       // 0 none; 1 no complementary base pair (3SPN2 EV); 2 complementary pairs only
     int base_stride;     // columns of base_par per particle (the base code is column 0)
     const double* base_par;
+    uint32_t last_part;  // participants.back()
+    int simple;          // 1: cell list without exclusions, molecule or base rule
 };
 
 
@@ -274,6 +276,9 @@ __device__ __forceinline__ bool pair_allowed(uint32_t lo, uint32_t hi, const Lis
                                              const uint32_t* __restrict__ pslot, const uint32_t* __restrict__ mol,
                                              const uint32_t* __restrict__ ex_start, const uint32_t* __restrict__ ex)
 {
+    // Only the leading-participant rule applies: lo leads unless it is the last participant (pslot
+    // maps participants one to one), decided without reading a table.
+    if(rule.simple) {return lo != rule.last_part;}
     const uint32_t slo = pslot[lo];
     if(rule.pairwise) {if(slo > pslot[hi]) {return false;}}
     else {if(slo == rule.last_slot) {return false;}}
