@@ -426,9 +426,10 @@ void Engine::make_exclusions(GlobalIA& g)
         all.insert(all.end(), tmp.begin(), tmp.end());
     }
     start[n_] = (uint32_t)all.size();
-    // a cell list with no exclusion, molecule or base rule decides a pair by its lower index alone
+    // a cell list with no exclusion (every particle's list holds only itself, which never matches a
+    // partner), molecule or base rule decides a pair by its lower index alone
     g.rule.last_part = g.part.empty() ? kNoSlot : g.part.back();
-    g.rule.simple = all.empty() && !g.rule.pairwise && g.rule.ignore_mol == 0 && g.rule.base_rule == 0;
+    g.rule.simple = all.size() == n_ && !g.rule.pairwise && g.rule.ignore_mol == 0 && g.rule.base_rule == 0;
     g.d_ex_start = dalloc<uint32_t>(n_ + 1);
     g.d_ex = dalloc<uint32_t>(all.size());
     upload(g.d_ex_start, start.data(), n_ + 1);
