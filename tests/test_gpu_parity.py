@@ -336,6 +336,55 @@ def test_engine_mixed_precision_forces_vs_reference(name):
             assert make_golden.pair_key_hash(p, spec.n) == str(g[f"pairhash_{k}"]), names[k]
 
 
+def test_large_cell_list_vs_oracle():
+    """Periodic LJ fluids above the one-workgroup rebuild's 4096 participants, whose cell lists come
+    from the multi-kernel rebuild (the counting sort of cell keys, each cell's slots then put in
+    participant order): 8000 particles with the gather kernel and with the cell-tile kernel forced
+    on, and 4913 particles after 300 Langevin steps through list rebuilds.  Against the oracle:
+    the neighbour pair set (count and hash) and FP64 forces within 1e-10; the tiled and gather
+    engines bit for bit equal after the steps."""
+    import os
+    import oracle
+    from mjolnir_amd import capi, workloads
+    spec = make_golden.perturb(workloads.lj_fluid(n_side=20, seed=51), 0.1, 52)
+    o = oracle.OracleEngine(spec, with_integrator=False)
+    f_ref, _ = o.force_and_energy()
+    p_ref = o.pairs(0)
+    for env in ({"MJG_NO_TILE": "1"}, {"MJG_TILE_MIN_NP": "0"}):
+        for k, v in env.items():
+            os.environ[k] = v
+        try:
+            e = _engine(spec, with_integrator=False)
+        finally:
+            for k in env:
+                del os.environ[k]
+        print(f"[gpu] large cell list {env}", flush=True)
+        e.calc_force()
+        _, _, f = e.state()
+        err = np.linalg.norm(f - f_ref, axis=1)
+        assert np.all(err <= force_tolerance(f_ref, 1e-10)), (env, err.max())
+        p = e.pairs(0)
+        assert len(p) == len(p_ref), env
+        assert make_golden.pair_key_hash(p, spec.n) == make_golden.pair_key_hash(p_ref, spec.n), env
+        e.close()
+    spec2 = make_golden.perturb(workloads.lj_fluid(n_side=17, seed=53), 0.1, 54)
+    runs = []
+    for env in ({"MJG_NO_TILE": "1"}, {"MJG_TILE_MIN_NP": "0"}):
+        for k, v in env.items():
+            os.environ[k] = v
+        try:
+            e = _engine(spec2)
+        finally:
+            for k in env:
+                del os.environ[k]
+        e.step(300)
+        assert e.counters()[1] > 1
+        runs.append(e.state())
+        e.close()
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
+
+
 def test_engine_mixed_precision_lj65536_vs_oracle():
     """A 65,536-particle LJ fluid (the size class of the served LJ jobs) in MJG_FP32_FIXED against
     the FP64 oracle: RMS force error <= 1e-5 of the RMS force; the same box run 200 steps in both
